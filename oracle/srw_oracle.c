@@ -1,0 +1,1323 @@
+/*
+ * srw_oracle.c -- FP64 CPU restatement of SRW's single-electron
+ * emit-and-propagate path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker of the GPU library:
+ * only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+ * load it (as liboracle.so).  The product (srw_amd/) never links or calls it.
+ *
+ * Every function cites the reference file:line whose algorithm it restates
+ * (paths relative to the reference source root).  Arithmetic is double
+ * precision throughout; the wavefront arrays are float32 like the reference's.
+ * The exported functions take the same C structures as the product
+ * (include/srwlib.h) and are prefixed "orc_".
+ *
+ * Pinning of each function (closed forms, invariants, brute force) lives in
+ * tests/test_oracle_*.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <complex.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "../include/srwlib.h"
+
+#define ORC_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------ */
+/* Constants (reference appendix of parity-critical values)            */
+/* ------------------------------------------------------------------ */
+static const double kElecEn0GeV = 0.51099890221e-03;   /* srwlib.cpp:332 */
+static const double kPI_SR = 3.141592653590;           /* srradint.cpp:48 */
+
+/* srebmdat.h:136-141: internal gamma from energy in GeV */
+static double orc_gamma_internal(double gammaApi, double relE0)
+{
+    double EGeV = gammaApi * relE0 * kElecEn0GeV;
+    return EGeV * 1000. / 0.511003414;
+}
+
+/* gmmeth.h:390-394 */
+static double orc_radical_one_plus_small(double x)
+{
+    if (fabs(x) > 0.01) return sqrt(1. + x);
+    return 1. + x * (0.5 + x * (-0.125 + x * (0.0625 + x * (-0.0390625 + x * (0.02734375 +
+           x * (-0.0205078125 + x * 0.01611328125))))));
+}
+
+/* ------------------------------------------------------------------ */
+/* Magnetic field: periodic undulator in a container                   */
+/* ------------------------------------------------------------------ */
+#define ORC_MAX_HARM 16
+#define ORC_MAX_UND 8
+typedef struct {
+    int nh;
+    int hn[ORC_MAX_HARM];
+    char xz[ORC_MAX_HARM];
+    double K[ORC_MAX_HARM], ph[ORC_MAX_HARM];
+    int s[ORC_MAX_HARM];
+    double per, totLen;
+    double cx, cy, cz;
+} orc_und;
+
+typedef struct {
+    int n;
+    orc_und u[ORC_MAX_UND];
+    double sStart, sEnd;
+} orc_fld;
+
+/* srmagcnt.cpp:20-140 (container of 'u' elements) and srmagfld.cpp:128-174 */
+static int orc_fld_from_c(const SRWLMagFldC* c, orc_fld* f)
+{
+    if (c == 0 || c->arMagFld == 0 || c->arMagFldTypes == 0) return SRWL_NO_FUNC_ARG_DATA;
+    if (c->nElem <= 0 || c->nElem > ORC_MAX_UND) return SRWL_AMD_ERR_UNSUPPORTED;
+    int cen = (c->arXc != 0) && (c->arYc != 0) && (c->arZc != 0);
+    f->n = c->nElem;
+    f->sStart = 1.e+23; f->sEnd = -1.e+23;
+    for (int i = 0; i < c->nElem; i++) {
+        if (c->arMagFldTypes[i] != 'u') return SRWL_AMD_ERR_UNSUPPORTED;
+        const SRWLMagFldU* U = (const SRWLMagFldU*)c->arMagFld[i];
+        orc_und* u = &f->u[i];
+        if (U->per <= 0) return SRWL_ERR_INCORRECT_MAG_FIELD_PER;
+        if (U->nPer <= 0) return SRWL_ERR_INCORRECT_PER_MAG_FIELD_LENGTH;
+        if (U->nHarm > ORC_MAX_HARM) return SRWL_AMD_ERR_UNSUPPORTED;
+        u->per = U->per;
+        u->totLen = (2 * U->nPer) * U->per * 0.5;
+        u->cx = cen ? c->arXc[i] : 0.; u->cy = cen ? c->arYc[i] : 0.; u->cz = cen ? c->arZc[i] : 0.;
+        u->nh = U->nHarm;
+        for (int k = 0; k < U->nHarm; k++) {
+            const SRWLMagFldH* H = &U->arHarm[k];
+            char hv = H->h_or_v;
+            u->xz[k] = (hv == 'h' || hv == 'H') ? 'x' : 'z';
+            u->K[k] = 93.37290417576577 * U->per * H->B;
+            u->hn[k] = H->n; u->ph[k] = H->ph; u->s[k] = H->s;
+        }
+        double half = 0.5 * u->totLen + 4 * u->per;
+        double s0 = u->cz - half, s1 = u->cz + half;
+        if (s0 < f->sStart) f->sStart = s0;
+        if (s1 > f->sEnd) f->sEnd = s1;
+    }
+    return 0;
+}
+
+/* srmagfld.h:401-502: adds the field of one undulator at local point */
+static void orc_und_addB(const orc_und* u, double xr, double yr, double zr, double* B)
+{
+    (void)xr;
+    const double pi = 3.1415926535898, twoPi = 6.2831853072, piE2 = pi * pi;
+    const double factK2B = 93.372904175766;
+    double halfTot = 0.5 * u->totLen;
+    double halfTotTerm = halfTot + 4 * u->per;
+    if (zr < -halfTotTerm || zr > halfTotTerm) return;
+    for (int i = 0; i < u->nh; i++) {
+        double ph = u->ph[i];
+        long long n2pi = (long long)(fabs(ph) / twoPi);
+        if (ph > 0) ph -= twoPi * n2pi; else if (ph < 0) ph += twoPi * n2pi;
+        double perH = u->per / u->hn[i];
+        double halfPerH = 0.5 * perH;
+        double s0 = -ph * perH / twoPi;
+        double zrr = zr - s0;
+        if (zrr < -halfTotTerm || zrr > halfTotTerm) continue;
+        double kw = twoPi / perH;
+        double Bm = (u->K[i] / (u->per * factK2B)) * cosh(kw * yr);
+        double arg = kw * zrr;
+        double dB;
+        double halfThis = halfTot;
+        if (u->s[i] == 1) halfThis += 0.25 * u->per;
+        if (zrr < -halfThis) {
+            double perE2 = perH * perH, p = 2. * piE2 / (3. * perE2);
+            double za = zrr + halfThis, za2 = za * za;
+            dB = (twoPi * Bm * za / perH) * (1. - 4. * piE2 * za2 / (9. * perE2)) * exp(-p * za2);
+            double zt = -halfThis + 0.5 * halfPerH;
+            double tt = (u->s[i] == 1) ? cos(kw * zt) : sin(kw * zt);
+            if (tt < 0) dB = -dB;
+        } else if (zrr > halfThis) {
+            double perE2 = perH * perH, p = 2. * piE2 / (3. * perE2);
+            double za = zrr - halfThis, za2 = za * za;
+            dB = (twoPi * Bm * za / perH) * (1. - 4. * piE2 * za2 / (9. * perE2)) * exp(-p * za2);
+            double zt = halfThis - 0.5 * halfPerH;
+            double tt = (u->s[i] == 1) ? cos(kw * zt) : sin(kw * zt);
+            if (tt > 0) dB = -dB;
+        } else {
+            dB = Bm * ((u->s[i] == 1) ? cos(arg) : sin(arg));
+        }
+        if (u->xz[i] == 'x') B[0] += dB; else B[1] += dB;
+    }
+}
+
+/* srmagcnt.h:84-98 with translation-only element frames */
+static void orc_fld_B(const orc_fld* f, double x, double y, double z, double* B)
+{
+    B[0] = B[1] = B[2] = 0.;
+    for (int i = 0; i < f->n; i++)
+        orc_und_addB(&f->u[i], x - f->u[i].cx, y - f->u[i].cy, z - f->u[i].cz, B);
+}
+
+ORC_API void orc_field_B(const SRWLMagFldC* c, double x, double y, double z, double* B)
+{
+    orc_fld f;
+    B[0] = B[1] = B[2] = 0;
+    if (orc_fld_from_c(c, &f)) return;
+    orc_fld_B(&f, x, y, z, B);
+}
+
+/* ------------------------------------------------------------------ */
+/* RK4 trajectory: srgtrjdt.cpp:24-365, gmintrk.h:129-200               */
+/* ------------------------------------------------------------------ */
+typedef struct { const orc_fld* f; double gEm2, mult; } orc_rk;
+
+/* srgtrjdt.h:279-293 */
+static void orc_rk_deriv(const orc_rk* r, const double* F, double* dF)
+{
+    double xd = F[1], yd = F[3];
+    double zd = orc_radical_one_plus_small(-(r->gEm2 + xd * xd + yd * yd));
+    double B[3];
+    orc_fld_B(r->f, F[0], F[2], F[4], B);
+    dF[0] = xd;
+    dF[1] = r->mult * (yd * B[2] - zd * B[1]);
+    dF[2] = yd;
+    dF[3] = r->mult * (zd * B[0] - xd * B[2]);
+    dF[4] = zd;
+}
+
+/* gmintrk.h:169-200 */
+static void orc_rk4_step(const orc_rk* r, double* y, const double* dydx, double h)
+{
+    double hh = 0.5 * h, h6 = h / 6., yt[5], dyt[5], dym[5];
+    for (int i = 0; i < 5; i++) yt[i] = y[i] + hh * dydx[i];
+    orc_rk_deriv(r, yt, dyt);
+    for (int i = 0; i < 5; i++) yt[i] = y[i] + hh * dyt[i];
+    orc_rk_deriv(r, yt, dym);
+    for (int i = 0; i < 5; i++) { yt[i] = y[i] + h * dym[i]; dym[i] += dyt[i]; }
+    orc_rk_deriv(r, yt, dyt);
+    for (int i = 0; i < 5; i++) y[i] += h6 * (dydx[i] + dyt[i] + 2. * dym[i]);
+}
+
+/* gmintrk.h:129-165: res[i*6 + {0:x-arg, 1..5: state}] */
+static void orc_rk_solve(const orc_rk* r, const double* init, double xmin, double xmax,
+                         long long np, double* res)
+{
+    double step = (xmax - xmin) / (double)(np - 1), x = xmin, y[5], d[5];
+    for (int k = 0; k < 5; k++) y[k] = init[k];
+    for (long long i = 0; i < np; i++) {
+        orc_rk_deriv(r, y, d);
+        double* t = res + i * 6;
+        t[0] = x;
+        for (int k = 0; k < 5; k++) t[k + 1] = y[k];
+        if (i != np - 1) { orc_rk4_step(r, y, d, step); x = x + step; }
+    }
+}
+
+/* srgtrjdt.cpp:24-365 (manual RK4 branch; the branches that first "arrive"
+ * at a range not containing ct = 0 are restated too). */
+static int orc_traj_rk(const orc_fld* f, double gammaInt, int nq, const double* init5,
+                       double sStart, double sEnd, long long ns,
+                       double* oBtx, double* oX, double* oBty, double* oY, double* oBtz, double* oZ)
+{
+    if (ns <= 0) return SRWL_INCORRECT_PARAM_FOR_TRJ_COMP;
+    double sStep = (ns <= 1) ? 0 : (sEnd - sStart) / (ns - 1);
+    orc_rk r;
+    r.f = f;
+    r.gEm2 = (gammaInt != 0) ? 1. / gammaInt / gammaInt : 0;
+    r.mult = nq * 1.602176462e-19 / (9.10938188e-31 * 2.99792458e+08 * gammaInt);
+    double sMin, sMax;
+    int rot = 0;
+    if (sStart <= sEnd) { sMin = sStart; sMax = sEnd; } else { sMin = sEnd; sMax = sStart; rot = 1; }
+    double tol = (sMax - sMin) * 1.e-12;
+    int left = (sMin < -tol), right = (sMax > tol);
+    double ic[5];
+    for (int k = 0; k < 5; k++) ic[k] = init5[k];
+    long long is0 = 0;
+    double s0Act = 0.;
+    double gEm2 = r.gEm2;
+#define ORC_STORE_FWD(buf, cnt, off)                                                      \
+    for (long long j = 0; j < (cnt); j++) {                                               \
+        const double* t = (buf) + j * 6;                                                  \
+        oX[(off) + j] = t[1]; oBtx[(off) + j] = t[2]; oY[(off) + j] = t[3];               \
+        oBty[(off) + j] = t[4];                                                           \
+        if (oZ) oZ[(off) + j] = t[5];                                                     \
+        if (oBtz) oBtz[(off) + j] = orc_radical_one_plus_small(-(gEm2 + t[2] * t[2] + t[4] * t[4])); \
+    }
+#define ORC_STORE_REV(buf, cnt, off)                                                      \
+    for (long long j = 0; j < (cnt); j++) {                                               \
+        const double* t = (buf) + ((cnt) - 1 - j) * 6;                                    \
+        oX[(off) + j] = t[1]; oBtx[(off) + j] = t[2]; oY[(off) + j] = t[3];               \
+        oBty[(off) + j] = t[4];                                                           \
+        if (oZ) oZ[(off) + j] = t[5];                                                     \
+        if (oBtz) oBtz[(off) + j] = orc_radical_one_plus_small(-(gEm2 + t[2] * t[2] + t[4] * t[4])); \
+    }
+    if (left) {
+        if (sMax < -tol) {
+            int auxNp = (int)fabs(sMax / sStep) + 1;
+            if (auxNp <= 1) auxNp = 2;
+            double* aux = (double*)malloc(sizeof(double) * (auxNp + ns) * 6);
+            if (!aux) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+            orc_rk_solve(&r, ic, 0., sMax, auxNp, aux);
+            double* pe = aux + auxNp * 6 - 5;
+            for (int i = 0; i < 5; i++) ic[i] = pe[i];
+            if (ns <= 1) { aux[0] = sMax; for (int i = 0; i < 5; i++) aux[1 + i] = ic[i]; }
+            else orc_rk_solve(&r, ic, sMax, sMin, ns, aux);
+            if (rot) { ORC_STORE_FWD(aux, ns, 0) } else { ORC_STORE_REV(aux, ns, 0) }
+            free(aux);
+        } else {
+            is0 = (int)fabs((-sMin + tol) / sStep);
+            if (is0 >= ns) is0 = ns - 1;
+            s0Act = sMin + is0 * sStep;
+            if (s0Act < -tol) {
+                double two[12];
+                orc_rk_solve(&r, ic, 0., s0Act, 2, two);
+                double* pe = two + 5 + 2;
+                for (int i = 0; i < 5; i++) ic[i] = pe[i];
+            }
+            long long nsL = is0 + 1;
+            double* aux = (double*)malloc(sizeof(double) * nsL * 6);
+            if (!aux) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+            if (nsL <= 1) { aux[0] = s0Act; for (int i = 0; i < 5; i++) aux[1 + i] = ic[i]; }
+            else orc_rk_solve(&r, ic, s0Act, sMin, nsL, aux);
+            if (rot) { ORC_STORE_FWD(aux, nsL, 0) } else { ORC_STORE_REV(aux, nsL, 0) }
+            free(aux);
+        }
+    }
+    if (right) {
+        if (sMin > tol) {
+            int auxNp = (int)fabs(sMin / sStep) + 1;
+            if (auxNp <= 1) auxNp = 2;
+            double* aux = (double*)malloc(sizeof(double) * (auxNp + ns) * 6);
+            if (!aux) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+            orc_rk_solve(&r, ic, 0., sMin, auxNp, aux);
+            double* pe = aux + auxNp * 6 - 5;
+            for (int i = 0; i < 5; i++) ic[i] = pe[i];
+            if (ns <= 1) { aux[0] = sMin; for (int i = 0; i < 5; i++) aux[1 + i] = ic[i]; }
+            else orc_rk_solve(&r, ic, sMin, sMax, ns, aux);
+            if (rot) { ORC_STORE_REV(aux, ns, is0) } else { ORC_STORE_FWD(aux, ns, is0) }
+            free(aux);
+        } else {
+            long long nsR = ns - is0;
+            if (nsR < 1) nsR = 1;
+            double* aux = (double*)malloc(sizeof(double) * nsR * 6);
+            if (!aux) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+            if (nsR <= 1) { aux[0] = s0Act; for (int i = 0; i < 5; i++) aux[1 + i] = ic[i]; }
+            else orc_rk_solve(&r, ic, s0Act, sMax, nsR, aux);
+            if (rot) { ORC_STORE_REV(aux, nsR, is0) } else { ORC_STORE_FWD(aux, nsR, is0) }
+            free(aux);
+        }
+    }
+#undef ORC_STORE_FWD
+#undef ORC_STORE_REV
+    return 0;
+}
+
+/* srwlib.cpp:328-373 */
+ORC_API int orc_srwlCalcPartTraj(SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld, double* precPar)
+{
+    (void)precPar;
+    if (pTrj == 0 || pMagFld == 0) return SRWL_NO_FUNC_ARG_DATA;
+    if (!pTrj->arX || !pTrj->arXp || !pTrj->arY || !pTrj->arYp || pTrj->np <= 0) return SRWL_INCORRECT_TRJ_STRUCT;
+    orc_fld f;
+    int res = orc_fld_from_c(pMagFld, &f);
+    if (res) return res;
+    SRWLParticle* p = &pTrj->partInitCond;
+    double sSt = pTrj->ctStart, sEn = pTrj->ctEnd;
+    if (sSt >= sEn) {
+        sSt = f.sStart; sEn = f.sEnd;
+        if (sSt > p->z) sSt = p->z;
+        if (sEn < p->z) sEn = p->z;
+        sSt -= p->z; sEn -= p->z;
+        pTrj->ctStart = sSt; pTrj->ctEnd = sEn;
+    }
+    double g = orc_gamma_internal(p->gamma, p->relE0);
+    double init[5] = {p->x, p->xp, p->y, p->yp, p->z};
+    return orc_traj_rk(&f, g, p->nq, init, pTrj->ctStart, pTrj->ctEnd, pTrj->np,
+                       pTrj->arXp, pTrj->arX, pTrj->arYp, pTrj->arY, pTrj->arZp, pTrj->arZ);
+}
+
+/* ------------------------------------------------------------------ */
+/* Trajectory interpolation structure: srtrjdat.cpp:1724-2088,          */
+/* srtrjdat.h:862-892, 955-1056                                         */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    long long np;
+    double sStart, sStep, invStep;
+    double gamma, gammaEm2, betaNorm, invBetaNorm;
+    double x0, dxds0, z0, dzds0, s0;
+    short horNZ, verNZ;
+    /* per interval: crd[6], bt[5], b[4], ib[6] for x (hor motion) and z (vert) */
+    double *xc, *xbt, *xb, *xib;
+    double *zc, *zbt, *zb, *zib;
+} orc_trj;
+
+/* srtrjdat.h:1037-1056 */
+static void orc_coefs5(double h, const double* f, double* c)
+{
+    double hi = 1. / h, hi2 = hi * hi;
+    double d1 = 0.016666666666667 * hi, d2 = 0.041666666666667 * hi2;
+    double d3 = d2 * hi, d4 = d3 * hi, d5 = 0.2 * d4 * hi;
+    double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5];
+    c[0] = f2;
+    c[1] = (3 * f0 - 30 * f1 - 20 * f2 + 60 * f3 - 15 * f4 + 2 * f5) * d1;
+    c[2] = -(f0 - 16 * f1 + 30 * f2 - 16 * f3 + f4) * d2;
+    c[3] = -(f0 + f1 - 10 * f2 + 14 * f3 - 7 * f4 + f5) * d3;
+    c[4] = (f0 - 4 * (f1 + f3) + 6 * f2 + f4) * d4;
+    c[5] = (-f0 + 5 * (f1 - f4) + 10 * (f3 - f2) + f5) * d5;
+}
+
+/* gminterp.h:608-613 */
+static double orc_interp_cub_half(const double* f, int i)
+{
+    if (i < 0) return 0.0625 * (5. * f[0] + 15. * f[1] - 5. * f[2] + f[3]);
+    else if (i == 0) return 0.0625 * (-f[0] + 9. * f[1] + 9. * f[2] - f[3]);
+    else return 0.0625 * (f[0] - 5. * f[1] + 15. * f[2] + 5. * f[3]);
+}
+
+static long long orc_offset5(long long i, long long np)
+{
+    if (i < 2) return 0;
+    if (i < np - 3) return i - 2;
+    if (i < np - 2) return i - 3;
+    return i - 4;
+}
+
+/* srtrjdat.cpp:1724-1805 (cumulative Simpson of the squared angle, zeroed at s0) */
+static void orc_quad_phase_terms(const double* ang, long long np, double ctStart, double ctEnd,
+                                 double sStart, double s0, double* out)
+{
+    long long n1 = np - 1, n2 = n1 - 1;
+    double sStep = (n1 > 0) ? (ctEnd - ctStart) / n1 : 0;
+    double stepMult = 0.333333333333 * (0.5 * sStep);
+    out[0] = 0;
+    const double* p = ang;
+    const double* t = ang + 1;
+    int mid = -1;
+    double f0 = 0., cur = 0.;
+    for (long long i = 1; i <= n1; i++) {
+        double a = orc_interp_cub_half(p, mid);
+        double f1 = a * a;
+        a = *(t++);
+        double f2 = a * a;
+        cur += stepMult * (f0 + 4. * f1 + f2);
+        out[i] = cur;
+        f0 = f2;
+        if ((i > 1) && (i < n2)) p++;
+        mid = (i < n1) ? 0 : 1;
+    }
+    /* srtrjdat.h:955-1003 FindOffestAndRelArgFromTrj, then InterpFuncAndDerivs */
+    long long idx = (long long)((s0 - sStart) / sStep);
+    if (idx >= np - 1) idx = np - 2; else if (idx < 0) idx = 0;
+    double rel = s0 - (sStart + sStep * idx);
+    long long off;
+    if (idx < 2) { off = 0; rel -= (2 - idx) * sStep; }
+    else if (idx < np - 3) off = idx - 2;
+    else if (idx < np - 2) { off = idx - 3; rel += sStep; }
+    else { off = idx - 4; rel += 2 * sStep; }
+    double c[6];
+    orc_coefs5(sStep, out + off, c);
+    double v = c[0] + rel * (c[1] + rel * (c[2] + rel * (c[3] + rel * (c[4] + rel * c[5]))));
+    for (long long j = 0; j <= n1; j++) out[j] -= v;
+}
+
+static void orc_trj_free(orc_trj* t)
+{
+    free(t->xc); free(t->xbt); free(t->xb); free(t->xib);
+    free(t->zc); free(t->zbt); free(t->zb); free(t->zib);
+    memset(t, 0, sizeof(*t));
+}
+
+/* srtrjdat.cpp:2006-2088 */
+static int orc_trj_build(orc_trj* T, const SRWLPrtTrj* trj)
+{
+    memset(T, 0, sizeof(*T));
+    if (trj->np < 6) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
+    const SRWLParticle* p = &trj->partInitCond;
+    T->gamma = orc_gamma_internal(p->gamma, p->relE0);
+    T->gammaEm2 = 1. / T->gamma / T->gamma;
+    T->x0 = p->x; T->dxds0 = p->xp; T->z0 = p->y; T->dzds0 = p->yp; T->s0 = p->z;
+    T->np = trj->np;
+    T->sStart = trj->ctStart + p->z;
+    T->sStep = (trj->ctEnd - trj->ctStart) / (trj->np - 1);
+    T->invStep = 1. / T->sStep;
+    long long np = trj->np;
+    /* srtrjdat.cpp:2048-2062 */
+    for (long long i = 0; i < np; i++) {
+        if (trj->arX[i] != 0. || trj->arXp[i] != 0.) T->verNZ = 1;
+        if (trj->arY[i] != 0. || trj->arYp[i] != 0.) T->horNZ = 1;
+        if (T->verNZ && T->horNZ) break;
+    }
+    /* srgtrjdt.h:247-258 */
+    T->betaNorm = -1.60217646263E-19 / (T->gamma * 9.1093818872E-31 * 2.99792458E+08);
+    T->invBetaNorm = 1. / T->betaNorm;
+    double* ibx = (double*)malloc(sizeof(double) * np);
+    double* ibz = (double*)malloc(sizeof(double) * np);
+    long long ni = np - 1;
+    T->xc = (double*)malloc(sizeof(double) * ni * 6); T->xbt = (double*)malloc(sizeof(double) * ni * 5);
+    T->xb = (double*)malloc(sizeof(double) * ni * 4); T->xib = (double*)malloc(sizeof(double) * ni * 6);
+    T->zc = (double*)malloc(sizeof(double) * ni * 6); T->zbt = (double*)malloc(sizeof(double) * ni * 5);
+    T->zb = (double*)malloc(sizeof(double) * ni * 4); T->zib = (double*)malloc(sizeof(double) * ni * 6);
+    if (!ibx || !ibz || !T->xc || !T->zc || !T->xbt || !T->zbt || !T->xb || !T->zb || !T->xib || !T->zib) {
+        free(ibx); free(ibz); orc_trj_free(T); return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    }
+    orc_quad_phase_terms(trj->arXp, np, trj->ctStart, trj->ctEnd, T->sStart, T->s0, ibx);
+    orc_quad_phase_terms(trj->arYp, np, trj->ctStart, trj->ctEnd, T->sStart, T->s0, ibz);
+    /* srtrjdat.cpp:1868-1926 */
+    double h = (trj->ctEnd - trj->ctStart) / (np - 1);
+    for (int pl = 0; pl < 2; pl++) {
+        const double* crd = pl == 0 ? trj->arX : trj->arY;
+        const double* ib = pl == 0 ? ibx : ibz;
+        double bm = pl == 0 ? -T->invBetaNorm : T->invBetaNorm;
+        double *C = pl == 0 ? T->xc : T->zc, *BT = pl == 0 ? T->xbt : T->zbt;
+        double *BB = pl == 0 ? T->xb : T->zb, *IB = pl == 0 ? T->xib : T->zib;
+        for (long long i = 0; i < ni; i++) {
+            long long off = orc_offset5(i, np);
+            double* c = C + i * 6;
+            orc_coefs5(h, ib + off, IB + i * 6);
+            orc_coefs5(h, crd + off, c);
+            double* bt = BT + i * 5;
+            bt[0] = c[1]; bt[1] = 2. * c[2]; bt[2] = 3. * c[3]; bt[3] = 4. * c[4]; bt[4] = 5. * c[5];
+            double* b = BB + i * 4;
+            b[0] = bt[1] * bm; b[1] = 2. * bt[2] * bm; b[2] = 3. * bt[3] * bm; b[3] = 4. * bt[4] * bm;
+        }
+    }
+    free(ibx); free(ibz);
+    return 0;
+}
+
+/* interval index and relative argument; byDiv selects "(s-S)/h" vs "(s-S)*invh" */
+static long long orc_trj_idx(const orc_trj* T, double s, int byDiv, double* sr)
+{
+    long long np = T->np;
+    long long idx = byDiv ? (long long)((s - T->sStart) / T->sStep) : (long long)((s - T->sStart) * T->invStep);
+    if (idx >= np - 1) idx = np - 2;
+    if (idx < 0) idx = 0;
+    double r = s - (T->sStart + T->sStep * idx);
+    if (idx < 2) r -= (2 - idx) * T->sStep;
+    else if (idx < np - 3) ;
+    else if (idx < np - 2) r += T->sStep;
+    else r += 2 * T->sStep;
+    *sr = r;
+    return idx;
+}
+
+static double orc_p5(const double* c, double r) { return c[0] + r * (c[1] + r * (c[2] + r * (c[3] + r * (c[4] + r * c[5])))); }
+static double orc_p4(const double* c, double r) { return c[0] + r * (c[1] + r * (c[2] + r * (c[3] + r * c[4]))); }
+static double orc_p3(const double* c, double r) { return c[0] + r * (c[1] + r * (c[2] + r * c[3])); }
+
+/* srtrjdat.h:431-472 (CompTrjDataDerivedAtPoint_FromTrj) for byDiv=0 and
+ * srtrjdat.cpp:1185-1248 (CompTotalTrjData_FromTrj) for byDiv=1 */
+static void orc_trj_eval(const orc_trj* T, double s, int byDiv, double* Btx, double* X, double* IBx,
+                         double* Btz, double* Z, double* IBz, double* Bx, double* Bz)
+{
+    double r;
+    long long i;
+    if (T->verNZ) {
+        i = orc_trj_idx(T, s, byDiv, &r);
+        *IBx = orc_p5(T->xib + i * 6, r); *X = orc_p5(T->xc + i * 6, r); *Btx = orc_p4(T->xbt + i * 5, r);
+        if (Bz) *Bz = orc_p3(T->xb + i * 4, r);
+    } else {
+        double d = s - T->s0;
+        *Btx = T->dxds0; *X = T->x0 + T->dxds0 * d; *IBx = T->dxds0 * T->dxds0 * d;
+        if (Bz) *Bz = 0.;
+    }
+    if (T->horNZ) {
+        i = orc_trj_idx(T, s, byDiv, &r);
+        *IBz = orc_p5(T->zib + i * 6, r); *Z = orc_p5(T->zc + i * 6, r); *Btz = orc_p4(T->zbt + i * 5, r);
+        if (Bx) *Bx = orc_p3(T->zb + i * 4, r);
+    } else {
+        double d = s - T->s0;
+        *Btz = T->dzds0; *Z = T->z0 + T->dzds0 * d; *IBz = T->dzds0 * T->dzds0 * d;
+        if (Bx) *Bx = 0.;
+    }
+}
+
+/* srtrjdat.h:1006-1034 (CompTrjDataAndFieldWithDerAtPoint_FromTrj), plane 0 = x, 1 = z */
+static void orc_trj_eval_fld(const orc_trj* T, int pl, double s, double* dBds, double* B,
+                             double* Bt, double* Crd, double* IB)
+{
+    double r;
+    long long i = orc_trj_idx(T, s, 1, &r);
+    const double* ci = (pl == 0 ? T->xib : T->zib) + i * 6;
+    const double* cc = (pl == 0 ? T->xc : T->zc) + i * 6;
+    const double* cb = (pl == 0 ? T->xbt : T->zbt) + i * 5;
+    const double* bb = (pl == 0 ? T->xb : T->zb) + i * 4;
+    *IB = orc_p5(ci, r); *Crd = orc_p5(cc, r); *Bt = orc_p4(cb, r); *B = orc_p3(bb, r);
+    *dBds = bb[1] + r * (bb[2] * 2 + r * (bb[3] * 3));
+}
+
+/* Test hook: evaluate the interpolation structure of a given trajectory. */
+ORC_API int orc_trj_interp(const SRWLPrtTrj* trj, const double* s, long long n, double* out8)
+{
+    orc_trj T;
+    int res = orc_trj_build(&T, trj);
+    if (res) return res;
+    for (long long k = 0; k < n; k++) {
+        double* o = out8 + k * 8;
+        orc_trj_eval(&T, s[k], 1, o + 0, o + 1, o + 2, o + 3, o + 4, o + 5, o + 6, o + 7);
+    }
+    orc_trj_free(&T);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* SR integral (srradint.cpp, srradint.h)                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const orc_trj* T;
+    double sIntegStart, sIntegFin, sIntegStep;
+    long long np;          /* AmOfPointsForManIntegr */
+    int slower;            /* UseManualSlower */
+    int termin;            /* m_CalcResidTerminTerms */
+    double normConst;
+    /* tables for the faster method */
+    double *Btx, *X, *IBx, *Btz, *Z, *IBz;
+    /* edge derivatives that persist across points (srradint.h:53) */
+    double complex initDer[2], finDer[2];
+    int warn;
+} orc_sr;
+
+/* srradint.h:867-880 with srradint.cpp:48-60 constants */
+static void orc_cos_sin(double x, double* c, double* s)
+{
+    const double PI = kPI_SR, TwoPI = 2. * PI, ThreePIdTwo = 1.5 * PI, HalfPI = 0.5 * PI;
+    const double One_dTwoPI = 1. / TwoPI;
+    const double a2c = -0.5, a4c = 1. / 24., a6c = -1. / 720., a8c = 1. / 40320., a10c = -1. / 3628800.;
+    const double a3s = -1. / 6., a5s = 1. / 120., a7s = -1. / 5040., a9s = 1. / 362880., a11s = -1. / 39916800.;
+    x -= TwoPI * (int)(x * One_dTwoPI);
+    if (x < 0.) x += TwoPI;
+    int chs = 0;
+    if (x > ThreePIdTwo) x -= TwoPI;
+    else if (x > HalfPI) { x -= PI; chs = 1; }
+    double xe2 = x * x;
+    double C = 1. + xe2 * (a2c + xe2 * (a4c + xe2 * (a6c + xe2 * (a8c + xe2 * a10c))));
+    double S = x * (1. + xe2 * (a3s + xe2 * (a5s + xe2 * (a7s + xe2 * (a9s + xe2 * a11s)))));
+    if (chs) { C = -C; S = -S; }
+    *c = C; *s = S;
+}
+
+/* srradint.h:884-919, coordinate representation */
+static void orc_fun_rad_int(const orc_sr* S, double s, double xo, double yo, double zo, double eph,
+                            double complex* F)
+{
+    double Btx, X, IBx, Btz, Z, IBz;
+    orc_trj_eval(S->T, s, 0, &Btx, &X, &IBx, &Btz, &Z, &IBz, 0, 0);
+    double kph = kPI_SR * 1.E+06 * 0.80654658 * eph;
+    double inv = 1. / (yo - s), dx = xo - X, dz = zo - Z;
+    double ph = kph * (s * S->T->gammaEm2 + IBx + IBz + (dx * dx + dz * dz) * inv);
+    double c, sn;
+    orc_cos_sin(ph, &c, &sn);
+    double ax = inv * (Btx - dx * inv), az = inv * (Btz - dz * inv);
+    F[0] = ax * c + I * ax * sn;
+    F[1] = az * c + I * az * sn;
+}
+
+/* srradint.cpp:398-871 (coordinate representation, far-field expansion with
+ * up to 3 terms; the near-field fallback is not restated: when the expansion
+ * is rejected the residual is zero and a warning is raised, and the edge
+ * derivative keeps its previous value exactly like the reference). */
+static void orc_residual(orc_sr* S, double s, int computeDer, double xo, double yo, double zo,
+                         double eph, double complex* R)
+{
+    const orc_trj* T = S->T;
+    double actNorm = S->normConst * eph * 0.80654658E-03;
+    double dBzds, Bz, Btx, X, IBx, dBxds, Bx, Btz, Z, IBz;
+    orc_trj_eval_fld(T, 0, s, &dBzds, &Bz, &Btx, &X, &IBx);
+    orc_trj_eval_fld(T, 1, s, &dBxds, &Bx, &Btz, &Z, &IBz);
+    double kph = kPI_SR * 1.E+06 * 0.80654658 * eph;
+    double inv = 1. / (yo - s);
+    double dx = xo - X, dz = zo - Z;
+    double a0 = IBx + IBz + (dx * dx + dz * dz) * inv;
+    double Ph = kph * (s * T->gammaEm2 + a0);
+    double cph, sph;
+    orc_cos_sin(Ph, &cph, &sph);
+    double complex eip = cph + I * sph;
+    double Nx = dx * inv, bxn = Btx - Nx;
+    double Nz = dz * inv, bzn = Btz - Nz;
+    double dPhds = kph * (T->gammaEm2 + bxn * bxn + bzn * bzn);
+    double odp = 1. / dPhds;
+    double Ax = bxn * inv, Az = bzn * inv;
+    double complex pX = Ax, pZ = Az, pX2, pZ2;
+    double t1x = Ax, t2x = 0, t3x = 0, t2z = 0, t3z = 0;
+    double cBxBz = T->betaNorm * Bz, cBzBx = -T->betaNorm * Bx;
+    double cxA = cBxBz + Ax, czA = cBzBx + Az, tk = 2. * kph;
+    double d2Phds2 = tk * (bxn * cxA + bzn * czA);
+    double dAxds = (2. * Ax + cBxBz) * inv, dAzds = (2. * Az + cBzBx) * inv;
+    double q = d2Phds2 * odp;
+    t2x = (dAxds - Ax * q) * odp;
+    t2z = (dAzds - Az * q) * odp;
+    pX += I * t2x; pZ += I * t2z;
+    pX2 = pX; pZ2 = pZ;
+    {
+        double cdBz = T->betaNorm * dBzds, cdBx = -T->betaNorm * dBxds;
+        double d3 = tk * (cxA * cxA + czA * czA + bxn * (inv * (2. * bxn * inv + cBxBz) + cdBz) +
+                          bzn * (inv * (2. * bzn * inv + cBzBx) + cdBx));
+        double d2Ax = inv * (3. * dAxds + cdBz), d2Az = inv * (3. * dAzds + cdBx);
+        double odp2 = odp * odp;
+        double qq = d2Phds2 * d2Phds2 * odp2;
+        t3x = (-d2Ax + (3. * dAxds * d2Phds2 + Ax * d3) * odp - 3. * Ax * qq) * odp2;
+        t3z = (-d2Az + (3. * dAzds * d2Phds2 + Az * d3) * odp - 3. * Az * qq) * odp2;
+        pX += t3x; pZ += t3z;
+    }
+    (void)t3z;
+    int ok2 = ((t2x == 0.) && (t1x == 0.)) || ((t2x != 0.) && (fabs(t2x) < 0.7 * fabs(t1x)));
+    int ok3 = ok2 && (((t3x == 0.) && (t2x == 0.)) || ((t3x != 0.) && (fabs(t3x) < 0.7 * fabs(t2x))));
+    int fail = 0;
+    if (!ok3) {
+        if (!ok2) fail = 1;
+        else { pX = pX2; pZ = pZ2; }
+    }
+    if (fail) {
+        R[0] = R[1] = 0;
+        S->warn = SRWL_WARN_BAD_RAD_INT_TERMINATIONS;
+        return;   /* edge derivative is left unchanged (srradint.cpp:737-746 return before :841) */
+    }
+    double complex m = I * odp;   /* mOne_d_IdPhds */
+    pX *= m; pZ *= m;
+    R[0] = actNorm * pX * eip;
+    R[1] = actNorm * pZ * eip;
+    double complex* D = computeDer == 1 ? S->initDer : S->finDer;
+    D[0] = (dAxds + I * (Ax * dPhds)) * eip;
+    D[1] = (dAzds + I * (Az * dPhds)) * eip;
+}
+
+/* srradint.cpp:1214-1258 (slower) and 1379-1604 (faster1): the composite
+ * rule with weights h/15*(7,16,14,...,16,7) + h^2/15*(D(start) - D(end)). */
+static void orc_manual(orc_sr* S, double xo, double yo, double zo, double eph, double complex* out)
+{
+    double actNorm = S->normConst * eph * 0.80654658E-03;
+    double h = S->sIntegStep;
+    long long loops = (S->np - 3) >> 1;
+    double complex s1[2] = {0, 0}, s2[2] = {0, 0}, fb[2], fe[2], f[2];
+    if (S->slower) {
+        double s = S->sIntegStart;
+        orc_fun_rad_int(S, s, xo, yo, zo, eph, fb);
+        for (long long i = 1; i <= loops; i++) {
+            s += h; orc_fun_rad_int(S, s, xo, yo, zo, eph, f); s1[0] += f[0]; s1[1] += f[1];
+            s += h; orc_fun_rad_int(S, s, xo, yo, zo, eph, f); s2[0] += f[0]; s2[1] += f[1];
+        }
+        s += h; orc_fun_rad_int(S, s, xo, yo, zo, eph, f); s1[0] += f[0]; s1[1] += f[1];
+        orc_fun_rad_int(S, S->sIntegFin, xo, yo, zo, eph, fe);
+        double w = actNorm * h * 0.0666666666667;
+        for (int k = 0; k < 2; k++)
+            out[k] += w * (7. * (fb[k] + fe[k]) + 16. * s1[k] + 14. * s2[k] + h * (S->initDer[k] - S->finDer[k]));
+        return;
+    }
+    double kph = kPI_SR * 1.E+06 * 0.80654658 * eph, gEm2 = S->T->gammaEm2;
+    double fr[2][2] = {{0, 0}, {0, 0}}, a1[2][2] = {{0, 0}, {0, 0}}, a2[2][2] = {{0, 0}, {0, 0}};
+    double sArg = S->sIntegStart;
+    for (long long j = 0; j < S->np; j++) {
+        if (j == S->np - 1) sArg = S->sIntegFin;
+        double inv = 1. / (yo - sArg), dx = xo - S->X[j], dz = zo - S->Z[j];
+        double ph = kph * (sArg * gEm2 + S->IBx[j] + S->IBz[j] + (dx * dx + dz * dz) * inv);
+        double c, sn;
+        orc_cos_sin(ph, &c, &sn);
+        double ax = (S->Btx[j] - dx * inv) * inv, az = (S->Btz[j] - dz * inv) * inv;
+        double (*acc)[2] = (j == 0 || j == S->np - 1) ? fr : ((j & 1) ? a1 : a2);
+        acc[0][0] += ax * c; acc[0][1] += ax * sn; acc[1][0] += az * c; acc[1][1] += az * sn;
+        sArg += h;
+    }
+    double w7 = h * (7. / 15.), wd = h * (h / 15.);
+    for (int k = 0; k < 2; k++) {
+        double re = w7 * (fr[k][0] + 2.2857142857143 * a1[k][0] + 2. * a2[k][0]);
+        double im = w7 * (fr[k][1] + 2.2857142857143 * a1[k][1] + 2. * a2[k][1]);
+        double complex dd = wd * (S->initDer[k] - S->finDer[k]);
+        out[k] += actNorm * (re + creal(dd)) + I * (actNorm * (im + cimag(dd)));
+    }
+}
+
+/* srradstr.cpp:2310-2501 FindAverageDistanceToSource */
+static void orc_find_rx(const orc_trj* T, double yObs, double xS, double xE, double zS, double zE,
+                        const double* precPar, double* Robs, double* RobsErr, double* xc, double* zc)
+{
+    double sStart = T->sStart, sRange = (T->np - 1) * T->sStep, sEnd = sStart + sRange;
+    long long N = T->np;
+    if (precPar) {
+        double a = precPar[2], b = precPar[3];
+        double s0 = sStart, s1 = sEnd;
+        int spec = (a < b);
+        if (spec && a > sStart && a < sEnd) sStart = a;
+        if (spec && b > sStart && b < sEnd) sEnd = b;
+        if (sStart != s0 || sEnd != s1) {
+            N = (int)((sEnd - sStart) / T->sStep + 0.00001) + 1;
+            sEnd = sStart + (N - 1) * T->sStep;
+            sRange = sEnd - sStart;
+        }
+    }
+    double* bx = (double*)malloc(sizeof(double) * N * 4);
+    double *bz = bx + N, *xa = bx + 2 * N, *za = bx + 3 * N;
+    /* srtrjdat.cpp:1416-1460 (polynomials used regardless of the zero-field flags) */
+    double s = sStart, st = (N > 1) ? (sEnd - sStart) / (N - 1) : 0.;
+    for (long long i = 0; i < N; i++) {
+        double r;
+        long long k = orc_trj_idx(T, s, 1, &r);
+        xa[i] = orc_p5(T->xc + k * 6, r); bx[i] = orc_p4(T->xbt + k * 5, r);
+        k = orc_trj_idx(T, s, 1, &r);
+        za[i] = orc_p5(T->zc + k * 6, r); bz[i] = orc_p4(T->zbt + k * 5, r);
+        s += st;
+    }
+    const double VL = 1.E+23;
+    long long L = N - 1;
+    double R = yObs - sEnd, iR = 1. / R;
+    double mxs = (xS - xa[L]) * iR - bx[L], mxf = (xE - xa[L]) * iR - bx[L];
+    double mzs = (zS - za[L]) * iR - bz[L], mzf = (zE - za[L]) * iR - bz[L];
+    double rxs = VL, rxf = VL, rzs = VL, rzf = VL;
+    for (long long is = 1; is < N; is++) {
+        R += T->sStep; iR = 1. / R;
+        long long j = L - is;
+        if (rxs == VL) { double m = (xS - xa[j]) * iR - bx[j]; if (m * mxs < 0.) rxs = R; }
+        if (rxf == VL) { double m = (xE - xa[j]) * iR - bx[j]; if (m * mxf < 0.) rxf = R; }
+        if (rzs == VL) { double m = (zS - za[j]) * iR - bz[j]; if (m * mzs < 0.) rzs = R; }
+        if (rzf == VL) { double m = (zE - za[j]) * iR - bz[j]; if (m * mzf < 0.) rzf = R; }
+        if (rxs != VL && rxf != VL && rzs != VL && rzf != VL) break;
+    }
+    double minRx = rxs < rxf ? rxs : rxf, minRz = rzs < rzf ? rzs : rzf;
+    double minR = minRx < minRz ? minRx : minRz;
+    R = yObs - sStart; iR = 1. / R;
+    mxs = (xS - xa[0]) * iR - bx[0]; mxf = (xE - xa[0]) * iR - bx[0];
+    mzs = (zS - za[0]) * iR - bz[0]; mzf = (zE - za[0]) * iR - bz[0];
+    rxs = rxf = rzs = rzf = VL;
+    for (long long is = 1; is < N; is++) {
+        R -= T->sStep; iR = 1. / R;
+        long long j = is;
+        if (rxs == VL) { double m = (xS - xa[j]) * iR - bx[j]; if (m * mxs < 0.) rxs = R; }
+        if (rxf == VL) { double m = (xE - xa[j]) * iR - bx[j]; if (m * mxf < 0.) rxf = R; }
+        if (rzs == VL) { double m = (zS - za[j]) * iR - bz[j]; if (m * mzs < 0.) rzs = R; }
+        if (rzf == VL) { double m = (zE - za[j]) * iR - bz[j]; if (m * mzf < 0.) rzf = R; }
+    }
+    double maxRx = rxs < rxf ? rxs : rxf, maxRz = rzs < rzf ? rzs : rzf;
+    double maxR = (minRx < minRz) ? maxRx : maxRz;
+    double avg = 0.5 * (minR + maxR);
+    if (minR == VL) avg = (maxR == VL) ? VL : maxR;
+    else if (maxR == VL) avg = minR;
+    double ysrc;
+    if (avg != VL) { *Robs = avg; *RobsErr = 0.25 * sRange; ysrc = yObs - *Robs; }
+    else {
+        if (sStart < 0. && sEnd > 0.) ysrc = 0.35 * sRange; else ysrc = sStart + 0.75 * sRange;
+        *Robs = yObs - ysrc; *RobsErr = 0.25 * sRange;
+    }
+    long long iy = (long long)((ysrc - sStart) / T->sStep + 0.00001);
+    if (iy < 0) iy = 0;
+    if (iy >= N) iy = N - 1;
+    *xc = xa[iy]; *zc = za[iy];
+    free(bx);
+}
+
+/* srwlib.cpp:403-452 */
+static int orc_setup_trj(SRWLPrtTrj* t, const SRWLParticle* p, SRWLMagFldC* mag, const double* precPar)
+{
+    long long np = precPar ? (long long)(int)precPar[4] : 100000;
+    memset(t, 0, sizeof(*t));
+    t->arX = (double*)calloc(np, 8); t->arXp = (double*)calloc(np, 8);
+    t->arY = (double*)calloc(np, 8); t->arYp = (double*)calloc(np, 8);
+    t->arZ = (double*)calloc(np, 8); t->arZp = (double*)calloc(np, 8);
+    t->partInitCond = *p;
+    t->np = np;
+    double a = precPar ? precPar[2] : 0., b = precPar ? precPar[3] : 0.;
+    t->ctStart = a - p->z; t->ctEnd = b - p->z;
+    return orc_srwlCalcPartTraj(t, mag, 0);
+}
+
+static void orc_free_trj_arrays(SRWLPrtTrj* t)
+{
+    free(t->arX); free(t->arXp); free(t->arY); free(t->arYp); free(t->arZ); free(t->arZp);
+}
+
+/* srwlib.cpp:456-540 + srradint.cpp:289-394, 3231-3350, 3551-3614.
+ * Method 0 (manual) only; nthreads > 1 parallelizes over mesh rows (the
+ * edge-derivative state is then per row, which only matters when the
+ * asymptotic expansion is rejected). */
+ORC_API int orc_srwlCalcElecFieldSR_mt(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
+                                      double* precPar, int nPrecPar, int nthreads)
+{
+    if (pWfr == 0 || precPar == 0) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
+    int trjDef = !(pTrj == 0 || ((!pTrj->arX || !pTrj->arXp) && (!pTrj->arY || !pTrj->arYp)) || pTrj->np <= 0);
+    if (!trjDef && (pMagFld == 0)) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
+    if ((int)precPar[0] != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (nPrecPar > 6 && precPar[6] > 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    SRWLPrtTrj own;
+    int res = 0;
+    if (!trjDef) {
+        res = orc_setup_trj(&own, &pWfr->partBeam.partStatMom1, pMagFld, precPar);
+        if (res) { orc_free_trj_arrays(&own); return res; }
+        pTrj = &own;
+    } else pWfr->partBeam.partStatMom1 = pTrj->partInitCond;
+
+    orc_trj T;
+    res = orc_trj_build(&T, pTrj);
+    if (res) goto done0;
+    {
+        SRWLRadMesh* m = &pWfr->mesh;
+        long nx = m->nx, nz = m->ny, ne = m->ne;
+        double xStep = nx > 1 ? (m->xFin - m->xStart) / (nx - 1) : 0;
+        double zStep = nz > 1 ? (m->yFin - m->yStart) / (nz - 1) : 0;
+        double eStep = ne > 1 ? (m->eFin - m->eStart) / (ne - 1) : 0;
+        double xS = m->xStart, xE = m->xStart + (nx - 1) * xStep;
+        double zS = m->yStart, zE = m->yStart + (nz - 1) * zStep;
+        double yObs = m->zStart;
+        /* wavefront radius and centre (srradstr.cpp:172-191, 324-375) */
+        double Robs, RobsErr, xcS, zcS;
+        orc_find_rx(&T, yObs, xS, xE, zS, zE, precPar, &Robs, &RobsErr, &xcS, &zcS);
+        pWfr->Rx = pWfr->Ry = Robs; pWfr->dRx = pWfr->dRy = RobsErr;
+        pWfr->xc = xcS; pWfr->yc = zcS;
+        /* precision parameters (srradint.cpp:3579-3614) */
+        orc_sr S;
+        memset(&S, 0, sizeof(S));
+        S.T = &T;
+        S.sIntegStep = precPar[1];
+        S.sIntegStart = T.sStart;
+        S.sIntegFin = T.sStart + T.sStep * (T.np - 1);
+        if (precPar[2] < precPar[3]) {
+            if (precPar[2] > S.sIntegStart && precPar[2] < S.sIntegFin) S.sIntegStart = precPar[2];
+            if (precPar[3] > S.sIntegStart && precPar[3] < S.sIntegFin) S.sIntegFin = precPar[3];
+        }
+        S.termin = (nPrecPar <= 0 || nPrecPar > 5) ? (int)(char)precPar[5] : 1;
+        S.normConst = sqrt((1. / 137.0360411) * pWfr->partBeam.Iavg * 1.E+09 / 1.602189246E-19);
+        /* srradint.cpp:92-123 */
+        if (T.gamma <= 1.) { res = SRWL_ERR_ELEC_BEAM_NOT_ULTRARELATIVISTIC; goto done1; }
+        if (T.s0 < T.sStart || T.s0 > T.sStart + T.sStep * (T.np - 1)) { res = SRWL_ERR_S0_OUT_OF_FIELD_LIMITS; goto done1; }
+        if (yObs < S.sIntegFin) { res = SRWL_ERR_OBS_WITHIN_INTEGR_LIMITS; goto done1; }
+        if ((yObs - S.sIntegFin) < 3. * 1.239842E-06 / m->eStart) { res = SRWL_ERR_WAVELENGTH_COMPARABLE_WITH_DISTANCE; goto done1; }
+        /* srradint.h:1043-1058 + 814-830 */
+        {
+            int b0 = (int)((S.sIntegFin - S.sIntegStart) / S.sIntegStep) >> 1;
+            long long npl = ((long long)b0 << 1) + 1;
+            S.slower = (10000 <= npl);
+            S.np = npl;
+            S.sIntegStep = (S.sIntegFin - S.sIntegStart) / (S.np - 1);
+        }
+        if (!S.slower) {
+            S.Btx = (double*)malloc(sizeof(double) * S.np * 6);
+            S.X = S.Btx + S.np; S.IBx = S.X + S.np; S.Btz = S.IBx + S.np; S.Z = S.Btz + S.np; S.IBz = S.Z + S.np;
+            double s = S.sIntegStart, st = (S.np > 1) ? (S.sIntegFin - S.sIntegStart) / (S.np - 1) : 0.;
+            for (long long i = 0; i < S.np; i++) {
+                orc_trj_eval(&T, s, 1, S.Btx + i, S.X + i, S.IBx + i, S.Btz + i, S.Z + i, S.IBz + i, 0, 0);
+                s += st;
+            }
+        }
+        /* srradint.cpp:3231-3260 */
+        int symX = 0, symZ = 0;
+        {
+            int fSymZ = !T.horNZ, fSymX = !T.verNZ;
+            if (fSymX && nx > 1) {
+                double tol = xStep * 0.01;
+                int small = fabs(T.dxds0 * (yObs - T.s0)) < tol;
+                symX = small && (fabs(0.5 * (xS + xE) - T.x0) < tol);
+            }
+            if (fSymZ && nz > 1) {
+                double tol = zStep * 0.01;
+                int small = fabs(T.dzds0 * (yObs - T.s0)) < tol;
+                symZ = small && (fabs(0.5 * (zS + zE) - T.z0) < tol);
+            }
+        }
+        float* ex = (float*)pWfr->arEx;
+        float* ez = (float*)pWfr->arEy;
+        long long perX = (long long)ne * 2, perZ = perX * nx;
+        double xc0 = T.x0, zc0 = T.z0, xTol = xStep * 0.001, zTol = zStep * 0.001;
+        /* number of rows / columns actually integrated (srradint.cpp:331-345) */
+        long nzc = nz, nxc = nx;
+        {
+            double z = zS;
+            for (long iz = 0; iz < nz; iz++) { if (symZ && (z - zc0) > zTol) { nzc = iz; break; } z += zStep; }
+            double x = xS;
+            for (long ix = 0; ix < nx; ix++) { if (symX && (x - xc0) > xTol) { nxc = ix; break; } x += xStep; }
+        }
+        if (nthreads < 1) nthreads = 1;
+        int warnAll = 0;
+#pragma omp parallel num_threads(nthreads) reduction(|:warnAll)
+        {
+            orc_sr L = S;
+            L.initDer[0] = L.initDer[1] = L.finDer[0] = L.finDer[1] = 0;
+            L.warn = 0;
+#pragma omp for schedule(dynamic, 1)
+            for (long iz = 0; iz < nzc; iz++) {
+                double z = zS;
+                for (long k = 0; k < iz; k++) z += zStep;
+                double x = xS;
+                for (long ix = 0; ix < nxc; ix++) {
+                    double e = m->eStart;
+                    for (long ie = 0; ie < ne; ie++) {
+                        double complex v[2] = {0, 0};
+                        if (L.termin > 0) {
+                            double complex lr[2] = {0, 0}, rr[2] = {0, 0};
+                            if (L.termin == 1 || L.termin == 2) orc_residual(&L, L.sIntegStart, 1, x, yObs, z, e, lr);
+                            if (L.termin == 1 || L.termin == 3) orc_residual(&L, L.sIntegFin, 2, x, yObs, z, e, rr);
+                            v[0] = rr[0] - lr[0]; v[1] = rr[1] - lr[1];
+                        }
+                        orc_manual(&L, x, yObs, z, e, v);
+                        long long o = iz * perZ + ix * perX + ie * 2;
+                        ex[o] = (float)creal(v[0]); ex[o + 1] = (float)cimag(v[0]);
+                        ez[o] = (float)creal(v[1]); ez[o + 1] = (float)cimag(v[1]);
+                        e += eStep;
+                    }
+                    x += xStep;
+                }
+            }
+            if (L.warn) warnAll = 1;
+        }
+        if (warnAll) S.warn = SRWL_WARN_BAD_RAD_INT_TERMINATIONS;
+        /* srradint.cpp:3264-3350 */
+        if (symZ || symX) {
+            long hz = nz >> 1, hx = nx >> 1;
+            for (long iz = 0; iz < nz; iz++)
+                for (long ix = 0; ix < nx; ix++) {
+                    long siz = iz, six = ix;
+                    int flipEx = 0, flipEz = 0;
+                    if (symZ && iz >= nz - hz) { siz = nz - 1 - iz; flipEz = !flipEz; }
+                    if (symX && ix >= nx - hx) { six = nx - 1 - ix; flipEx = !flipEx; }
+                    if (siz == iz && six == ix) continue;
+                    for (long ie = 0; ie < ne; ie++) {
+                        long long d = iz * perZ + ix * perX + ie * 2, sidx = siz * perZ + six * perX + ie * 2;
+                        float fx = flipEx ? -1.f : 1.f, fz = flipEz ? -1.f : 1.f;
+                        ex[d] = fx * ex[sidx]; ex[d + 1] = fx * ex[sidx + 1];
+                        ez[d] = fz * ez[sidx]; ez[d + 1] = fz * ez[sidx + 1];
+                    }
+                }
+        }
+        pWfr->avgPhotEn = 0.5 * (m->eStart + m->eFin);
+        pWfr->presCA = 0; pWfr->presFT = 0;
+        if (!S.slower) free(S.Btx);
+        if (!res && S.warn) res = S.warn;
+    }
+done1:
+    orc_trj_free(&T);
+done0:
+    if (!trjDef) orc_free_trj_arrays(&own);
+    return res;
+}
+
+ORC_API int orc_srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
+                                    double* precPar, int nPrecPar)
+{
+    return orc_srwlCalcElecFieldSR_mt(pWfr, pTrj, pMagFld, precPar, nPrecPar, 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* Centred 2D FFT (gmfft.cpp:295-893, gmfft.h:232-472) in FP64          */
+/* ------------------------------------------------------------------ */
+static int orc_is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* in-place standard DFT of a strided line: out[k] = sum_n in[n] e^{sgn i 2pi k n/N} */
+static void orc_dft_line(double complex* a, long n, long stride, int sgn, double complex* tmp)
+{
+    for (long i = 0; i < n; i++) tmp[i] = a[i * stride];
+    if (orc_is_pow2(n)) {
+        /* iterative radix-2 Cooley-Tukey */
+        for (long i = 1, j = 0; i < n; i++) {
+            long bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) { double complex t = tmp[i]; tmp[i] = tmp[j]; tmp[j] = t; }
+        }
+        for (long len = 2; len <= n; len <<= 1) {
+            double ang = sgn * 2. * M_PI / len;
+            for (long i = 0; i < n; i += len)
+                for (long k = 0; k < len / 2; k++) {
+                    double complex w = cos(ang * k) + I * sin(ang * k);
+                    double complex u = tmp[i + k], v = tmp[i + k + len / 2] * w;
+                    tmp[i + k] = u + v; tmp[i + k + len / 2] = u - v;
+                }
+        }
+        for (long i = 0; i < n; i++) a[i * stride] = tmp[i];
+    } else {
+        for (long k = 0; k < n; k++) {
+            double complex s = 0;
+            for (long m = 0; m < n; m++) {
+                double ang = sgn * 2. * M_PI * (double)((k * m) % n) / n;
+                s += tmp[m] * (cos(ang) + I * sin(ang));
+            }
+            a[k * stride] = s;
+        }
+    }
+}
+
+/* Centred transform of an [ny][nx] complex array:
+ * out[ky][kx] = mult * sum in[ny][nx] exp(-+ i 2pi ((kx-Nx/2)(nx-Nx/2)/Nx + (ky-Ny/2)(ny-Ny/2)/Ny))
+ * which equals the reference's FFTW call bracketed by RepairSignAfter2DFFT and
+ * RotateDataAfter2DFFT (gmfft.cpp:579-615 forward, 724-738 inverse). */
+static void orc_centred_fft2(double complex* a, long nx, long ny, int dir, double mult)
+{
+    int sgn = dir > 0 ? -1 : 1;
+    double complex* tmp = (double complex*)malloc(sizeof(double complex) * (nx > ny ? nx : ny));
+    for (long iy = 0; iy < ny; iy++)
+        for (long ix = 0; ix < nx; ix++)
+            if ((ix + iy) & 1) a[iy * nx + ix] = -a[iy * nx + ix];
+    for (long iy = 0; iy < ny; iy++) orc_dft_line(a + iy * nx, nx, 1, sgn, tmp);
+    for (long ix = 0; ix < nx; ix++) orc_dft_line(a + ix, ny, nx, sgn, tmp);
+    long hs = (nx / 2 + ny / 2) & 1;
+    for (long iy = 0; iy < ny; iy++)
+        for (long ix = 0; ix < nx; ix++) {
+            double s = ((ix + iy + hs) & 1) ? -mult : mult;
+            a[iy * nx + ix] *= s;
+        }
+    free(tmp);
+}
+
+/* CGenMathFFT2D::Make2DFFT on one [ny][nx] slice with the mesh bookkeeping of
+ * SetupLimitsTr (gmfft.h:232-248) and the linear-phase shifts (gmfft.h:250-276).
+ * mesh[6] = xStart, xStep, yStart, yStep (input), updated to the output mesh. */
+static void orc_make2dfft(double complex* a, long nx, long ny, int dir, double* mesh)
+{
+    double xStart = mesh[0], xStep = mesh[1], yStart = mesh[2], yStep = mesh[3];
+    double xStartTr = -0.5 / xStep, yStartTr = -0.5 / yStep;
+    double xStepTr = -xStartTr / (nx >> 1), yStepTr = -yStartTr / (ny >> 1);
+    double x0 = xStart + 0.5 * nx * xStep, y0 = yStart + 0.5 * ny * yStep;
+    int shX = fabs(x0) > 1.E-06 * nx * xStep, shY = fabs(y0) > 1.E-06 * ny * yStep;
+    orc_centred_fft2(a, nx, ny, dir, xStep * yStep);
+    double t0s = dir > 0 ? -1. : 1.;
+    if (shX || shY) {
+        for (long iy = 0; iy < ny; iy++) {
+            double complex my = 1.;
+            if (shY) { double q = (iy - (ny >> 1)) * yStepTr; double ph = t0s * y0 * q * 2. * M_PI; my = cos(ph) + I * sin(ph); }
+            for (long ix = 0; ix < nx; ix++) {
+                double complex mx = 1.;
+                if (shX) { double q = (ix - (nx >> 1)) * xStepTr; double ph = t0s * x0 * q * 2. * M_PI; mx = cos(ph) + I * sin(ph); }
+                a[iy * nx + ix] *= mx * my;
+            }
+        }
+    }
+    mesh[0] = xStartTr; mesh[1] = xStepTr; mesh[2] = yStartTr; mesh[3] = yStepTr;
+}
+
+/* srwlib.cpp:1090-1204 for typeData 'f' (float complex), 2D */
+ORC_API int orc_srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMesh, int dir)
+{
+    if (!pcData || !arMesh || nMesh < 6 || typeData != 'f') return SRWL_INCORRECT_PARAM_FOR_FFT;
+    long nx = (long)arMesh[2], ny = (long)arMesh[5];
+    if (nx < 2 || ny < 2 || (nx & 1) || (ny & 1)) return SRWL_INCORRECT_PARAM_FOR_FFT;
+    float* f = (float*)pcData;
+    double complex* a = (double complex*)malloc(sizeof(double complex) * nx * ny);
+    for (long i = 0; i < nx * ny; i++) a[i] = f[2 * i] + I * f[2 * i + 1];
+    double mesh[4] = {arMesh[0], arMesh[1], arMesh[3], arMesh[4]};
+    orc_make2dfft(a, nx, ny, dir, mesh);
+    for (long i = 0; i < nx * ny; i++) { f[2 * i] = (float)creal(a[i]); f[2 * i + 1] = (float)cimag(a[i]); }
+    arMesh[0] = mesh[0]; arMesh[1] = mesh[1]; arMesh[3] = mesh[2]; arMesh[4] = mesh[3];
+    free(a);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Drift propagation (sroptdrf.h:116-409, 600-880; sroptdrf.cpp:726-1130) */
+/* ------------------------------------------------------------------ */
+static double orc_sign(double x) { return x < 0 ? -1. : 1.; }
+
+typedef struct { double* ex; double* ez; } orc_slice;
+
+/* Propagates one photon-energy slice (both components, FP64) through a
+ * drift of length L.  mode 0: angular representation; mode 3: analytic
+ * quadratic phase (subtype 2 radius estimate).  mesh: xStart,xStep,zStart,zStep. */
+static void orc_drift_slice(double complex* ex, double complex* ez, long nx, long nz, double e,
+                            double L, int treat, int mode, const double* bv, const double* mesh)
+{
+    double lam = 1.239842e-06 / e;
+    long n = nx * nz;
+    double complex* comp[2] = {ex, ez};
+    if (mode == 0) {
+        double m[4] = {-(nx >> 1) * mesh[1], mesh[1], -(nz >> 1) * mesh[3], mesh[3]};
+        for (int c = 0; c < 2; c++) { double mm[4] = {m[0], m[1], m[2], m[3]}; orc_make2dfft(comp[c], nx, nz, 1, mm); }
+        double qx0 = -0.5 / mesh[1], qdx = 1. / (nx * mesh[1]), qz0 = -0.5 / mesh[3], qdz = 1. / (nz * mesh[3]);
+        double c0 = -3.1415926536 * L * lam, c1 = 0.25 * lam * lam;
+        for (long iz = 0; iz < nz; iz++) {
+            double qz = qz0 + iz * qdz;
+            for (long ix = 0; ix < nx; ix++) {
+                double qx = qx0 + ix * qdx, q2 = qx * qx + qz * qz, cq = c1 * q2;
+                double ph = c0 * q2 * (1. + cq + 2. * cq * cq);
+                if (treat == 1) ph += (5.067730652e+06) * L * e;
+                double complex f = cos(ph) + I * sin(ph);
+                ex[iz * nx + ix] *= f; ez[iz * nx + ix] *= f;
+            }
+        }
+        double ma[4] = {qx0, qdx, qz0, qdz};
+        for (int c = 0; c < 2; c++) { double mm[4] = {ma[0], ma[1], ma[2], ma[3]}; orc_make2dfft(comp[c], nx, nz, -1, mm); }
+        (void)n;
+        return;
+    }
+    /* mode 3, buffer variables bv: xc, zc, invRx, invRz, Lx, Lz, invRxL, invRzL, sqrtLxLz_d_L, phTerm, kx, kxc, kz, kzc */
+    double xc = bv[0], zc = bv[1], invRx = bv[2], invRz = bv[3], Lx = bv[4], Lz = bv[5];
+    double invRxL = bv[6], invRzL = bv[7], mult = bv[8], phT = bv[9];
+    double pidl = 3.1415926536 / lam, pil = 3.1415926536 * lam;
+    for (long iz = 0; iz < nz; iz++) {
+        double rz = mesh[2] + iz * mesh[3] - zc;
+        for (long ix = 0; ix < nx; ix++) {
+            double rx = mesh[0] + ix * mesh[1] - xc;
+            double ph = -pidl * (invRx * rx * rx + invRz * rz * rz);
+            double complex f = cos(ph) + I * sin(ph);
+            ex[iz * nx + ix] *= f; ez[iz * nx + ix] *= f;
+        }
+    }
+    double m[4] = {-(nx >> 1) * mesh[1], mesh[1], -(nz >> 1) * mesh[3], mesh[3]};
+    for (int c = 0; c < 2; c++) { double mm[4] = {m[0], m[1], m[2], m[3]}; orc_make2dfft(comp[c], nx, nz, 1, mm); }
+    double qx0 = -0.5 / mesh[1], qdx = 1. / (nx * mesh[1]), qz0 = -0.5 / mesh[3], qdz = 1. / (nz * mesh[3]);
+    for (long iz = 0; iz < nz; iz++) {
+        double qz = qz0 + iz * qdz;
+        for (long ix = 0; ix < nx; ix++) {
+            double qx = qx0 + ix * qdx;
+            double ph = -pil * Lx * qx * qx - pil * Lz * qz * qz + phT;
+            double complex f = (cos(ph) + I * sin(ph)) * mult;
+            ex[iz * nx + ix] *= f; ez[iz * nx + ix] *= f;
+        }
+    }
+    double ma[4] = {qx0, qdx, qz0, qdz};
+    for (int c = 0; c < 2; c++) { double mm[4] = {ma[0], ma[1], ma[2], ma[3]}; orc_make2dfft(comp[c], nx, nz, -1, mm); }
+    double kx = bv[10], kxc = bv[11], kz = bv[12], kzc = bv[13];
+    double nxS = kx * mesh[0] - kxc * xc, nxD = mesh[1] * kx;
+    double nzS = kz * mesh[2] - kzc * zc, nzD = mesh[3] * kz;
+    for (long iz = 0; iz < nz; iz++) {
+        double rz = nzS + iz * nzD - zc;
+        for (long ix = 0; ix < nx; ix++) {
+            double rx = nxS + ix * nxD - xc;
+            double ph = pidl * (invRxL * rx * rx + invRzL * rz * rz);
+            if (treat == 1) ph += 2 * pidl * L;
+            double complex f = cos(ph) + I * sin(ph);
+            ex[iz * nx + ix] *= f; ez[iz * nx + ix] *= f;
+        }
+    }
+}
+
+/* sroptdrf.cpp:889-1081 (UseExactRxRz off) with sroptdrf.cpp:1085-1116 (subtype 2) */
+static void orc_mode3_bufvars(const SRWLWfr* w, double L, double* bv)
+{
+    double Rx = w->Rx, Rz = w->Ry;
+    double mx = 3. * w->dRx, mz = 3. * w->dRy;
+    double RxL = Rx + L;
+    if (fabs(Rx) < mx) Rx = mx * (Rx >= 0 ? 1. : -1.);
+    else if (fabs(RxL) < mx) Rx = mx * (RxL >= 0 ? 1. : -1.) - L;
+    double RzL = Rz + L;
+    if (fabs(Rz) < mz) Rz = mz * (Rz >= 0 ? 1. : -1.);
+    else if (fabs(RzL) < mz) Rz = mz * (RzL >= 0 ? 1. : -1.) - L;
+    const double inf = 1E+23;
+    double invRx = 0, invRz = 0, Lx = 0, Lz = 0, invRxL = 1. / L, invRzL = 1. / L;
+    double kx = inf, kxc = inf, kz = inf, kzc = inf;
+    if (Rx != 0) {
+        invRx = 1. / Rx; kx = (Rx + L) * invRx; kxc = L / Rx;
+        if (-L != Rx) { Lx = L / (1. + L * invRx); invRxL = 1. / (L + Rx); } else { Lx = inf; invRxL = inf; }
+    }
+    if (Rz != 0) {
+        invRz = 1. / Rz; kz = (Rz + L) * invRz; kzc = L / Rz;
+        if (-L != Rz) { Lz = L / (1. + L * invRz); invRzL = 1. / (L + Rz); } else { Lz = inf; invRzL = inf; }
+    }
+    bv[0] = w->xc; bv[1] = w->yc; bv[2] = invRx; bv[3] = invRz; bv[4] = Lx; bv[5] = Lz;
+    bv[6] = invRxL; bv[7] = invRzL;
+    bv[8] = sqrt(fabs(Lx * Lz)) / L;
+    bv[9] = 0.25 * 3.141592653589793 * (2. - orc_sign(Lx) - orc_sign(Lz));
+    bv[10] = kx; bv[11] = kxc; bv[12] = kz; bv[13] = kzc;
+}
+
+/* srwlib.cpp:1023-1074 + sroptcnt.cpp:87-425 for drift-only containers */
+ORC_API int orc_srwlPropagElecField(SRWLWfr* w, SRWLOptC* opt)
+{
+    if (!w || !opt) return SRWL_INCORRECT_PARAM_FOR_WFR_PROP;
+    if (w->presCA != 0 || w->presFT != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    for (int iel = 0; iel < opt->nElem; iel++) {
+        if (strcmp(opt->arOptTypes[iel], "drift") != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+        const SRWLOptD* d = (const SRWLOptD*)opt->arOpt[iel];
+        double* pp = (opt->arProp && iel < opt->nProp) ? opt->arProp[iel] : 0;
+        int mode = 0, sub = 0;
+        if (pp) {
+            if (pp[0] != 0 || pp[1] != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+            if (fabs(pp[5] - 1) > 1e-4 || fabs(pp[6] - 1) > 1e-4 || fabs(pp[7] - 1) > 1e-4 || fabs(pp[8] - 1) > 1e-4)
+                return SRWL_AMD_ERR_UNSUPPORTED;
+            int at = (int)(char)pp[3];
+            if (at == 0) mode = 0;
+            else if (at == 2) { mode = 3; sub = 2; }
+            else return SRWL_AMD_ERR_UNSUPPORTED;
+        }
+        long nx = w->mesh.nx, nz = w->mesh.ny, ne = w->mesh.ne;
+        if ((nx & 1) || (nz & 1) || nx < 4 || nz < 4) return SRWL_ERR_IN_FFT;
+        double mesh[4] = {w->mesh.xStart, (w->mesh.xFin - w->mesh.xStart) / (nx - 1),
+                          w->mesh.yStart, (w->mesh.yFin - w->mesh.yStart) / (nz - 1)};
+        double eStep = ne > 1 ? (w->mesh.eFin - w->mesh.eStart) / (ne - 1) : 0;
+        double bv[14] = {0};
+        if (mode == 3) orc_mode3_bufvars(w, d->L, bv);
+        (void)sub;
+        double complex* ex = (double complex*)malloc(sizeof(double complex) * nx * nz);
+        double complex* ez = (double complex*)malloc(sizeof(double complex) * nx * nz);
+        float* fx = (float*)w->arEx;
+        float* fz = (float*)w->arEy;
+        for (long ie = 0; ie < ne; ie++) {
+            double e = w->mesh.eStart + ie * eStep;
+            for (long iz = 0; iz < nz; iz++)
+                for (long ix = 0; ix < nx; ix++) {
+                    long long o = ((long long)(iz * nx + ix) * ne + ie) * 2;
+                    ex[iz * nx + ix] = fx[o] + I * fx[o + 1];
+                    ez[iz * nx + ix] = fz[o] + I * fz[o + 1];
+                }
+            orc_drift_slice(ex, ez, nx, nz, e, d->L, d->treat, mode, bv, mesh);
+            for (long iz = 0; iz < nz; iz++)
+                for (long ix = 0; ix < nx; ix++) {
+                    long long o = ((long long)(iz * nx + ix) * ne + ie) * 2;
+                    fx[o] = (float)creal(ex[iz * nx + ix]); fx[o + 1] = (float)cimag(ex[iz * nx + ix]);
+                    fz[o] = (float)creal(ez[iz * nx + ix]); fz[o + 1] = (float)cimag(ez[iz * nx + ix]);
+                }
+        }
+        free(ex); free(ez);
+        if (mode == 3) {
+            double nS = bv[10] * mesh[0] - bv[11] * w->xc, nD = mesh[1] * bv[10];
+            double zS2 = bv[12] * mesh[2] - bv[13] * w->yc, zD = mesh[3] * bv[12];
+            if (bv[10] < 0 || bv[12] < 0) return SRWL_AMD_ERR_UNSUPPORTED;
+            w->mesh.xStart = nS; w->mesh.xFin = nS + (nx - 1) * nD;
+            w->mesh.yStart = zS2; w->mesh.yFin = zS2 + (nz - 1) * zD;
+        }
+        /* sroptdrf.h:967-979 */
+        w->Rx += d->L; w->Ry += d->L;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Intensity (srradmnp.h:367-430), single-electron, vs x and y          */
+/* ------------------------------------------------------------------ */
+static double orc_int_comp(double xr, double xi, double zr, double zi, int pol)
+{
+    switch (pol) {
+    case 0: return xr * xr + xi * xi;
+    case 1: return zr * zr + zi * zi;
+    case 2: { double a = xr + zr, b = xi + zi; return 0.5 * (a * a + b * b); }
+    case 3: { double a = xr - zr, b = xi - zi; return 0.5 * (a * a + b * b); }
+    case 5: { double a = xr - zi, b = xi + zr; return 0.5 * (a * a + b * b); }
+    case 4: { double a = xr + zi, b = xi - zr; return 0.5 * (a * a + b * b); }
+    case -1: return xr * xr + xi * xi + zr * zr + zi * zi;
+    case -2: return xr * xr + xi * xi - (zr * zr + zi * zi);
+    case -3: return 2. * (xr * zr + xi * zi);
+    case -4: return 2. * (xr * zi - xi * zr);
+    default: return xr * xr + xi * xi + zr * zr + zi * zi;
+    }
+}
+
+ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e)
+{
+    if (!pInt || !w) return SRWL_INCORRECT_PARAM_FOR_INT_EXTR;
+    if (intType != 0 || depType != 3) return SRWL_AMD_ERR_UNSUPPORTED;
+    long nx = w->mesh.nx, nz = w->mesh.ny, ne = w->mesh.ne;
+    double eStep = ne > 1 ? (w->mesh.eFin - w->mesh.eStart) / (ne - 1) : 0;
+    long ie0 = 0, ie1 = 0;
+    double t = 0;
+    if (ne > 1) {
+        double r = (e - w->mesh.eStart) / eStep;
+        ie0 = (long)r;
+        if (ie0 < 0) ie0 = 0;
+        if (ie0 > ne - 2) ie0 = ne - 2;
+        ie1 = ie0 + 1;
+        t = r - ie0;
+    }
+    const float* fx = (const float*)w->arEx;
+    const float* fz = (const float*)w->arEy;
+    float* out = (float*)pInt;
+    for (long iz = 0; iz < nz; iz++)
+        for (long ix = 0; ix < nx; ix++) {
+            long long o0 = ((long long)(iz * nx + ix) * ne + ie0) * 2, o1 = ((long long)(iz * nx + ix) * ne + ie1) * 2;
+            double v0 = orc_int_comp(fx[o0], fx[o0 + 1], fz[o0], fz[o0 + 1], pol);
+            double v = v0;
+            if (ne > 1) { double v1 = orc_int_comp(fx[o1], fx[o1 + 1], fz[o1], fz[o1 + 1], pol); v = v0 + (v1 - v0) * t; }
+            out[iz * nx + ix] = (float)v;
+        }
+    return 0;
+}
+
+/* Layout probe: sizes of the C structures as compiled from include/srwlib.h,
+ * compared with the Python ctypes mirror by the CPU tests. */
+ORC_API void orc_struct_sizes(long* out)
+{
+    out[0] = sizeof(SRWLParticle); out[1] = sizeof(SRWLPartBeam); out[2] = sizeof(SRWLMagFldH);
+    out[3] = sizeof(SRWLMagFldU); out[4] = sizeof(SRWLMagFldC); out[5] = sizeof(SRWLPrtTrj);
+    out[6] = sizeof(SRWLRadMesh); out[7] = sizeof(SRWLWfr); out[8] = sizeof(SRWLStokes);
+    out[9] = sizeof(SRWLOptD); out[10] = sizeof(SRWLOptC);
+}

@@ -1,0 +1,752 @@
+// sr.hip - FP64 trajectory (RK4), trajectory interpolation tables and the
+// near-field SR integral for gfx950.
+//
+// Reference algorithms (paths relative to the reference source root):
+//   field          cpp/src/core/srmagfld.h:401-502
+//   RK4 trajectory cpp/src/core/srgtrjdt.cpp:24-365, ext/genmath/gmintrk.h:129-200
+//   interpolation  cpp/src/core/srtrjdat.cpp:1724-2088, srtrjdat.h:955-1056
+//   SR integral    cpp/src/core/srradint.cpp:1214-1604 (method 0), srradint.h:884-919
+//   end terms      cpp/src/core/srradint.cpp:398-871
+//
+// Precision: trajectory, tables and the per-chunk phase reference are FP64;
+// the per-point phase increment (|t| < ~2 revolutions) and the integrand sum
+// are FP32, with Kahan-compensated accumulation of 16-point block sums.
+#include "srw_dev.h"
+#include <math.h>
+
+// ------------------------------------------------------------------------
+// Field and RK4 (FP64)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double radical_one_plus_small(double x)
+{   // gmmeth.h:390-394
+    if (fabs(x) > 0.01) return sqrt(1. + x);
+    return 1. + x * (0.5 + x * (-0.125 + x * (0.0625 + x * (-0.0390625 + x * (0.02734375 +
+           x * (-0.0205078125 + x * 0.01611328125))))));
+}
+
+__device__ void und_addB(const srwd_und& u, double yr, double zr, double* B)
+{   // srmagfld.h:401-502
+    const double pi = 3.1415926535898, twoPi = 6.2831853072, piE2 = pi * pi;
+    const double factK2B = 93.372904175766;
+    double halfTot = 0.5 * u.totLen;
+    double halfTotTerm = halfTot + 4 * u.per;
+    if (zr < -halfTotTerm || zr > halfTotTerm) return;
+    for (int i = 0; i < u.nh; i++) {
+        double ph = u.ph[i];
+        long long n2pi = (long long)(fabs(ph) / twoPi);
+        if (ph > 0) ph -= twoPi * n2pi; else if (ph < 0) ph += twoPi * n2pi;
+        double perH = u.per / u.hn[i];
+        double halfPerH = 0.5 * perH;
+        double zrr = zr + ph * perH / twoPi;
+        if (zrr < -halfTotTerm || zrr > halfTotTerm) continue;
+        double kw = twoPi / perH;
+        double Bm = (u.K[i] / (u.per * factK2B)) * cosh(kw * yr);
+        double halfThis = halfTot + ((u.sym[i] == 1) ? 0.25 * u.per : 0.);
+        double dB;
+        if (zrr < -halfThis || zrr > halfThis) {
+            double perE2 = perH * perH, p = 2. * piE2 / (3. * perE2);
+            double za = (zrr < -halfThis) ? zrr + halfThis : zrr - halfThis;
+            double za2 = za * za;
+            dB = (twoPi * Bm * za / perH) * (1. - 4. * piE2 * za2 / (9. * perE2)) * exp(-p * za2);
+            double zt = (zrr < -halfThis) ? (-halfThis + 0.5 * halfPerH) : (halfThis - 0.5 * halfPerH);
+            double tt = (u.sym[i] == 1) ? cos(kw * zt) : sin(kw * zt);
+            if (zrr < -halfThis) { if (tt < 0) dB = -dB; }
+            else { if (tt > 0) dB = -dB; }
+        } else {
+            double arg = kw * zrr;
+            dB = Bm * ((u.sym[i] == 1) ? cos(arg) : sin(arg));
+        }
+        if (u.isX[i]) B[0] += dB; else B[1] += dB;
+    }
+}
+
+__device__ void field_B(const srwd_field& f, double x, double y, double z, double* B)
+{
+    B[0] = B[1] = B[2] = 0.;
+    (void)x;
+    for (int i = 0; i < f.n; i++) und_addB(f.u[i], y - f.u[i].cy, z - f.u[i].cz, B);
+}
+
+struct rk_ctx { const srwd_field* f; double gEm2, mult; };
+
+__device__ __forceinline__ void rk_deriv(const rk_ctx& c, const double* F, double* dF)
+{   // srgtrjdt.h:279-293
+    double xd = F[1], yd = F[3];
+    double zd = radical_one_plus_small(-(c.gEm2 + xd * xd + yd * yd));
+    double B[3];
+    field_B(*c.f, F[0], F[2], F[4], B);
+    dF[0] = xd;
+    dF[1] = c.mult * (yd * B[2] - zd * B[1]);
+    dF[2] = yd;
+    dF[3] = c.mult * (zd * B[0] - xd * B[2]);
+    dF[4] = zd;
+}
+
+__device__ void rk4_step(const rk_ctx& c, double* y, const double* d, double h)
+{   // gmintrk.h:169-200
+    double hh = 0.5 * h, h6 = h / 6., yt[5], dyt[5], dym[5];
+    for (int i = 0; i < 5; i++) yt[i] = y[i] + hh * d[i];
+    rk_deriv(c, yt, dyt);
+    for (int i = 0; i < 5; i++) yt[i] = y[i] + hh * dyt[i];
+    rk_deriv(c, yt, dym);
+    for (int i = 0; i < 5; i++) { yt[i] = y[i] + h * dym[i]; dym[i] += dyt[i]; }
+    rk_deriv(c, yt, dyt);
+    for (int i = 0; i < 5; i++) y[i] += h6 * (d[i] + dyt[i] + 2. * dym[i]);
+}
+
+// Integrates np points from xmin to xmax starting at y0 (gmintrk.h:129-165);
+// point j is stored at output index base + dir*j.
+__device__ void rk_chain(const rk_ctx& c, const double* y0, double xmin, double xmax, long long np,
+                         double* oX, double* oXp, double* oY, double* oYp, double* oZ,
+                         long long base, int dir)
+{
+    double step = (xmax - xmin) / (double)(np - 1);
+    double y[5], d[5];
+    for (int k = 0; k < 5; k++) y[k] = y0[k];
+    for (long long j = 0; j < np; j++) {
+        long long o = base + dir * j;
+        oX[o] = y[0]; oXp[o] = y[1]; oY[o] = y[2]; oYp[o] = y[3];
+        if (oZ) oZ[o] = y[4];
+        if (j != np - 1) { rk_deriv(c, y, d); rk4_step(c, y, d, step); }
+    }
+}
+
+// "Arrive" at xmax from 0 with n points, returning the end state.
+__device__ void rk_arrive(const rk_ctx& c, double* y, double xmax, long long n)
+{
+    double step = xmax / (double)(n - 1), d[5];
+    for (long long j = 0; j < n - 1; j++) { rk_deriv(c, y, d); rk4_step(c, y, d, step); }
+}
+
+// One thread per (trajectory, side): side 0 fills the part below ct = 0,
+// side 1 the part above it (srgtrjdt.cpp:90-340; the "rotated" branches are
+// unreachable because ctStart < ctEnd is enforced by the caller).
+__global__ void __launch_bounds__(64) k_traj_rk4(const srwd_traj_job* __restrict__ jobs,
+                                                 const srwd_field* __restrict__ fields, int nTraj,
+                                                 double* X, double* Xp, double* Y, double* Yp, double* Z,
+                                                 long long ld)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int tr = t >> 1, side = t & 1;
+    if (tr >= nTraj) return;
+    const srwd_traj_job J = jobs[tr];
+    rk_ctx c;
+    c.f = &fields[J.field];
+    c.gEm2 = 1. / J.gammaInt / J.gammaInt;
+    c.mult = J.mult;
+    double* oX = X + tr * ld; double* oXp = Xp + tr * ld; double* oY = Y + tr * ld;
+    double* oYp = Yp + tr * ld; double* oZ = Z ? Z + tr * ld : nullptr;
+    long long ns = J.np;
+    double sMin = J.ctStart, sMax = J.ctEnd;
+    double sStep = (ns <= 1) ? 0 : (sMax - sMin) / (ns - 1);
+    double tol = (sMax - sMin) * 1.E-12;
+    bool left = sMin < -tol, right = sMax > tol;
+    double ic[5];
+    for (int k = 0; k < 5; k++) ic[k] = J.init[k];
+    if (left && sMax < -tol) {
+        if (side != 0) return;
+        long long auxNp = (long long)(int)fabs(sMax / sStep) + 1;
+        if (auxNp <= 1) auxNp = 2;
+        rk_arrive(c, ic, sMax, auxNp);
+        if (ns <= 1) { oX[0] = ic[0]; oXp[0] = ic[1]; oY[0] = ic[2]; oYp[0] = ic[3]; if (oZ) oZ[0] = ic[4]; }
+        else rk_chain(c, ic, sMax, sMin, ns, oX, oXp, oY, oYp, oZ, ns - 1, -1);
+        return;
+    }
+    if (right && sMin > tol) {
+        if (side != 1) return;
+        long long auxNp = (long long)(int)fabs(sMin / sStep) + 1;
+        if (auxNp <= 1) auxNp = 2;
+        rk_arrive(c, ic, sMin, auxNp);
+        if (ns <= 1) { oX[0] = ic[0]; oXp[0] = ic[1]; oY[0] = ic[2]; oYp[0] = ic[3]; if (oZ) oZ[0] = ic[4]; }
+        else rk_chain(c, ic, sMin, sMax, ns, oX, oXp, oY, oYp, oZ, 0, 1);
+        return;
+    }
+    long long is0 = 0;
+    double s0Act = 0.;
+    if (left) {
+        is0 = (int)fabs((-sMin + tol) / sStep);
+        if (is0 >= ns) is0 = ns - 1;
+        s0Act = sMin + is0 * sStep;
+        if (s0Act < -tol) rk_arrive(c, ic, s0Act, 2);
+    }
+    if (side == 0) {
+        if (!left) return;
+        long long nsL = is0 + 1;
+        if (nsL <= 1) { oX[0] = ic[0]; oXp[0] = ic[1]; oY[0] = ic[2]; oYp[0] = ic[3]; if (oZ) oZ[0] = ic[4]; }
+        else rk_chain(c, ic, s0Act, sMin, nsL, oX, oXp, oY, oYp, oZ, nsL - 1, -1);
+    } else {
+        if (!right) return;
+        long long nsR = ns - is0;
+        if (nsR < 1) nsR = 1;
+        if (nsR <= 1) { oX[is0] = ic[0]; oXp[is0] = ic[1]; oY[is0] = ic[2]; oYp[is0] = ic[3]; if (oZ) oZ[is0] = ic[4]; }
+        else rk_chain(c, ic, s0Act, sMax, nsR, oX, oXp, oY, oYp, oZ, is0, 1);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Interpolation structure (FP64)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void coefs5(double h, const double* f, double* c)
+{   // srtrjdat.h:1037-1056
+    double hi = 1. / h, hi2 = hi * hi;
+    double d1 = 0.016666666666667 * hi, d2 = 0.041666666666667 * hi2;
+    double d3 = d2 * hi, d4 = d3 * hi, d5 = 0.2 * d4 * hi;
+    double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5];
+    c[0] = f2;
+    c[1] = (3 * f0 - 30 * f1 - 20 * f2 + 60 * f3 - 15 * f4 + 2 * f5) * d1;
+    c[2] = -(f0 - 16 * f1 + 30 * f2 - 16 * f3 + f4) * d2;
+    c[3] = -(f0 + f1 - 10 * f2 + 14 * f3 - 7 * f4 + f5) * d3;
+    c[4] = (f0 - 4 * (f1 + f3) + 6 * f2 + f4) * d4;
+    c[5] = (-f0 + 5 * (f1 - f4) + 10 * (f3 - f2) + f5) * d5;
+}
+
+__device__ __forceinline__ long long offset5(long long i, long long np)
+{
+    if (i < 2) return 0;
+    if (i < np - 3) return i - 2;
+    if (i < np - 2) return i - 3;
+    return i - 4;
+}
+
+__device__ __forceinline__ double interp_cub_half(const double* f, int i)
+{   // gminterp.h:608-613
+    if (i < 0) return 0.0625 * (5. * f[0] + 15. * f[1] - 5. * f[2] + f[3]);
+    else if (i == 0) return 0.0625 * (-f[0] + 9. * f[1] + 9. * f[2] - f[3]);
+    else return 0.0625 * (f[0] - 5. * f[1] + 15. * f[2] + 5. * f[3]);
+}
+
+// Cumulative Simpson of the squared angle, zeroed at s0 (srtrjdat.cpp:1724-1805).
+// One thread per (trajectory, plane).
+__global__ void __launch_bounds__(64) k_quad_terms(const srwd_trj_info* __restrict__ info, int nTraj,
+                                                   const double* Xp, const double* Yp, long long ld,
+                                                   double* IBx, double* IBz)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int tr = t >> 1, pl = t & 1;
+    if (tr >= nTraj) return;
+    const srwd_trj_info I = info[tr];
+    const double* ang = (pl == 0 ? Xp : Yp) + tr * ld;
+    double* out = (pl == 0 ? IBx : IBz) + tr * ld;
+    long long np = I.np, n1 = np - 1, n2 = n1 - 1;
+    double sStep = I.sStep;
+    double stepMult = 0.333333333333 * (0.5 * sStep);
+    out[0] = 0;
+    const double* p = ang;
+    int mid = -1;
+    double f0 = 0., cur = 0.;
+    for (long long i = 1; i <= n1; i++) {
+        double a = interp_cub_half(p, mid);
+        double f1 = a * a;
+        a = ang[i];
+        double f2 = a * a;
+        cur += stepMult * (f0 + 4. * f1 + f2);
+        out[i] = cur;
+        f0 = f2;
+        if ((i > 1) && (i < n2)) p++;
+        mid = (i < n1) ? 0 : 1;
+    }
+    long long idx = (long long)((I.s0 - I.sStart) / sStep);
+    if (idx >= np - 1) idx = np - 2; else if (idx < 0) idx = 0;
+    double rel = I.s0 - (I.sStart + sStep * idx);
+    long long off;
+    if (idx < 2) { off = 0; rel -= (2 - idx) * sStep; }
+    else if (idx < np - 3) off = idx - 2;
+    else if (idx < np - 2) { off = idx - 3; rel += sStep; }
+    else { off = idx - 4; rel += 2 * sStep; }
+    double c[6];
+    coefs5(sStep, out + off, c);
+    double v = c[0] + rel * (c[1] + rel * (c[2] + rel * (c[3] + rel * (c[4] + rel * c[5]))));
+    for (long long j = 0; j <= n1; j++) out[j] -= v;
+}
+
+// Evaluates coordinate, angle, integral of squared angle and field of one
+// plane at s (srtrjdat.cpp:1185-1248 with byDiv, srtrjdat.h:431-472 without).
+struct trj_pt { double crd, bt, ib, b, dbds; };
+
+__device__ __forceinline__ trj_pt eval_plane(const srwd_trj_info& I, const double* crd, const double* ib,
+                                             double bmult, double s, bool byDiv)
+{
+    long long np = I.np;
+    long long idx = byDiv ? (long long)((s - I.sStart) / I.sStep) : (long long)((s - I.sStart) * I.invStep);
+    if (idx >= np - 1) idx = np - 2;
+    if (idx < 0) idx = 0;
+    double r = s - (I.sStart + I.sStep * idx);
+    if (idx < 2) r -= (2 - idx) * I.sStep;
+    else if (idx < np - 3) ;
+    else if (idx < np - 2) r += I.sStep;
+    else r += 2 * I.sStep;
+    long long off = offset5(idx, np);
+    double c[6], q[6];
+    coefs5(I.sStep, crd + off, c);
+    coefs5(I.sStep, ib + off, q);
+    double bt[5] = {c[1], 2. * c[2], 3. * c[3], 4. * c[4], 5. * c[5]};
+    double bb[4] = {bt[1] * bmult, 2. * bt[2] * bmult, 3. * bt[3] * bmult, 4. * bt[4] * bmult};
+    trj_pt P;
+    P.ib = q[0] + r * (q[1] + r * (q[2] + r * (q[3] + r * (q[4] + r * q[5]))));
+    P.crd = c[0] + r * (c[1] + r * (c[2] + r * (c[3] + r * (c[4] + r * c[5]))));
+    P.bt = bt[0] + r * (bt[1] + r * (bt[2] + r * (bt[3] + r * bt[4])));
+    P.b = bb[0] + r * (bb[1] + r * (bb[2] + r * bb[3]));
+    P.dbds = bb[1] + r * (bb[2] * 2 + r * (bb[3] * 3));
+    return P;
+}
+
+__device__ __forceinline__ double int_point_s(const srwd_trj_info& I, long long i)
+{
+    return (i == I.nInt - 1) ? I.sIntegFin : I.sIntegStart + (double)i * I.h;
+}
+
+// Integration-point tables.  One thread per (trajectory, integration point).
+// Writes the FP32 record, the FP64 phase base Q_i and, for the first point of
+// each chunk, the FP64 chunk reference.
+__global__ void __launch_bounds__(256) k_sr_tables(const srwd_trj_info* __restrict__ info,
+                                                   const double* X, const double* Y, const double* IBx,
+                                                   const double* IBz, long long ldT,
+                                                   srwd_rec* rec, srwd_chunk* chunk, double* Qout,
+                                                   long long ldI, long long ldC, long long maxInt)
+{
+    long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int tr = blockIdx.y;
+    if (gi >= maxInt) return;
+    const srwd_trj_info I = info[tr];
+    if (gi >= I.nInt) return;
+    bool byDiv = I.nInt < 10000;   // srradint.h:1046-1050: manual "faster" below 10000 points
+    const double* cx = X + tr * ldT; const double* cz = Y + tr * ldT;
+    const double* ix = IBx + tr * ldT; const double* iz = IBz + tr * ldT;
+    double bmx = -1. / I.betaNorm, bmz = 1. / I.betaNorm;
+    auto eval = [&](long long i, double& s, double& Xv, double& Zv, double& Btx, double& Btz, double& IB) {
+        s = int_point_s(I, i);
+        double IBxv, IBzv;
+        if (I.verNZ) { trj_pt p = eval_plane(I, cx, ix, bmx, s, byDiv); Xv = p.crd; Btx = p.bt; IBxv = p.ib; }
+        else { double d = s - I.s0; Btx = I.dxds0; Xv = I.x0 + I.dxds0 * d; IBxv = I.dxds0 * I.dxds0 * d; }
+        if (I.horNZ) { trj_pt p = eval_plane(I, cz, iz, bmz, s, byDiv); Zv = p.crd; Btz = p.bt; IBzv = p.ib; }
+        else { double d = s - I.s0; Btz = I.dzds0; Zv = I.z0 + I.dzds0 * d; IBzv = I.dzds0 * I.dzds0 * d; }
+        IB = IBxv + IBzv;
+    };
+    double s, Xv, Zv, Btx, Btz, IB;
+    eval(gi, s, Xv, Zv, Btx, Btz, IB);
+    double inv = 1. / (I.yObs - s);
+    double U = Xv * inv, V = Zv * inv;
+    double Q = s * I.gammaEm2 + IB + (Xv * Xv + Zv * Zv) * inv;
+    long long c0 = (gi / SRWD_CHUNK) * SRWD_CHUNK;
+    double invC, UC, VC;
+    if (c0 == gi) { invC = inv; UC = U; VC = V; }
+    else {
+        double s2, X2, Z2, bt1, bt2, ib2;
+        eval(c0, s2, X2, Z2, bt1, bt2, ib2);
+        invC = 1. / (I.yObs - s2); UC = X2 * invC; VC = Z2 * invC;
+    }
+    // composite-rule weight in units of h/15 (srradint.cpp:1576-1588)
+    double w = (gi == 0 || gi == I.nInt - 1) ? 7. : ((gi & 1) ? 16. : 14.);
+    srwd_rec R;
+    R.dInv = (float)(inv - invC);
+    R.dU = (float)(U - UC);
+    R.dV = (float)(V - VC);
+    R.X = (float)Xv;
+    R.Z = (float)Zv;
+    R.a1 = (float)(w * Btx * inv);
+    R.a2 = (float)(w * Btz * inv);
+    R.b = (float)(w * inv * inv);
+    rec[tr * ldI + gi] = R;
+    Qout[tr * ldI + gi] = Q;
+    if (c0 == gi) {
+        srwd_chunk C; C.inv = inv; C.U = U; C.V = V; C.pad = 0;
+        chunk[tr * ldC + gi / SRWD_CHUNK] = C;
+    }
+}
+
+// Edge data at the integration limits (srtrjdat.h:1006-1034).  One thread per (trajectory, end).
+__global__ void k_sr_edges(const srwd_trj_info* __restrict__ info, int nTraj, const double* X, const double* Y,
+                           const double* IBx, const double* IBz, long long ldT, srwd_edge* edges)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int tr = t >> 1, end = t & 1;
+    if (tr >= nTraj) return;
+    const srwd_trj_info I = info[tr];
+    double s = end ? I.sIntegFin : I.sIntegStart;
+    trj_pt px = eval_plane(I, X + tr * ldT, IBx + tr * ldT, -1. / I.betaNorm, s, true);
+    trj_pt pz = eval_plane(I, Y + tr * ldT, IBz + tr * ldT, 1. / I.betaNorm, s, true);
+    double* v = edges[tr].v[end];
+    v[0] = px.dbds; v[1] = px.b; v[2] = px.bt; v[3] = px.crd; v[4] = px.ib;
+    v[5] = pz.dbds; v[6] = pz.b; v[7] = pz.bt; v[8] = pz.crd; v[9] = pz.ib;
+}
+
+// Phase table A[row][i] = frac(K'*Q_i) with K' the phase constant in
+// revolutions per metre for the row's photon energy.
+#define SRWD_KPH(e) (3.141592653590 * 1.E+06 * 0.80654658 * (e))
+__global__ void __launch_bounds__(256) k_phase_table(const double* __restrict__ Q, long long ldI,
+                                                     const int* __restrict__ rowTraj,
+                                                     const double* __restrict__ rowE,
+                                                     const long long* __restrict__ rowN, float* A,
+                                                     long long maxInt)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int row = blockIdx.y;
+    if (i >= maxInt || i >= rowN[row]) return;
+    double k = SRWD_KPH(rowE[row]) * (0.5 / M_PI);
+    double t = k * Q[rowTraj[row] * ldI + i];
+    A[row * ldI + i] = (float)(t - rint(t));
+}
+
+// ------------------------------------------------------------------------
+// SR integral main kernel
+// ------------------------------------------------------------------------
+// Block = 256 observation points of one item; blockIdx.y = split of the
+// integration range.  Per-point data of the trajectory are uniform across the
+// block and are read with scalar loads.
+struct kahan4 { float s[4], c[4]; };
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x)
+{
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict__ items,
+                                                     const int* __restrict__ blockItem,
+                                                     const srwd_trj_info* __restrict__ info,
+                                                     const srwd_rec* __restrict__ rec,
+                                                     const srwd_chunk* __restrict__ chunks,
+                                                     const float* __restrict__ A,
+                                                     long long ldI, long long ldC, int nSplit,
+                                                     long long nPointsPad, double* __restrict__ part)
+{
+    const int it = blockItem[blockIdx.x];
+    const srwd_item& IT = items[it];
+    const int local = (blockIdx.x - IT.blockBase) * 256 + threadIdx.x;
+    const long long flat = IT.pointBase + local;
+    const bool valid = local < IT.npts;
+    const srwd_trj_info& I = info[IT.traj];
+    int ix = valid ? local % IT.nxc : 0, iz = valid ? local / IT.nxc : 0;
+    double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
+    double kph = SRWD_KPH(IT.eph) * (0.5 / M_PI);
+    double r2 = xo * xo + zo * zo;
+    float alpha = (float)(kph * r2), beta = (float)(-2. * kph * xo), gam = (float)(-2. * kph * zo);
+    float fxo = (float)xo, fzo = (float)zo;
+    long long nInt = I.nInt;
+    long long nChunk = (nInt + SRWD_CHUNK - 1) / SRWD_CHUNK;
+    long long cBeg = (nChunk * blockIdx.y) / nSplit, cEnd = (nChunk * (blockIdx.y + 1)) / nSplit;
+    const srwd_rec* R = rec + (long long)IT.traj * ldI;
+    const srwd_chunk* CH = chunks + (long long)IT.traj * ldC;
+    const float* Ar = A + (long long)IT.arow * ldI;
+    kahan4 acc;
+    for (int k = 0; k < 4; k++) { acc.s[k] = 0.f; acc.c[k] = 0.f; }
+    for (long long c = cBeg; c < cEnd; c++) {
+        const srwd_chunk C = CH[c];
+        double tc = kph * (r2 * C.inv - 2. * xo * C.U - 2. * zo * C.V);
+        float Tc = (float)(tc - rint(tc));
+        long long i0 = c * SRWD_CHUNK, i1 = i0 + SRWD_CHUNK;
+        if (i1 > nInt) i1 = nInt;
+        for (long long b0 = i0; b0 < i1; b0 += 16) {
+            float bx = 0.f, bxi = 0.f, bz = 0.f, bzi = 0.f;
+            long long b1 = b0 + 16 < i1 ? b0 + 16 : i1;
+#pragma unroll 4
+            for (long long i = b0; i < b1; i++) {
+                const srwd_rec r = R[i];
+                float t = Ar[i] + Tc;
+                t = fmaf(alpha, r.dInv, t);
+                t = fmaf(beta, r.dU, t);
+                t = fmaf(gam, r.dV, t);
+                t = t - rintf(t);
+                float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
+                float dx = fxo - r.X, dz = fzo - r.Z;
+                float wax = fmaf(-dx, r.b, r.a1), waz = fmaf(-dz, r.b, r.a2);
+                bx = fmaf(wax, cs, bx); bxi = fmaf(wax, sn, bxi);
+                bz = fmaf(waz, cs, bz); bzi = fmaf(waz, sn, bzi);
+            }
+            kahan_add(acc.s[0], acc.c[0], bx); kahan_add(acc.s[1], acc.c[1], bxi);
+            kahan_add(acc.s[2], acc.c[2], bz); kahan_add(acc.s[3], acc.c[3], bzi);
+        }
+    }
+    if (!valid) return;
+    double* o = part + ((long long)blockIdx.y * nPointsPad + flat) * 4;
+    for (int k = 0; k < 4; k++) o[k] = (double)acc.s[k] - (double)acc.c[k];
+}
+
+// ------------------------------------------------------------------------
+// End terms (FP64) and finalisation
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void cos_sin_ref(double x, double& c, double& s)
+{
+    s = sin(x); c = cos(x);
+}
+
+struct cplx { double re, im; };
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// srradint.cpp:398-871, coordinate representation, 3-term far-field expansion.
+// Returns false when the expansion is rejected (residual zero, no derivative).
+__device__ bool residual(const srwd_trj_info& I, const double* ev, double s, double xo, double yo, double zo,
+                         double eph, double actNorm, cplx* R, cplx* D)
+{
+    double dBzds = ev[0], Bz = ev[1], Btx = ev[2], X = ev[3], IBx = ev[4];
+    double dBxds = ev[5], Bx = ev[6], Btz = ev[7], Z = ev[8], IBz = ev[9];
+    double kph = SRWD_KPH(eph);
+    double inv = 1. / (yo - s), dx = xo - X, dz = zo - Z;
+    double Ph = kph * (s * I.gammaEm2 + IBx + IBz + (dx * dx + dz * dz) * inv);
+    double cph, sph;
+    cos_sin_ref(Ph, cph, sph);
+    cplx eip = {cph, sph};
+    double bxn = Btx - dx * inv, bzn = Btz - dz * inv;
+    double dPhds = kph * (I.gammaEm2 + bxn * bxn + bzn * bzn);
+    double odp = 1. / dPhds;
+    double Ax = bxn * inv, Az = bzn * inv;
+    double cBxBz = I.betaNorm * Bz, cBzBx = -I.betaNorm * Bx;
+    double cxA = cBxBz + Ax, czA = cBzBx + Az, tk = 2. * kph;
+    double d2 = tk * (bxn * cxA + bzn * czA);
+    double dAxds = (2. * Ax + cBxBz) * inv, dAzds = (2. * Az + cBzBx) * inv;
+    double q = d2 * odp;
+    double t2x = (dAxds - Ax * q) * odp, t2z = (dAzds - Az * q) * odp;
+    double cdBz = I.betaNorm * dBzds, cdBx = -I.betaNorm * dBxds;
+    double d3 = tk * (cxA * cxA + czA * czA + bxn * (inv * (2. * bxn * inv + cBxBz) + cdBz) +
+                      bzn * (inv * (2. * bzn * inv + cBzBx) + cdBx));
+    double d2Ax = inv * (3. * dAxds + cdBz), d2Az = inv * (3. * dAzds + cdBx);
+    double odp2 = odp * odp, qq = d2 * d2 * odp2;
+    double t3x = (-d2Ax + (3. * dAxds * d2 + Ax * d3) * odp - 3. * Ax * qq) * odp2;
+    double t3z = (-d2Az + (3. * dAzds * d2 + Az * d3) * odp - 3. * Az * qq) * odp2;
+    double t1x = Ax;
+    bool ok2 = ((t2x == 0.) && (t1x == 0.)) || ((t2x != 0.) && (fabs(t2x) < 0.7 * fabs(t1x)));
+    bool ok3 = ok2 && (((t3x == 0.) && (t2x == 0.)) || ((t3x != 0.) && (fabs(t3x) < 0.7 * fabs(t2x))));
+    if (!ok2) return false;
+    cplx pX = {Ax, t2x}, pZ = {Az, t2z};
+    if (ok3) { pX.re += t3x; pZ.re += t3z; }
+    // multiply by i/dPhds then by exp(i Ph)
+    cplx mX = {-pX.im * odp, pX.re * odp}, mZ = {-pZ.im * odp, pZ.re * odp};
+    cplx ex = cmul(mX, eip), ez = cmul(mZ, eip);
+    R[0] = {actNorm * ex.re, actNorm * ex.im};
+    R[1] = {actNorm * ez.re, actNorm * ez.im};
+    D[0] = cmul({dAxds, Ax * dPhds}, eip);
+    D[1] = cmul({dAzds, Az * dPhds}, eip);
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict__ items,
+                                                     const int* __restrict__ blockItem,
+                                                     const srwd_trj_info* __restrict__ info,
+                                                     const srwd_edge* __restrict__ edges, int termin,
+                                                     const double* __restrict__ part, int nSplit,
+                                                     long long nPointsPad, float2* Ex, float2* Ez, int* warn)
+{
+    const int it = blockItem[blockIdx.x];
+    const srwd_item& IT = items[it];
+    const int local = (blockIdx.x - IT.blockBase) * 256 + threadIdx.x;
+    if (local >= IT.npts) return;
+    const long long flat = IT.pointBase + local;
+    const srwd_trj_info& I = info[IT.traj];
+    int ix = local % IT.nxc, iz = local / IT.nxc;
+    double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
+    double S[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nSplit; k++) {
+        const double* p = part + ((long long)k * nPointsPad + flat) * 4;
+        S[0] += p[0]; S[1] += p[1]; S[2] += p[2]; S[3] += p[3];
+    }
+    double h = I.h;
+    double w = IT.actNorm * h / 15.;
+    cplx ex = {w * S[0], w * S[1]}, ez = {w * S[2], w * S[3]};
+    if (termin > 0) {
+        cplx RL[2] = {{0, 0}, {0, 0}}, RR[2] = {{0, 0}, {0, 0}}, DL[2] = {{0, 0}, {0, 0}}, DR[2] = {{0, 0}, {0, 0}};
+        const srwd_edge& E = edges[IT.traj];
+        bool bad = false;
+        if (termin == 1 || termin == 2)
+            bad |= !residual(I, E.v[0], I.sIntegStart, xo, I.yObs, zo, IT.eph, IT.actNorm, RL, DL);
+        if (termin == 1 || termin == 3)
+            bad |= !residual(I, E.v[1], I.sIntegFin, xo, I.yObs, zo, IT.eph, IT.actNorm, RR, DR);
+        if (bad) atomicOr(warn, 1);
+        double wd = IT.actNorm * h * h / 15.;
+        ex.re += RR[0].re - RL[0].re + wd * (DL[0].re - DR[0].re);
+        ex.im += RR[0].im - RL[0].im + wd * (DL[0].im - DR[0].im);
+        ez.re += RR[1].re - RL[1].re + wd * (DL[1].re - DR[1].re);
+        ez.im += RR[1].im - RL[1].im + wd * (DL[1].im - DR[1].im);
+    }
+    float2 vx = make_float2((float)ex.re, (float)ex.im), vz = make_float2((float)ez.re, (float)ez.im);
+    long long ne = IT.ne;
+    auto put = [&](int jx, int jz, float sx, float sz) {
+        long long o = IT.outOffset + ((long long)jz * IT.nx + jx) * ne;
+        Ex[o] = make_float2(sx * vx.x, sx * vx.y);
+        Ez[o] = make_float2(sz * vz.x, sz * vz.y);
+    };
+    put(ix, iz, 1.f, 1.f);
+    // symmetric parts (srradint.cpp:3264-3350): mirror in z flips Ez, in x flips Ex
+    int hz = IT.nz >> 1, hx = IT.nx >> 1;
+    bool mz = IT.symZ && iz < hz, mx = IT.symX && ix < hx;
+    if (mz) put(ix, IT.nz - 1 - iz, 1.f, -1.f);
+    if (mx) put(IT.nx - 1 - ix, iz, -1.f, 1.f);
+    if (mz && mx) put(IT.nx - 1 - ix, IT.nz - 1 - iz, -1.f, -1.f);
+}
+
+// ------------------------------------------------------------------------
+// Launch wrappers (internal C ABI)
+// ------------------------------------------------------------------------
+extern "C" {
+
+int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int nTraj, double* dX,
+                     double* dXp, double* dY, double* dYp, double* dZ, long long ld, hipStream_t st)
+{
+    int threads = 2 * nTraj;
+    hipLaunchKernelGGL(k_traj_rk4, dim3((threads + 63) / 64), dim3(64), 0, st, dJobs, dFields, nTraj,
+                       dX, dXp, dY, dYp, dZ, ld);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_tables(const srwd_trj_info* dInfo, int nTraj, long long maxInt, const double* dX,
+                       const double* dXp, const double* dY, const double* dYp, long long ldT,
+                       double* dIBx, double* dIBz, srwd_rec* dRec, srwd_chunk* dChunk, double* dQ,
+                       long long ldI, long long ldC, srwd_edge* dEdge, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_quad_terms, dim3((2 * nTraj + 63) / 64), dim3(64), 0, st, dInfo, nTraj, dXp, dYp,
+                       ldT, dIBx, dIBz);
+    hipLaunchKernelGGL(k_sr_tables, dim3((unsigned)((maxInt + 255) / 256), nTraj), dim3(256), 0, st, dInfo,
+                       dX, dY, dIBx, dIBz, ldT, dRec, dChunk, dQ, ldI, ldC, maxInt);
+    hipLaunchKernelGGL(k_sr_edges, dim3((2 * nTraj + 63) / 64), dim3(64), 0, st, dInfo, nTraj, dX, dY,
+                       dIBx, dIBz, ldT, dEdge);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj, const double* dRowE,
+                            const long long* dRowN, int nRows, long long maxInt, float* dA, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_phase_table, dim3((unsigned)((maxInt + 255) / 256), nRows), dim3(256), 0, st, dQ,
+                       ldI, dRowTraj, dRowE, dRowN, dA, maxInt);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
+                   const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA, long long ldI,
+                   long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sr_integral, dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlockItem, dInfo, dRec,
+                       dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
+                         const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
+                         long long nPointsPad, float* dEx, float* dEz, int* dWarn, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sr_finalize, dim3(nBlocks), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin,
+                       dPart, nSplit, nPointsPad, (float2*)dEx, (float2*)dEz, dWarn);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// Zero-field flags and wavefront radius
+// ------------------------------------------------------------------------
+// srtrjdat.cpp:2048-2062: a plane is treated as moving when any position or
+// angle sample is non-zero.  One block per trajectory.
+__global__ void __launch_bounds__(256) k_nz_flags(srwd_trj_info* info, const double* X, const double* Xp,
+                                                  const double* Y, const double* Yp, long long ld)
+{
+    int tr = blockIdx.x;
+    long long np = info[tr].np;
+    __shared__ int fv, fh;
+    if (threadIdx.x == 0) { fv = 0; fh = 0; }
+    __syncthreads();
+    int v = 0, h = 0;
+    for (long long i = threadIdx.x; i < np; i += blockDim.x) {
+        long long o = tr * ld + i;
+        v |= (X[o] != 0. || Xp[o] != 0.);
+        h |= (Y[o] != 0. || Yp[o] != 0.);
+    }
+    if (v) atomicOr(&fv, 1);
+    if (h) atomicOr(&fh, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) { info[tr].verNZ = fv; info[tr].horNZ = fh; }
+}
+
+// Trajectory samples for the radius search (srtrjdat.cpp:1416-1460: the
+// polynomials are used regardless of the zero-field flags).
+__global__ void __launch_bounds__(256) k_rx_samples(const srwd_trj_info* __restrict__ info, const double* X,
+                                                    const double* Y, const double* IBx, const double* IBz,
+                                                    long long ldT, double* samp, long long ldS)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int tr = blockIdx.y;
+    const srwd_trj_info I = info[tr];
+    if (i >= I.np) return;
+    double s = I.sStart + (double)i * I.sStep;
+    trj_pt px = eval_plane(I, X + tr * ldT, IBx + tr * ldT, -1. / I.betaNorm, s, true);
+    trj_pt pz = eval_plane(I, Y + tr * ldT, IBz + tr * ldT, 1. / I.betaNorm, s, true);
+    double* o = samp + tr * ldS * 4;
+    o[i] = px.bt; o[ldS + i] = pz.bt; o[2 * ldS + i] = px.crd; o[3 * ldS + i] = pz.crd;
+}
+
+// srradstr.cpp:2310-2501 (FindAverageDistanceToSource) with the full
+// trajectory range.  mesh[4*tr] = xStart, xEnd, zStart, zEnd; out[4*tr] =
+// Robs, RobsAbsErr, xc, zc.  One thread per trajectory.
+__global__ void k_find_rx(const srwd_trj_info* __restrict__ info, int nTraj, const double* samp, long long ldS,
+                          const double* mesh, double* out)
+{
+    int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tr >= nTraj) return;
+    const srwd_trj_info I = info[tr];
+    const double* bx = samp + tr * ldS * 4; const double* bz = bx + ldS;
+    const double* xa = bx + 2 * ldS; const double* za = bx + 3 * ldS;
+    double xS = mesh[4 * tr], xE = mesh[4 * tr + 1], zS = mesh[4 * tr + 2], zE = mesh[4 * tr + 3];
+    long long N = I.np;
+    double sStart = I.sStart, sRange = (N - 1) * I.sStep, sEnd = sStart + sRange;
+    const double VL = 1.E+23;
+    long long L = N - 1;
+    double R = I.yObs - sEnd, iR = 1. / R;
+    double mxs = (xS - xa[L]) * iR - bx[L], mxf = (xE - xa[L]) * iR - bx[L];
+    double mzs = (zS - za[L]) * iR - bz[L], mzf = (zE - za[L]) * iR - bz[L];
+    double rxs = VL, rxf = VL, rzs = VL, rzf = VL;
+    for (long long is = 1; is < N; is++) {
+        R += I.sStep; iR = 1. / R;
+        long long j = L - is;
+        if (rxs == VL) { double m = (xS - xa[j]) * iR - bx[j]; if (m * mxs < 0.) rxs = R; }
+        if (rxf == VL) { double m = (xE - xa[j]) * iR - bx[j]; if (m * mxf < 0.) rxf = R; }
+        if (rzs == VL) { double m = (zS - za[j]) * iR - bz[j]; if (m * mzs < 0.) rzs = R; }
+        if (rzf == VL) { double m = (zE - za[j]) * iR - bz[j]; if (m * mzf < 0.) rzf = R; }
+        if (rxs != VL && rxf != VL && rzs != VL && rzf != VL) break;
+    }
+    double minRx = rxs < rxf ? rxs : rxf, minRz = rzs < rzf ? rzs : rzf;
+    double minR = minRx < minRz ? minRx : minRz;
+    R = I.yObs - sStart; iR = 1. / R;
+    mxs = (xS - xa[0]) * iR - bx[0]; mxf = (xE - xa[0]) * iR - bx[0];
+    mzs = (zS - za[0]) * iR - bz[0]; mzf = (zE - za[0]) * iR - bz[0];
+    rxs = rxf = rzs = rzf = VL;
+    for (long long j = 1; j < N; j++) {
+        R -= I.sStep; iR = 1. / R;
+        if (rxs == VL) { double m = (xS - xa[j]) * iR - bx[j]; if (m * mxs < 0.) rxs = R; }
+        if (rxf == VL) { double m = (xE - xa[j]) * iR - bx[j]; if (m * mxf < 0.) rxf = R; }
+        if (rzs == VL) { double m = (zS - za[j]) * iR - bz[j]; if (m * mzs < 0.) rzs = R; }
+        if (rzf == VL) { double m = (zE - za[j]) * iR - bz[j]; if (m * mzf < 0.) rzf = R; }
+    }
+    double maxRx = rxs < rxf ? rxs : rxf, maxRz = rzs < rzf ? rzs : rzf;
+    double maxR = (minRx < minRz) ? maxRx : maxRz;
+    double avg = 0.5 * (minR + maxR);
+    if (minR == VL) avg = (maxR == VL) ? VL : maxR;
+    else if (maxR == VL) avg = minR;
+    double Robs, ysrc;
+    if (avg != VL) { Robs = avg; ysrc = I.yObs - Robs; }
+    else {
+        ysrc = (sStart < 0. && sEnd > 0.) ? 0.35 * sRange : sStart + 0.75 * sRange;
+        Robs = I.yObs - ysrc;
+    }
+    long long iy = (long long)((ysrc - sStart) / I.sStep + 0.00001);
+    if (iy < 0) iy = 0;
+    if (iy >= N) iy = N - 1;
+    out[4 * tr] = Robs; out[4 * tr + 1] = 0.25 * sRange; out[4 * tr + 2] = xa[iy]; out[4 * tr + 3] = za[iy];
+}
+
+extern "C" {
+int srwd_launch_nz(srwd_trj_info* dInfo, int nTraj, const double* dX, const double* dXp, const double* dY,
+                   const double* dYp, long long ldT, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_nz_flags, dim3(nTraj), dim3(256), 0, st, dInfo, dX, dXp, dY, dYp, ldT);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_rx(const srwd_trj_info* dInfo, int nTraj, long long maxNp, const double* dX, const double* dY,
+                   const double* dIBx, const double* dIBz, long long ldT, double* dSamp, const double* dMesh,
+                   double* dRx, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rx_samples, dim3((unsigned)((maxNp + 255) / 256), nTraj), dim3(256), 0, st, dInfo, dX,
+                       dY, dIBx, dIBz, ldT, dSamp, ldT);
+    hipLaunchKernelGGL(k_find_rx, dim3((nTraj + 63) / 64), dim3(64), 0, st, dInfo, nTraj, dSamp, ldT, dMesh, dRx);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+}

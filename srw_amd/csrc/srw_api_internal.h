@@ -1,0 +1,69 @@
+// srw_api_internal.h - host-side state shared by the C ABI implementation files.
+#pragma once
+#include "../../include/srwlib.h"
+#include "srw_dev.h"
+#include <vector>
+
+namespace srwamd {
+
+// Grow-only device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    void* get(size_t bytes);
+    void release();
+};
+
+struct Buffers {
+    DBuf X, Xp, Y, Yp, IBx, IBz, Samp, Jobs, Fields, Info;
+    DBuf Rec, Chunk, Q, Edge, Mesh, Rx, Items, BlockItem, RowTraj, RowE, RowN, A, Part, Warn;
+    DBuf Ex, Ez;                  // field outputs
+    DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
+    DBuf Stokes, Draws, Wt, Net, Grad;
+};
+
+struct Context {
+    bool ready = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    Buffers buf;
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    float lastSrMs = 0;
+    long long lastSrEvals = 0;
+    int lastSrBlocks = 0, lastSrSplit = 0;
+    int forceSplit = 0;
+    double lastFftMs = 0, lastFftBytes = 0;
+};
+
+extern Context g_ctx;
+extern int (*g_wfrModif)(int, SRWLWfr*, char);
+extern char* (*g_allocArray)(char, long long);
+extern int (*g_progress)(double);
+
+int ensure_device();
+
+// One single-electron SR request.
+struct SrJob {
+    SRWLParticle part;
+    double Iavg = 0;
+    int field = 0;
+    long long np = 0;
+    double ctStart = 0, ctEnd = 0;
+    const SRWLPrtTrj* userTrj = nullptr;
+    SRWLRadMesh mesh;
+    double step = 0, sLim0 = 0, sLim1 = 0;
+};
+
+struct SrResult {
+    double Robs = 0, RobsErr = 0, xc = 0, zc = 0;
+    int warn = 0;
+};
+
+int field_from_c(const SRWLMagFldC* c, srwd_field& f, double& gsStart, double& gsEnd);
+int setup_job_from_field(SrJob& J, const SRWLParticle& p, const SRWLMagFldC* mag, const double* precPar,
+                         std::vector<srwd_field>& fields);
+int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
+           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin);
+
+}  // namespace srwamd

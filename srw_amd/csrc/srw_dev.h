@@ -1,0 +1,108 @@
+// srw_dev.h - device-side data structures and the internal C ABI between the
+// host library (srwlib_api.cpp) and the HIP kernels.  Plain structs and
+// pointers only; every launch wrapper takes an explicit stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SRWD_MAX_HARM 8
+#define SRWD_MAX_UND 4
+
+// Periodic undulator in a container (reference srmagfld.h:401-502,
+// srmagfld.cpp:128-174).  Element frames are translations only.
+struct srwd_und {
+    int nh;
+    int hn[SRWD_MAX_HARM];
+    int isX[SRWD_MAX_HARM];     // 1: horizontal field ('h'), 0: vertical
+    int sym[SRWD_MAX_HARM];     // 1 symmetric (cos), -1 anti-symmetric (sin)
+    double K[SRWD_MAX_HARM], ph[SRWD_MAX_HARM];
+    double per, totLen, cx, cy, cz;
+};
+
+struct srwd_field {
+    int n;
+    srwd_und u[SRWD_MAX_UND];
+};
+
+// One electron's trajectory job: RK4 over np points in ct in [ctStart, ctEnd]
+// (reference srgtrjdt.cpp:24-365).
+struct srwd_traj_job {
+    double init[5];      // x, x', y, y', z at ct = 0
+    double gammaInt;     // internal gamma (srebmdat.h:136-141)
+    double mult;         // nq*e/(m c gamma)
+    double ctStart, ctEnd;
+    long long np;
+    int field;           // index into the field array
+    int pad;
+};
+
+// Interpolation / integration setup of one trajectory (srtrjdat.cpp:2006-2088,
+// srradint.cpp:3579-3614).
+struct srwd_trj_info {
+    long long np;            // trajectory points
+    double sStart, sStep, invStep;
+    double gammaEm2, betaNorm;
+    double x0, dxds0, z0, dzds0, s0;
+    int horNZ, verNZ;
+    double sIntegStart, sIntegFin, h;   // integration range and step
+    long long nInt;          // integration points (odd)
+    double yObs;             // longitudinal observation position
+};
+
+// Per integration point, energy independent (FP32), 32 bytes.
+struct srwd_rec {
+    float dInv, dU, dV, X, Z, a1, a2, b;
+};
+
+// Per chunk reference (FP64).
+struct srwd_chunk {
+    double inv, U, V, pad;
+};
+
+// Trajectory values with field at an integration edge:
+// [0..4] dBzds, Bz, Btx, X, IBx ; [5..9] dBxds, Bx, Btz, Z, IBz
+struct srwd_edge {
+    double v[2][10];
+};
+
+// A field computation item: one (trajectory, photon energy) pair on a mesh.
+struct srwd_item {
+    int traj;                 // trajectory index (tables, info, edges)
+    int arow;                 // row of the per-(trajectory, energy) phase table
+    double eph;               // photon energy [eV]
+    double xStart, xStep, zStart, zStep;
+    int nx, nz;               // full mesh
+    int nxc, nzc;             // computed region (symmetry)
+    int symX, symZ;
+    int ne, ie;               // output layout: energies per point and this energy's index
+    long long outOffset;      // float2 offset of (iz=0, ix=0, ie) in the output arrays
+    long long pointBase;      // first point of this item in the partial-sum workspace
+    int blockBase;            // first 256-point block of this item
+    int npts;                 // nxc * nzc
+    double actNorm;           // normalisation constant * eph * 0.80654658e-3
+};
+
+#define SRWD_CHUNK 256
+
+extern "C" {
+// Launch wrappers implemented in sr.hip; all asynchronous on stream st.
+int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int nTraj, double* dX,
+                     double* dXp, double* dY, double* dYp, double* dZ, long long ld, hipStream_t st);
+int srwd_launch_nz(srwd_trj_info* dInfo, int nTraj, const double* dX, const double* dXp, const double* dY,
+                   const double* dYp, long long ldT, hipStream_t st);
+int srwd_launch_tables(const srwd_trj_info* dInfo, int nTraj, long long maxInt, const double* dX,
+                       const double* dXp, const double* dY, const double* dYp, long long ldT,
+                       double* dIBx, double* dIBz, srwd_rec* dRec, srwd_chunk* dChunk, double* dQ,
+                       long long ldI, long long ldC, srwd_edge* dEdge, hipStream_t st);
+int srwd_launch_rx(const srwd_trj_info* dInfo, int nTraj, long long maxNp, const double* dX, const double* dY,
+                   const double* dIBx, const double* dIBz, long long ldT, double* dSamp, const double* dMesh,
+                   double* dRx, hipStream_t st);
+int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj, const double* dRowE,
+                            const long long* dRowN, int nRows, long long maxInt, float* dA, hipStream_t st);
+int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
+                   const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA, long long ldI,
+                   long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st);
+int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
+                         const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
+                         long long nPointsPad, float* dEx, float* dEz, int* dWarn, hipStream_t st);
+}

@@ -1,0 +1,106 @@
+"""srwlpy-compatible Python entry points over libsrw_amd.so.
+
+Method names and argument order follow the reference binding
+(cpp/src/clients/python/srwlpy.cpp:5895-5917): CalcElecFieldSR (:4622),
+CalcPartTraj, CalcIntFromElecField (:4873), PropagElecField (:5214), UtiFFT.
+Errors raise RuntimeError with the library's error text, like the
+reference's ProcRes (srwlpy.cpp:148); warnings are returned silently.
+
+The compute path is the HIP library only: importing this module without the
+built library raises ImportError.
+"""
+import ctypes as C
+import os
+
+from . import _marshal as M
+from . import _structs as S
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsrw_amd.so")
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError("libsrw_amd.so is not built (run python -m srw_amd.build)")
+    lib = C.CDLL(_LIB_PATH)
+    P = C.POINTER
+    lib.srwlCalcElecFieldSR.argtypes = [P(S.SRWLWfr), P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double), C.c_int]
+    lib.srwlCalcPartTraj.argtypes = [P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double)]
+    lib.srwlCalcElecFieldSRBatch.argtypes = [P(P(S.SRWLWfr)), P(P(S.SRWLMagFldC)), C.c_int, P(C.c_double), C.c_int]
+    lib.srwlUtiGetErrText.argtypes = [C.c_char_p, C.c_int]
+    lib.srwlUtiGPUProc.argtypes = [C.c_int, P(C.c_double)]
+    for name in ("srwlPropagElecField",):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [P(S.SRWLWfr), P(S.SRWLOptC), C.c_int, P(C.c_char_p), P(S.SRWLRadMesh),
+                                           P(C.c_char_p), P(C.c_double)]
+    if hasattr(lib, "srwlCalcIntFromElecField"):
+        lib.srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_char, C.c_char, C.c_char,
+                                                 C.c_double, C.c_double, C.c_double, P(C.c_double),
+                                                 C.c_void_p, P(C.c_double)]
+    if hasattr(lib, "srwlUtiFFT"):
+        lib.srwlUtiFFT.argtypes = [C.c_void_p, C.c_char, P(C.c_double), C.c_int, C.c_int, P(C.c_double)]
+    if hasattr(lib, "srwlEmitPropMultiE"):
+        lib.srwlEmitPropMultiE.argtypes = [P(S.SRWLStokes), P(S.SRWLWfr), P(S.SRWLMagFldC), P(S.SRWLOptC),
+                                           P(C.c_double), C.c_int, C.c_longlong, P(C.c_double), P(C.c_double)]
+    lib.srwlAmdGetStats.argtypes = [P(C.c_double), C.c_int]
+    return lib
+
+
+lib = _load()
+LIB_PATH = _LIB_PATH
+
+
+def err_text(code):
+    buf = C.create_string_buffer(2048)
+    lib.srwlUtiGetErrText(buf, int(code))
+    return buf.value.decode()
+
+
+def _check(code):
+    if code > 0:
+        raise RuntimeError(err_text(code))
+    return code
+
+
+def CalcPartTraj(_prt_trj, _mag, _prec_par=None):
+    keep = M.Keep()
+    ct = M.prt_trj(_prt_trj, keep)
+    cm = M.mag_cnt(_mag, keep)
+    _check(lib.srwlCalcPartTraj(C.byref(ct), C.byref(cm), None))
+    M.prt_trj_back(ct, _prt_trj)
+    return _prt_trj
+
+
+def CalcElecFieldSR(_wfr, _trj, _mag, _prec_par):
+    """Near-field SR electric field (reference srwlpy.cpp:4622-4680)."""
+    keep = M.Keep()
+    cw = M.wfr(_wfr, keep)
+    ct = None
+    if _trj is not None and _trj != 0 and getattr(_trj, "arX", None) is not None:
+        ct = C.byref(M.prt_trj(_trj, keep))
+    cm = C.byref(M.mag_cnt(_mag, keep)) if (_mag is not None and _mag != 0) else None
+    prec = M.dvec(_prec_par, keep)
+    rc = lib.srwlCalcElecFieldSR(C.byref(cw), ct, cm, prec, len(_prec_par))
+    _check(rc)
+    M.wfr_back(cw, _wfr)
+    return _wfr
+
+
+def CalcElecFieldSRBatch(_arWfr, _arMag, _prec_par):
+    """Batch of independent single-electron fields in one device pass."""
+    keep = M.Keep()
+    n = len(_arWfr)
+    cws = [M.wfr(w, keep) for w in _arWfr]
+    cms = [M.mag_cnt(m, keep) for m in _arMag]
+    pw = (C.POINTER(S.SRWLWfr) * n)(*[C.pointer(c) for c in cws])
+    pm = (C.POINTER(S.SRWLMagFldC) * n)(*[C.pointer(c) for c in cms])
+    prec = M.dvec(_prec_par, keep)
+    _check(lib.srwlCalcElecFieldSRBatch(pw, pm, n, prec, len(_prec_par)))
+    for c, w in zip(cws, _arWfr):
+        M.wfr_back(c, w)
+    return _arWfr
+
+
+def stats():
+    out = (C.c_double * 8)()
+    lib.srwlAmdGetStats(out, 8)
+    return list(out)

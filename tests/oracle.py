@@ -1,0 +1,89 @@
+"""ctypes loader of the FP64 CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is test infrastructure: it is built by oracle/Makefile (or
+__graft_entry__.build()) and only tests, smoke() and the CPU-baseline leg of
+bench.py load it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+from srw_amd import _marshal as M
+from srw_amd import _structs as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+def load():
+    if not os.path.exists(PATH):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
+    lib = C.CDLL(PATH)
+    P = C.POINTER
+    lib.orc_srwlCalcElecFieldSR_mt.argtypes = [P(S.SRWLWfr), P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double),
+                                               C.c_int, C.c_int]
+    lib.orc_srwlCalcPartTraj.argtypes = [P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double)]
+    lib.orc_srwlPropagElecField.argtypes = [P(S.SRWLWfr), P(S.SRWLOptC)]
+    lib.orc_srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_char, C.c_char, C.c_char, C.c_double]
+    lib.orc_srwlUtiFFT.argtypes = [C.c_void_p, C.c_char, P(C.c_double), C.c_int, C.c_int]
+    lib.orc_field_B.argtypes = [P(S.SRWLMagFldC), C.c_double, C.c_double, C.c_double, P(C.c_double)]
+    lib.orc_trj_interp.argtypes = [P(S.SRWLPrtTrj), P(C.c_double), C.c_longlong, P(C.c_double)]
+    lib.orc_struct_sizes.argtypes = [P(C.c_long)]
+    return lib
+
+
+lib = load()
+
+
+def calc_field(w, mag, prec, trj=None, nthreads=8):
+    keep = M.Keep()
+    cw = M.wfr(w, keep)
+    cm = M.mag_cnt(mag, keep) if mag is not None else None
+    ct = M.prt_trj(trj, keep) if trj is not None else None
+    cp = M.dvec(prec, keep)
+    rc = lib.orc_srwlCalcElecFieldSR_mt(C.byref(cw), C.byref(ct) if ct is not None else None,
+                                        C.byref(cm) if cm is not None else None, cp, len(prec), nthreads)
+    M.wfr_back(cw, w)
+    return rc
+
+
+def calc_traj(trj, mag):
+    keep = M.Keep()
+    ct = M.prt_trj(trj, keep)
+    cm = M.mag_cnt(mag, keep)
+    rc = lib.orc_srwlCalcPartTraj(C.byref(ct), C.byref(cm), None)
+    M.prt_trj_back(ct, trj)
+    return rc
+
+
+def propagate(w, optc):
+    keep = M.Keep()
+    cw = M.wfr(w, keep)
+    co = M.opt_cnt(optc, keep)
+    rc = lib.orc_srwlPropagElecField(C.byref(cw), C.byref(co))
+    M.wfr_back(cw, w)
+    return rc
+
+
+def intensity(arI, w, pol=6, int_type=0, dep_type=3, e=None):
+    keep = M.Keep()
+    cw = M.wfr(w, keep)
+    e = w.mesh.eStart if e is None else e
+    return lib.orc_srwlCalcIntFromElecField(M._addr(arI), C.byref(cw), pol, int_type, dep_type, e)
+
+
+def uti_fft(data, mesh, direction):
+    keep = M.Keep()
+    cm = M.dvec(mesh, keep)
+    rc = lib.orc_srwlUtiFFT(M._addr(data), b"f", cm, len(mesh), direction)
+    for i in range(len(mesh)):
+        mesh[i] = cm[i]
+    return rc
+
+
+def field_B(mag, x, y, z):
+    keep = M.Keep()
+    cm = M.mag_cnt(mag, keep)
+    out = (C.c_double * 3)()
+    lib.orc_field_B(C.byref(cm), x, y, z, out)
+    return list(out)
