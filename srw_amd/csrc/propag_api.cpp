@@ -1,0 +1,353 @@
+// propag_api.cpp - host side of the drift propagation, the standalone SRW
+// FFT and intensity extraction (C ABI in include/srwlib.h).
+//
+// Reference call semantics:
+//   srwlPropagElecField      cpp/src/lib/srwlib.cpp:1023-1074, sroptcnt.cpp:87-425
+//   drift buffer variables   cpp/src/core/sroptdrf.cpp:889-1116
+//   srwlUtiFFT               cpp/src/lib/srwlib.cpp:1090-1204, gmfft.cpp:295-893
+//   srwlCalcIntFromElecField cpp/src/lib/srwlib.cpp:757-856, srradmnp.h:367-430
+#include "srw_api_internal.h"
+#include "srw_fft.h"
+#include "srw_prop.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace srwamd {
+
+static const double kTwoPi = 6.283185307179586;
+static const double kPiSrw = 3.1415926536;   // sroptdrf.h:619, 812
+
+static int log2_exact(long n)
+{
+    if (n <= 0 || (n & (n - 1))) return -1;
+    int l = 0;
+    while ((1L << l) < n) l++;
+    return l;
+}
+
+// Twiddle tables tw[k] = exp(-i 2pi k / N), k < N/2, cached per size.
+const float2* twiddles(int logn)
+{
+    static std::map<int, DBuf> cache;
+    DBuf& b = cache[logn];
+    if (b.p) return (const float2*)b.p;
+    int n = 1 << logn;
+    std::vector<float2> h(n / 2);
+    for (int k = 0; k < n / 2; k++) {
+        double a = -kTwoPi * k / n;
+        h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    void* p = b.get(sizeof(float2) * h.size());
+    if (!p) return nullptr;
+    if (hipMemcpy(p, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return (const float2*)p;
+}
+
+int fft_ready()
+{
+    static bool done = false;
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (!done) {
+        if (srwf_init_attributes()) return SRWL_AMD_ERR_GPU;
+        done = true;
+    }
+    return 0;
+}
+
+// Drift element parameters (sroptcnt.cpp:183-211: arProp layout)
+int drift_mode_from_prop(const double* pp, int& mode)
+{
+    mode = 0;
+    if (pp == nullptr) return 0;
+    if (pp[0] != 0 || pp[1] != 0) return SRWL_AMD_ERR_UNSUPPORTED;    // automatic resizing
+    int at = (int)(char)pp[3];
+    if (at == 0) mode = 0;
+    else if (at == 2) mode = 3;
+    else return SRWL_AMD_ERR_UNSUPPORTED;   // 1: moments-based radius, 3-5: waist modes
+    return 0;
+}
+
+// sroptdrf.cpp:889-1081 with the radius estimate of sroptdrf.cpp:1085-1116.
+void mode3_setup(double Rx, double Rz, double dRx, double dRz, double L, DriftGeom& g)
+{
+    double mx = 3. * dRx, mz = 3. * dRz;
+    double RxL = Rx + L;
+    if (std::fabs(Rx) < mx) Rx = mx * (Rx >= 0 ? 1. : -1.);
+    else if (std::fabs(RxL) < mx) Rx = mx * (RxL >= 0 ? 1. : -1.) - L;
+    double RzL = Rz + L;
+    if (std::fabs(Rz) < mz) Rz = mz * (Rz >= 0 ? 1. : -1.);
+    else if (std::fabs(RzL) < mz) Rz = mz * (RzL >= 0 ? 1. : -1.) - L;
+    const double inf = 1E+23;
+    g.invRx = 0; g.invRz = 0; g.Lx = 0; g.Lz = 0; g.invRxL = 1. / L; g.invRzL = 1. / L;
+    g.kx = inf; g.kxc = inf; g.kz = inf; g.kzc = inf;
+    if (Rx != 0) {
+        g.invRx = 1. / Rx; g.kx = (Rx + L) * g.invRx; g.kxc = L / Rx;
+        if (-L != Rx) { g.Lx = L / (1. + L * g.invRx); g.invRxL = 1. / (L + Rx); } else { g.Lx = inf; g.invRxL = inf; }
+    }
+    if (Rz != 0) {
+        g.invRz = 1. / Rz; g.kz = (Rz + L) * g.invRz; g.kzc = L / Rz;
+        if (-L != Rz) { g.Lz = L / (1. + L * g.invRz); g.invRzL = 1. / (L + Rz); } else { g.Lz = inf; g.invRzL = inf; }
+    }
+    g.mult = std::sqrt(std::fabs(g.Lx * g.Lz)) / L;
+    g.phT = 0.25 * 3.141592653589793 * (2. - (g.Lx < 0 ? -1. : 1.) - (g.Lz < 0 ? -1. : 1.));
+}
+
+// Slice parameters of one photon energy for a drift of length L.
+void drift_slice(srwf_slice& S, int mode, const DriftGeom& g, double L, int treat, double e, long nx, long nz,
+                 double xStart, double xStep, double zStart, double zStep, double xc, double zc)
+{
+    std::memset(&S, 0, sizeof(S));
+    double lam = 1.239842e-06 / e;
+    S.xStart = xStart; S.xStep = xStep; S.zStart = zStart; S.zStep = zStep;
+    S.qStepX = 1. / (nx * xStep); S.qStepZ = 1. / (nz * zStep);
+    S.xc = xc; S.zc = zc;
+    S.ny = (int)nz;
+    double norm = 1. / ((double)nx * (double)nz);   // dx dz (forward) * dqx dqz (inverse)
+    if (mode == 0) {
+        S.c0_rev = -kPiSrw * L * lam / kTwoPi;
+        S.c1 = 0.25 * lam * lam;
+        S.path_rev = (treat == 1) ? (5.067730652e+06) * L * e / kTwoPi : 0.;
+        S.midScale = (float)norm;
+    } else {
+        S.invRx = g.invRx; S.invRz = g.invRz; S.invRxL = g.invRxL; S.invRzL = g.invRzL;
+        S.pre_rev = -(kPiSrw / lam) / kTwoPi;
+        S.post_rev = (kPiSrw / lam) / kTwoPi;
+        S.path3_rev = (treat == 1) ? 2. * (kPiSrw / lam) * L / kTwoPi : 0.;
+        S.c0x_rev = -kPiSrw * lam * g.Lx / kTwoPi;
+        S.c0z_rev = -kPiSrw * lam * g.Lz / kTwoPi;
+        S.phT_rev = g.phT / kTwoPi;
+        S.midScale = (float)(norm * g.mult);
+        S.nxStart = g.kx * xStart - g.kxc * xc; S.nxStep = xStep * g.kx;
+        S.nzStart = g.kz * zStart - g.kzc * zc; S.nzStep = zStep * g.kz;
+    }
+}
+
+// Runs passes A-C over nSlices N x N slices in place.
+int drift_passes(float2* d, long long sliceStride, int nSlices, const srwf_slice* dSp, int lognx, int lognz,
+                 int mode, int postOp, float* inten, long long intenStride, hipStream_t st)
+{
+    const float2* twx = twiddles(lognx);
+    const float2* twz = twiddles(lognz);
+    if (!twx || !twz) return SRWL_AMD_ERR_GPU;
+    int nz = 1 << lognz;
+    if (srwf_launch_rows_fwd(d, sliceStride, nSlices, dSp, twx, lognx, nz, mode == 3 ? 1 : 0, 0, st))
+        return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_cols(d, sliceStride, nSlices, dSp, twz, lognz, lognx, mode == 3 ? 1 : 0, 0, nullptr, st))
+        return SRWL_AMD_ERR_GPU;
+    int pop = postOp;
+    if (pop == 1 && mode != 3) pop = 0;
+    if (srwf_launch_rows_inv(d, sliceStride, nSlices, dSp, twx, lognx, nz, pop, inten, intenStride, st))
+        return SRWL_AMD_ERR_GPU;
+    return 0;
+}
+
+}  // namespace srwamd
+
+using namespace srwamd;
+
+#define HIPCHK(x) do { if ((x) != hipSuccess) return SRWL_AMD_ERR_GPU; } while (0)
+
+extern "C" {
+
+SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, char** arID, SRWLRadMesh* arIM,
+                                    char** arI, double* arParGPU)
+{
+    (void)arID; (void)arIM; (void)arI; (void)arParGPU;
+    if (pWfr == nullptr || pOpt == nullptr) return SRWL_INCORRECT_PARAM_FOR_WFR_PROP;
+    if (nInt != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (pWfr->presCA != 0 || pWfr->presFT != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
+    long nx = pWfr->mesh.nx, nz = pWfr->mesh.ny, ne = pWfr->mesh.ne;
+    int lx = log2_exact(nx), lz = log2_exact(nz);
+    if (lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_ERR_IN_FFT;
+    for (int i = 0; i < pOpt->nElem; i++) {
+        if (std::strcmp(pOpt->arOptTypes[i], "drift") != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+        int mode;
+        const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
+        int rc = drift_mode_from_prop(pp, mode);
+        if (rc) return rc;
+        if (pp && (std::fabs(pp[5] - 1) > 1e-4 || std::fabs(pp[6] - 1) > 1e-4 || std::fabs(pp[7] - 1) > 1e-4 ||
+                   std::fabs(pp[8] - 1) > 1e-4))
+            return SRWL_AMD_ERR_UNSUPPORTED;   // resize instructions
+    }
+    if (pOpt->nProp > pOpt->nElem) {
+        const double* pp = pOpt->arProp[pOpt->nElem];
+        if (pp && (std::fabs(pp[5] - 1) > 1e-4 || std::fabs(pp[6] - 1) > 1e-4 || std::fabs(pp[7] - 1) > 1e-4 ||
+                   std::fabs(pp[8] - 1) > 1e-4))
+            return SRWL_AMD_ERR_UNSUPPORTED;
+    }
+    int rc = fft_ready();
+    if (rc) return rc;
+    hipStream_t st = g_ctx.stream;
+    long long nxy = (long long)nx * nz;
+    long long n = nxy * ne;
+    float2* dIn = (float2*)g_ctx.buf.W0.get(sizeof(float2) * 2 * n);
+    float2* dS = (float2*)g_ctx.buf.W1.get(sizeof(float2) * 2 * n);
+    srwf_slice* dSp = (srwf_slice*)g_ctx.buf.W2.get(sizeof(srwf_slice) * 2 * ne);
+    if (!dIn || !dS || !dSp) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    HIPCHK(hipMemcpyAsync(dIn, pWfr->arEx, sizeof(float2) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dIn + n, pWfr->arEy, sizeof(float2) * n, hipMemcpyHostToDevice, st));
+    // slice-major layout: slice (2*ie + comp) holds [iy][ix]
+    for (int c = 0; c < 2; c++) {
+        if (ne == 1) { HIPCHK(hipMemcpyAsync(dS + c * nxy, dIn + c * n, sizeof(float2) * nxy, hipMemcpyDeviceToDevice, st)); continue; }
+    }
+    if (ne > 1) {
+        // gather each component into [ie][iy][ix], then interleave components per energy
+        float2* tmp = (float2*)g_ctx.buf.W3.get(sizeof(float2) * 2 * n);
+        if (!tmp) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        for (int c = 0; c < 2; c++)
+            if (srwf_launch_gather(dIn + c * n, tmp + c * n, (int)ne, nxy, 0, st)) return SRWL_AMD_ERR_GPU;
+        for (long ie = 0; ie < ne; ie++)
+            for (int c = 0; c < 2; c++)
+                HIPCHK(hipMemcpyAsync(dS + (2 * ie + c) * nxy, tmp + c * n + ie * nxy, sizeof(float2) * nxy,
+                                      hipMemcpyDeviceToDevice, st));
+    }
+    double xStep = (pWfr->mesh.xFin - pWfr->mesh.xStart) / (nx - 1);
+    double zStep = (pWfr->mesh.yFin - pWfr->mesh.yStart) / (nz - 1);
+    double eStep = ne > 1 ? (pWfr->mesh.eFin - pWfr->mesh.eStart) / (ne - 1) : 0;
+    std::vector<srwf_slice> sp(2 * ne);
+    for (int i = 0; i < pOpt->nElem; i++) {
+        const SRWLOptD* d = (const SRWLOptD*)pOpt->arOpt[i];
+        const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
+        int mode;
+        drift_mode_from_prop(pp, mode);
+        DriftGeom g;
+        std::memset(&g, 0, sizeof(g));
+        if (mode == 3) {
+            mode3_setup(pWfr->Rx, pWfr->Ry, pWfr->dRx, pWfr->dRy, d->L, g);
+            if (g.kx < 0 || g.kz < 0) return SRWL_AMD_ERR_UNSUPPORTED;   // mirrored output mesh
+        }
+        for (long ie = 0; ie < ne; ie++) {
+            double e = pWfr->mesh.eStart + ie * eStep;
+            drift_slice(sp[2 * ie], mode, g, d->L, d->treat, e, nx, nz, pWfr->mesh.xStart, xStep, pWfr->mesh.yStart,
+                        zStep, pWfr->xc, pWfr->yc);
+            sp[2 * ie + 1] = sp[2 * ie];
+        }
+        HIPCHK(hipMemcpyAsync(dSp, sp.data(), sizeof(srwf_slice) * 2 * ne, hipMemcpyHostToDevice, st));
+        if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
+        rc = drift_passes(dS, nxy, (int)(2 * ne), dSp, lx, lz, mode, 1, nullptr, 0, st);
+        if (rc) return rc;
+        if (g_ctx.timing) {
+            HIPCHK(hipEventRecord(g_ctx.ev1, st));
+            HIPCHK(hipEventSynchronize(g_ctx.ev1));
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
+            g_ctx.lastFftMs = ms;
+            g_ctx.lastFftBytes = 3. * 2. * 8. * (double)n * 2.;
+        }
+        if (mode == 3) {
+            pWfr->mesh.xStart = sp[0].nxStart;
+            pWfr->mesh.xFin = sp[0].nxStart + (nx - 1) * sp[0].nxStep;
+            pWfr->mesh.yStart = sp[0].nzStart;
+            pWfr->mesh.yFin = sp[0].nzStart + (nz - 1) * sp[0].nzStep;
+            xStep = sp[0].nxStep; zStep = sp[0].nzStep;
+        }
+        // sroptdrf.h:967-979
+        pWfr->Rx += d->L;
+        pWfr->Ry += d->L;
+    }
+    if (ne == 1) {
+        HIPCHK(hipMemcpyAsync(pWfr->arEx, dS, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(pWfr->arEy, dS + nxy, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
+    } else {
+        float2* tmp = (float2*)g_ctx.buf.W3.get(sizeof(float2) * 2 * n);
+        for (long ie = 0; ie < ne; ie++)
+            for (int c = 0; c < 2; c++)
+                HIPCHK(hipMemcpyAsync(tmp + c * n + ie * nxy, dS + (2 * ie + c) * nxy, sizeof(float2) * nxy,
+                                      hipMemcpyDeviceToDevice, st));
+        for (int c = 0; c < 2; c++)
+            if (srwf_launch_gather(tmp + c * n, dIn + c * n, (int)ne, nxy, 1, st)) return SRWL_AMD_ERR_GPU;
+        HIPCHK(hipMemcpyAsync(pWfr->arEx, dIn, sizeof(float2) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(pWfr->arEy, dIn + n, sizeof(float2) * n, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+SRWL_EXPORT int srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMesh, int dir, double* arParGPU)
+{
+    (void)arParGPU;
+    if (pcData == nullptr || arMesh == nullptr || nMesh < 6) return SRWL_INCORRECT_PARAM_FOR_FFT;
+    if (typeData != 'f') return SRWL_AMD_ERR_UNSUPPORTED;
+    long nx = (long)arMesh[2], nz = (long)arMesh[5];
+    int lx = log2_exact(nx), lz = log2_exact(nz);
+    if (nx < 2 || nz < 2 || (nx & 1) || (nz & 1)) return SRWL_INCORRECT_PARAM_FOR_FFT;
+    if (lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_AMD_ERR_UNSUPPORTED;
+    int rc = fft_ready();
+    if (rc) return rc;
+    hipStream_t st = g_ctx.stream;
+    long long nxy = (long long)nx * nz;
+    float2* d = (float2*)g_ctx.buf.W0.get(sizeof(float2) * nxy);
+    float2* o = (float2*)g_ctx.buf.W1.get(sizeof(float2) * nxy);
+    srwf_slice* dSp = (srwf_slice*)g_ctx.buf.W2.get(sizeof(srwf_slice));
+    if (!d || !o || !dSp) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    // gmfft.h:232-248 SetupLimitsTr and gmfft.cpp:303-317 shift decision
+    double xStart = arMesh[0], xStep = arMesh[1], zStart = arMesh[3], zStep = arMesh[4];
+    double xStartTr = -0.5 / xStep, zStartTr = -0.5 / zStep;
+    double xStepTr = -xStartTr / (nx >> 1), zStepTr = -zStartTr / (nz >> 1);
+    double x0 = xStart + 0.5 * nx * xStep, z0 = zStart + 0.5 * nz * zStep;
+    bool shX = std::fabs(x0) > 1.E-06 * nx * xStep, shZ = std::fabs(z0) > 1.E-06 * nz * zStep;
+    double t0s = dir > 0 ? -1. : 1.;
+    srwf_slice S;
+    std::memset(&S, 0, sizeof(S));
+    S.xStart = xStart; S.xStep = xStep; S.zStart = zStart; S.zStep = zStep;
+    S.qStepX = xStepTr; S.qStepZ = zStepTr;
+    S.midScale = (float)(xStep * zStep);
+    S.postShift = (shX || shZ) ? 1 : 0;
+    S.sh1x = shX ? t0s * x0 : 0.;
+    S.sh1y = shZ ? t0s * z0 : 0.;
+    S.ny = (int)nz;
+    HIPCHK(hipMemcpyAsync(dSp, &S, sizeof(S), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d, pcData, sizeof(float2) * nxy, hipMemcpyHostToDevice, st));
+    const float2* twx = twiddles(lx);
+    const float2* twz = twiddles(lz);
+    if (!twx || !twz) return SRWL_AMD_ERR_GPU;
+    int inv = dir > 0 ? 0 : 1;
+    if (srwf_launch_rows_fwd(d, nxy, 1, dSp, twx, lx, (int)nz, 0, inv, st)) return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_cols(d, nxy, 1, dSp, twz, lz, lx, 2, inv, o, st)) return SRWL_AMD_ERR_GPU;
+    HIPCHK(hipMemcpyAsync(pcData, o, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    arMesh[0] = xStartTr; arMesh[1] = xStepTr; arMesh[3] = zStartTr; arMesh[4] = zStepTr;
+    return 0;
+}
+
+SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, char intType, char depType, double e,
+                                         double x, double y, double* arMeth, void* pFldTrj, double* arParGPU)
+{
+    (void)x; (void)y; (void)arMeth; (void)arParGPU;
+    if (pWfr == nullptr || pInt == nullptr) return SRWL_INCORRECT_PARAM_FOR_INT_EXTR;
+    if (intType != 0 || depType != 3 || pFldTrj != nullptr) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t st = g_ctx.stream;
+    long nx = pWfr->mesh.nx, nz = pWfr->mesh.ny, ne = pWfr->mesh.ne;
+    long long n = (long long)nx * nz * ne;
+    float2* d = (float2*)g_ctx.buf.W0.get(sizeof(float2) * 2 * n);
+    float* o = (float*)g_ctx.buf.W1.get(sizeof(float) * nx * nz);
+    if (!d || !o) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    HIPCHK(hipMemcpyAsync(d, pWfr->arEx, sizeof(float2) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d + n, pWfr->arEy, sizeof(float2) * n, hipMemcpyHostToDevice, st));
+    int ie0 = 0, ie1 = 0;
+    double t = 0;
+    if (ne > 1) {
+        double eStep = (pWfr->mesh.eFin - pWfr->mesh.eStart) / (ne - 1);
+        double r = (e - pWfr->mesh.eStart) / eStep;
+        ie0 = (int)r;
+        if (ie0 < 0) ie0 = 0;
+        if (ie0 > ne - 2) ie0 = (int)ne - 2;
+        ie1 = ie0 + 1;
+        t = r - ie0;
+    }
+    if (srwp_launch_intensity(d, d + n, (int)ne, ie0, ie1, (float)t, (int)pol, (long long)nx * nz, o, st))
+        return SRWL_AMD_ERR_GPU;
+    HIPCHK(hipMemcpyAsync(pInt, o, sizeof(float) * nx * nz, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // extern "C"

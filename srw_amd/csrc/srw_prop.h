@@ -1,0 +1,27 @@
+// srw_prop.h - host helpers of the drift propagation shared by the C ABI files.
+#pragma once
+#include "srw_fft.h"
+
+namespace srwamd {
+
+// Drift buffer variables of the analytic quadratic-phase mode
+// (reference sroptdrf.cpp:889-1081).
+struct DriftGeom {
+    double invRx, invRz, Lx, Lz, invRxL, invRzL, kx, kxc, kz, kzc, mult, phT;
+};
+
+const float2* twiddles(int logn);
+int fft_ready();
+int drift_mode_from_prop(const double* pp, int& mode);
+void mode3_setup(double Rx, double Rz, double dRx, double dRz, double L, DriftGeom& g);
+void drift_slice(srwf_slice& S, int mode, const DriftGeom& g, double L, int treat, double e, long nx, long nz,
+                 double xStart, double xStep, double zStart, double zStep, double xc, double zc);
+int drift_passes(float2* d, long long sliceStride, int nSlices, const srwf_slice* dSp, int lognx, int lognz,
+                 int mode, int postOp, float* inten, long long intenStride, hipStream_t st);
+
+}  // namespace srwamd
+
+extern "C" {
+int srwp_launch_intensity(const float2* ex, const float2* ez, int ne, int ie0, int ie1, float t, int pol,
+                          long long nxy, float* out, hipStream_t st);
+}

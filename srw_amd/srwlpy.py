@@ -33,7 +33,7 @@ def _load():
             getattr(lib, name).argtypes = [P(S.SRWLWfr), P(S.SRWLOptC), C.c_int, P(C.c_char_p), P(S.SRWLRadMesh),
                                            P(C.c_char_p), P(C.c_double)]
     if hasattr(lib, "srwlCalcIntFromElecField"):
-        lib.srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_char, C.c_char, C.c_char,
+        lib.srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_byte, C.c_byte, C.c_byte,
                                                  C.c_double, C.c_double, C.c_double, P(C.c_double),
                                                  C.c_void_p, P(C.c_double)]
     if hasattr(lib, "srwlUtiFFT"):
@@ -104,3 +104,35 @@ def stats():
     out = (C.c_double * 8)()
     lib.srwlAmdGetStats(out, 8)
     return list(out)
+
+
+def PropagElecField(_wfr, _opt_cnt, _int_list=None, _gpu_idx=None):
+    """Propagation through a container of drift spaces (reference srwlpy.cpp:5214-5320)."""
+    if _int_list is not None:
+        raise RuntimeError(err_text(23301))
+    keep = M.Keep()
+    cw = M.wfr(_wfr, keep)
+    co = M.opt_cnt(_opt_cnt, keep)
+    _check(lib.srwlPropagElecField(C.byref(cw), C.byref(co), 0, None, None, None, None))
+    M.wfr_back(cw, _wfr)
+    return _wfr
+
+
+def CalcIntFromElecField(_arI, _wfr, _pol, _int_type, _dep_type, _e, _x, _y, _meth=None, _fldtrj=None,
+                         _gpu_idx=None):
+    """Intensity from the electric field (reference srwlpy.cpp:4873)."""
+    keep = M.Keep()
+    cw = M.wfr(_wfr, keep)
+    _check(lib.srwlCalcIntFromElecField(M._addr(_arI), C.byref(cw), int(_pol), int(_int_type), int(_dep_type),
+                                        float(_e), float(_x), float(_y), None, None, None))
+    return _arI
+
+
+def UtiFFT(_data, _mesh, _dir):
+    """SRW-convention 2D FFT of float complex data; _mesh = [xStart, xStep, nx, yStart, yStep, ny] is updated."""
+    keep = M.Keep()
+    cm = M.dvec(_mesh, keep)
+    _check(lib.srwlUtiFFT(M._addr(_data), b"f", cm, len(_mesh), int(_dir), None))
+    for i in range(len(_mesh)):
+        _mesh[i] = cm[i]
+    return _data

@@ -24,7 +24,7 @@ def load():
                                                C.c_int, C.c_int]
     lib.orc_srwlCalcPartTraj.argtypes = [P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double)]
     lib.orc_srwlPropagElecField.argtypes = [P(S.SRWLWfr), P(S.SRWLOptC)]
-    lib.orc_srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_char, C.c_char, C.c_char, C.c_double]
+    lib.orc_srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_byte, C.c_byte, C.c_byte, C.c_double]
     lib.orc_srwlUtiFFT.argtypes = [C.c_void_p, C.c_char, P(C.c_double), C.c_int, C.c_int]
     lib.orc_field_B.argtypes = [P(S.SRWLMagFldC), C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.orc_trj_interp.argtypes = [P(S.SRWLPrtTrj), P(C.c_double), C.c_longlong, P(C.c_double)]

@@ -1,0 +1,83 @@
+"""GPU parity of the SRW 2D FFT, the drift propagation and intensity extraction."""
+from array import array
+
+import numpy as np
+import pytest
+
+from srw_amd import workloads as W
+from tests import oracle
+from tests.util import cfield, clone_wfr, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+# FP32 FFT of an FP32 field compared with an FP64 oracle: the error is set by
+# FP32 rounding of O(log N) butterfly stages.
+TOL_FFT = 2e-6
+TOL_DRIFT = 5e-6
+
+
+def _rand_data(n, ny, seed):
+    rng = np.random.default_rng(seed)
+    z = (rng.standard_normal((ny, n)) + 1j * rng.standard_normal((ny, n))).astype(np.complex64)
+    return array('f', z.view(np.float32).ravel().tobytes())
+
+
+@pytest.mark.parametrize("n,ny,xs,dirn", [(64, 64, None, 1), (128, 32, None, -1), (256, 256, 3e-4, 1),
+                                         (1024, 16, -2e-4, -1), (2048, 64, None, 1)])
+def test_uti_fft(n, ny, xs, dirn):
+    from srw_amd import srwlpy
+    print("case: UtiFFT", n, ny, xs, dirn, flush=True)
+    d1 = _rand_data(n, ny, n + ny)
+    d2 = array('f', d1)
+    dx, dy = 1e-5, 2e-5
+    x0 = -0.5 * n * dx if xs is None else xs
+    mesh1 = [x0, dx, n, -0.5 * ny * dy, dy, ny]
+    mesh2 = list(mesh1)
+    srwlpy.UtiFFT(d1, mesh1, dirn)
+    assert oracle.uti_fft(d2, mesh2, dirn) == 0
+    a = np.frombuffer(d1, np.float32).view(np.complex64)
+    b = np.frombuffer(d2, np.float32).view(np.complex64)
+    err = rel_l2(a, b)
+    print("rel-L2 %.3e" % err, flush=True)
+    assert err < TOL_FFT
+    assert mesh1 == pytest.approx(mesh2, rel=1e-12)
+
+
+def _sr_wfr(n, ne=1, de=0.0):
+    w, mag, prec = W.single_electron(n=n, ne=ne, de_rel=de)
+    assert oracle.calc_field(w, mag, prec) == 0
+    return w
+
+
+@pytest.mark.parametrize("n,ne,mode,L", [(128, 1, 2, 20.), (256, 1, 0, 10.), (128, 3, 2, 30.), (64, 2, 0, 5.)])
+def test_drift(n, ne, mode, L):
+    from srw_amd import srwlpy
+    print("case: drift n=%d ne=%d mode=%d L=%g" % (n, ne, mode, L), flush=True)
+    w = _sr_wfr(n, ne, 0.03 if ne > 1 else 0.)
+    wg, wo = clone_wfr(w), clone_wfr(w)
+    opt = W.drift_container(L, mode)
+    srwlpy.PropagElecField(wg, opt)
+    assert oracle.propagate(wo, opt) == 0
+    for attr in ("xStart", "xFin", "yStart", "yFin"):
+        assert getattr(wg.mesh, attr) == pytest.approx(getattr(wo.mesh, attr), rel=1e-12)
+    assert wg.Rx == pytest.approx(wo.Rx)
+    ex_e = rel_l2(cfield(wg.arEx, wg), cfield(wo.arEx, wo))
+    ig = np.abs(cfield(wg.arEx, wg)) ** 2 + np.abs(cfield(wg.arEy, wg)) ** 2
+    io = np.abs(cfield(wo.arEx, wo)) ** 2 + np.abs(cfield(wo.arEy, wo)) ** 2
+    i_e = rel_l2(ig, io)
+    print("rel-L2 Ex %.3e I %.3e" % (ex_e, i_e), flush=True)
+    assert ex_e < TOL_DRIFT
+    assert i_e < 2 * TOL_DRIFT
+
+
+@pytest.mark.parametrize("pol", [0, 1, 2, 3, 4, 5, 6, -1, -2, -3, -4])
+def test_intensity(pol):
+    from srw_amd import srwlpy
+    w = _sr_wfr(16, ne=3, de=0.02)
+    e = w.mesh.eStart + 0.37 * (w.mesh.eFin - w.mesh.eStart)
+    a1 = array('f', [0] * (16 * 16))
+    a2 = array('f', [0] * (16 * 16))
+    srwlpy.CalcIntFromElecField(a1, w, pol, 0, 3, e, 0, 0)
+    assert oracle.intensity(a2, w, pol, 0, 3, e) == 0
+    x, y = np.array(a1), np.array(a2)
+    assert rel_l2(x, y) < 1e-6
