@@ -1,0 +1,115 @@
+"""Benchmark of the single-electron emit-and-propagate path on one MI355X.
+
+Workload (one step): the SR electric field of a 3 GeV electron in a planar
+undulator (lambda_u = 20 mm, 100 periods, K = 1.5), 10001-point FP64
+trajectory, observed at 20 m on a 201 x 201 mesh over +/-1 mm at 201 photon
+energies spanning +/-5 % around the first harmonic; results stay in HBM.
+
+Prints one JSON line with the single-electron field throughput (points/s),
+the roofline of the dominant kernel (the SR integral) and a CPU baseline
+(the FP64 oracle on the host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_VALU_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+FLOPS_PER_EVAL = 64                 # algorithmic FP32-equivalent flops per integrand evaluation
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=201, help="mesh points per axis")
+    ap.add_argument("--ne", type=int, default=201, help="photon energies")
+    ap.add_argument("--cpu-energies", type=int, default=3, help="energies in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(n, ne_sample, threads):
+    """FP64 oracle on a bounded sample of the same workload (fewer energies)."""
+    from tests import oracle
+    from srw_amd import workloads as W
+    w, mag, prec = W.single_electron(n=n, ne=ne_sample, de_rel=0.05)
+    t0 = time.perf_counter()
+    rc = oracle.calc_field(w, mag, prec, nthreads=threads)
+    dt = time.perf_counter() - t0
+    if rc > 0:
+        raise RuntimeError("oracle failed: %d" % rc)
+    pts = n * n * ne_sample
+    return {"value": pts / dt, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": "%dx%d mesh, %d photon energies of the same undulator (%.1f s)" % (n, n, ne_sample, dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    from srw_amd import srwlpy
+    from srw_amd import workloads as W
+
+    w, mag, prec = W.single_electron(n=a.n, ne=a.ne, de_rel=0.05)
+    srwlpy.set_keep_on_device(True)
+    srwlpy.set_timing(True)
+    for _ in range(a.warmup):
+        srwlpy.CalcElecFieldSR(w, 0, mag, prec)
+    srwlpy.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        srwlpy.CalcElecFieldSR(w, 0, mag, prec)
+    dt = time.perf_counter() - t0
+    st = srwlpy.stats()
+    pts = a.n * a.n * a.ne
+    value = pts * a.steps / dt
+    k_ms = st["sum_sr_ms"] / max(st["n_sr"], 1)
+    evals = st["sum_sr_evals"] / max(st["n_sr"], 1)
+    achieved = FLOPS_PER_EVAL * evals / (k_ms * 1e-3) / 1e12
+    # PCIe-inclusive rate (results copied to host buffers), reported aside
+    srwlpy.set_keep_on_device(False)
+    t1 = time.perf_counter()
+    srwlpy.CalcElecFieldSR(w, 0, mag, prec)
+    pcie_rate = pts / (time.perf_counter() - t1)
+    cpu = None
+    if not a.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(a.n, a.cpu_energies, threads)
+    out = {
+        "metric": "single-e field pts/s",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * dt / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 trajectory and phase reference)",
+        "data": "synthetic",
+        "config": {"workload": "planar undulator 3 GeV, lambda_u 20 mm x 100, K 1.5; %dx%d mesh at 20 m, %d photon "
+                               "energies +/-5%% around the 1st harmonic; 10001-point trajectory, method 0"
+                               % (a.n, a.n, a.ne),
+                   "mesh": [a.n, a.n], "energies": a.ne, "trajectory_points": 10001, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_sr_integral", "kernel_ms": k_ms, "evals_per_launch": evals,
+                     "flops_per_eval": FLOPS_PER_EVAL},
+        "pcie_inclusive_points_per_s": pcie_rate,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1321,3 +1321,87 @@ ORC_API void orc_struct_sizes(long* out)
     out[6] = sizeof(SRWLRadMesh); out[7] = sizeof(SRWLWfr); out[8] = sizeof(SRWLStokes);
     out[9] = sizeof(SRWLOptD); out[10] = sizeof(SRWLOptC);
 }
+
+/* ------------------------------------------------------------------ */
+/* Partially coherent intensity (srwlib.py:8506-10559 restated):        */
+/* per electron SR field -> drifts -> total intensity -> bilinear onto  */
+/* the result mesh (avg_update_interp, order 1) -> average.             */
+/* Electrons are drawn with the shared counter-based sampler so the      */
+/* draws equal the library's.                                            */
+/* ------------------------------------------------------------------ */
+#include "../include/srw_rng.h"
+
+ORC_API int orc_emit_prop_multi_e(SRWLStokes* st, SRWLWfr* w0, SRWLMagFldC* mag, SRWLOptC* opt, double* prec,
+                                  int nprec, long long nMacro, unsigned long long seed, long long first,
+                                  int nthreads, double* out2)
+{
+    long nx = w0->mesh.nx, nz = w0->mesh.ny;
+    long rnx = st->mesh.nx, rnz = st->mesh.ny;
+    double rxS = st->mesh.xStart, rxD = (st->mesh.xFin - st->mesh.xStart) / (rnx - 1);
+    double rzS = st->mesh.yStart, rzD = (st->mesh.yFin - st->mesh.yStart) / (rnz - 1);
+    /* srwlib.py:9000-9029 */
+    const double* m = w0->partBeam.arStatMom2;
+    double dx = m[0] * m[2] - m[1] * m[1], dy = m[3] * m[5] - m[4] * m[4];
+    double mx = 0.5 / dx, my = 0.5 / dy;
+    double BX = m[0] * mx, GX = m[2] * mx, AX = m[1] * mx;
+    double BY = m[3] * my, GY = m[5] * my, AY = m[4] * my;
+    double SigPX = 1. / sqrt(2. * GX), SigQX = sqrt(GX / (2. * (BX * GX - AX * AX)));
+    double SigPY = 1. / sqrt(2. * GY), SigQY = sqrt(GY / (2. * (BY * GY - AY * AY)));
+    double relE = sqrt(m[10]);
+    double* S = (double*)calloc(rnx * rnz, sizeof(double));
+    double* S2 = (double*)calloc(rnx * rnz, sizeof(double));
+    int err = 0;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+    {
+        double* Sl = (double*)calloc(rnx * rnz, sizeof(double));
+        double* S2l = (double*)calloc(rnx * rnz, sizeof(double));
+        float* ex = (float*)malloc(sizeof(float) * 2 * nx * nz);
+        float* ez = (float*)malloc(sizeof(float) * 2 * nx * nz);
+        float* Iarr = (float*)malloc(sizeof(float) * nx * nz);
+#pragma omp for schedule(dynamic, 1)
+        for (long long i = 0; i < nMacro; i++) {
+            double g[6];
+            srw_normals6(seed, (unsigned long long)(first + i), g);
+            SRWLWfr w = *w0;
+            w.arEx = (char*)ex; w.arEy = (char*)ez;
+            SRWLParticle* p = &w.partBeam.partStatMom1;
+            /* srwlib.py:9793-9803 */
+            double pxp = SigQX * g[0], px = SigPX * g[1] + AX * pxp / GX;
+            double pyp = SigQY * g[2], py = SigPY * g[3] + AY * pyp / GY;
+            p->x = w0->partBeam.partStatMom1.x + px; p->xp = w0->partBeam.partStatMom1.xp + pxp;
+            p->y = w0->partBeam.partStatMom1.y + py; p->yp = w0->partBeam.partStatMom1.yp + pyp;
+            p->gamma = w0->partBeam.partStatMom1.gamma * (1. + relE * g[4]);
+            int rc = orc_srwlCalcElecFieldSR_mt(&w, 0, mag, prec, nprec, 1);
+            if (rc > 0) { err = rc; continue; }
+            if (opt) { rc = orc_srwlPropagElecField(&w, opt); if (rc) { err = rc; continue; } }
+            orc_srwlCalcIntFromElecField((char*)Iarr, &w, 6, 0, 3, w.mesh.eStart);
+            double wxS = w.mesh.xStart, wxD = (w.mesh.xFin - w.mesh.xStart) / (nx - 1);
+            double wzS = w.mesh.yStart, wzD = (w.mesh.yFin - w.mesh.yStart) / (nz - 1);
+            for (long iz = 0; iz < rnz; iz++) {
+                double zr = rzS + iz * rzD;
+                for (long ix = 0; ix < rnx; ix++) {
+                    double xr = rxS + ix * rxD, f = 0.;
+                    int ix0 = (int)trunc((xr - wxS) / wxD + 1.e-09);
+                    int iz0 = (int)trunc((zr - wzS) / wzD + 1.e-09);
+                    if (ix0 >= 0 && ix0 < nx - 1 && iz0 >= 0 && iz0 < nz - 1) {
+                        double tx = (xr - (wxS + wxD * ix0)) / wxD, tz = (zr - (wzS + wzD * iz0)) / wzD;
+                        const float* P = Iarr + iz0 * nx + ix0;
+                        double a00 = P[0], f10 = P[1], f01 = P[nx], f11 = P[nx + 1];
+                        f = a00 + tx * ((f10 - a00) + tz * (a00 - f01 - f10 + f11)) + tz * (f01 - a00);
+                    }
+                    Sl[iz * rnx + ix] += f;
+                    S2l[iz * rnx + ix] += f * f;
+                }
+            }
+        }
+#pragma omp critical
+        for (long k = 0; k < rnx * rnz; k++) { S[k] += Sl[k]; S2[k] += S2l[k]; }
+        free(Sl); free(S2l); free(ex); free(ez); free(Iarr);
+    }
+    float* o = (float*)st->arS0;
+    for (long k = 0; k < rnx * rnz; k++) o[k] = (float)(S[k] / (double)nMacro);
+    if (out2) for (long k = 0; k < rnx * rnz; k++) { out2[2 * k] = S[k]; out2[2 * k + 1] = S2[k]; }
+    free(S); free(S2);
+    return err;
+}

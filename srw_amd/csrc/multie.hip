@@ -54,3 +54,72 @@ int srwp_launch_intensity(const float2* ex, const float2* ez, int ne, int ie0, i
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 }
+
+namespace {
+
+// Bilinear re-gridding of per-electron intensities onto the result mesh and
+// accumulation of the weighted sums S1 = sum w*I and S2 = sum (w*I)^2
+// (srwlib.py avg_update_interp, _ord = 1: points outside the electron's mesh
+// contribute zero).  One thread per result point; loops over the batch.
+__global__ void __launch_bounds__(256) k_accum_interp(const float* __restrict__ I, long long iStride,
+                                                      const srwm_emesh* __restrict__ em, const float* __restrict__ w,
+                                                      int nb, srwm_rmesh rm, double* S1, double* S2, double* norm2)
+{
+    long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long nr = (long long)rm.nx * rm.nz;
+    bool valid = r < nr;
+    int ix = valid ? (int)(r % rm.nx) : 0, iz = valid ? (int)(r / rm.nx) : 0;
+    double xr = rm.xStart + ix * rm.xStep, zr = rm.zStart + iz * rm.zStep;
+    double s1 = 0., s2 = 0.;
+    for (int j = 0; j < nb; j++) {
+        const srwm_emesh E = em[j];
+        float f = 0.f;
+        int ix0 = (int)trunc((xr - E.xStart) / E.xStep + 1.e-09);
+        int iz0 = (int)trunc((zr - E.zStart) / E.zStep + 1.e-09);
+        if (valid && ix0 >= 0 && ix0 < E.nx - 1 && iz0 >= 0 && iz0 < E.nz - 1) {
+            double tx = (xr - (E.xStart + E.xStep * ix0)) / E.xStep;
+            double tz = (zr - (E.zStart + E.zStep * iz0)) / E.zStep;
+            const float* P = I + j * iStride + (long long)iz0 * E.nx + ix0;
+            float a00 = P[0], f10 = P[1], f01 = P[E.nx], f11 = P[E.nx + 1];
+            float a10 = f10 - a00, a01 = f01 - a00, a11 = a00 - f01 - f10 + f11;
+            f = (float)(a00 + tx * (a10 + tz * a11) + tz * a01);
+        }
+        double v = (double)w[j] * (double)f;
+        s1 += v;
+        s2 += v * v;
+        if (norm2) {
+            // per-electron squared L2 norm of the re-gridded intensity (training signal)
+            double q = (double)f * (double)f;
+            for (int off = 32; off > 0; off >>= 1) q += __shfl_down(q, off);
+            if ((threadIdx.x & 63) == 0) atomicAdd(&norm2[j], q);
+        }
+    }
+    if (!valid) return;
+    S1[r] += s1;
+    S2[r] += s2;
+}
+
+__global__ void k_stokes_out(const double* __restrict__ S1, double scale, long long n, float* out)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)(S1[i] * scale);
+}
+
+}  // namespace
+
+extern "C" {
+int srwm_launch_accum(const float* I, long long iStride, const srwm_emesh* em, const float* w, int nb,
+                      srwm_rmesh rm, double* S1, double* S2, double* norm2, hipStream_t st)
+{
+    long long nr = (long long)rm.nx * rm.nz;
+    hipLaunchKernelGGL(k_accum_interp, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, I, iStride, em, w, nb,
+                       rm, S1, S2, norm2);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwm_launch_stokes_out(const double* S1, double scale, long long n, float* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_stokes_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S1, scale, n, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+}

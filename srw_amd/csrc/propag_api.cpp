@@ -238,6 +238,7 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
             (void)hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
             g_ctx.lastFftMs = ms;
             g_ctx.lastFftBytes = 3. * 2. * 8. * (double)n * 2.;
+            g_ctx.sumFftMs += ms; g_ctx.sumFftBytes += g_ctx.lastFftBytes; g_ctx.nFftCalls += 1;
         }
         if (mode == 3) {
             pWfr->mesh.xStart = sp[0].nxStart;

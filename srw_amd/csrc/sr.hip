@@ -18,46 +18,77 @@
 // Field and RK4 (FP64)
 // ------------------------------------------------------------------------
 __device__ __forceinline__ double radical_one_plus_small(double x)
-{   // gmmeth.h:390-394
+{   // gmmeth.h:390-394; the degree-8 polynomial is evaluated in Estrin form
+    // to shorten the dependency chain of the serial RK4 loop
     if (fabs(x) > 0.01) return sqrt(1. + x);
-    return 1. + x * (0.5 + x * (-0.125 + x * (0.0625 + x * (-0.0390625 + x * (0.02734375 +
-           x * (-0.0205078125 + x * 0.01611328125))))));
+    double x2 = x * x, x4 = x2 * x2;
+    double p01 = fma(x, 0.5, 1.), p23 = fma(x, 0.0625, -0.125);
+    double p45 = fma(x, 0.02734375, -0.0390625), p67 = fma(x, 0.01611328125, -0.0205078125);
+    double lo = fma(x2, p23, p01), hi = fma(x2, p67, p45);
+    return fma(x4 * x2, hi, lo);
 }
 
-__device__ void und_addB(const srwd_und& u, double yr, double zr, double* B)
-{   // srmagfld.h:401-502
-    const double pi = 3.1415926535898, twoPi = 6.2831853072, piE2 = pi * pi;
-    const double factK2B = 93.372904175766;
-    double halfTot = 0.5 * u.totLen;
-    double halfTotTerm = halfTot + 4 * u.per;
-    if (zr < -halfTotTerm || zr > halfTotTerm) return;
-    for (int i = 0; i < u.nh; i++) {
-        double ph = u.ph[i];
-        long long n2pi = (long long)(fabs(ph) / twoPi);
-        if (ph > 0) ph -= twoPi * n2pi; else if (ph < 0) ph += twoPi * n2pi;
-        double perH = u.per / u.hn[i];
-        double halfPerH = 0.5 * perH;
-        double zrr = zr + ph * perH / twoPi;
-        if (zrr < -halfTotTerm || zrr > halfTotTerm) continue;
-        double kw = twoPi / perH;
-        double Bm = (u.K[i] / (u.per * factK2B)) * cosh(kw * yr);
-        double halfThis = halfTot + ((u.sym[i] == 1) ? 0.25 * u.per : 0.);
-        double dB;
-        if (zrr < -halfThis || zrr > halfThis) {
-            double perE2 = perH * perH, p = 2. * piE2 / (3. * perE2);
-            double za = (zrr < -halfThis) ? zrr + halfThis : zrr - halfThis;
-            double za2 = za * za;
-            dB = (twoPi * Bm * za / perH) * (1. - 4. * piE2 * za2 / (9. * perE2)) * exp(-p * za2);
-            double zt = (zrr < -halfThis) ? (-halfThis + 0.5 * halfPerH) : (halfThis - 0.5 * halfPerH);
-            double tt = (u.sym[i] == 1) ? cos(kw * zt) : sin(kw * zt);
-            if (zrr < -halfThis) { if (tt < 0) dB = -dB; }
-            else { if (tt > 0) dB = -dB; }
-        } else {
-            double arg = kw * zrr;
-            dB = Bm * ((u.sym[i] == 1) ? cos(arg) : sin(arg));
-        }
-        if (u.isX[i]) B[0] += dB; else B[1] += dB;
+// cos(pi a) and sin(pi a) for the field harmonics: exact reduction to
+// r = a - rint(a), then Taylor series of cos/sin(pi r) (|pi r| <= pi/2,
+// truncation below 1e-17) evaluated in Estrin form.
+__device__ __forceinline__ void sincospi_fast(double a, double& s, double& c)
+{
+    double n = rint(a);
+    double x = (a - n) * 3.141592653589793;
+    double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4;
+    // cos: 1 - x2/2 + x4/24 - ... + x20/20!
+    double c0 = fma(x2, -0.5, 1.);
+    double c1 = fma(x2, -1.3888888888888889e-03, 4.1666666666666664e-02);
+    double c2 = fma(x2, -2.7557319223985893e-07, 2.4801587301587302e-05);
+    double c3 = fma(x2, -1.1470745597729725e-11, 2.0876756987868100e-09);
+    double c4 = fma(x4, 4.1103176233121648e-19, fma(x2, -1.5619206968586225e-16, 4.7794773323873853e-14));
+    double clo = fma(x4, c1, c0), chi = fma(x4, c3, c2);
+    c = fma(x8, fma(x8, c4, chi), clo);
+    // sin: x (1 - x2/6 + ... - x20/21!)
+    double s0 = fma(x2, -1.6666666666666666e-01, 1.);
+    double s1 = fma(x2, -1.9841269841269841e-04, 8.3333333333333333e-03);
+    double s2 = fma(x2, -2.5052108385441720e-08, 2.7557319223985893e-06);
+    double s3 = fma(x2, -7.6471637318198164e-13, 1.6059043836821613e-10);
+    double s4 = fma(x4, 1.9572941063391263e-20, fma(x2, -8.2206352466243295e-18, 2.8114572543455206e-15));
+    double slo = fma(x4, s1, s0), shi = fma(x4, s3, s2);
+    s = x * fma(x8, fma(x8, s4, shi), slo);
+    if (((long long)n) & 1) { s = -s; c = -c; }
+}
+
+__device__ __forceinline__ double cosh_small(double a)
+{
+    if (fabs(a) > 0.1) return cosh(a);
+    double a2 = a * a;
+    return 1. + 0.5 * a2 * (1. + a2 / 12. * (1. + a2 / 30. * (1. + a2 / 56. * (1. + a2 / 90.))));
+}
+
+// srmagfld.h:401-502 with the per-harmonic constants precomputed on the host
+// (srwd_harm); cos/sin of k*z use cospi/sinpi of the argument in half turns.
+__device__ __forceinline__ void harm_addB(const srwd_harm& h, double halfTotTerm, double yr, double zr, double* B)
+{
+    double zrr = zr + h.shift;
+    if (zrr < -halfTotTerm || zrr > halfTotTerm) return;
+    double Bm = h.Bm0;
+    if (yr != 0.) Bm *= cosh_small(h.kw * yr);
+    double dB;
+    if (zrr < -h.halfThis || zrr > h.halfThis) {
+        bool lft = zrr < -h.halfThis;
+        double za = lft ? zrr + h.halfThis : zrr - h.halfThis;
+        double za2 = za * za;
+        dB = (h.kw * Bm * za) * (1. - h.c4 * za2) * exp(-h.p * za2);
+        dB *= lft ? h.sL : h.sR;
+    } else {
+        double sn, cs;
+        sincospi_fast(h.kwpi * zrr, sn, cs);
+        dB = Bm * ((h.sym == 1) ? cs : sn);
     }
+    if (h.isX) B[0] += dB; else B[1] += dB;
+}
+
+__device__ __forceinline__ void und_addB(const srwd_und& u, double yr, double zr, double* B)
+{
+    if (zr < -u.halfTotTerm || zr > u.halfTotTerm) return;
+    for (int i = 0; i < u.nh; i++) harm_addB(u.h[i], u.halfTotTerm, yr, zr, B);
 }
 
 __device__ void field_B(const srwd_field& f, double x, double y, double z, double* B)
@@ -67,14 +98,24 @@ __device__ void field_B(const srwd_field& f, double x, double y, double z, doubl
     for (int i = 0; i < f.n; i++) und_addB(f.u[i], y - f.u[i].cy, z - f.u[i].cz, B);
 }
 
-struct rk_ctx { const srwd_field* f; double gEm2, mult; };
+struct rk_ctx {
+    const srwd_field* f;
+    double gEm2, mult;
+    bool one;              // single undulator with a single harmonic: constants in registers
+    srwd_harm h;
+    double halfTotTerm, cy, cz;
+};
 
 __device__ __forceinline__ void rk_deriv(const rk_ctx& c, const double* F, double* dF)
 {   // srgtrjdt.h:279-293
     double xd = F[1], yd = F[3];
     double zd = radical_one_plus_small(-(c.gEm2 + xd * xd + yd * yd));
     double B[3];
-    field_B(*c.f, F[0], F[2], F[4], B);
+    if (c.one) {
+        B[0] = B[1] = B[2] = 0.;
+        double zr = F[4] - c.cz;
+        if (zr >= -c.halfTotTerm && zr <= c.halfTotTerm) harm_addB(c.h, c.halfTotTerm, F[2] - c.cy, zr, B);
+    } else field_B(*c.f, F[0], F[2], F[4], B);
     dF[0] = xd;
     dF[1] = c.mult * (yd * B[2] - zd * B[1]);
     dF[2] = yd;
@@ -134,6 +175,13 @@ __global__ void __launch_bounds__(64) k_traj_rk4(const srwd_traj_job* __restrict
     c.f = &fields[J.field];
     c.gEm2 = 1. / J.gammaInt / J.gammaInt;
     c.mult = J.mult;
+    c.one = (c.f->n == 1 && c.f->u[0].nh == 1);
+    if (c.one) {
+        c.h = c.f->u[0].h[0];
+        c.halfTotTerm = c.f->u[0].halfTotTerm;
+        c.cy = c.f->u[0].cy;
+        c.cz = c.f->u[0].cz;
+    }
     double* oX = X + tr * ld; double* oXp = Xp + tr * ld; double* oY = Y + tr * ld;
     double* oYp = Yp + tr * ld; double* oZ = Z ? Z + tr * ld : nullptr;
     long long ns = J.np;
@@ -216,47 +264,68 @@ __device__ __forceinline__ double interp_cub_half(const double* f, int i)
 }
 
 // Cumulative Simpson of the squared angle, zeroed at s0 (srtrjdat.cpp:1724-1805).
-// One thread per (trajectory, plane).
-__global__ void __launch_bounds__(64) k_quad_terms(const srwd_trj_info* __restrict__ info, int nTraj,
-                                                   const double* Xp, const double* Yp, long long ld,
-                                                   double* IBx, double* IBz)
+// One 256-thread block per (trajectory, plane): the interval terms are formed
+// in parallel and summed with a block prefix scan.  Interval i (1-based) uses
+// the half-step cubic interpolant centred on nodes (i-1, i); as in the
+// reference, the first interval's left end term is 0.
+#define SRWD_QT_THREADS 256
+__global__ void __launch_bounds__(SRWD_QT_THREADS) k_quad_terms(const srwd_trj_info* __restrict__ info,
+                                                                const double* Xp, const double* Yp, long long ld,
+                                                                double* IBx, double* IBz)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int tr = t >> 1, pl = t & 1;
-    if (tr >= nTraj) return;
+    const int tr = blockIdx.x >> 1, pl = blockIdx.x & 1;
     const srwd_trj_info I = info[tr];
     const double* ang = (pl == 0 ? Xp : Yp) + tr * ld;
     double* out = (pl == 0 ? IBx : IBz) + tr * ld;
-    long long np = I.np, n1 = np - 1, n2 = n1 - 1;
-    double sStep = I.sStep;
-    double stepMult = 0.333333333333 * (0.5 * sStep);
-    out[0] = 0;
-    const double* p = ang;
-    int mid = -1;
-    double f0 = 0., cur = 0.;
-    for (long long i = 1; i <= n1; i++) {
+    const long long np = I.np, n1 = np - 1;
+    const double stepMult = 0.333333333333 * (0.5 * I.sStep);
+    const int T = SRWD_QT_THREADS;
+    long long seg = (n1 + T - 1) / T;
+    long long i0 = 1 + threadIdx.x * seg, i1 = i0 + seg;
+    if (i1 > n1 + 1) i1 = n1 + 1;
+    auto term = [&](long long i) {
+        const double* p;
+        int mid;
+        if (i == 1) { p = ang; mid = -1; }
+        else if (i < n1) { p = ang + (i - 2); mid = 0; }
+        else { p = ang + (n1 - 3); mid = 1; }
         double a = interp_cub_half(p, mid);
-        double f1 = a * a;
-        a = ang[i];
-        double f2 = a * a;
-        cur += stepMult * (f0 + 4. * f1 + f2);
-        out[i] = cur;
-        f0 = f2;
-        if ((i > 1) && (i < n2)) p++;
-        mid = (i < n1) ? 0 : 1;
+        double f0 = (i == 1) ? 0. : ang[i - 1] * ang[i - 1];
+        return stepMult * (f0 + 4. * a * a + ang[i] * ang[i]);
+    };
+    double loc = 0.;
+    for (long long i = i0; i < i1; i++) loc += term(i);
+    __shared__ double sc[SRWD_QT_THREADS];
+    __shared__ double sv;
+    sc[threadIdx.x] = loc;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        double v = threadIdx.x >= off ? sc[threadIdx.x - off] : 0.;
+        __syncthreads();
+        sc[threadIdx.x] += v;
+        __syncthreads();
     }
-    long long idx = (long long)((I.s0 - I.sStart) / sStep);
-    if (idx >= np - 1) idx = np - 2; else if (idx < 0) idx = 0;
-    double rel = I.s0 - (I.sStart + sStep * idx);
-    long long off;
-    if (idx < 2) { off = 0; rel -= (2 - idx) * sStep; }
-    else if (idx < np - 3) off = idx - 2;
-    else if (idx < np - 2) { off = idx - 3; rel += sStep; }
-    else { off = idx - 4; rel += 2 * sStep; }
-    double c[6];
-    coefs5(sStep, out + off, c);
-    double v = c[0] + rel * (c[1] + rel * (c[2] + rel * (c[3] + rel * (c[4] + rel * c[5]))));
-    for (long long j = 0; j <= n1; j++) out[j] -= v;
+    double run = threadIdx.x > 0 ? sc[threadIdx.x - 1] : 0.;
+    if (threadIdx.x == 0) out[0] = 0.;
+    for (long long i = i0; i < i1; i++) { run += term(i); out[i] = run; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sStep = I.sStep;
+        long long idx = (long long)((I.s0 - I.sStart) / sStep);
+        if (idx >= np - 1) idx = np - 2; else if (idx < 0) idx = 0;
+        double rel = I.s0 - (I.sStart + sStep * idx);
+        long long off;
+        if (idx < 2) { off = 0; rel -= (2 - idx) * sStep; }
+        else if (idx < np - 3) off = idx - 2;
+        else if (idx < np - 2) { off = idx - 3; rel += sStep; }
+        else { off = idx - 4; rel += 2 * sStep; }
+        double c[6];
+        coefs5(sStep, out + off, c);
+        sv = c[0] + rel * (c[1] + rel * (c[2] + rel * (c[3] + rel * (c[4] + rel * c[5]))));
+    }
+    __syncthreads();
+    double v = sv;
+    for (long long j = threadIdx.x; j <= n1; j += T) out[j] -= v;
 }
 
 // Evaluates coordinate, angle, integral of squared angle and field of one
@@ -337,15 +406,16 @@ __global__ void __launch_bounds__(256) k_sr_tables(const srwd_trj_info* __restri
     }
     // composite-rule weight in units of h/15 (srradint.cpp:1576-1588)
     double w = (gi == 0 || gi == I.nInt - 1) ? 7. : ((gi & 1) ? 16. : 14.);
+    // w*A_x = w*inv^2*(Px - xo) with the projected position Px = X + Btx*(yObs - s)
     srwd_rec R;
     R.dInv = (float)(inv - invC);
     R.dU = (float)(U - UC);
     R.dV = (float)(V - VC);
-    R.X = (float)Xv;
-    R.Z = (float)Zv;
-    R.a1 = (float)(w * Btx * inv);
-    R.a2 = (float)(w * Btz * inv);
+    R.Px = (float)(Xv + Btx * (I.yObs - s));
+    R.Pz = (float)(Zv + Btz * (I.yObs - s));
     R.b = (float)(w * inv * inv);
+    R.pad0 = 0.f;
+    R.pad1 = 0.f;
     rec[tr * ldI + gi] = R;
     Qout[tr * ldI + gi] = Q;
     if (c0 == gi) {
@@ -394,6 +464,10 @@ __global__ void __launch_bounds__(256) k_phase_table(const double* __restrict__ 
 // integration range.  Per-point data of the trajectory are uniform across the
 // block and are read with scalar loads.
 struct kahan4 { float s[4], c[4]; };
+#define SRWD_BLOCK 32
+#ifndef SRWD_REDUCE_PHASE
+#define SRWD_REDUCE_PHASE 0
+#endif
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
 {
@@ -438,9 +512,9 @@ __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict
         float Tc = (float)(tc - rint(tc));
         long long i0 = c * SRWD_CHUNK, i1 = i0 + SRWD_CHUNK;
         if (i1 > nInt) i1 = nInt;
-        for (long long b0 = i0; b0 < i1; b0 += 16) {
+        for (long long b0 = i0; b0 < i1; b0 += SRWD_BLOCK) {
             float bx = 0.f, bxi = 0.f, bz = 0.f, bzi = 0.f;
-            long long b1 = b0 + 16 < i1 ? b0 + 16 : i1;
+            long long b1 = b0 + SRWD_BLOCK < i1 ? b0 + SRWD_BLOCK : i1;
 #pragma unroll 4
             for (long long i = b0; i < b1; i++) {
                 const srwd_rec r = R[i];
@@ -448,10 +522,11 @@ __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict
                 t = fmaf(alpha, r.dInv, t);
                 t = fmaf(beta, r.dU, t);
                 t = fmaf(gam, r.dV, t);
+#if SRWD_REDUCE_PHASE
                 t = t - rintf(t);
+#endif
                 float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
-                float dx = fxo - r.X, dz = fzo - r.Z;
-                float wax = fmaf(-dx, r.b, r.a1), waz = fmaf(-dz, r.b, r.a2);
+                float wax = (r.Px - fxo) * r.b, waz = (r.Pz - fzo) * r.b;
                 bx = fmaf(wax, cs, bx); bxi = fmaf(wax, sn, bxi);
                 bz = fmaf(waz, cs, bz); bzi = fmaf(waz, sn, bzi);
             }
@@ -594,8 +669,8 @@ int srwd_launch_tables(const srwd_trj_info* dInfo, int nTraj, long long maxInt, 
                        double* dIBx, double* dIBz, srwd_rec* dRec, srwd_chunk* dChunk, double* dQ,
                        long long ldI, long long ldC, srwd_edge* dEdge, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_quad_terms, dim3((2 * nTraj + 63) / 64), dim3(64), 0, st, dInfo, nTraj, dXp, dYp,
-                       ldT, dIBx, dIBz);
+    hipLaunchKernelGGL(k_quad_terms, dim3(2 * nTraj), dim3(SRWD_QT_THREADS), 0, st, dInfo, dXp, dYp, ldT, dIBx,
+                       dIBz);
     hipLaunchKernelGGL(k_sr_tables, dim3((unsigned)((maxInt + 255) / 256), nTraj), dim3(256), 0, st, dInfo,
                        dX, dY, dIBx, dIBz, ldT, dRec, dChunk, dQ, ldI, ldC, maxInt);
     hipLaunchKernelGGL(k_sr_edges, dim3((2 * nTraj + 63) / 64), dim3(64), 0, st, dInfo, nTraj, dX, dY,
@@ -675,47 +750,54 @@ __global__ void __launch_bounds__(256) k_rx_samples(const srwd_trj_info* __restr
 
 // srradstr.cpp:2310-2501 (FindAverageDistanceToSource) with the full
 // trajectory range.  mesh[4*tr] = xStart, xEnd, zStart, zEnd; out[4*tr] =
-// Robs, RobsAbsErr, xc, zc.  One thread per trajectory.
-__global__ void k_find_rx(const srwd_trj_info* __restrict__ info, int nTraj, const double* samp, long long ldS,
-                          const double* mesh, double* out)
+// Robs, RobsAbsErr, xc, zc.  One block per trajectory: each of the four
+// misfit criteria keeps the first sample (scanning from the downstream end,
+// then from the upstream end) whose misfit changes sign relative to the end
+// sample; the per-criterion minimum index is found with LDS atomics.
+__global__ void __launch_bounds__(256) k_find_rx(const srwd_trj_info* __restrict__ info, const double* samp,
+                                                 long long ldS, const double* mesh, double* out)
 {
-    int tr = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tr >= nTraj) return;
+    const int tr = blockIdx.x;
     const srwd_trj_info I = info[tr];
     const double* bx = samp + tr * ldS * 4; const double* bz = bx + ldS;
     const double* xa = bx + 2 * ldS; const double* za = bx + 3 * ldS;
-    double xS = mesh[4 * tr], xE = mesh[4 * tr + 1], zS = mesh[4 * tr + 2], zE = mesh[4 * tr + 3];
-    long long N = I.np;
-    double sStart = I.sStart, sRange = (N - 1) * I.sStep, sEnd = sStart + sRange;
+    const double xS = mesh[4 * tr], xE = mesh[4 * tr + 1], zS = mesh[4 * tr + 2], zE = mesh[4 * tr + 3];
+    const long long N = I.np;
+    const double sStart = I.sStart, sRange = (N - 1) * I.sStep, sEnd = sStart + sRange;
+    const long long L = N - 1;
+    __shared__ unsigned long long first[8];
+    if (threadIdx.x < 8) first[threadIdx.x] = ~0ull;
+    __syncthreads();
+    const double R0b = I.yObs - sEnd, R0f = I.yObs - sStart;
+    double crd[4] = {xS, xE, zS, zE};
+    double ref[8];
+    for (int k = 0; k < 4; k++) {
+        bool zp = k >= 2;
+        ref[k] = (crd[k] - (zp ? za[L] : xa[L])) * (1. / R0b) - (zp ? bz[L] : bx[L]);
+        ref[4 + k] = (crd[k] - (zp ? za[0] : xa[0])) * (1. / R0f) - (zp ? bz[0] : bx[0]);
+    }
+    for (long long is = 1 + threadIdx.x; is < N; is += blockDim.x) {
+        double Rb = R0b + (double)is * I.sStep, Rf = R0f - (double)is * I.sStep;
+        long long jb = L - is, jf = is;
+        for (int k = 0; k < 4; k++) {
+            bool zp = k >= 2;
+            double mb = (crd[k] - (zp ? za[jb] : xa[jb])) * (1. / Rb) - (zp ? bz[jb] : bx[jb]);
+            if (mb * ref[k] < 0.) atomicMin(&first[k], (unsigned long long)is);
+            double mf = (crd[k] - (zp ? za[jf] : xa[jf])) * (1. / Rf) - (zp ? bz[jf] : bx[jf]);
+            if (mf * ref[4 + k] < 0.) atomicMin(&first[4 + k], (unsigned long long)is);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     const double VL = 1.E+23;
-    long long L = N - 1;
-    double R = I.yObs - sEnd, iR = 1. / R;
-    double mxs = (xS - xa[L]) * iR - bx[L], mxf = (xE - xa[L]) * iR - bx[L];
-    double mzs = (zS - za[L]) * iR - bz[L], mzf = (zE - za[L]) * iR - bz[L];
-    double rxs = VL, rxf = VL, rzs = VL, rzf = VL;
-    for (long long is = 1; is < N; is++) {
-        R += I.sStep; iR = 1. / R;
-        long long j = L - is;
-        if (rxs == VL) { double m = (xS - xa[j]) * iR - bx[j]; if (m * mxs < 0.) rxs = R; }
-        if (rxf == VL) { double m = (xE - xa[j]) * iR - bx[j]; if (m * mxf < 0.) rxf = R; }
-        if (rzs == VL) { double m = (zS - za[j]) * iR - bz[j]; if (m * mzs < 0.) rzs = R; }
-        if (rzf == VL) { double m = (zE - za[j]) * iR - bz[j]; if (m * mzf < 0.) rzf = R; }
-        if (rxs != VL && rxf != VL && rzs != VL && rzf != VL) break;
+    double Rv[8];
+    for (int k = 0; k < 4; k++) {
+        Rv[k] = first[k] == ~0ull ? VL : R0b + (double)first[k] * I.sStep;
+        Rv[4 + k] = first[4 + k] == ~0ull ? VL : R0f - (double)first[4 + k] * I.sStep;
     }
-    double minRx = rxs < rxf ? rxs : rxf, minRz = rzs < rzf ? rzs : rzf;
+    double minRx = Rv[0] < Rv[1] ? Rv[0] : Rv[1], minRz = Rv[2] < Rv[3] ? Rv[2] : Rv[3];
     double minR = minRx < minRz ? minRx : minRz;
-    R = I.yObs - sStart; iR = 1. / R;
-    mxs = (xS - xa[0]) * iR - bx[0]; mxf = (xE - xa[0]) * iR - bx[0];
-    mzs = (zS - za[0]) * iR - bz[0]; mzf = (zE - za[0]) * iR - bz[0];
-    rxs = rxf = rzs = rzf = VL;
-    for (long long j = 1; j < N; j++) {
-        R -= I.sStep; iR = 1. / R;
-        if (rxs == VL) { double m = (xS - xa[j]) * iR - bx[j]; if (m * mxs < 0.) rxs = R; }
-        if (rxf == VL) { double m = (xE - xa[j]) * iR - bx[j]; if (m * mxf < 0.) rxf = R; }
-        if (rzs == VL) { double m = (zS - za[j]) * iR - bz[j]; if (m * mzs < 0.) rzs = R; }
-        if (rzf == VL) { double m = (zE - za[j]) * iR - bz[j]; if (m * mzf < 0.) rzf = R; }
-    }
-    double maxRx = rxs < rxf ? rxs : rxf, maxRz = rzs < rzf ? rzs : rzf;
+    double maxRx = Rv[4] < Rv[5] ? Rv[4] : Rv[5], maxRz = Rv[6] < Rv[7] ? Rv[6] : Rv[7];
     double maxR = (minRx < minRz) ? maxRx : maxRz;
     double avg = 0.5 * (minR + maxR);
     if (minR == VL) avg = (maxR == VL) ? VL : maxR;
@@ -746,7 +828,7 @@ int srwd_launch_rx(const srwd_trj_info* dInfo, int nTraj, long long maxNp, const
 {
     hipLaunchKernelGGL(k_rx_samples, dim3((unsigned)((maxNp + 255) / 256), nTraj), dim3(256), 0, st, dInfo, dX,
                        dY, dIBx, dIBz, ldT, dSamp, ldT);
-    hipLaunchKernelGGL(k_find_rx, dim3((nTraj + 63) / 64), dim3(64), 0, st, dInfo, nTraj, dSamp, ldT, dMesh, dRx);
+    hipLaunchKernelGGL(k_find_rx, dim3(nTraj), dim3(256), 0, st, dInfo, dSamp, ldT, dMesh, dRx);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 }

@@ -33,7 +33,12 @@ struct Context {
     long long lastSrEvals = 0;
     int lastSrBlocks = 0, lastSrSplit = 0;
     int forceSplit = 0;
+    bool keepOnDevice = false;    // benchmark switch: leave results in HBM
+    double sumSrMs = 0, sumSrEvals = 0;
+    int nSrCalls = 0;
     double lastFftMs = 0, lastFftBytes = 0;
+    double sumFftMs = 0, sumFftBytes = 0;
+    int nFftCalls = 0;
 };
 
 extern Context g_ctx;

@@ -9,14 +9,24 @@
 #define SRWD_MAX_UND 4
 
 // Periodic undulator in a container (reference srmagfld.h:401-502,
-// srmagfld.cpp:128-174).  Element frames are translations only.
+// srmagfld.cpp:128-174).  Element frames are translations only.  Per-harmonic
+// constants are precomputed on the host from the reference's formulas.
+struct srwd_harm {
+    double kw;        // 2pi/(per/n) with the reference's 2pi = 6.2831853072
+    double kwpi;      // kw/pi, for cospi/sinpi
+    double shift;     // phase offset along z: zrr = zr + shift
+    double Bm0;       // K/(per*93.372904175766)
+    double halfThis;  // half length of the periodic part (+per/4 if symmetric)
+    double p, c4;     // termination exponent and polynomial coefficients
+    double sL, sR;    // termination sign factors
+    int isX, sym;
+};
+
 struct srwd_und {
     int nh;
-    int hn[SRWD_MAX_HARM];
-    int isX[SRWD_MAX_HARM];     // 1: horizontal field ('h'), 0: vertical
-    int sym[SRWD_MAX_HARM];     // 1 symmetric (cos), -1 anti-symmetric (sin)
-    double K[SRWD_MAX_HARM], ph[SRWD_MAX_HARM];
-    double per, totLen, cx, cy, cz;
+    int pad;
+    srwd_harm h[SRWD_MAX_HARM];
+    double halfTotTerm, cx, cy, cz;
 };
 
 struct srwd_field {
@@ -49,9 +59,10 @@ struct srwd_trj_info {
     double yObs;             // longitudinal observation position
 };
 
-// Per integration point, energy independent (FP32), 32 bytes.
+// Per integration point, energy independent (FP32), 32 bytes: phase increments
+// relative to the chunk reference, projected positions and the weighted 1/R^2.
 struct srwd_rec {
-    float dInv, dU, dV, X, Z, a1, a2, b;
+    float dInv, dU, dV, Px, Pz, b, pad0, pad1;
 };
 
 // Per chunk reference (FP64).

@@ -80,20 +80,42 @@ int field_from_c(const SRWLMagFldC* c, srwd_field& f, double& gsStart, double& g
         if (U->nPer <= 0) return SRWL_ERR_INCORRECT_PER_MAG_FIELD_LENGTH;
         if (U->nHarm > SRWD_MAX_HARM || U->nHarm < 0) return SRWL_AMD_ERR_UNSUPPORTED;
         srwd_und& u = f.u[i];
-        u.per = U->per;
-        u.totLen = (2 * U->nPer) * U->per * 0.5;
+        const double pi = 3.1415926535898, twoPi = 6.2831853072, piE2 = pi * pi;
+        const double factK2B = 93.372904175766;
+        double per = U->per;
+        double totLen = (2 * U->nPer) * U->per * 0.5;
+        double halfTot = 0.5 * totLen;
+        u.halfTotTerm = halfTot + 4 * per;
         u.cx = cen ? c->arXc[i] : 0.; u.cy = cen ? c->arYc[i] : 0.; u.cz = cen ? c->arZc[i] : 0.;
         u.nh = U->nHarm;
         for (int k = 0; k < U->nHarm; k++) {
             const SRWLMagFldH& H = U->arHarm[k];
             if (H.n <= 0) return SRWL_AMD_ERR_UNSUPPORTED;
-            u.isX[k] = (H.h_or_v == 'h' || H.h_or_v == 'H') ? 1 : 0;
-            u.K[k] = 93.37290417576577 * U->per * H.B;
-            u.hn[k] = H.n; u.ph[k] = H.ph; u.sym[k] = H.s;
+            srwd_harm& h = u.h[k];
+            h.isX = (H.h_or_v == 'h' || H.h_or_v == 'H') ? 1 : 0;
+            h.sym = H.s;
+            double K = 93.37290417576577 * per * H.B;
+            double ph = H.ph;
+            long long n2pi = (long long)(std::fabs(ph) / twoPi);
+            if (ph > 0) ph -= twoPi * n2pi; else if (ph < 0) ph += twoPi * n2pi;
+            double perH = per / H.n;
+            h.kw = twoPi / perH;
+            h.kwpi = h.kw / 3.141592653589793;
+            h.shift = ph * perH / twoPi;
+            h.Bm0 = K / (per * factK2B);
+            h.halfThis = halfTot + ((H.s == 1) ? 0.25 * per : 0.);
+            double perE2 = perH * perH;
+            h.p = 2. * piE2 / (3. * perE2);
+            h.c4 = 4. * piE2 / (9. * perE2);
+            double halfPerH = 0.5 * perH;
+            double ztL = -h.halfThis + 0.5 * halfPerH, ztR = h.halfThis - 0.5 * halfPerH;
+            double ttL = (H.s == 1) ? std::cos(h.kw * ztL) : std::sin(h.kw * ztL);
+            double ttR = (H.s == 1) ? std::cos(h.kw * ztR) : std::sin(h.kw * ztR);
+            h.sL = (ttL < 0) ? -1. : 1.;
+            h.sR = (ttR > 0) ? -1. : 1.;
         }
-        double half = 0.5 * u.totLen + 4 * u.per;
-        gsStart = std::min(gsStart, u.cz - half);
-        gsEnd = std::max(gsEnd, u.cz + half);
+        gsStart = std::min(gsStart, u.cz - u.halfTotTerm);
+        gsEnd = std::max(gsEnd, u.cz + u.halfTotTerm);
     }
     return 0;
 }
@@ -334,6 +356,9 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         long long ev = 0;
         for (auto& it : items) ev += (long long)it.npts * info[it.traj].nInt;
         g_ctx.lastSrEvals = ev;
+        g_ctx.sumSrMs += ms;
+        g_ctx.sumSrEvals += (double)ev;
+        g_ctx.nSrCalls += 1;
     }
     if (warn)
         for (auto& r : res) if (!r.warn) r.warn = SRWL_WARN_BAD_RAD_INT_TERMINATIONS;
@@ -538,8 +563,10 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     std::vector<SrResult> res;
     rc = sr_run(jobs, fields, dEx, dEz, off, res, termin);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(pWfr->arEx, dEx, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
-    HIPCHK(hipMemcpyAsync(pWfr->arEy, dEz, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
+    if (!g_ctx.keepOnDevice) {
+        HIPCHK(hipMemcpyAsync(pWfr->arEx, dEx, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
+        HIPCHK(hipMemcpyAsync(pWfr->arEy, dEz, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
+    }
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
     pWfr->Rx = pWfr->Ry = res[0].Robs;
     pWfr->dRx = pWfr->dRy = res[0].RobsErr;
@@ -612,9 +639,23 @@ SRWL_EXPORT int srwlAmdSetTiming(int on)
 SRWL_EXPORT int srwlAmdGetStats(double* out, int n)
 {
     if (out == nullptr) return 0;
-    double v[8] = {g_ctx.lastSrMs, (double)g_ctx.lastSrEvals, (double)g_ctx.lastSrBlocks, (double)g_ctx.lastSrSplit,
-                   g_ctx.lastFftMs, g_ctx.lastFftBytes, 0, 0};
-    for (int i = 0; i < n && i < 8; i++) out[i] = v[i];
+    double v[12] = {g_ctx.lastSrMs, (double)g_ctx.lastSrEvals, (double)g_ctx.lastSrBlocks, (double)g_ctx.lastSrSplit,
+                    g_ctx.lastFftMs, g_ctx.lastFftBytes, g_ctx.sumSrMs, g_ctx.sumSrEvals, (double)g_ctx.nSrCalls,
+                    g_ctx.sumFftMs, g_ctx.sumFftBytes, (double)g_ctx.nFftCalls};
+    for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdResetStats()
+{
+    g_ctx.sumSrMs = g_ctx.sumSrEvals = 0; g_ctx.nSrCalls = 0;
+    g_ctx.sumFftMs = g_ctx.sumFftBytes = 0; g_ctx.nFftCalls = 0;
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetKeepOnDevice(int on)
+{
+    g_ctx.keepOnDevice = on != 0;
     return 0;
 }
 

@@ -42,6 +42,9 @@ def _load():
         lib.srwlEmitPropMultiE.argtypes = [P(S.SRWLStokes), P(S.SRWLWfr), P(S.SRWLMagFldC), P(S.SRWLOptC),
                                            P(C.c_double), C.c_int, C.c_longlong, P(C.c_double), P(C.c_double)]
     lib.srwlAmdGetStats.argtypes = [P(C.c_double), C.c_int]
+    lib.srwlAmdSetTiming.argtypes = [C.c_int]
+    lib.srwlAmdSetKeepOnDevice.argtypes = [C.c_int]
+    lib.srwlAmdNormals.argtypes = [C.c_ulonglong, C.c_longlong, P(C.c_double)]
     return lib
 
 
@@ -101,9 +104,23 @@ def CalcElecFieldSRBatch(_arWfr, _arMag, _prec_par):
 
 
 def stats():
-    out = (C.c_double * 8)()
-    lib.srwlAmdGetStats(out, 8)
-    return list(out)
+    out = (C.c_double * 12)()
+    lib.srwlAmdGetStats(out, 12)
+    keys = ("sr_ms", "sr_evals", "sr_blocks", "sr_split", "fft_ms", "fft_bytes", "sum_sr_ms", "sum_sr_evals",
+            "n_sr", "sum_fft_ms", "sum_fft_bytes", "n_fft")
+    return dict(zip(keys, list(out)))
+
+
+def set_timing(on=True):
+    _check(lib.srwlAmdSetTiming(1 if on else 0))
+
+
+def reset_stats():
+    lib.srwlAmdResetStats()
+
+
+def set_keep_on_device(on=True):
+    lib.srwlAmdSetKeepOnDevice(1 if on else 0)
 
 
 def PropagElecField(_wfr, _opt_cnt, _int_list=None, _gpu_idx=None):
@@ -136,3 +153,29 @@ def UtiFFT(_data, _mesh, _dir):
     for i in range(len(_mesh)):
         _mesh[i] = cm[i]
     return _data
+
+
+def EmitPropMultiE(_stokes, _wfr0, _mag, _opt_cnt, _prec_par, _n_macro, _seed=1, _batch=0, _proposal=0,
+                   _target=0., _train=0):
+    """Partially coherent intensity averaged over _n_macro electrons drawn from _wfr0.partBeam
+    (replaces the per-electron loop of srwl_wfr_emit_prop_multi_e, reference srwlib.py:8506).
+    _stokes.arS (float, nx*ny of _stokes.mesh) receives the averaged total intensity.
+    Returns a dict of run information."""
+    keep = M.Keep()
+    cs = M.stokes(_stokes, keep)
+    cw = M.wfr(_wfr0, keep)
+    cm = M.mag_cnt(_mag, keep)
+    co = M.opt_cnt(_opt_cnt, keep) if _opt_cnt is not None else None
+    prec = M.dvec(_prec_par, keep)
+    par = M.dvec([_seed, _batch, _proposal, _target, _train, 0, 0, 0], keep)
+    out = (C.c_double * 16)()
+    _check(lib.srwlEmitPropMultiE(C.byref(cs), C.byref(cw), C.byref(cm), C.byref(co) if co is not None else None,
+                                  prec, len(_prec_par), int(_n_macro), par, out))
+    keys = ("electrons", "ess", "sum_w", "seconds", "rel_l2_est", "batch", "train_loss", "reserved")
+    return dict(zip(keys, list(out)[:8]))
+
+
+def normals(seed, i):
+    g = (C.c_double * 6)()
+    lib.srwlAmdNormals(C.c_ulonglong(seed), C.c_longlong(i), g)
+    return list(g)

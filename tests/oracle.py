@@ -29,6 +29,9 @@ def load():
     lib.orc_field_B.argtypes = [P(S.SRWLMagFldC), C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.orc_trj_interp.argtypes = [P(S.SRWLPrtTrj), P(C.c_double), C.c_longlong, P(C.c_double)]
     lib.orc_struct_sizes.argtypes = [P(C.c_long)]
+    lib.orc_emit_prop_multi_e.argtypes = [P(S.SRWLStokes), P(S.SRWLWfr), P(S.SRWLMagFldC), P(S.SRWLOptC),
+                                          P(C.c_double), C.c_int, C.c_longlong, C.c_ulonglong, C.c_longlong,
+                                          C.c_int, P(C.c_double)]
     return lib
 
 
@@ -87,3 +90,20 @@ def field_B(mag, x, y, z):
     out = (C.c_double * 3)()
     lib.orc_field_B(C.byref(cm), x, y, z, out)
     return list(out)
+
+
+def multi_e(stokes, w0, mag, optc, prec, n_macro, seed=1, first=0, nthreads=8, sums=None):
+    keep = M.Keep()
+    cs = M.stokes(stokes, keep)
+    cw = M.wfr(w0, keep)
+    cm = M.mag_cnt(mag, keep)
+    co = M.opt_cnt(optc, keep) if optc is not None else None
+    cp = M.dvec(prec, keep)
+    out2 = None
+    if sums is not None:
+        out2 = (C.c_double * (2 * stokes.mesh.nx * stokes.mesh.ny))()
+    rc = lib.orc_emit_prop_multi_e(C.byref(cs), C.byref(cw), C.byref(cm), C.byref(co) if co is not None else None,
+                                   cp, len(prec), n_macro, seed, first, nthreads, out2)
+    if sums is not None:
+        sums.extend(list(out2))
+    return rc
