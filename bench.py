@@ -32,12 +32,44 @@ def parse():
     ap.add_argument("--ne", type=int, default=201, help="photon energies")
     ap.add_argument("--cpu-energies", type=int, default=3, help="energies in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--multi-e", type=int, default=1, help="also measure the multi-electron time to 1%% rel-L2")
+    ap.add_argument("--me-n", type=int, default=512, help="multi-electron emission mesh")
+    ap.add_argument("--me-max", type=int, default=4096, help="electron cap per multi-electron run")
     return ap.parse_args()
+
+
+def multi_e_runs(srwlpy, W, n, nmax):
+    """Time to 1 % rel-L2 of the partially coherent intensity (Gaussian 6-D beam,
+    sigma_x 30 um, sigma_x' 5 urad, sigma_y 5 um, sigma_y' 2 urad, sigma_E/E 1e-3;
+    n x n mesh at 20 m, 30 m drift), plain Gaussian MC vs the learned proposal.
+    The rel-L2 error is the estimator's own standard error (per-pixel sample
+    variance of w*I), the stop criterion of the run."""
+    from srw_amd.srwlib import SRWLStokes
+    beam = W.beam_for_multi_e()
+    e1 = W.first_harmonic_eV(beam.partStatMom1.gamma, 1.5, 0.02)
+    w0 = W.wavefront(beam, 1, e1, e1, n, n, 1e-3, 1e-3, 20.)
+    mag, prec, opt = W.planar_undulator(), W.traj_prec(10001), W.drift_container(30., 2)
+    res = {}
+    for name, prop in (("plain_mc", 0), ("learned", 1)):
+        st = SRWLStokes()
+        st.allocate(1, n, n, _n_comp=1)
+        st.mesh.eStart = st.mesh.eFin = e1
+        st.mesh.xStart, st.mesh.xFin = -2.5e-3, 2.5e-3
+        st.mesh.yStart, st.mesh.yFin = -2.5e-3, 2.5e-3
+        info = srwlpy.EmitPropMultiE(st, w0, mag, opt, prec, nmax, _seed=20251202, _batch=256, _proposal=prop,
+                                     _target=0.01, _train=4)
+        res[name] = {"seconds": info["seconds"], "electrons": int(info["electrons"]),
+                     "rel_l2_est": info["rel_l2_est"], "ess": info["ess"]}
+    p, q = res["plain_mc"], res["learned"]
+    res["speedup_learned_vs_plain"] = p["seconds"] / q["seconds"] if q["seconds"] > 0 else None
+    res["workload"] = ("Gaussian beam 4096-electron cap, %dx%d mesh at 20 m, 30 m drift (analytic quadratic phase), "
+                       "intensity re-gridded to %dx%d over +/-2.5 mm" % (n, n, n, n))
+    return res
 
 
 def cpu_baseline(n, ne_sample, threads):
     """FP64 oracle on a bounded sample of the same workload (fewer energies)."""
-    from tests import oracle
+    from tests import oracle_lib as oracle
     from srw_amd import workloads as W
     w, mag, prec = W.single_electron(n=n, ne=ne_sample, de_rel=0.05)
     t0 = time.perf_counter()
@@ -80,6 +112,7 @@ def main():
     t1 = time.perf_counter()
     srwlpy.CalcElecFieldSR(w, 0, mag, prec)
     pcie_rate = pts / (time.perf_counter() - t1)
+    me = multi_e_runs(srwlpy, W, a.me_n, a.me_max) if a.multi_e else None
     cpu = None
     if not a.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -106,6 +139,7 @@ def main():
                      "kernel": "k_sr_integral", "kernel_ms": k_ms, "evals_per_launch": evals,
                      "flops_per_eval": FLOPS_PER_EVAL},
         "pcie_inclusive_points_per_s": pcie_rate,
+        "multi_electron": me,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1,0 +1,56 @@
+/*
+ * srwl_amd_ext.h - instrumentation and test hooks of libsrw_amd.so.
+ *
+ * None of these has a counterpart in the reference API.  They let a caller
+ * time the device kernels, keep results resident in device memory between
+ * calls, and check individual device components (the electron sampler and
+ * the proposal network) against a CPU implementation.
+ */
+#ifndef SRW_AMD_EXT_H
+#define SRW_AMD_EXT_H
+
+#include "srwlib.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Enables (on != 0) HIP-event timing of the SR-integral and FFT kernels on
+ * the library's stream.  Returns 0 or an SRWL error code. */
+SRWL_EXPORT int srwlAmdSetTiming(int on);
+
+/* Copies up to n (<= 12) statistics into out: [0] last SR-kernel ms,
+ * [1] last integrand evaluations, [2] last SR work blocks, [3] last s-range
+ * split, [4] last FFT-pass ms, [5] last FFT-pass bytes, [6] sum SR ms,
+ * [7] sum SR evaluations, [8] SR calls, [9] sum FFT ms, [10] sum FFT bytes,
+ * [11] FFT calls (sums since srwlAmdResetStats). */
+SRWL_EXPORT int srwlAmdGetStats(double* out, int n);
+SRWL_EXPORT int srwlAmdResetStats(void);
+
+/* on != 0: srwlCalcElecFieldSR leaves the field in device memory and skips
+ * the copy into pWfr->arEx/arEy (throughput measurement of the device path). */
+SRWL_EXPORT int srwlAmdSetKeepOnDevice(int on);
+
+/* Forces the number of longitudinal partitions of the SR integral
+ * (0: automatic). */
+SRWL_EXPORT int srwlAmdSetSplit(int nSplit);
+
+/* Six standard normals of electron i of the counter-based sampler used by
+ * srwlEmitPropMultiE (Philox4x32-10 + Box-Muller, see srw_rng.h). */
+SRWL_EXPORT int srwlAmdNormals(unsigned long long seed, long long i, double* g6);
+
+/* Initial parameters (11368 floats) of the proposal network for a seed;
+ * returns the parameter count. */
+SRWL_EXPORT int srwlAmdProposalInit(float* P, unsigned long long seed);
+
+/* Device forward pass, mixture-loss gradient and backward pass of the
+ * proposal network: P (11368 parameters), in6 (beam descriptor), g (nb x 6
+ * standardized samples), wh (nb weights), alpha (defensive weight).
+ * Outputs G (11368 gradient values), out104 (network output), loss. */
+SRWL_EXPORT int srwlAmdProposalGrad(const float* P, const float* in6, const float* g, const float* wh,
+                                    int nb, float alpha, float* G, float* out104, float* loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

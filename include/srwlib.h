@@ -232,7 +232,8 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
                                          double* precPar, int nPrecPar);
 
 /* Partially coherent intensity: nMacroElec electrons drawn from the Gaussian
- * beam pBeam (moments as in srwlib.py:9000-9029, mapping srwlib.py:9793-9803),
+ * beam pWfr0->partBeam (moments as in srwlib.py:9000-9029, mapping
+ * srwlib.py:9793-9803),
  * each emitting on pWfr0's mesh, propagated through pOpt (drifts) and
  * bilinearly re-gridded onto pStokes->mesh (srwlib.py avg_update_interp),
  * averaged into pStokes->arS0 (float32, nx*ny).
@@ -240,7 +241,8 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
  *   mode (0 plain Gaussian MC, 1 learned mixture proposal), [3] target
  *   rel-L2 for early stop (0: none), [4] training steps per batch.
  * arOut (optional, >= 8 doubles): [0] electrons used, [1] effective sample
- *   size, [2] sum of weights, [3] device seconds, ... */
+ *   size, [2] sum of weights, [3] device seconds, [4] rel-L2 error estimate,
+ *   [5] batch size used, [6] last training loss, [7] training steps taken. */
 SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagFldC* pMagFld,
                                    SRWLOptC* pOpt, double* precPar, int nPrecPar,
                                    long long nMacroElec, double* arPar, double* arOut);

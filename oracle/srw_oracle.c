@@ -1405,3 +1405,11 @@ ORC_API int orc_emit_prop_multi_e(SRWLStokes* st, SRWLWfr* w0, SRWLMagFldC* mag,
     free(S); free(S2);
     return err;
 }
+
+/* Test hook: one Philox4x32-10 block (known-answer tests of the sampler). */
+ORC_API void orc_philox(const unsigned* ctr, unsigned k0, unsigned k1, unsigned* out)
+{
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]}, o[4];
+    srw_philox4x32_10(c, k0, k1, o);
+    for (int i = 0; i < 4; i++) out[i] = o[i];
+}

@@ -12,6 +12,7 @@
 #include "srw_api_internal.h"
 #include "srw_multie.h"
 #include "srw_prop.h"
+#include "srw_learn.h"
 #include "../../include/srw_rng.h"
 
 #include <chrono>
@@ -57,6 +58,96 @@ struct BeamSampler {
     }
 };
 
+// Learned proposal state: network parameters and Adam moments on the device,
+// the current mixture on the host (for sampling and p/q weights).
+struct Proposal {
+    float* P = nullptr; float* m = nullptr; float* v = nullptr; float* G = nullptr;
+    float* act = nullptr; float* gout = nullptr; float* loss = nullptr; float* in = nullptr;
+    float* gs = nullptr; float* wh = nullptr;
+    int step = 0;
+    double alpha = 0.25;
+    double logit[SRWL_K], mu[SRWL_K][SRWL_D], ls[SRWL_K][SRWL_D];
+};
+
+// Initial parameters: scaled normal hidden weights, zero output weights and a
+// fixed spread of component means (so the components are distinguishable).
+void proposal_init_params(std::vector<float>& P, unsigned long long seed)
+{
+    P.assign(SRWL_NPARAM, 0.f);
+    auto fill = [&](int off, int n, double sc, unsigned long long sub) {
+        for (int i = 0; i < n; i += 6) {
+            double g[6];
+            srw_normals6(seed ^ (0x5151ull * sub), (unsigned long long)i, g);
+            for (int k = 0; k < 6 && i + k < n; k++) P[off + i + k] = (float)(sc * g[k]);
+        }
+    };
+    fill(SRWL_OFF_W1, SRWL_H * SRWL_IN, 1. / std::sqrt((double)SRWL_IN), 1);
+    fill(SRWL_OFF_W2, SRWL_H * SRWL_H, 1. / std::sqrt((double)SRWL_H), 2);
+    fill(SRWL_OFF_W3, SRWL_OUT * SRWL_H, 0.01 / std::sqrt((double)SRWL_H), 3);
+    fill(SRWL_OFF_B3 + SRWL_K, SRWL_K * SRWL_D, 0.6, 4);   // component means
+}
+
+void proposal_set(Proposal& Q, const float* o)
+{
+    for (int k = 0; k < SRWL_K; k++) {
+        Q.logit[k] = o[k];
+        for (int d = 0; d < SRWL_D; d++) {
+            Q.mu[k][d] = o[SRWL_K + k * SRWL_D + d];
+            double v = o[SRWL_K + SRWL_K * SRWL_D + k * SRWL_D + d];
+            Q.ls[k][d] = std::min(std::max(v, -3.), 1.);
+        }
+    }
+}
+
+// log p(g) and log q(g) of the defensive mixture (same formulas as k_gmm_grad)
+void proposal_logpq(const Proposal& Q, const double* g, double& lp, double& lq)
+{
+    const double L2P = 1.8378770664093453;
+    lp = -0.5 * SRWL_D * L2P;
+    for (int d = 0; d < SRWL_D; d++) lp -= 0.5 * g[d] * g[d];
+    double lmax = -1e300;
+    for (int k = 0; k < SRWL_K; k++) lmax = std::max(lmax, Q.logit[k]);
+    double se = 0;
+    for (int k = 0; k < SRWL_K; k++) se += std::exp(Q.logit[k] - lmax);
+    double logZ = lmax + std::log(se);
+    double lN[SRWL_K], mm = -1e300;
+    for (int k = 0; k < SRWL_K; k++) {
+        double s = Q.logit[k] - logZ;
+        for (int d = 0; d < SRWL_D; d++) {
+            double z = (g[d] - Q.mu[k][d]) * std::exp(-Q.ls[k][d]);
+            s += -0.5 * z * z - Q.ls[k][d] - 0.5 * L2P;
+        }
+        lN[k] = s;
+        mm = std::max(mm, s);
+    }
+    double ms = 0;
+    for (int k = 0; k < SRWL_K; k++) ms += std::exp(lN[k] - mm);
+    double lm = mm + std::log(ms);
+    double top = std::max(lm, lp);
+    lq = top + std::log(Q.alpha * std::exp(lp - top) + (1. - Q.alpha) * std::exp(lm - top));
+}
+
+// Draw of electron i from q: the base normals of plain sampling, a mixture
+// choice from a third Philox block.
+void proposal_draw(const Proposal& Q, unsigned long long seed, unsigned long long i, double* g)
+{
+    double z[6];
+    srw_normals6(seed, i, z);
+    uint32_t ctr[4] = {(uint32_t)i, 2u, (uint32_t)(i >> 32), 0u}, u[4];
+    srw_philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32), u);
+    double a = srw_u01(u[0]);
+    if (a < Q.alpha) { for (int d = 0; d < 6; d++) g[d] = z[d]; return; }
+    double b = srw_u01(u[1]);
+    double lmax = -1e300;
+    for (int k = 0; k < SRWL_K; k++) lmax = std::max(lmax, Q.logit[k]);
+    double pk[SRWL_K], se = 0;
+    for (int k = 0; k < SRWL_K; k++) { pk[k] = std::exp(Q.logit[k] - lmax); se += pk[k]; }
+    int kk = SRWL_K - 1;
+    double cum = 0;
+    for (int k = 0; k < SRWL_K; k++) { cum += pk[k] / se; if (b < cum) { kk = k; break; } }
+    for (int d = 0; d < SRWL_D; d++) g[d] = Q.mu[kk][d] + std::exp(Q.ls[kk][d]) * z[d];
+}
+
 }  // namespace srwamd
 
 using namespace srwamd;
@@ -64,6 +155,43 @@ using namespace srwamd;
 #define HIPCHK(x) do { if ((x) != hipSuccess) return SRWL_AMD_ERR_GPU; } while (0)
 
 extern "C" {
+
+// Forward + mixture gradient + backward of the proposal network for given
+// parameters, descriptor, samples g (nb x 6) and weights (test hook used to
+// check the hand-written kernels against a CPU implementation).
+SRWL_EXPORT int srwlAmdProposalGrad(const float* P, const float* in6, const float* g, const float* wh, int nb,
+                                    float alpha, float* G, float* out104, float* loss)
+{
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t st = g_ctx.stream;
+    size_t n = 2 * SRWL_NPARAM + SRWL_NACT + SRWL_OUT + 2 + 8 + 7 * (size_t)nb;
+    float* b = (float*)g_ctx.buf.Grad.get(sizeof(float) * n);
+    if (!b) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    float *dP = b, *dG = dP + SRWL_NPARAM, *dAct = dG + SRWL_NPARAM, *dGo = dAct + SRWL_NACT, *dL = dGo + SRWL_OUT;
+    float *dIn = dL + 2, *dg = dIn + 8, *dw = dg + 6 * nb;
+    HIPCHK(hipMemcpyAsync(dP, P, sizeof(float) * SRWL_NPARAM, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dIn, in6, sizeof(float) * 6, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dg, g, sizeof(float) * 6 * nb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dw, wh, sizeof(float) * nb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(dGo, 0, sizeof(float) * (SRWL_OUT + 2), st));
+    if (srwl_launch_mlp_forward(dP, dIn, dAct, st)) return SRWL_AMD_ERR_GPU;
+    if (srwl_launch_gmm_grad(dAct, dg, dw, nb, alpha, dGo, dL, st)) return SRWL_AMD_ERR_GPU;
+    if (srwl_launch_mlp_backward(dP, dAct, dGo, dG, st)) return SRWL_AMD_ERR_GPU;
+    HIPCHK(hipMemcpyAsync(G, dG, sizeof(float) * SRWL_NPARAM, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(out104, dAct + SRWL_ACT_OUT, sizeof(float) * SRWL_OUT, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(loss, dL, sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdProposalInit(float* P, unsigned long long seed)
+{
+    std::vector<float> v;
+    proposal_init_params(v, seed);
+    std::memcpy(P, v.data(), sizeof(float) * SRWL_NPARAM);
+    return SRWL_NPARAM;
+}
 
 // Draws the standard-normal vector of electron i (test hook of the sampler).
 SRWL_EXPORT int srwlAmdNormals(unsigned long long seed, long long i, double* g6)
@@ -118,6 +246,30 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     double* dN2 = (double*)Bf.Grad.get(sizeof(double) * B);
     if (!dF || !dI || !dS || !dEm || !dW || !dSp || !dN2) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     HIPCHK(hipMemsetAsync(dS, 0, sizeof(double) * 2 * nr, st));
+    int useQ = (arPar && arPar[2] > 0) ? 1 : 0;
+    int trainSteps = (arPar && arPar[4] > 0) ? (int)arPar[4] : 4;
+    Proposal Q;
+    std::vector<float> hOut(SRWL_OUT);
+    double lastLoss = 0;
+    if (useQ) {
+        float* base = (float*)Bf.Net.get(sizeof(float) * (4 * SRWL_NPARAM + SRWL_NACT + SRWL_OUT + 8 + 7 * B + 64));
+        if (!base) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        Q.P = base; Q.m = Q.P + SRWL_NPARAM; Q.v = Q.m + SRWL_NPARAM; Q.G = Q.v + SRWL_NPARAM;
+        Q.act = Q.G + SRWL_NPARAM; Q.gout = Q.act + SRWL_NACT; Q.loss = Q.gout + SRWL_OUT; Q.in = Q.loss + 2;
+        Q.gs = Q.in + 8; Q.wh = Q.gs + 6 * B;
+        std::vector<float> P0;
+        proposal_init_params(P0, 0x6c65617264ull);
+        HIPCHK(hipMemcpyAsync(Q.P, P0.data(), sizeof(float) * SRWL_NPARAM, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(Q.m, 0, sizeof(float) * 2 * SRWL_NPARAM, st));
+        // beam / beamline descriptor: log-scaled rms sizes, divergences, energy spread, total drift
+        const double* m2 = pWfr0->partBeam.arStatMom2;
+        double Ltot = 0;
+        for (int d = 0; d < nDrift; d++) Ltot += ((const SRWLOptD*)pOpt->arOpt[d])->L;
+        float in[8] = {(float)std::log(std::sqrt(m2[0]) / 1e-5 + 1e-30), (float)std::log(std::sqrt(m2[2]) / 1e-6 + 1e-30),
+                       (float)std::log(std::sqrt(m2[3]) / 1e-5 + 1e-30), (float)std::log(std::sqrt(m2[5]) / 1e-6 + 1e-30),
+                       (float)std::log(std::sqrt(m2[10]) / 1e-3 + 1e-30), (float)(Ltot / 10.), 0.f, 0.f};
+        HIPCHK(hipMemcpyAsync(Q.in, in, sizeof(float) * 8, hipMemcpyHostToDevice, st));
+    }
     std::vector<srwd_field> fields(1);
     double gs0, gs1;
     rc = field_from_c(pMagFld, fields[0], gs0, gs1);
@@ -135,9 +287,22 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         std::vector<SrJob> jobs(nb);
         std::vector<long long> off(nb);
         std::vector<float> w(nb, 1.f);
+        std::vector<float> gsamp(6 * nb);
+        if (useQ) {
+            if (srwl_launch_mlp_forward(Q.P, Q.in, Q.act, st)) return SRWL_AMD_ERR_GPU;
+            HIPCHK(hipMemcpyAsync(hOut.data(), Q.act + SRWL_ACT_OUT, sizeof(float) * SRWL_OUT, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            proposal_set(Q, hOut.data());
+        }
         for (long long j = 0; j < nb; j++) {
             double g[6];
-            srw_normals6(seed, (unsigned long long)(done + j), g);
+            if (useQ) {
+                proposal_draw(Q, seed, (unsigned long long)(done + j), g);
+                double lp, lq;
+                proposal_logpq(Q, g, lp, lq);
+                w[j] = (float)std::exp(lp - lq);
+                for (int d = 0; d < 6; d++) gsamp[6 * j + d] = (float)g[d];
+            } else srw_normals6(seed, (unsigned long long)(done + j), g);
             SRWLParticle p = pWfr0->partBeam.partStatMom1;
             bs.map(g, p);
             std::vector<srwd_field> dummy;
@@ -206,7 +371,34 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         }
         HIPCHK(hipMemcpyAsync(dEm, em.data(), sizeof(srwm_emesh) * nb, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dW, w.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st));
-        if (srwm_launch_accum(dI, nxy, dEm, dW, (int)nb, rm, dS, dS + nr, nullptr, st)) return SRWL_AMD_ERR_GPU;
+        if (useQ) HIPCHK(hipMemsetAsync(dN2, 0, sizeof(double) * nb, st));
+        if (srwm_launch_accum(dI, nxy, dEm, dW, (int)nb, rm, dS, dS + nr, useQ ? dN2 : nullptr, st))
+            return SRWL_AMD_ERR_GPU;
+        if (useQ) {
+            // online training on this batch: target q* ~ p * ||I||
+            std::vector<double> n2(nb);
+            HIPCHK(hipMemcpyAsync(n2.data(), dN2, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            std::vector<float> wh(nb);
+            double tot = 0;
+            for (long long j = 0; j < nb; j++) { double c = w[j] * std::sqrt(std::max(n2[j], 0.)); wh[j] = (float)c; tot += c; }
+            for (long long j = 0; j < nb; j++) wh[j] = tot > 0 ? (float)(wh[j] / tot) : 0.f;
+            HIPCHK(hipMemcpyAsync(Q.gs, gsamp.data(), sizeof(float) * 6 * nb, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(Q.wh, wh.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st));
+            for (int e = 0; e < trainSteps; e++) {
+                if (srwl_launch_mlp_forward(Q.P, Q.in, Q.act, st)) return SRWL_AMD_ERR_GPU;
+                HIPCHK(hipMemsetAsync(Q.gout, 0, sizeof(float) * (SRWL_OUT + 2), st));
+                if (srwl_launch_gmm_grad(Q.act, Q.gs, Q.wh, (int)nb, (float)Q.alpha, Q.gout, Q.loss, st))
+                    return SRWL_AMD_ERR_GPU;
+                if (srwl_launch_mlp_backward(Q.P, Q.act, Q.gout, Q.G, st)) return SRWL_AMD_ERR_GPU;
+                Q.step++;
+                if (srwl_launch_adam(Q.P, Q.G, Q.m, Q.v, SRWL_NPARAM, 3e-3f, Q.step, st)) return SRWL_AMD_ERR_GPU;
+            }
+            float lossv = 0;
+            HIPCHK(hipMemcpyAsync(&lossv, Q.loss, sizeof(float), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            lastLoss = lossv;
+        }
         done += nb;
         if (target > 0 || done == nMacroElec) {
             S1h.resize(nr); S2h.resize(nr);
@@ -232,8 +424,8 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         arOut[3] = secs;
         arOut[4] = errEst;
         arOut[5] = (double)B;
-        arOut[6] = 0;
-        arOut[7] = 0;
+        arOut[6] = lastLoss;
+        arOut[7] = useQ ? Q.step : 0;
     }
     return 0;
 }

@@ -1,6 +1,7 @@
 // srw_api_internal.h - host-side state shared by the C ABI implementation files.
 #pragma once
 #include "../../include/srwlib.h"
+#include "../../include/srwl_amd_ext.h"
 #include "srw_dev.h"
 #include <vector>
 

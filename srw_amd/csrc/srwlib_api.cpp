@@ -646,7 +646,7 @@ SRWL_EXPORT int srwlAmdGetStats(double* out, int n)
     return 0;
 }
 
-SRWL_EXPORT int srwlAmdResetStats()
+SRWL_EXPORT int srwlAmdResetStats(void)
 {
     g_ctx.sumSrMs = g_ctx.sumSrEvals = 0; g_ctx.nSrCalls = 0;
     g_ctx.sumFftMs = g_ctx.sumFftBytes = 0; g_ctx.nFftCalls = 0;

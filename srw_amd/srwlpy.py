@@ -179,3 +179,27 @@ def normals(seed, i):
     g = (C.c_double * 6)()
     lib.srwlAmdNormals(C.c_ulonglong(seed), C.c_longlong(i), g)
     return list(g)
+
+
+def proposal_grad(P, c6, g, wh, alpha=0.25):
+    """Runs the proposal kernels (forward, mixture gradient, backward) once; returns (G, out, loss)."""
+    import numpy as np
+    P = np.ascontiguousarray(P, np.float32)
+    c6 = np.ascontiguousarray(c6, np.float32)
+    g = np.ascontiguousarray(g, np.float32)
+    wh = np.ascontiguousarray(wh, np.float32)
+    G = np.zeros_like(P)
+    out = np.zeros(104, np.float32)
+    loss = np.zeros(1, np.float32)
+    fp = C.POINTER(C.c_float)
+    _check(lib.srwlAmdProposalGrad(P.ctypes.data_as(fp), c6.ctypes.data_as(fp), g.ctypes.data_as(fp),
+                                   wh.ctypes.data_as(fp), len(wh), C.c_float(alpha), G.ctypes.data_as(fp),
+                                   out.ctypes.data_as(fp), loss.ctypes.data_as(fp)))
+    return G, out, float(loss[0])
+
+
+def proposal_init(seed=0x6c65617264):
+    import numpy as np
+    P = np.zeros(11368, np.float32)
+    lib.srwlAmdProposalInit(P.ctypes.data_as(C.POINTER(C.c_float)), C.c_ulonglong(seed))
+    return P

@@ -4,7 +4,7 @@ import pytest
 
 from srw_amd import workloads as W
 from srw_amd.srwlib import SRWLStokes
-from tests import oracle
+from tests import oracle_lib as oracle
 from tests.util import rel_l2
 
 pytestmark = pytest.mark.gpu

@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 from srw_amd import workloads as W
-from tests import oracle
+from tests import oracle_lib as oracle
 from tests.util import cfield, clone_wfr, rel_l2
 
 pytestmark = pytest.mark.gpu
