@@ -35,6 +35,11 @@ SRWL_EXPORT int srwlAmdSetKeepOnDevice(int on);
  * (0: automatic). */
 SRWL_EXPORT int srwlAmdSetSplit(int nSplit);
 
+/* on != 0: integrate trajectories with the serial RK4 kernel (two threads per
+ * trajectory) instead of the parallel-in-s solver; both give the serial RK4
+ * recurrence, the switch exists to compare them. */
+SRWL_EXPORT int srwlAmdSetTrajSerial(int on);
+
 /* Six standard normals of electron i of the counter-based sampler used by
  * srwlEmitPropMultiE (Philox4x32-10 + Box-Muller, see srw_rng.h). */
 SRWL_EXPORT int srwlAmdNormals(unsigned long long seed, long long i, double* g6);

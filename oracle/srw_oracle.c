@@ -27,6 +27,9 @@
 
 #define ORC_API __attribute__((visibility("default")))
 
+ORC_API void orc_compute_rad_moments(SRWLWfr* w, double* momX, double* momZ);
+ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e);
+
 /* ------------------------------------------------------------------ */
 /* Constants (reference appendix of parity-critical values)            */
 /* ------------------------------------------------------------------ */
@@ -972,6 +975,8 @@ ORC_API int orc_srwlCalcElecFieldSR_mt(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
                     }
                 }
         }
+        /* srradint.cpp:3570-3574: moments (in place phase treatment included), mean energy */
+        orc_compute_rad_moments(pWfr, pWfr->arMomX, pWfr->arMomY);
         pWfr->avgPhotEn = 0.5 * (m->eStart + m->eFin);
         pWfr->presCA = 0; pWfr->presFT = 0;
         if (!S.slower) free(S.Btx);
@@ -1172,8 +1177,41 @@ static void orc_drift_slice(double complex* ex, double complex* ez, long nx, lon
     }
 }
 
-/* sroptdrf.cpp:889-1081 (UseExactRxRz off) with sroptdrf.cpp:1085-1116 (subtype 2) */
-static void orc_mode3_bufvars(const SRWLWfr* w, double L, double* bv)
+typedef struct {
+    long nx, nz, ne;
+    double xStart, xStep, zStart, zStep, eStart, eStep;
+} orc_mesh3;
+
+static orc_mesh3 orc_mesh_of(const SRWLWfr* w)
+{
+    orc_mesh3 m;
+    m.nx = w->mesh.nx; m.nz = w->mesh.ny; m.ne = w->mesh.ne;
+    m.xStart = w->mesh.xStart; m.zStart = w->mesh.yStart; m.eStart = w->mesh.eStart;
+    m.xStep = m.nx > 1 ? (w->mesh.xFin - w->mesh.xStart) / (m.nx - 1) : 0.;
+    m.zStep = m.nz > 1 ? (w->mesh.yFin - w->mesh.yStart) / (m.nz - 1) : 0.;
+    m.eStep = m.ne > 1 ? (w->mesh.eFin - w->mesh.eStart) / (m.ne - 1) : 0.;
+    return m;
+}
+
+/* srstraux.h:348-372 (ComputeCentralMoments) for one 11-value record;
+ * c = {Mxx, Mxpxp, Mzz, Mzpzp, SqrtMxpxp, SqrtMzpzp}; returns precCenMomIsOK. */
+static int orc_central_moments(const double* mom, double* c)
+{
+    double Mxx = mom[5] - mom[1] * mom[1], Mxpxp = mom[7] - mom[2] * mom[2];
+    double Mzz = mom[8] - mom[3] * mom[3], Mzpzp = mom[10] - mom[4] * mom[4];
+    int ok = !(Mxx < 0 || Mxpxp < 0 || Mzz < 0 || Mzpzp < 0);
+    if (Mxx < 0.1 * mom[5]) Mxx = 0.1 * mom[5];
+    if (Mxpxp < 0.1 * mom[7]) Mxpxp = 0.1 * mom[7];
+    if (Mzz < 0.1 * mom[8]) Mzz = 0.1 * mom[8];
+    if (Mzpzp < 0.1 * mom[10]) Mzpzp = 0.1 * mom[10];
+    c[0] = Mxx; c[1] = Mxpxp; c[2] = Mzz; c[3] = Mzpzp;
+    c[4] = sqrt(fabs(Mxpxp)); c[5] = sqrt(fabs(Mzpzp));
+    return ok;
+}
+
+/* sroptdrf.cpp:1085-1116 (EstimateWfrRadToSub2): radius kept away from 0 and -L
+ * by three times its estimated error */
+static void orc_rad_sub2(const SRWLWfr* w, double L, double* pRx, double* pRz)
 {
     double Rx = w->Rx, Rz = w->Ry;
     double mx = 3. * w->dRx, mz = 3. * w->dRy;
@@ -1183,6 +1221,71 @@ static void orc_mode3_bufvars(const SRWLWfr* w, double L, double* bv)
     double RzL = Rz + L;
     if (fabs(Rz) < mz) Rz = mz * (Rz >= 0 ? 1. : -1.);
     else if (fabs(RzL) < mz) Rz = mz * (RzL >= 0 ? 1. : -1.) - L;
+    *pRx = Rx; *pRz = Rz;
+}
+
+/* One axis of sroptdrf.cpp:1226-1368: effective radius from the distance to
+ * the source s1, the drift L and the Rayleigh-like length smin. */
+static double orc_rad_sub1_axis(double s1, double L, double smin)
+{
+    double Rmin = 2. * smin, s2 = s1 + L, eff;
+    if (fabs(s1) < smin) {
+        if (fabs(s2) < smin) {
+            if (fabs(s1) < fabs(s2)) eff = (L > 0) ? Rmin : -Rmin;
+            else eff = (L > 0) ? -Rmin - L : Rmin - L;
+        } else eff = (L > 0) ? Rmin : -Rmin;
+    } else {
+        eff = s1 + smin * smin / s1;
+        double effL = eff + L;
+        double minR = 0.007 * fabs(eff);
+        if (Rmin < minR) Rmin = minR;
+        if (fabs(effL) < Rmin) eff = (effL < 0) ? -Rmin - L : Rmin - L;
+    }
+    return eff;
+}
+
+/* sroptdrf.cpp:1129-1370 (EstimateWfrRadToSub): radius from the angular
+ * spread of the wavefront moments at the central photon energy.  momX/momZ
+ * are the wavefront's moment arrays (computed first when empty or unusable). */
+static void orc_rad_sub1(SRWLWfr* w, double L, double* momX, double* momZ, double* pRx, double* pRz)
+{
+    const double infLarge = 1E+23, coefAngRange = 0.2, fourPi = 4. * 3.14159265359;
+    orc_mesh3 m = orc_mesh_of(w);
+    double photEn0 = m.eStart;
+    long ieCen = 0;
+    if (m.ne > 1 && m.eStep > 0) {
+        photEn0 = w->avgPhotEn;
+        double d = (photEn0 - m.eStart) / m.eStep;
+        ieCen = (long)d;
+        if ((d - ieCen) > 0.5) ieCen++;
+        if (ieCen < 0) ieCen = 0; else if (ieCen >= m.ne) ieCen = m.ne - 1;
+    }
+    double* mx = momX + 11 * ieCen;
+    double* mz = momZ + 11 * ieCen;
+    if (mx[0] == 0 && mz[0] == 0) orc_compute_rad_moments(w, momX, momZ);
+    double* use = (mx[0] >= mz[0]) ? mx : mz;
+    double c[6];
+    int ok = orc_central_moments(use, c);
+    if (!ok || c[4] == 0 || c[5] == 0) {
+        orc_compute_rad_moments(w, momX, momZ);
+        orc_central_moments(use, c);
+    }
+    double sigXp = c[4], sigZp = c[5];
+    if (sigXp == 0 || sigZp == 0) {
+        *pRx = (w->Rx == 0) ? infLarge : w->Rx;
+        *pRz = (w->Ry == 0) ? infLarge : w->Ry;
+        return;
+    }
+    double lam = 1.239842E-06 / photEn0;
+    double ax = coefAngRange * sigXp, az = coefAngRange * sigZp;
+    *pRx = orc_rad_sub1_axis(w->Rx, L, lam / (fourPi * ax * ax));
+    *pRz = orc_rad_sub1_axis(w->Ry, L, lam / (fourPi * az * az));
+}
+
+/* sroptdrf.cpp:889-1081 with UseExactRxRz off: buffer variables of the
+ * analytic quadratic-phase drift for the radii Rx, Rz */
+static void orc_mode3_bufvars(const SRWLWfr* w, double L, double Rx, double Rz, double* bv)
+{
     const double inf = 1E+23;
     double invRx = 0, invRz = 0, Lx = 0, Lz = 0, invRxL = 1. / L, invRzL = 1. / L;
     double kx = inf, kxc = inf, kz = inf, kzc = inf;
@@ -1201,6 +1304,44 @@ static void orc_mode3_bufvars(const SRWLWfr* w, double L, double* bv)
     bv[10] = kx; bv[11] = kxc; bv[12] = kz; bv[13] = kzc;
 }
 
+/* sroptdrf.cpp:21-110 (AuxPropagateRadMoments) for a drift of length L, with
+ * the diffraction-limited spot floor of sroptdrf.h:1018-1027 on the (new) mesh */
+static void orc_drift_moments(const SRWLWfr* w, double L, double* momArr)
+{
+    orc_mesh3 m = orc_mesh_of(w);
+    for (long ie = 0; ie < m.ne; ie++) {
+        double* M = momArr + 11 * ie;
+        if (M[0] == 0.) continue;
+        double X = M[1] + L * M[2], Z = M[3] + L * M[4];
+        double XX = M[5] + 2. * L * M[6] + L * L * M[7], XXP = M[6] + L * M[7];
+        double ZZ = M[8] + 2. * L * M[9] + L * L * M[10], ZZP = M[9] + L * M[10];
+        M[1] = X; M[3] = Z; M[5] = XX; M[6] = XXP; M[8] = ZZ; M[9] = ZZP;
+        double lam = 1.239842e-06 / (m.eStart + ie * m.eStep);
+        double mSx = lam * L / ((m.nx - 1) * m.xStep), mSz = lam * L / ((m.nz - 1) * m.zStep);
+        if (M[5] - M[1] * M[1] < mSx * mSx) M[5] = mSx * mSx + M[1] * M[1];
+        if (M[8] - M[3] * M[3] < mSz * mSz) M[8] = mSz * mSz + M[3] * M[3];
+    }
+}
+
+/* sroptdrf.h:1002-1012 with sroptelm.cpp:5594-5640: M <- D*M, V <- D*V for the
+ * 4x4 drift matrix D; arElecPropMatr holds M (16, row-major) then V (4). */
+static void orc_drift_4x4(double* P, double L)
+{
+    double D[16] = {1., L, 0., 0., 0., 1., 0., 0., 0., 0., 1., L, 0., 0., 0., 1.};
+    double R[16], V[4];
+    for (int i = 0; i < 4; i++) {
+        V[i] = 0.;
+        for (int j = 0; j < 4; j++) {
+            double r = 0.;
+            for (int k = 0; k < 4; k++) r += D[4 * i + k] * P[4 * k + j];
+            R[4 * i + j] = r;
+            V[i] += D[4 * i + j] * P[16 + j];
+        }
+    }
+    for (int i = 0; i < 16; i++) P[i] = R[i];
+    for (int i = 0; i < 4; i++) P[16 + i] = V[i];
+}
+
 /* srwlib.cpp:1023-1074 + sroptcnt.cpp:87-425 for drift-only containers */
 ORC_API int orc_srwlPropagElecField(SRWLWfr* w, SRWLOptC* opt)
 {
@@ -1217,7 +1358,7 @@ ORC_API int orc_srwlPropagElecField(SRWLWfr* w, SRWLOptC* opt)
                 return SRWL_AMD_ERR_UNSUPPORTED;
             int at = (int)(char)pp[3];
             if (at == 0) mode = 0;
-            else if (at == 2) { mode = 3; sub = 2; }
+            else if (at == 1 || at == 2) { mode = 3; sub = at; }
             else return SRWL_AMD_ERR_UNSUPPORTED;
         }
         long nx = w->mesh.nx, nz = w->mesh.ny, ne = w->mesh.ne;
@@ -1226,8 +1367,21 @@ ORC_API int orc_srwlPropagElecField(SRWLWfr* w, SRWLOptC* opt)
                           w->mesh.yStart, (w->mesh.yFin - w->mesh.yStart) / (nz - 1)};
         double eStep = ne > 1 ? (w->mesh.eFin - w->mesh.eStart) / (ne - 1) : 0;
         double bv[14] = {0};
-        if (mode == 3) orc_mode3_bufvars(w, d->L, bv);
-        (void)sub;
+        long nMom = 11 * (ne > 0 ? ne : 1);
+        double* tmpMom = 0;
+        double *momX = w->arMomX, *momZ = w->arMomY;
+        if (!momX || !momZ) {   /* moments are needed by subtype 1 and updated below */
+            tmpMom = (double*)calloc(2 * nMom, sizeof(double));
+            if (!tmpMom) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+            if (!momX) momX = tmpMom;
+            if (!momZ) momZ = tmpMom + nMom;
+        }
+        if (mode == 3) {
+            double Rx, Rz;
+            if (sub == 1) orc_rad_sub1(w, d->L, momX, momZ, &Rx, &Rz);
+            else orc_rad_sub2(w, d->L, &Rx, &Rz);
+            orc_mode3_bufvars(w, d->L, Rx, Rz, bv);
+        }
         double complex* ex = (double complex*)malloc(sizeof(double complex) * nx * nz);
         double complex* ez = (double complex*)malloc(sizeof(double complex) * nx * nz);
         float* fx = (float*)w->arEx;
@@ -1252,12 +1406,16 @@ ORC_API int orc_srwlPropagElecField(SRWLWfr* w, SRWLOptC* opt)
         if (mode == 3) {
             double nS = bv[10] * mesh[0] - bv[11] * w->xc, nD = mesh[1] * bv[10];
             double zS2 = bv[12] * mesh[2] - bv[13] * w->yc, zD = mesh[3] * bv[12];
-            if (bv[10] < 0 || bv[12] < 0) return SRWL_AMD_ERR_UNSUPPORTED;
+            if (bv[10] < 0 || bv[12] < 0) { free(tmpMom); return SRWL_AMD_ERR_UNSUPPORTED; }
             w->mesh.xStart = nS; w->mesh.xFin = nS + (nx - 1) * nD;
             w->mesh.yStart = zS2; w->mesh.yFin = zS2 + (nz - 1) * zD;
         }
-        /* sroptdrf.h:967-979 */
+        /* sroptdrf.h:181-194: moments, radius and 4x4 matrix after the field */
+        orc_drift_moments(w, d->L, momX);
+        orc_drift_moments(w, d->L, momZ);
+        free(tmpMom);
         w->Rx += d->L; w->Ry += d->L;
+        if (w->arElecPropMatr) orc_drift_4x4(w->arElecPropMatr, d->L);
     }
     return 0;
 }
@@ -1413,3 +1571,249 @@ ORC_API void orc_philox(const unsigned* ctr, unsigned k0, unsigned k1, unsigned*
     srw_philox4x32_10(c, k0, k1, o);
     for (int i = 0; i < 4; i++) out[i] = o[i];
 }
+
+/* ------------------------------------------------------------------ */
+/* Wavefront statistical moments (sroptelm.cpp:1608-2090) with the     */
+/* quadratic-phase treatment (sroptelm.cpp:4692-4990) and the 90 %     */
+/* power limits (srmatsta.cpp:350-630)                                  */
+/* ------------------------------------------------------------------ */
+
+/* float sin/cos of the optics elements: sroptelm.h:856-872 with the
+ * constants of sroptelm.h:105-115 */
+static void orc_cos_sin_f(double x, float* c, float* s)
+{
+    const double HalfPI = 1.5707963267949, PI = 3.141592653590, TwoPI = 6.2831853071796;
+    const double ThreePIdTwo = 4.7123889803847, One_dTwoPI = 0.1591549430919;
+    if (x < -1.E+08 || x > 1.E+08) { *c = (float)cos(x); *s = (float)sin(x); return; }
+    x -= TwoPI * (double)(long long)(x * One_dTwoPI);
+    if (x < 0.) x += TwoPI;
+    int neg = 0;
+    if (x > ThreePIdTwo) x -= TwoPI;
+    else if (x > HalfPI) { x -= PI; neg = 1; }
+    double q = x * x;
+    float cc = (float)(1. + q * (-0.5 + q * (0.041666666666667 + q * (-0.0013888888888889 +
+                       q * (0.000024801587301587 + q * -2.755731922E-07)))));
+    float ss = (float)(x * (1. + q * (-0.16666666666667 + q * (0.0083333333333333 + q * (-0.0001984126984127 +
+                       q * (2.755731922E-06 + q * -2.505210839E-08))))));
+    *c = neg ? -cc : cc;
+    *s = neg ? -ss : ss;
+}
+
+/* Multiplies both field components by exp(+-i pi/lambda ((x-xc)^2/Rx + (z-zc)^2/Rz)) in
+ * place ('a' adds the term, 'r' removes it), for the axes whose flag is set;
+ * float storage as in sroptelm.cpp:4774-4990 (coordinate, frequency representation). */
+static void orc_treat_quad_phase(SRWLWfr* w, int add, int fx, int fz)
+{
+    orc_mesh3 m = orc_mesh_of(w);
+    const double Const = 3.14159265358979 * 1.E+06 / 1.23984186;
+    double cRx = Const / w->Rx, cRz = Const / w->Ry;
+    if (!add) { cRx = -cRx; cRz = -cRz; }
+    float* pX = (float*)w->arEx;
+    float* pZ = (float*)w->arEy;
+    long long perX = m.ne * 2, perZ = perX * m.nx;
+    for (long ie = 0; ie < m.ne; ie++) {
+        double ePh = m.eStart + m.eStep * ie;
+        double cRxE = cRx * ePh, cRzE = cRz * ePh;
+        double z = m.zStart - w->yc;
+        double phZ = fz ? cRzE * z * z : 0.;
+        for (long iz = 0; iz < m.nz; iz++) {
+            double x = m.xStart - w->xc;
+            for (long ix = 0; ix < m.nx; ix++) {
+                double ph = phZ;
+                if (fx) ph += cRxE * x * x;
+                float c, s;
+                orc_cos_sin_f(ph, &c, &s);
+                long long o = iz * perZ + ix * perX + ie * 2;
+                if (pX) {
+                    double re = pX[o], im = pX[o + 1];
+                    pX[o] = (float)(re * c - im * s); pX[o + 1] = (float)(re * s + im * c);
+                }
+                if (pZ) {
+                    double re = pZ[o], im = pZ[o + 1];
+                    pZ[o] = (float)(re * c - im * s); pZ[o + 1] = (float)(re * s + im * c);
+                }
+                x += m.xStep;
+            }
+            z += m.zStep;
+            phZ = fz ? cRzE * z * z : 0.;
+        }
+    }
+}
+
+/* srmatsta.h:437-445 */
+static double orc_sum_f(const float* p, long long i0, long long i1, long long step)
+{
+    if (i0 >= i1 || i1 <= 0) return 0.;
+    if (i0 < 0) i0 = 0;
+    double s = 0.;
+    for (long long i = i0; i <= i1; i += step) s += p[i];
+    return s;
+}
+
+static double orc_sum_d(const double* p, long long i0, long long i1)
+{
+    if (i0 >= i1 || i1 <= 0) return 0.;
+    double s = 0.;
+    for (long long i = i0; i <= i1; i++) s += p[i];
+    return s;
+}
+
+/* srmatsta.cpp:400-450 */
+static long long orc_limit_left(const double* p, long long n, double stop)
+{
+    double s = 0.;
+    for (long long i = 0; i < n; i++) { s += p[i]; if (s > stop) return i; }
+    return 0;
+}
+
+static long long orc_limit_right(const double* p, long long n, double stop)
+{
+    double s = 0.;
+    for (long long i = 0; i < n; i++) { s += p[n - 1 - i]; if (s > stop) return (n - 1) - i; }
+    return n - 1;
+}
+
+/* Index limits of the rectangle holding RelPow = 0.9 of the slice's total
+ * intensity: srmatsta.cpp:572-630 with FindIntensityLimits2D (:452-545).
+ * On any degenerate case the full mesh is kept, as there. */
+static void orc_intensity_limits(const SRWLWfr* w, const orc_mesh3* m, long ie, int* L)
+{
+    L[0] = 0; L[1] = (int)m->nx - 1; L[2] = 0; L[3] = (int)m->nz - 1;
+    long nx = m->nx, nz = m->nz;
+    float* Iv = (float*)malloc(sizeof(float) * nx * nz);
+    double* ax = (double*)malloc(sizeof(double) * (nz > nx ? nz : nx));
+    if (!Iv || !ax) { free(Iv); free(ax); return; }
+    /* single-electron intensity, total polarisation, at this slice's energy (srradmnp.h:367-415) */
+    orc_srwlCalcIntFromElecField((char*)Iv, (SRWLWfr*)w, 6, 0, 3, m->eStart + ie * m->eStep);
+    float tot = (float)(orc_sum_f(Iv, 0, nx * nz - 1, 1) * m->xStep * m->zStep);
+    double stopAbs = (double)tot * (1. - 0.9) * 0.25;
+    float lim[5];
+    if (nx - 1 <= 0) goto out;
+    for (long iz = 0; iz < nz; iz++) ax[iz] = m->xStep * orc_sum_f(Iv + iz * nx, 0, nx - 1, 1);
+    {
+        double stop = stopAbs / m->zStep;
+        long long iz0 = orc_limit_left(ax, nz, stop), iz1 = orc_limit_right(ax, nz, stop);
+        if (iz1 <= iz0) goto out;
+        lim[3] = (float)(m->zStart + m->zStep * iz0);
+        lim[4] = (float)(m->zStart + m->zStep * iz1);
+        for (long ix = 0; ix < nx; ix++) ax[ix] = m->zStep * orc_sum_f(Iv + ix, iz0 * nx, iz1 * nx, nx);
+        stop = stopAbs / m->xStep;
+        long long ix0 = orc_limit_left(ax, nx, stop), ix1 = orc_limit_right(ax, nx, stop);
+        lim[1] = (float)(m->xStart + m->xStep * ix0);
+        lim[2] = (float)(m->xStart + m->xStep * ix1);
+    }
+    L[0] = (int)((lim[1] - m->xStart) * 1.0000001 / m->xStep); if (L[0] < 0) L[0] = 0;
+    L[1] = (int)((lim[2] - m->xStart) * 1.0000001 / m->xStep); if (L[1] >= nx) L[1] = (int)nx - 1;
+    L[2] = (int)((lim[3] - m->zStart) * 1.0000001 / m->zStep); if (L[2] < 0) L[2] = 0;
+    L[3] = (int)((lim[4] - m->zStart) * 1.0000001 / m->zStep); if (L[3] >= nz) L[3] = (int)nz - 1;
+out:
+    free(Iv); free(ax);
+    (void)orc_sum_d;
+}
+
+/* The 22 sums of one slice (sroptelm.cpp:1688-1941): [0..10] from Ex, [11..21] from Ez. */
+static void orc_moment_sums(const SRWLWfr* w, const orc_mesh3* m, long ie, const int* L, double* SumsZ)
+{
+    const double TwoPi = 3.141592653590 * 2., FourPi = TwoPi * 2.;
+    const float* pX0 = (const float*)w->arEx;
+    const float* pZ0 = (const float*)w->arEy;
+    long long perX = m->ne * 2, perZ = perX * m->nx;
+    double ePh = m->eStart + m->eStep * ie;
+    double lam4 = 1.239842E-06 / (FourPi * ePh), lam_m = lam4 * FourPi;
+    double rx = w->Rx != 0. ? w->Rx : 100. * lam_m, rz = w->Ry != 0. ? w->Ry : 100. * lam_m;
+    double kx = 0.5 * m->xStep * (1. / lam4) / rx, kz = 0.5 * m->zStep * (1. / lam4) / rz;
+    double kx2 = kx * kx, kz2 = kz * kz;
+    for (int k = 0; k < 22; k++) SumsZ[k] = 0.;
+    for (long iz = 0; iz < m->nz; iz++) {
+        int inZ = iz >= L[2] && iz <= L[3];
+        double SumsX[22], ff[22];
+        for (int k = 0; k < 22; k++) SumsX[k] = 0.;
+        double z = m->zStart + iz * m->zStep;
+        for (long ix = 0; ix < m->nx; ix++) {
+            int in = inZ && ix >= L[0] && ix <= L[1];
+            long long o = iz * perZ + ix * perX + ie * 2;
+            double a[4] = {0, 0, 0, 0};   /* ExRe, ExIm, EzRe, EzIm */
+            if (pX0) { a[0] = pX0[o]; a[1] = pX0[o + 1]; }
+            if (pZ0) { a[2] = pZ0[o]; a[3] = pZ0[o + 1]; }
+            double x = m->xStart + ix * m->xStep;
+            for (int c = 0; c < 2; c++) {
+                double* f = ff + 11 * c;
+                double re = a[2 * c], im = a[2 * c + 1];
+                f[0] = re * re + im * im;
+                f[1] = x * f[0];
+                f[3] = z * f[0];
+                f[5] = in ? x * f[1] : 0.;
+                f[8] = in ? z * f[3] : 0.;
+                const float* base = c == 0 ? pX0 : pZ0;
+                if (ix > 0) {
+                    double reM = base ? base[o - perX] : 0., imM = base ? base[o - perX + 1] : 0.;
+                    double dRe = re - reM, dIm = im - imM, cr = dIm * re - dRe * im;
+                    f[2] = cr + kx * x * f[0];
+                    f[6] = in ? x * f[2] : 0.;
+                    f[7] = in ? (dRe * dRe + dIm * dIm) + cr * kx * x + kx2 * x * x * f[0] : 0.;
+                } else f[2] = f[6] = f[7] = 0.;
+                if (iz > 0) {
+                    double reM = base ? base[o - perZ] : 0., imM = base ? base[o - perZ + 1] : 0.;
+                    double dRe = re - reM, dIm = im - imM, cr = dIm * re - dRe * im;
+                    f[4] = cr + kz * z * f[0];
+                    f[9] = in ? z * f[4] : 0.;
+                    f[10] = in ? (dRe * dRe + dIm * dIm) + cr * kz * z + kz2 * z * z * f[0] : 0.;
+                } else f[4] = f[9] = f[10] = 0.;
+            }
+            if (ix == 0 || ix == m->nx - 1) for (int k = 0; k < 22; k++) ff[k] *= 0.5;
+            if (ix == 1) { ff[2] *= 0.5; ff[6] *= 0.5; ff[7] *= 0.5; ff[13] *= 0.5; ff[17] *= 0.5; ff[18] *= 0.5; }
+            for (int k = 0; k < 22; k++) SumsX[k] += ff[k];
+        }
+        if (iz == 0 || iz == m->nz - 1) for (int k = 0; k < 22; k++) SumsX[k] *= 0.5;
+        if (iz == 1) { SumsX[4] *= 0.5; SumsX[9] *= 0.5; SumsX[10] *= 0.5; SumsX[15] *= 0.5; SumsX[20] *= 0.5; SumsX[21] *= 0.5; }
+        for (int k = 0; k < 22; k++) SumsZ[k] += SumsX[k];
+    }
+}
+
+/* Moments of one component from its 11 sums (sroptelm.cpp:1948-2078, coordinate
+ * representation); values are stored through float as there. */
+static void orc_moments_from_sums(const double* S, const orc_mesh3* m, double ePh, double* mom)
+{
+    const double TwoPi = 3.141592653590 * 2., FourPi = TwoPi * 2.;
+    double lam2 = 2. * (1.239842E-06 / (FourPi * ePh)), lam2e2 = lam2 * lam2;
+    int scX = m->nx > 1, scZ = m->nz > 1;
+    if (S[0] == 0.) { for (int k = 0; k < 11; k++) mom[k] = 0.; return; }
+    double inv = 1. / S[0];
+    double iX = 1. / m->xStep, iZ = 1. / m->zStep;
+    mom[0] = (float)((scX && scZ) ? S[0] * m->xStep * m->zStep * 1.E+06 : 0.);
+    mom[1] = (float)(scX ? S[1] * inv : m->xStart);
+    mom[2] = (float)(scX ? S[2] * (inv * lam2 * iX) : 0.);
+    mom[3] = (float)(scZ ? S[3] * inv : m->zStart);
+    mom[4] = (float)(scZ ? S[4] * (inv * lam2 * iZ) : 0.);
+    mom[5] = (float)(scX ? S[5] * inv : m->xStart * m->xStart);
+    mom[6] = (float)(scX ? S[6] * (inv * lam2 * iX) : 0.);
+    mom[7] = (float)(scX ? S[7] * (inv * lam2e2 * (iX * iX)) : 0.);
+    mom[8] = (float)(scZ ? S[8] * inv : m->zStart * m->zStart);
+    mom[9] = (float)(scZ ? S[9] * (inv * lam2 * iZ) : 0.);
+    mom[10] = (float)(scZ ? S[10] * (inv * lam2e2 * (iZ * iZ)) : 0.);
+}
+
+/* srTGenOptElem::ComputeRadMoments (sroptelm.cpp:1608-2090) for a wavefront
+ * in coordinate / frequency representation; arMomX/arMomY hold 11 values per
+ * photon energy.  The quadratic phase is removed before and added after, in
+ * place, when the radius is known well enough (WaveFrontTermCanBeTreated,
+ * sroptelm.cpp:4692-4770, without the benefit check). */
+ORC_API void orc_compute_rad_moments(SRWLWfr* w, double* momX, double* momZ)
+{
+    orc_mesh3 m = orc_mesh_of(w);
+    if (m.nx <= 1 && m.nz <= 1) return;
+    int fx = fabs(w->dRx) < 0.4 * fabs(w->Rx), fz = fabs(w->dRy) < 0.4 * fabs(w->Ry);
+    int treated = fx || fz;
+    if (treated) orc_treat_quad_phase(w, 0, fx, fz);
+    for (long ie = 0; ie < m.ne; ie++) {
+        int L[4];
+        double S[22];
+        orc_intensity_limits(w, &m, ie, L);
+        orc_moment_sums(w, &m, ie, L, S);
+        double ePh = m.eStart + m.eStep * ie;
+        if (w->arEx && momX) orc_moments_from_sums(S, &m, ePh, momX + 11 * ie);
+        if (w->arEy && momZ) orc_moments_from_sums(S + 11, &m, ePh, momZ + 11 * ie);
+    }
+    if (treated) orc_treat_quad_phase(w, 1, fx, fz);
+}
+

@@ -136,6 +136,8 @@ def wfr(w, keep):
     c.partBeam = part_beam(w.partBeam)
     c.arMomX = _dptr(getattr(w, "arMomX", None))
     c.arMomY = _dptr(getattr(w, "arMomY", None))
+    c.arElecPropMatr = _dptr(getattr(w, "arElecPropMatr", None))
+    c.arWfrAuxData = _dptr(getattr(w, "arWfrAuxData", None))
     keep.append(c)
     return c
 

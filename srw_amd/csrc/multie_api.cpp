@@ -11,6 +11,7 @@
 // run is reproducible and independent of the batch size.
 #include "srw_api_internal.h"
 #include "srw_multie.h"
+#include "srw_moments.h"
 #include "srw_prop.h"
 #include "srw_learn.h"
 #include "../../include/srw_rng.h"
@@ -214,11 +215,11 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     while ((1L << lz) < nz) lz++;
     if ((1L << lx) != nx || (1L << lz) != nz || lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_ERR_IN_FFT;
     int nDrift = pOpt ? pOpt->nElem : 0;
-    std::vector<int> modes(nDrift);
+    std::vector<int> modes(nDrift), subs(nDrift);
     for (int i = 0; i < nDrift; i++) {
         if (std::strcmp(pOpt->arOptTypes[i], "drift") != 0) return SRWL_AMD_ERR_UNSUPPORTED;
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
-        int rc = drift_mode_from_prop(pp, modes[i]);
+        int rc = drift_mode_from_prop(pp, modes[i], subs[i]);
         if (rc) return rc;
     }
     BeamSampler bs;
@@ -339,13 +340,64 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
             Rx[j] = Rz[j] = res[j].Robs;
         }
         std::vector<srwf_slice> sp(2 * nb);
+        // per-electron wavefront state seen by the drifts (ne = 1)
+        std::vector<SRWLWfr> ws(nb);
+        for (long long j = 0; j < nb; j++) {
+            SRWLWfr& q = ws[j];
+            std::memset(&q, 0, sizeof(q));
+            q.mesh = pWfr0->mesh;
+            q.Rx = q.Ry = Rx[j]; q.dRx = q.dRy = res[j].RobsErr;
+            q.xc = res[j].xc; q.yc = res[j].zc;
+            q.avgPhotEn = pWfr0->mesh.eStart;
+        }
+        // moments of each electron's field, formed when a drift first needs them
+        // and then carried through the drifts like the reference does
+        std::vector<double> mom;
+        bool momValid = false;
+        auto emesh_to_wfr = [&](long long j) {
+            SRWLWfr& q = ws[j];
+            q.mesh.xStart = em[j].xStart; q.mesh.xFin = em[j].xStart + (nx - 1) * em[j].xStep;
+            q.mesh.yStart = em[j].zStart; q.mesh.yFin = em[j].zStart + (nz - 1) * em[j].zStep;
+        };
+        struct OneCtx { SRWLWfr* w; const float2* d; long long nxy, nx; double *mx, *mz; };
+        auto recompute = [](void* c) -> int {
+            OneCtx* m = (OneCtx*)c;
+            return wfr_moments(*m->w, m->d, m->d + m->nxy, 0, 1, m->nx, m->mx, m->mz);
+        };
         for (int d = 0; d < nDrift; d++) {
             const SRWLOptD* D = (const SRWLOptD*)pOpt->arOpt[d];
+            if (modes[d] == 3 && subs[d] == 1 && !momValid) {
+                mom.assign(22 * (size_t)nb, 0.);
+                std::vector<srwm_mslice> sl(nb);
+                for (long long j = 0; j < nb; j++) {
+                    emesh_to_wfr(j);
+                    // electron j's Ex is slice 2j of dF, its Ez slice 2j + 1, each [iz][ix]
+                    const float2* base = (const float2*)dF + 2 * j * nxy;
+                    sl[j] = wfr_slice(ws[j], base, base + nxy, 0, 1, nx, 0);
+                }
+                std::vector<double> sums;
+                std::vector<int> lims;
+                rc = device_moments(sl, sums, lims);
+                if (rc) return rc;
+                for (long long j = 0; j < nb; j++) {
+                    moments_from_sums(&sums[24 * j], sl[j], &mom[22 * j]);
+                    moments_from_sums(&sums[24 * j + 11], sl[j], &mom[22 * j + 11]);
+                }
+                momValid = true;
+            }
             for (long long j = 0; j < nb; j++) {
                 DriftGeom g;
                 std::memset(&g, 0, sizeof(g));
                 if (modes[d] == 3) {
-                    mode3_setup(Rx[j], Rz[j], res[j].RobsErr, res[j].RobsErr, D->L, g);
+                    emesh_to_wfr(j);
+                    double Rxe, Rze;
+                    const float2* base = (const float2*)dF + 2 * j * nxy;
+                    OneCtx oc;
+                    oc.w = &ws[j]; oc.d = base; oc.nxy = nxy; oc.nx = nx;
+                    oc.mx = momValid ? &mom[22 * j] : nullptr; oc.mz = momValid ? &mom[22 * j + 11] : nullptr;
+                    rc = drift_radius(ws[j], subs[d], D->L, oc.mx, oc.mz, recompute, &oc, Rxe, Rze);
+                    if (rc) return rc;
+                    mode3_setup(Rxe, Rze, D->L, g);
                     if (g.kx < 0 || g.kz < 0) return SRWL_AMD_ERR_UNSUPPORTED;
                 }
                 drift_slice(sp[2 * j], modes[d], g, D->L, D->treat, pWfr0->mesh.eStart, nx, nz, em[j].xStart,
@@ -355,7 +407,13 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
                     em[j].xStart = sp[2 * j].nxStart; em[j].xStep = sp[2 * j].nxStep;
                     em[j].zStart = sp[2 * j].nzStart; em[j].zStep = sp[2 * j].nzStep;
                 }
+                emesh_to_wfr(j);
+                if (momValid) {
+                    drift_moments(ws[j], D->L, &mom[22 * j]);
+                    drift_moments(ws[j], D->L, &mom[22 * j + 11]);
+                }
                 Rx[j] += D->L; Rz[j] += D->L;
+                ws[j].Rx += D->L; ws[j].Ry += D->L;
             }
             HIPCHK(hipMemcpyAsync(dSp, sp.data(), sizeof(srwf_slice) * 2 * nb, hipMemcpyHostToDevice, st));
             bool last = (d == nDrift - 1);

@@ -59,28 +59,23 @@ int fft_ready()
 }
 
 // Drift element parameters (sroptcnt.cpp:183-211: arProp layout)
-int drift_mode_from_prop(const double* pp, int& mode)
+// mode: 0 angular representation, 3 analytic quadratic phase (sroptdrf.h:223-283);
+// sub: the radius estimate of the analytic mode (arProp[3] = 1 or 2).
+int drift_mode_from_prop(const double* pp, int& mode, int& sub)
 {
-    mode = 0;
+    mode = 0; sub = 0;
     if (pp == nullptr) return 0;
     if (pp[0] != 0 || pp[1] != 0) return SRWL_AMD_ERR_UNSUPPORTED;    // automatic resizing
     int at = (int)(char)pp[3];
     if (at == 0) mode = 0;
-    else if (at == 2) mode = 3;
-    else return SRWL_AMD_ERR_UNSUPPORTED;   // 1: moments-based radius, 3-5: waist modes
+    else if (at == 1 || at == 2) { mode = 3; sub = at; }
+    else return SRWL_AMD_ERR_UNSUPPORTED;   // 3-5: waist modes
     return 0;
 }
 
-// sroptdrf.cpp:889-1081 with the radius estimate of sroptdrf.cpp:1085-1116.
-void mode3_setup(double Rx, double Rz, double dRx, double dRz, double L, DriftGeom& g)
+// sroptdrf.cpp:889-1081 for the radii Rx, Rz chosen by drift_radius().
+void mode3_setup(double Rx, double Rz, double L, DriftGeom& g)
 {
-    double mx = 3. * dRx, mz = 3. * dRz;
-    double RxL = Rx + L;
-    if (std::fabs(Rx) < mx) Rx = mx * (Rx >= 0 ? 1. : -1.);
-    else if (std::fabs(RxL) < mx) Rx = mx * (RxL >= 0 ? 1. : -1.) - L;
-    double RzL = Rz + L;
-    if (std::fabs(Rz) < mz) Rz = mz * (Rz >= 0 ? 1. : -1.);
-    else if (std::fabs(RzL) < mz) Rz = mz * (RzL >= 0 ? 1. : -1.) - L;
     const double inf = 1E+23;
     g.invRx = 0; g.invRz = 0; g.Lx = 0; g.Lz = 0; g.invRxL = 1. / L; g.invRzL = 1. / L;
     g.kx = inf; g.kxc = inf; g.kz = inf; g.kzc = inf;
@@ -166,9 +161,9 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
     if (lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_ERR_IN_FFT;
     for (int i = 0; i < pOpt->nElem; i++) {
         if (std::strcmp(pOpt->arOptTypes[i], "drift") != 0) return SRWL_AMD_ERR_UNSUPPORTED;
-        int mode;
+        int mode, sub;
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
-        int rc = drift_mode_from_prop(pp, mode);
+        int rc = drift_mode_from_prop(pp, mode, sub);
         if (rc) return rc;
         if (pp && (std::fabs(pp[5] - 1) > 1e-4 || std::fabs(pp[6] - 1) > 1e-4 || std::fabs(pp[7] - 1) > 1e-4 ||
                    std::fabs(pp[8] - 1) > 1e-4))
@@ -210,15 +205,33 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
     double zStep = (pWfr->mesh.yFin - pWfr->mesh.yStart) / (nz - 1);
     double eStep = ne > 1 ? (pWfr->mesh.eFin - pWfr->mesh.eStart) / (ne - 1) : 0;
     std::vector<srwf_slice> sp(2 * ne);
+    // moments: the caller's arrays, or scratch ones when the structure has none
+    std::vector<double> tmpMom;
+    double* momX = pWfr->arMomX;
+    double* momZ = pWfr->arMomY;
+    if (!momX || !momZ) {
+        tmpMom.assign(22 * (size_t)ne, 0.);
+        if (!momX) momX = tmpMom.data();
+        if (!momZ) momZ = tmpMom.data() + 11 * ne;
+    }
+    struct MomCtx { SRWLWfr* w; const float2* d; long long nxy, nx; double *mx, *mz; } mctx;
+    mctx.w = pWfr; mctx.d = dS; mctx.nxy = nxy; mctx.nx = nx; mctx.mx = momX; mctx.mz = momZ;
+    auto recompute = [](void* c) -> int {
+        MomCtx* m = (MomCtx*)c;   // slice-major device field: slice 2*ie + component, [iz][ix]
+        return wfr_moments(*m->w, m->d, m->d + m->nxy, 2 * m->nxy, 1, m->nx, m->mx, m->mz);
+    };
     for (int i = 0; i < pOpt->nElem; i++) {
         const SRWLOptD* d = (const SRWLOptD*)pOpt->arOpt[i];
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
-        int mode;
-        drift_mode_from_prop(pp, mode);
+        int mode, sub;
+        drift_mode_from_prop(pp, mode, sub);
         DriftGeom g;
         std::memset(&g, 0, sizeof(g));
         if (mode == 3) {
-            mode3_setup(pWfr->Rx, pWfr->Ry, pWfr->dRx, pWfr->dRy, d->L, g);
+            double Rx, Rz;
+            rc = drift_radius(*pWfr, sub, d->L, momX, momZ, recompute, &mctx, Rx, Rz);
+            if (rc) return rc;
+            mode3_setup(Rx, Rz, d->L, g);
             if (g.kx < 0 || g.kz < 0) return SRWL_AMD_ERR_UNSUPPORTED;   // mirrored output mesh
         }
         for (long ie = 0; ie < ne; ie++) {
@@ -247,9 +260,12 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
             pWfr->mesh.yFin = sp[0].nzStart + (nz - 1) * sp[0].nzStep;
             xStep = sp[0].nxStep; zStep = sp[0].nzStep;
         }
-        // sroptdrf.h:967-979
+        // sroptdrf.h:181-194: moments, radius and 4x4 matrix follow the field
+        drift_moments(*pWfr, d->L, momX);
+        drift_moments(*pWfr, d->L, momZ);
         pWfr->Rx += d->L;
         pWfr->Ry += d->L;
+        if (pWfr->arElecPropMatr) drift_4x4(pWfr->arElecPropMatr, d->L);
     }
     if (ne == 1) {
         HIPCHK(hipMemcpyAsync(pWfr->arEx, dS, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));

@@ -232,6 +232,264 @@ __global__ void __launch_bounds__(64) k_traj_rk4(const srwd_traj_job* __restrict
 }
 
 // ------------------------------------------------------------------------
+// Parallel-in-s RK4.  A chain of n points y_{j+1} = y_j + D(y_j) (D = one RK4
+// increment, rk4_step) is the fixed point of y = y_0 + prefix-sum(D(y)).  One
+// workgroup iterates that map: every thread forms the increments of its
+// contiguous segment of steps from the current iterate, a block scan turns
+// them into positions, and the loop stops when no component moved by more
+// than 1e-15 of its magnitude (the planar-undulator chains here reach an exact
+// fixed point in 7-8 sweeps).  The result equals the serial recurrence up to
+// the association order of the sums.  A chain that has not converged after
+// SRWD_PIC_MAXIT sweeps is recomputed serially by one thread, so every
+// trajectory ends with the serial answer or its converged equivalent.
+// ------------------------------------------------------------------------
+#define SRWD_PIC_THREADS 256
+#define SRWD_PIC_MAXIT 48
+
+__device__ __forceinline__ void rk4_increment(const rk_ctx& c, const double* y, double h, double* D)
+{   // the update of rk4_step, returned instead of added
+    double d[5], hh = 0.5 * h, h6 = h / 6., yt[5], dyt[5], dym[5];
+    rk_deriv(c, y, d);
+    for (int i = 0; i < 5; i++) yt[i] = y[i] + hh * d[i];
+    rk_deriv(c, yt, dyt);
+    for (int i = 0; i < 5; i++) yt[i] = y[i] + hh * dyt[i];
+    rk_deriv(c, yt, dym);
+    for (int i = 0; i < 5; i++) { yt[i] = y[i] + h * dym[i]; dym[i] += dyt[i]; }
+    rk_deriv(c, yt, dyt);
+    for (int i = 0; i < 5; i++) D[i] = h6 * (d[i] + dyt[i] + 2. * dym[i]);
+}
+
+// Exclusive block scan of 5 doubles per thread (256 threads = 4 waves).
+__device__ __forceinline__ void block_excl_scan5(const double* v, double* ex, double* lds /* [4][5] */)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double inc[5];
+    for (int k = 0; k < 5; k++) {
+        double x = v[k];
+        for (int off = 1; off < 64; off <<= 1) {
+            double t = __shfl_up(x, off, 64);
+            if (lane >= off) x += t;
+        }
+        inc[k] = x;
+    }
+    if (lane == 63) for (int k = 0; k < 5; k++) lds[w * 5 + k] = inc[k];
+    __syncthreads();
+    for (int k = 0; k < 5; k++) {
+        double e = __shfl_up(inc[k], 1, 64);
+        if (lane == 0) e = 0.;
+        for (int q = 0; q < w; q++) e += lds[q * 5 + k];
+        ex[k] = e;
+    }
+    __syncthreads();
+}
+
+// Block max-reduction of 10 non-negative doubles (5 changes, 5 magnitudes);
+// every thread receives the result.
+__device__ __forceinline__ void block_max10(double* v, double* lds /* [4][10] */)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = 0; k < 10; k++) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+        v[k] = x;
+    }
+    if (lane == 0) for (int k = 0; k < 10; k++) lds[w * 10 + k] = v[k];
+    __syncthreads();
+    for (int k = 0; k < 10; k++) {
+        double x = lds[k];
+        for (int q = 1; q < SRWD_PIC_THREADS / 64; q++) x = fmax(x, lds[q * 10 + k]);
+        v[k] = x;
+    }
+    __syncthreads();
+}
+
+// Solves one chain of n points from y0 with step h (component k of point j at
+// buf[k*cap + j]) using the two iterate buffers Ya and Yb, and returns the one
+// that holds the result.  All threads of the block call it with the same
+// arguments.
+__device__ double* picard_chain(const rk_ctx& c, const double* y0, double h, long long n, double* Ya, double* Yb,
+                             long long cap, double* lds)
+{
+    const int t = threadIdx.x;
+    const long long nSteps = n - 1;
+    {   // initial iterate: straight line along the initial derivative
+        double d0[5];
+        rk_deriv(c, y0, d0);
+        for (long long j = t; j < n; j += SRWD_PIC_THREADS)
+            for (int k = 0; k < 5; k++) Ya[k * cap + j] = (j == 0) ? y0[k] : y0[k] + (double)j * h * d0[k];
+    }
+    __syncthreads();
+    if (nSteps <= 0) return Ya;
+    const long long per = (nSteps + SRWD_PIC_THREADS - 1) / SRWD_PIC_THREADS;
+    const long long j0 = t * per, j1 = (j0 + per < nSteps) ? j0 + per : nSteps;
+    bool done = false;
+    for (int it = 0; it < SRWD_PIC_MAXIT && !done; it++) {
+        double tot[5] = {0., 0., 0., 0., 0.};
+        for (long long j = j0; j < j1; j++) {
+            double y[5], D[5];
+            for (int k = 0; k < 5; k++) y[k] = Ya[k * cap + j];
+            rk4_increment(c, y, h, D);
+            for (int k = 0; k < 5; k++) { Yb[k * cap + j + 1] = D[k]; tot[k] += D[k]; }
+        }
+        double ex[5];
+        block_excl_scan5(tot, ex, lds);
+        double m[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+        double acc[5];
+        for (int k = 0; k < 5; k++) acc[k] = y0[k] + ex[k];
+        for (long long j = j0; j < j1; j++) {
+            for (int k = 0; k < 5; k++) {
+                acc[k] += Yb[k * cap + j + 1];
+                m[k] = fmax(m[k], fabs(acc[k] - Ya[k * cap + j + 1]));
+                m[5 + k] = fmax(m[5 + k], fabs(acc[k]));
+                Yb[k * cap + j + 1] = acc[k];
+            }
+        }
+        if (t == 0) for (int k = 0; k < 5; k++) Yb[k * cap] = y0[k];
+        block_max10(m, lds + 20);
+        double* sw = Ya; Ya = Yb; Yb = sw;
+        done = true;
+        for (int k = 0; k < 5; k++) done = done && (m[k] <= 1e-15 * m[5 + k]);
+    }
+    if (!done) {
+        // serial recurrence, one thread
+        if (t == 0) {
+            double y[5], D[5];
+            for (int k = 0; k < 5; k++) { y[k] = y0[k]; Ya[k * cap] = y[k]; }
+            for (long long j = 0; j < nSteps; j++) {
+                rk4_increment(c, y, h, D);
+                for (int k = 0; k < 5; k++) { y[k] += D[k]; Ya[k * cap + j + 1] = y[k]; }
+            }
+        }
+        __syncthreads();
+    }
+    return Ya;
+}
+
+__device__ __forceinline__ void rk_ctx_setup(rk_ctx& c, const srwd_traj_job& J, const srwd_field* fields)
+{
+    c.f = &fields[J.field];
+    c.gEm2 = 1. / J.gammaInt / J.gammaInt;
+    c.mult = J.mult;
+    c.one = (c.f->n == 1 && c.f->u[0].nh == 1);
+    if (c.one) {
+        c.h = c.f->u[0].h[0];
+        c.halfTotTerm = c.f->u[0].halfTotTerm;
+        c.cy = c.f->u[0].cy;
+        c.cz = c.f->u[0].cz;
+    }
+}
+
+// Stores chain point j of buffer R at output index base + dir*j.
+__device__ __forceinline__ void picard_store(const double* R, long long cap, long long n, long long base, int dir,
+                                             double* oX, double* oXp, double* oY, double* oYp, double* oZ)
+{
+    for (long long j = threadIdx.x; j < n; j += SRWD_PIC_THREADS) {
+        long long o = base + dir * j;
+        oX[o] = R[j]; oXp[o] = R[cap + j]; oY[o] = R[2 * cap + j]; oYp[o] = R[3 * cap + j];
+        if (oZ) oZ[o] = R[4 * cap + j];
+    }
+}
+
+// Advances y by a chain of n points with step h (the reference's "arrive"
+// integrations), in pieces that fit the scratch buffers.
+__device__ void picard_arrive(const rk_ctx& c, double* y, double h, long long n, double* Ya, double* Yb,
+                              long long cap, double* lds)
+{
+    long long left = n;
+    if (cap < 2) return;
+    while (left > 1) {
+        long long m = left < cap ? left : cap;
+        double* R = picard_chain(c, y, h, m, Ya, Yb, cap, lds);
+        __syncthreads();
+        for (int k = 0; k < 5; k++) y[k] = R[k * cap + m - 1];
+        __syncthreads();
+        left -= m - 1;
+    }
+}
+
+// Same trajectory as k_traj_rk4 (srgtrjdt.cpp:90-340), one workgroup per
+// trajectory, each chain solved by picard_chain.  scratch holds 10*ld doubles
+// per trajectory.
+__global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_traj_job* __restrict__ jobs,
+                                                                   const srwd_field* __restrict__ fields,
+                                                                   double* X, double* Xp, double* Y,
+                                                                   double* Yp, double* Z, long long ld,
+                                                                   double* scratch)
+{
+    __shared__ double lds[64];
+    const int tr = blockIdx.x;
+    const srwd_traj_job J = jobs[tr];
+    rk_ctx c;
+    rk_ctx_setup(c, J, fields);
+    double* oX = X + tr * ld; double* oXp = Xp + tr * ld; double* oY = Y + tr * ld;
+    double* oYp = Yp + tr * ld; double* oZ = Z ? Z + tr * ld : nullptr;
+    double* Ya = scratch + (size_t)tr * 10 * ld;
+    double* Yb = Ya + 5 * ld;
+    const long long cap = ld;
+    const long long ns = J.np;
+    const double sMin = J.ctStart, sMax = J.ctEnd;
+    const double sStep = (ns <= 1) ? 0 : (sMax - sMin) / (ns - 1);
+    const double tol = (sMax - sMin) * 1.E-12;
+    const bool left = sMin < -tol, right = sMax > tol;
+    double ic[5];
+    for (int k = 0; k < 5; k++) ic[k] = J.init[k];
+    if ((left && sMax < -tol) || (right && sMin > tol)) {
+        // the whole range lies on one side of ct = 0
+        const bool below = left && sMax < -tol;
+        const double sNear = below ? sMax : sMin, sFar = below ? sMin : sMax;
+        long long auxNp = (long long)(int)fabs(sNear / sStep) + 1;
+        if (auxNp <= 1) auxNp = 2;
+        picard_arrive(c, ic, sNear / (double)(auxNp - 1), auxNp, Ya, Yb, cap, lds);
+        if (ns <= 1) {
+            if (threadIdx.x == 0) {
+                oX[0] = ic[0]; oXp[0] = ic[1]; oY[0] = ic[2]; oYp[0] = ic[3]; if (oZ) oZ[0] = ic[4];
+            }
+            return;
+        }
+        double* R = picard_chain(c, ic, (sFar - sNear) / (double)(ns - 1), ns, Ya, Yb, cap, lds);
+        if (below) picard_store(R, cap, ns, ns - 1, -1, oX, oXp, oY, oYp, oZ);
+        else picard_store(R, cap, ns, 0, 1, oX, oXp, oY, oYp, oZ);
+        return;
+    }
+    long long is0 = 0;
+    double s0Act = 0.;
+    if (left) {
+        is0 = (int)fabs((-sMin + tol) / sStep);
+        if (is0 >= ns) is0 = ns - 1;
+        s0Act = sMin + is0 * sStep;
+        if (s0Act < -tol) {   // one RK4 step from 0 to s0Act, computed by every thread alike
+            double D[5];
+            rk4_increment(c, ic, s0Act, D);
+            for (int k = 0; k < 5; k++) ic[k] += D[k];
+        }
+    }
+    if (left) {
+        long long nsL = is0 + 1;
+        if (nsL <= 1) {
+            if (threadIdx.x == 0) {
+                oX[0] = ic[0]; oXp[0] = ic[1]; oY[0] = ic[2]; oYp[0] = ic[3]; if (oZ) oZ[0] = ic[4];
+            }
+        } else {
+            double* R = picard_chain(c, ic, (sMin - s0Act) / (double)(nsL - 1), nsL, Ya, Yb, cap, lds);
+            picard_store(R, cap, nsL, nsL - 1, -1, oX, oXp, oY, oYp, oZ);
+        }
+        __syncthreads();
+    }
+    if (right) {
+        long long nsR = ns - is0;
+        if (nsR < 1) nsR = 1;
+        if (nsR <= 1) {
+            if (threadIdx.x == 0) {
+                oX[is0] = ic[0]; oXp[is0] = ic[1]; oY[is0] = ic[2]; oYp[is0] = ic[3]; if (oZ) oZ[is0] = ic[4];
+            }
+        } else {
+            double* R = picard_chain(c, ic, (sMax - s0Act) / (double)(nsR - 1), nsR, Ya, Yb, cap, lds);
+            picard_store(R, cap, nsR, is0, 1, oX, oXp, oY, oYp, oZ);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // Interpolation structure (FP64)
 // ------------------------------------------------------------------------
 __device__ __forceinline__ void coefs5(double h, const double* f, double* c)
@@ -656,11 +914,17 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
 extern "C" {
 
 int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int nTraj, double* dX,
-                     double* dXp, double* dY, double* dYp, double* dZ, long long ld, hipStream_t st)
+                     double* dXp, double* dY, double* dYp, double* dZ, long long ld, double* dScratch,
+                     hipStream_t st)
 {
-    int threads = 2 * nTraj;
-    hipLaunchKernelGGL(k_traj_rk4, dim3((threads + 63) / 64), dim3(64), 0, st, dJobs, dFields, nTraj,
-                       dX, dXp, dY, dYp, dZ, ld);
+    if (dScratch && ld >= 16) {   // picard_arrive needs buffers of at least 2 points
+        hipLaunchKernelGGL(k_traj_picard, dim3(nTraj), dim3(SRWD_PIC_THREADS), 0, st, dJobs, dFields, dX, dXp, dY,
+                           dYp, dZ, ld, dScratch);
+    } else {
+        int threads = 2 * nTraj;
+        hipLaunchKernelGGL(k_traj_rk4, dim3((threads + 63) / 64), dim3(64), 0, st, dJobs, dFields, nTraj,
+                           dX, dXp, dY, dYp, dZ, ld);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

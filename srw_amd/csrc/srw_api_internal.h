@@ -3,6 +3,7 @@
 #include "../../include/srwlib.h"
 #include "../../include/srwl_amd_ext.h"
 #include "srw_dev.h"
+#include "srw_moments.h"
 #include <vector>
 
 namespace srwamd {
@@ -21,6 +22,8 @@ struct Buffers {
     DBuf Ex, Ez;                  // field outputs
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
     DBuf Stokes, Draws, Wt, Net, Grad;
+    DBuf Pic;                     // parallel-in-s RK4 iterates
+    DBuf MomSl, MomWork, MomSums, MomLims;   // wavefront moments
 };
 
 struct Context {
@@ -34,6 +37,7 @@ struct Context {
     long long lastSrEvals = 0;
     int lastSrBlocks = 0, lastSrSplit = 0;
     int forceSplit = 0;
+    bool trajSerial = false;      // test switch: serial RK4 kernel instead of the parallel one
     bool keepOnDevice = false;    // benchmark switch: leave results in HBM
     double sumSrMs = 0, sumSrEvals = 0;
     int nSrCalls = 0;
@@ -71,5 +75,18 @@ int setup_job_from_field(SrJob& J, const SRWLParticle& p, const SRWLMagFldC* mag
                          std::vector<srwd_field>& fields);
 int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
            const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin);
+
+// Wavefront moments (moments_api.cpp).  Component element (ix, iz, ie) of a
+// device wavefront is at base[ie*perE + iz*perZ + ix*perX] (float2 units).
+int wfr_moments(const SRWLWfr& w, const float2* dEx, const float2* dEz, long long perE, long long perX,
+                long long perZ, double* momX, double* momZ);
+int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums, std::vector<int>& lims);
+void moments_from_sums(const double* S, const srwm_mslice& s, double* mom);
+srwm_mslice wfr_slice(const SRWLWfr& w, const float2* dEx, const float2* dEz, long long perE, long long perX,
+                      long long perZ, long ie);
+int drift_radius(const SRWLWfr& w, int sub, double L, double* momX, double* momZ, int (*recompute)(void*),
+                 void* ctx, double& Rx, double& Rz);
+void drift_moments(const SRWLWfr& w, double L, double* mom);
+void drift_4x4(double* P, double L);
 
 }  // namespace srwamd

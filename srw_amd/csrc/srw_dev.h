@@ -97,8 +97,11 @@ struct srwd_item {
 
 extern "C" {
 // Launch wrappers implemented in sr.hip; all asynchronous on stream st.
+// dScratch (10*ld doubles per trajectory) selects the parallel-in-s solver;
+// nullptr runs the serial two-thread-per-trajectory kernel.
 int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int nTraj, double* dX,
-                     double* dXp, double* dY, double* dYp, double* dZ, long long ld, hipStream_t st);
+                     double* dXp, double* dY, double* dYp, double* dZ, long long ld, double* dScratch,
+                     hipStream_t st);
 int srwd_launch_nz(srwd_trj_info* dInfo, int nTraj, const double* dX, const double* dXp, const double* dY,
                    const double* dYp, long long ldT, hipStream_t st);
 int srwd_launch_tables(const srwd_trj_info* dInfo, int nTraj, long long maxInt, const double* dX,

@@ -12,8 +12,8 @@ struct DriftGeom {
 
 const float2* twiddles(int logn);
 int fft_ready();
-int drift_mode_from_prop(const double* pp, int& mode);
-void mode3_setup(double Rx, double Rz, double dRx, double dRz, double L, DriftGeom& g);
+int drift_mode_from_prop(const double* pp, int& mode, int& sub);
+void mode3_setup(double Rx, double Rz, double L, DriftGeom& g);
 void drift_slice(srwf_slice& S, int mode, const DriftGeom& g, double L, int treat, double e, long nx, long nz,
                  double xStart, double xStep, double zStart, double zStep, double xc, double zc);
 int drift_passes(float2* d, long long sliceStride, int nSlices, const srwf_slice* dSp, int lognx, int lognz,

@@ -212,7 +212,12 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     }
     if (anyRK) {
         // user-supplied trajectories are re-uploaded after the RK pass overwrote them
-        if (srwd_launch_traj(dJobs, dFields, nT, dX, dXp, dY, dYp, nullptr, ldT, st)) return SRWL_AMD_ERR_GPU;
+        double* dPic = nullptr;
+        if (!g_ctx.trajSerial) {
+            dPic = (double*)B.Pic.get(sizeof(double) * 10 * (size_t)ldT * nT);
+            if (!dPic) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        }
+        if (srwd_launch_traj(dJobs, dFields, nT, dX, dXp, dY, dYp, nullptr, ldT, dPic, st)) return SRWL_AMD_ERR_GPU;
         for (int j = 0; j < nT; j++) {
             if (!jobs[j].userTrj) continue;
             const SRWLPrtTrj* u = jobs[j].userTrj;
@@ -501,7 +506,12 @@ SRWL_EXPORT int srwlCalcPartTraj(SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld, double*
     if (!dX || !dXp || !dY || !dYp || !dZ || !dJ || !dF) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     HIPCHK(hipMemcpyAsync(dJ, &J, sizeof(J), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dF, &f, sizeof(f), hipMemcpyHostToDevice, st));
-    if (srwd_launch_traj(dJ, dF, 1, dX, dXp, dY, dYp, dZ, pTrj->np, st)) return SRWL_AMD_ERR_GPU;
+    double* dPic = nullptr;
+    if (!g_ctx.trajSerial) {
+        dPic = (double*)B.Pic.get(sizeof(double) * 10 * (size_t)pTrj->np);
+        if (!dPic) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    }
+    if (srwd_launch_traj(dJ, dF, 1, dX, dXp, dY, dYp, dZ, pTrj->np, dPic, st)) return SRWL_AMD_ERR_GPU;
     HIPCHK(hipMemcpyAsync(pTrj->arX, dX, b, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(pTrj->arXp, dXp, b, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(pTrj->arY, dY, b, hipMemcpyDeviceToHost, st));
@@ -567,10 +577,17 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
         HIPCHK(hipMemcpyAsync(pWfr->arEx, dEx, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
         HIPCHK(hipMemcpyAsync(pWfr->arEy, dEz, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
     }
-    HIPCHK(hipStreamSynchronize(g_ctx.stream));
     pWfr->Rx = pWfr->Ry = res[0].Robs;
     pWfr->dRx = pWfr->dRy = res[0].RobsErr;
     pWfr->xc = res[0].xc; pWfr->yc = res[0].zc;
+    // srradint.cpp:3570-3574: statistical moments of every energy slice, then the mean energy
+    if (pWfr->arMomX || pWfr->arMomY) {
+        long long ne = J.mesh.ne;
+        rc = wfr_moments(*pWfr, (const float2*)dEx, (const float2*)dEz, 1, ne, ne * J.mesh.nx, pWfr->arMomX,
+                         pWfr->arMomY);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(g_ctx.stream));
     pWfr->avgPhotEn = 0.5 * (pWfr->mesh.eStart + pWfr->mesh.eFin);
     pWfr->presCA = 0; pWfr->presFT = 0;
     return res[0].warn;
@@ -615,6 +632,12 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
         HIPCHK(hipMemcpyAsync(w->arEy, dEz + 2 * off[i], sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
         w->Rx = w->Ry = res[i].Robs; w->dRx = w->dRy = res[i].RobsErr;
         w->xc = res[i].xc; w->yc = res[i].zc;
+        if (w->arMomX || w->arMomY) {
+            long long ne = w->mesh.ne;
+            rc = wfr_moments(*w, (const float2*)dEx + off[i], (const float2*)dEz + off[i], 1, ne, ne * w->mesh.nx,
+                             w->arMomX, w->arMomY);
+            if (rc) return rc;
+        }
         w->avgPhotEn = 0.5 * (w->mesh.eStart + w->mesh.eFin);
         w->presCA = 0; w->presFT = 0;
         if (res[i].warn) warn = res[i].warn;
@@ -662,6 +685,12 @@ SRWL_EXPORT int srwlAmdSetKeepOnDevice(int on)
 SRWL_EXPORT int srwlAmdSetSplit(int nSplit)
 {
     g_ctx.forceSplit = nSplit;
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetTrajSerial(int on)
+{
+    g_ctx.trajSerial = on != 0;
     return 0;
 }
 

@@ -123,6 +123,10 @@ def set_keep_on_device(on=True):
     lib.srwlAmdSetKeepOnDevice(1 if on else 0)
 
 
+def set_traj_serial(on=True):
+    lib.srwlAmdSetTrajSerial(1 if on else 0)
+
+
 def PropagElecField(_wfr, _opt_cnt, _int_list=None, _gpu_idx=None):
     """Propagation through a container of drift spaces (reference srwlpy.cpp:5214-5320)."""
     if _int_list is not None:

@@ -49,11 +49,15 @@ def _sr_wfr(n, ne=1, de=0.0):
     return w
 
 
-@pytest.mark.parametrize("n,ne,mode,L", [(128, 1, 2, 20.), (256, 1, 0, 10.), (128, 3, 2, 30.), (64, 2, 0, 5.)])
+@pytest.mark.parametrize("n,ne,mode,L", [(128, 1, 2, 20.), (256, 1, 0, 10.), (128, 3, 2, 30.), (64, 2, 0, 5.),
+                                         (128, 1, 1, 20.), (128, 3, 1, 30.)])
 def test_drift(n, ne, mode, L):
     from srw_amd import srwlpy
     print("case: drift n=%d ne=%d mode=%d L=%g" % (n, ne, mode, L), flush=True)
     w = _sr_wfr(n, ne, 0.03 if ne > 1 else 0.)
+    for i in range(20):
+        w.arElecPropMatr[i] = 1. if i in (0, 5, 10, 15) else 0.
+    w.arElecPropMatr[16], w.arElecPropMatr[17] = 1e-6, 2e-6
     wg, wo = clone_wfr(w), clone_wfr(w)
     opt = W.drift_container(L, mode)
     srwlpy.PropagElecField(wg, opt)
@@ -68,6 +72,34 @@ def test_drift(n, ne, mode, L):
     print("rel-L2 Ex %.3e I %.3e" % (ex_e, i_e), flush=True)
     assert ex_e < TOL_DRIFT
     assert i_e < 2 * TOL_DRIFT
+    # moments and the 4x4 matrix are carried through the drift the same way
+    assert np.allclose(np.array(wg.arMomX), np.array(wo.arMomX), rtol=1e-12, atol=0)
+    assert np.allclose(np.array(wg.arMomY), np.array(wo.arMomY), rtol=1e-12, atol=0)
+    assert np.allclose(np.array(wg.arElecPropMatr), np.array(wo.arElecPropMatr), rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("n,ne", [(128, 1), (64, 3)])
+def test_drift_radius_from_recomputed_moments(n, ne):
+    # arProp[3] = 1 with empty moment arrays: both sides compute the moments of
+    # the input field first; the device sums (other order, FP32 input) move the
+    # radius, hence the output mesh, by ~1e-7 at most
+    from srw_amd import srwlpy
+    print("case: drift subtype 1, moments recomputed, n=%d ne=%d" % (n, ne), flush=True)
+    w = _sr_wfr(n, ne, 0.03 if ne > 1 else 0.)
+    for i in range(len(w.arMomX)):
+        w.arMomX[i] = 0.
+        w.arMomY[i] = 0.
+    wg, wo = clone_wfr(w), clone_wfr(w)
+    opt = W.drift_container(25., 1)
+    srwlpy.PropagElecField(wg, opt)
+    assert oracle.propagate(wo, opt) == 0
+    for attr in ("xStart", "xFin", "yStart", "yFin"):
+        assert getattr(wg.mesh, attr) == pytest.approx(getattr(wo.mesh, attr), rel=1e-6)
+    ig = np.abs(cfield(wg.arEx, wg)) ** 2 + np.abs(cfield(wg.arEy, wg)) ** 2
+    io = np.abs(cfield(wo.arEx, wo)) ** 2 + np.abs(cfield(wo.arEy, wo)) ** 2
+    i_e = rel_l2(ig, io)
+    print("rel-L2 I %.3e" % i_e, flush=True)
+    assert i_e < 1e-4
 
 
 @pytest.mark.parametrize("pol", [0, 1, 2, 3, 4, 5, 6, -1, -2, -3, -4])

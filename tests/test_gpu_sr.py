@@ -37,6 +37,29 @@ def _check(wg, wo, tol=TOL):
         assert rel_l2(eyg, eyo) < tol
     assert wg.Rx == pytest.approx(wo.Rx, rel=1e-12)
     assert wg.xc == pytest.approx(wo.xc, rel=1e-9, abs=1e-15)
+    _check_moments(wg, wo)
+
+
+def _check_moments(wg, wo):
+    """Statistical moments (11 per energy and component) against the oracle.
+    First moments are compared on the scale of the rms size / divergence."""
+    for name in ("arMomX", "arMomY"):
+        g = np.array(getattr(wg, name)).reshape(-1, 11)[:wg.mesh.ne]
+        o = np.array(getattr(wo, name)).reshape(-1, 11)[:wo.mesh.ne]
+        for ie in range(wg.mesh.ne):
+            a, b = g[ie], o[ie]
+            if b[0] == 0:
+                assert a[0] == 0
+                continue
+            sx, sz = np.sqrt(abs(b[5])), np.sqrt(abs(b[8]))
+            sxp, szp = np.sqrt(abs(b[7])), np.sqrt(abs(b[10]))
+            d = [abs(a[0] - b[0]) / abs(b[0]), abs(a[1] - b[1]) / sx, abs(a[2] - b[2]) / sxp,
+                 abs(a[3] - b[3]) / sz, abs(a[4] - b[4]) / szp,
+                 abs(a[5] - b[5]) / b[5], abs(a[6] - b[6]) / (sx * sxp), abs(a[7] - b[7]) / b[7],
+                 abs(a[8] - b[8]) / b[8], abs(a[9] - b[9]) / (sz * szp), abs(a[10] - b[10]) / b[10]]
+            if ie == 0:
+                print("%s moment deviations: max %.2e" % (name, max(d)), flush=True)
+            assert max(d) < 1e-4, (name, ie, d)
 
 
 def test_planar_undulator_small_mesh():
