@@ -30,12 +30,27 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=201, help="mesh points per axis")
     ap.add_argument("--ne", type=int, default=201, help="photon energies")
-    ap.add_argument("--cpu-energies", type=int, default=3, help="energies in the CPU-baseline sample")
+    ap.add_argument("--cpu-energies", type=int, default=24, help="energies in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--multi-e", type=int, default=1, help="also measure the multi-electron time to 1%% rel-L2")
     ap.add_argument("--me-n", type=int, default=512, help="multi-electron emission mesh")
     ap.add_argument("--me-max", type=int, default=4096, help="electron cap per multi-electron run")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size, 5 times, for scaling the
+    HBM byte counters of a profiler run (read MIB MiB, write MIB MiB per copy)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}))
 
 
 def multi_e_runs(srwlpy, W, n, nmax):
@@ -84,6 +99,9 @@ def cpu_baseline(n, ne_sample, threads):
 
 def main():
     a = parse()
+    if a.copy_calib:
+        copy_calibration(a.copy_calib)
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % a.gpus,
               file=sys.stderr)

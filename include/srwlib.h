@@ -211,6 +211,13 @@ SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, ch
 SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, char** arID,
                                     SRWLRadMesh* arIM, char** arI, double* arParGPU);
 
+/* reference srwlib.h:775 (srwlib.cpp:860-900): resampling of the transverse
+ * mesh, type 'c'; par = [0 (no FFT-side resizing), pxm, pxd, pzm, pzd, 0.5,
+ * 0.5] (range and resolution factors, centred).  A size change reallocates
+ * the field arrays through the callback of srwlUtiSetWfrModifFunc (action 2);
+ * without one the call returns SRWL_WFR_EXT_FUNC_NOT_DEFINED. */
+SRWL_EXPORT int srwlResizeElecField(SRWLWfr* pWfr, char type, double* par);
+
 /* reference srwlib.h:871 (srwlib.cpp:1090-1204): SRW-convention 2D FFT of
  * float complex data; arMesh = [xStart, xStep, nx, yStart, yStep, ny]. */
 SRWL_EXPORT int srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMesh, int dir,

@@ -1817,3 +1817,270 @@ ORC_API void orc_compute_rad_moments(SRWLWfr* w, double* momX, double* momZ)
     if (treated) orc_treat_quad_phase(w, 1, fx, fz);
 }
 
+
+/* ------------------------------------------------------------------ */
+/* Resampling of the wavefront mesh (srwlib.cpp:860-900 with            */
+/* sroptelm.cpp:2242-3339, RelCenPos 0.5, no FFT-side resizing)         */
+/* ------------------------------------------------------------------ */
+
+/* gmfft.cpp:156-225 (NextCorrectNumberForFFT) for n <= 100000: the smallest
+ * size >= n in the reference's table of good FFT sizes, which is exactly the
+ * set of even numbers 2^a 3^b 5^c 7^d 11^e 13^f with e, f <= 1. */
+ORC_API long orc_next_fft_size(long n)
+{
+    if (n < 4) return 4;
+    for (long m = n;; m++) {
+        if (m & 1) continue;
+        long r = m;
+        int e11 = 0, e13 = 0;
+        while (r % 2 == 0) r /= 2;
+        while (r % 3 == 0) r /= 3;
+        while (r % 5 == 0) r /= 5;
+        while (r % 7 == 0) r /= 7;
+        while (r % 11 == 0) { r /= 11; e11++; }
+        while (r % 13 == 0) { r /= 13; e13++; }
+        if (r == 1 && e11 <= 1 && e13 <= 1) return m;
+    }
+}
+
+/* New mesh of one axis (sroptelm.cpp:2321-2340 size, 2432-2500 limits).
+ * out: [0] n, [1] start, [2] step, [3] first, [4] last index filled. */
+static void orc_resize_axis(long nOld, double startOld, double stepOld, double pm, double pd, double* out)
+{
+    double pTot = pm * pd;
+    long nNew = nOld;
+    if (pTot != 1.) {
+        double d = pTot * nNew;
+        long l = (long)d;
+        nNew = ((d - l) >= 0.5) ? l + 1 : l;
+        nNew = orc_next_fft_size(nNew);
+    }
+    double magOld = (nOld - 1) * stepOld;
+    double start, step;
+    long i0, i1;
+    if (pm > 1.) {
+        long inner = (pd != 1.) ? (long)(nNew / pm) : nOld;
+        long outer = nNew - inner;
+        long left = outer >> 1;
+        if ((left << 1) != outer) left++;
+        step = magOld / (inner - 1);
+        start = startOld - left * step;
+        i0 = left; i1 = left + inner - 1;
+    } else {
+        double endOld = startOld + (nOld - 1) * stepOld;
+        double mid = 0.5 * (startOld + endOld);
+        double magNew = magOld * pm;
+        step = magNew / (nNew - 1);
+        if (pd == 1) { step = stepOld; magNew = (nNew - 1) * step; }
+        start = mid - 0.5 * magNew;
+        i0 = 0; i1 = nNew - 1;
+    }
+    out[0] = (double)nNew; out[1] = start; out[2] = step; out[3] = (double)i0; out[4] = (double)i1;
+}
+
+/* The new mesh of srwlResizeElecField(w, 'c', par), par = [fft side (0), pxm,
+ * pxd, pzm, pzd, (rel. centre x 0.5), (rel. centre z 0.5)].  Writes nx, ny,
+ * xStart, xFin, yStart, yFin of the result and the filled index ranges
+ * (ix0, ix1, iz0, iz1) into out[10].  Returns 0, or an error for options
+ * outside this restatement. */
+ORC_API int orc_resize_mesh(const SRWLWfr* w, const double* par, int npar, double* out)
+{
+    if (!w || !par || npar < 5) return SRWL_INCORRECT_PARAM_FOR_RESIZE;
+    if (par[0] != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (npar > 5 && fabs(par[5] - 0.5) > 1.E-06) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (npar > 6 && fabs(par[6] - 0.5) > 1.E-06) return SRWL_AMD_ERR_UNSUPPORTED;
+    orc_mesh3 m = orc_mesh_of(w);
+    double ax[5], az[5];
+    orc_resize_axis(m.nx, m.xStart, m.xStep, par[1], par[2], ax);
+    orc_resize_axis(m.nz, m.zStart, m.zStep, par[3], par[4], az);
+    out[0] = ax[0]; out[1] = az[0];
+    out[2] = ax[1]; out[3] = ax[1] + (ax[0] - 1) * ax[2];
+    out[4] = az[1]; out[5] = az[1] + (az[0] - 1) * az[2];
+    out[6] = ax[3]; out[7] = ax[4]; out[8] = az[3]; out[9] = az[4];
+    return 0;
+}
+
+/* 4x4 cell of one component (re, im) starting at old point (ixSt, izSt):
+ * c[0..15] = Re f(ix, iz) with index 4*iz + ix, c[16..31] = Im
+ * (sroptelm.h:460-482). */
+static void orc_cell(const float* base, long long perX, long long perZ, float* c)
+{
+    for (int j = 0; j < 4; j++)
+        for (int i = 0; i < 4; i++) {
+            const float* p = base + j * perZ + i * perX;
+            c[4 * j + i] = p[0];
+            c[16 + 4 * j + i] = p[1];
+        }
+}
+
+/* 16 bicubic coefficients of a 4x4 cell f[4*iz + ix] (sroptelm.h:526-546);
+ * a[4*px + pz] multiplies x^px z^pz.  Sums in float, as there. */
+static void orc_bicubic(const float* F, double* a)
+{
+#define FF(i, j) F[4 * (j) + (i)]
+    float f00 = FF(0, 0), f10 = FF(1, 0), f20 = FF(2, 0), f30 = FF(3, 0);
+    float f01 = FF(0, 1), f11 = FF(1, 1), f21 = FF(2, 1), f31 = FF(3, 1);
+    float f02 = FF(0, 2), f12 = FF(1, 2), f22 = FF(2, 2), f32 = FF(3, 2);
+    float f03 = FF(0, 3), f13 = FF(1, 3), f23 = FF(2, 3), f33 = FF(3, 3);
+#undef FF
+    const double c6 = 0.1666666667, c36 = 0.027777777778, c12 = 0.083333333333;
+    a[0] = f11;
+    a[1] = (-2 * f10 - 3 * f11 + 6 * f12 - f13) * c6;
+    a[2] = (f10 + f12 - 2 * f11) * 0.5;
+    a[3] = (f13 - f10 + 3 * (f11 - f12)) * c6;
+    a[4] = (-2 * f01 - 3 * f11 + 6 * f21 - f31) * c6;
+    a[5] = (4 * f00 + 6 * (f01 + f10 - f23 - f32) - 12 * (f02 + f20) + 2 * (f03 + f30) + 9 * f11 - 18 * (f12 + f21) +
+            3 * (f13 + f31) + 36 * f22 + f33) * c36;
+    a[6] = (-2 * (f00 + f02 - f31) + 4 * f01 - 3 * (f10 + f12) + 6 * (f11 + f20 + f22) - 12 * f21 - f30 - f32) * c12;
+    a[7] = (2 * (f00 - f03) + 6 * (-f01 + f02 - f20 + f23) + 3 * (f10 - f13 - f31 + f32) + 9 * (f12 - f11) +
+            18 * (f21 - f22) + f30 - f33) * c36;
+    a[8] = (f01 + f21 - 2 * f11) * 0.5;
+    a[9] = (2 * (-f00 + f13 - f20) - 3 * (f21 + f01) + 6 * (f02 + f11 + f22) + 4 * f10 - 12 * f12 - f23 - f03) * c12;
+    a[10] = (f00 + f02 + f22 + f20 - 2 * (f01 + f10 + f12 + f21) + 4 * f11) * 0.25;
+    a[11] = (-f00 + f03 - f20 + f23 + 3 * (f01 - f02 + f21 - f22) + 2 * (f10 - f13) + 6 * (f12 - f11)) * c12;
+    a[12] = (f31 - f01 + 3 * (f11 - f21)) * 0.16666666667;
+    a[13] = (2 * (f00 - f30) + 3 * (f01 - f13 + f23 - f31) + 6 * (-f02 - f10 + f20 + f32) + 9 * (f21 - f11) +
+             18 * (f12 - f22) + f03 - f33) * c36;
+    a[14] = (f30 + f32 - f00 - f02 + 2 * (f01 - f31) + 3 * (f10 + f12 - f20 - f22) + 6 * (f21 - f11)) * c12;
+    a[15] = (f00 - f03 - f30 + f33 + 3 * (-f01 + f02 - f10 + f13 + f20 - f23 + f31 - f32) +
+             9 * (f11 - f12 - f21 + f22)) * c36;
+}
+
+/* sroptelm.h:594-606 polynomial sum in the reference's term order */
+static double orc_bicubic_eval(const double* a, double x, double z)
+{
+    double xE2 = x * x, xz = x * z, zE2 = z * z;
+    double xE3 = xE2 * x, xE2z = xE2 * z, xzE2 = x * zE2, zE3 = zE2 * z, xE2zE2 = xE2 * zE2;
+    double xE3z = xE3 * z, xE3zE2 = xE3 * zE2, xE3zE3 = xE3 * zE3, xE2zE3 = xE2 * zE3, xzE3 = x * zE3;
+    return a[15] * xE3zE3 + a[14] * xE3zE2 + a[13] * xE3z + a[12] * xE3 + a[11] * xE2zE3 + a[10] * xE2zE2 +
+           a[9] * xE2z + a[8] * xE2 + a[7] * xzE3 + a[6] * xzE2 + a[5] * xz + a[4] * x + a[3] * zE3 + a[2] * zE2 +
+           a[1] * z + a[0];
+}
+
+/* bilinear form on the sub-cell (sroptelm.h:563-569, 650-657): a = {A00, A10, A01, A11} */
+static void orc_bilinear(const float* F, int ix, int iz, double* a)
+{
+    float f00 = F[4 * iz + ix], f10 = F[4 * iz + ix + 1], f01 = F[4 * iz + 4 + ix], f11 = F[4 * iz + 4 + ix + 1];
+    a[0] = f00; a[1] = 1. * (f10 - f00); a[2] = 1. * (f01 - f00); a[3] = 1. * (f00 - f01 - f10 + f11);
+}
+
+/* Resampled value of one component at relative cell position (xr, zr)
+ * (sroptelm.cpp:3140-3300 per component): cubic interpolation of Re and Im
+ * and of the intensity (normalised by its cell average when above 1), the
+ * complex value then rescaled to the interpolated intensity; bilinear
+ * interpolation inside the sub-cell when the cell holds an exact zero. */
+static void orc_resample_point(const float* cell, int ixRel, int izRel, double xr, double zr, float* out)
+{
+    float In[16];   /* intensity cell, sroptelm.h:487-507 */
+    for (int k = 0; k < 16; k++) In[k] = cell[k] * cell[k] + cell[16 + k] * cell[16 + k];
+    /* srstraux.h:1663-1683: cell average, normalisation above 1 */
+    float avg = (float)(0.0625 * (double)(In[0] + In[1] + In[2] + In[3] + In[4] + In[5] + In[6] + In[7] + In[8] +
+                                           In[9] + In[10] + In[11] + In[12] + In[13] + In[14] + In[15]));
+    float norm = 1.f;
+    if (fabs(avg) > 1.f) {
+        float a = (float)(1. / avg);
+        for (int k = 0; k < 16; k++) In[k] *= a;
+        norm = avg;
+    }
+    int low = 0;
+    for (int k = 0; k < 32 && !low; k++) low = (cell[k] == 0.f);
+    float F[2], FI;
+    if (!low) {
+        double a[16];
+        orc_bicubic(cell, a); F[0] = (float)orc_bicubic_eval(a, xr, zr);
+        orc_bicubic(cell + 16, a); F[1] = (float)orc_bicubic_eval(a, xr, zr);
+        orc_bicubic(In, a);
+        double b = orc_bicubic_eval(a, xr, zr);
+        FI = (float)((b > 0.) ? b : 0.);
+    } else {
+        if (ixRel < 0) ixRel = 0;
+        if (ixRel > 2) ixRel = 2;
+        if (izRel < 0) izRel = 0;
+        if (izRel > 2) izRel = 2;
+        double a[4];
+        double xz = xr * zr;
+        orc_bilinear(cell, ixRel, izRel, a); F[0] = (float)(a[3] * xz + a[1] * xr + a[2] * zr + a[0]);
+        orc_bilinear(cell + 16, ixRel, izRel, a); F[1] = (float)(a[3] * xz + a[1] * xr + a[2] * zr + a[0]);
+        orc_bilinear(In, ixRel, izRel, a);
+        double b = a[3] * xz + a[1] * xr + a[2] * zr + a[0];
+        FI = (float)((b > 0.) ? b : 0.);
+    }
+    FI *= norm;
+    /* sroptelm.h:688-697 */
+    float appI = F[0] * F[0] + F[1] * F[1];
+    if (appI != 0.f) {
+        float f = (float)sqrt(FI / appI);
+        F[0] *= f; F[1] *= f;
+    }
+    out[0] = F[0]; out[1] = F[1];
+}
+
+/* Fills the resampled wavefront wNew (mesh and zeroed arrays prepared by the
+ * caller from orc_resize_mesh) from wOld; the old arrays are modified in
+ * place by the phase treatment, as in the reference. */
+ORC_API int orc_resize_fill(SRWLWfr* wOld, SRWLWfr* wNew, const double* par, const double* meshOut)
+{
+    orc_mesh3 mo = orc_mesh_of(wOld), mn = orc_mesh_of(wNew);
+    if (mo.nx < 4 || mo.nz < 4) return SRWL_INCORRECT_PARAM_FOR_RESIZE;
+    long ix0 = (long)meshOut[6], ix1 = (long)meshOut[7], iz0 = (long)meshOut[8], iz1 = (long)meshOut[9];
+    long long perX = mo.ne * 2, perZo = perX * mo.nx, perZn = perX * mn.nx;
+    const float* oX = (const float*)wOld->arEx;
+    const float* oZ = (const float*)wOld->arEy;
+    float* nX = (float*)wNew->arEx;
+    float* nZ = (float*)wNew->arEy;
+    /* sroptelm.cpp:3040-3046: range-only enlargement copies the old points */
+    int onlyLarger = (par[2] == 1. && par[4] == 1. && par[1] >= 1. && par[3] >= 1.) &&
+                     (fabs(mo.xStep - mn.xStep) < 1.e-05 && fabs(mo.zStep - mn.zStep) < 1.e-05);
+    if (onlyLarger) {
+        for (long ie = 0; ie < mn.ne; ie++)
+            for (long iz = iz0; iz <= iz1; iz++) {
+                double zAbs = mn.zStart + iz * mn.zStep;
+                long izo = (long)((zAbs - mo.zStart) * (1. / mo.zStep) + 1.E-08);
+                for (long ix = ix0; ix <= ix1; ix++) {
+                    double xAbs = mn.xStart + ix * mn.xStep;
+                    long ixo = (long)((xAbs - mo.xStart) * (1. / mo.xStep) + 1.E-08);
+                    long long on = iz * perZn + ix * perX + 2 * ie, oo = izo * perZo + ixo * perX + 2 * ie;
+                    if (nX) { nX[on] = oX[oo]; nX[on + 1] = oX[oo + 1]; }
+                    if (nZ) { nZ[on] = oZ[oo]; nZ[on + 1] = oZ[oo + 1]; }
+                }
+            }
+        return 0;
+    }
+    int fx = fabs(wOld->dRx) < 0.4 * fabs(wOld->Rx), fz = fabs(wOld->dRy) < 0.4 * fabs(wOld->Ry);
+    int treated = fx || fz;
+    if (treated) orc_treat_quad_phase(wOld, 0, fx, fz);
+    double ixs = 1. / mo.xStep, izs = 1. / mo.zStep;
+    for (long ie = 0; ie < mn.ne; ie++)
+        for (long iz = iz0; iz <= iz1; iz++) {
+            double zAbs = mn.zStart + iz * mn.zStep;
+            int izc = (int)((zAbs - mo.zStart) * izs + 1.E-06);
+            double zRel = zAbs - (mo.zStart + izc * mo.zStep);
+            int izSt;
+            if (izc == mo.nz - 1) { izSt = izc - 3; zRel += 2. * mo.zStep; }
+            else if (izc == mo.nz - 2) { izSt = izc - 2; zRel += mo.zStep; }
+            else if (izc == 0) { izSt = izc; zRel -= mo.zStep; }
+            else izSt = izc - 1;
+            zRel *= izs;
+            for (long ix = ix0; ix <= ix1; ix++) {
+                double xAbs = mn.xStart + ix * mn.xStep;
+                int ixc = (int)((xAbs - mo.xStart) * ixs + 1.E-06);
+                double xRel = xAbs - (mo.xStart + ixc * mo.xStep);
+                int ixSt;
+                if (ixc == mo.nx - 1) { ixSt = ixc - 3; xRel += 2. * mo.xStep; }
+                else if (ixc == mo.nx - 2) { ixSt = ixc - 2; xRel += mo.xStep; }
+                else if (ixc == 0) { ixSt = ixc; xRel -= mo.xStep; }
+                else ixSt = ixc - 1;
+                xRel *= ixs;
+                long long oo = izSt * perZo + ixSt * perX + 2 * ie, on = iz * perZn + ix * perX + 2 * ie;
+                float cell[32];
+                if (nX) { orc_cell(oX + oo, perX, perZo, cell); orc_resample_point(cell, ixc - ixSt, izc - izSt, xRel, zRel, nX + on); }
+                if (nZ) { orc_cell(oZ + oo, perX, perZo, cell); orc_resample_point(cell, ixc - ixSt, izc - izSt, xRel, zRel, nZ + on); }
+            }
+        }
+    if (treated) {
+        wNew->Rx = wOld->Rx; wNew->Ry = wOld->Ry; wNew->dRx = wOld->dRx; wNew->dRy = wOld->dRy;
+        wNew->xc = wOld->xc; wNew->yc = wOld->yc;
+        orc_treat_quad_phase(wNew, 1, fx, fz);
+    }
+    return 0;
+}

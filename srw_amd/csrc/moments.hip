@@ -9,35 +9,17 @@
 //
 // One 256-thread workgroup per slice.  The slice is read three times: row
 // sums of the intensity, column sums inside the row limits, and the moment
-// sums, where each thread walks whole rows so the backward x difference uses
-// the value it just formed.  The phase-treated field is formed on the fly;
-// the stored field is not modified.
+// sums; rows are contiguous in x (perX = 1) for coalesced reads.  The
+// phase-treated field is formed on the fly; the stored field is not
+// modified.
 #include "srw_moments.h"
+#include "srw_optics.h"
 
 #include <math.h>
 
 namespace {
 
 #define SRWM_T 256
-
-__device__ __forceinline__ void cos_sin_optics(double x, float& c, float& s)
-{
-    const double HalfPI = 1.5707963267949, PI = 3.141592653590, TwoPI = 6.2831853071796;
-    const double ThreePIdTwo = 4.7123889803847, One_dTwoPI = 0.1591549430919;
-    if (x < -1.E+08 || x > 1.E+08) { c = (float)cos(x); s = (float)sin(x); return; }
-    x -= TwoPI * (double)(long long)(x * One_dTwoPI);
-    if (x < 0.) x += TwoPI;
-    bool neg = false;
-    if (x > ThreePIdTwo) x -= TwoPI;
-    else if (x > HalfPI) { x -= PI; neg = true; }
-    double q = x * x;
-    float cc = (float)(1. + q * (-0.5 + q * (0.041666666666667 + q * (-0.0013888888888889 +
-                       q * (0.000024801587301587 + q * -2.755731922E-07)))));
-    float ss = (float)(x * (1. + q * (-0.16666666666667 + q * (0.0083333333333333 + q * (-0.0001984126984127 +
-                       q * (2.755731922E-06 + q * -2.505210839E-08))))));
-    c = neg ? -cc : cc;
-    s = neg ? -ss : ss;
-}
 
 __device__ __forceinline__ double intensity(const srwm_mslice& S, long long o)
 {   // total polarisation, as the float extraction of the reference
@@ -75,7 +57,7 @@ __device__ __forceinline__ float2 treated(const float2* p, long long o, const tr
     if (t.fx) { double x = t.x0 + ix * t.xStep; ph += t.cRxE * x * x; }
     float c, s;
     cos_sin_optics(ph, c, s);
-    return make_float2((float)((double)a.x * c - (double)a.y * s), (float)((double)a.x * s + (double)a.y * c));
+    return rotate_f(a, c, s);
 }
 
 __global__ void __launch_bounds__(SRWM_T) k_rad_moments(const srwm_mslice* __restrict__ slices, double* work,
@@ -156,36 +138,48 @@ __global__ void __launch_bounds__(SRWM_T) k_rad_moments(const srwm_mslice* __res
     const double kx = 0.5 * (S.xStep * ((1. / lam4) / rx)), kz = 0.5 * (S.zStep * ((1. / lam4) / rz));
     const double kx2 = kx * kx, kz2 = kz * kz;
     treat_ctx tc;
-    const double Const = 3.14159265358979 * 1.E+06 / 1.23984186;
-    tc.cRxE = -(Const / S.Rx) * S.ePh; tc.cRzE = -(Const / S.Rz) * S.ePh;
+    tc.cRxE = -(SRW_QUAD_CONST / S.Rx) * S.ePh; tc.cRzE = -(SRW_QUAD_CONST / S.Rz) * S.ePh;
     tc.x0 = S.xStart - S.xc; tc.z0 = S.zStart - S.zc; tc.xStep = S.xStep; tc.zStep = S.zStep;
     tc.fx = S.fx; tc.fz = S.fz;
     const bool any = S.fx || S.fz;
     double acc[SRWM_NSUM];
     for (int k = 0; k < SRWM_NSUM; k++) acc[k] = 0.;
-    for (int iz = t; iz < nz; iz += SRWM_T) {
+    // one wave per row, lanes across x: the backward x difference comes from
+    // the neighbouring lane (lane 0 from the previous 64-point chunk)
+    for (int iz = w; iz < nz; iz += SRWM_T / 64) {
         const bool inZ = iz >= L2 && iz <= L3;
         const double z = S.zStart + iz * S.zStep;
-        double row[SRWM_NSUM];
-        for (int k = 0; k < SRWM_NSUM; k++) row[k] = 0.;
-        float2 prev[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-        for (int ix = 0; ix < nx; ix++) {
+        const double rAll = (iz == 0 || iz == nz - 1) ? 0.5 : 1.;
+        const double rDer = (iz == 1) ? 0.5 * rAll : rAll;
+        float2 carry[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+        for (int base = 0; base < nx; base += 64) {
+            const int ix = base + lane;
+            const bool valid = ix < nx;
+            const long long o = iz * S.perZ + (long long)ix * S.perX;
             const bool in = inZ && ix >= L0 && ix <= L1;
-            const long long o = iz * S.perZ + ix * S.perX;
             const double x = S.xStart + ix * S.xStep;
-            double ff[SRWM_NSUM];
+            float2 e[2], l[2];
             for (int c = 0; c < 2; c++) {
                 const float2* p = c == 0 ? S.ex : S.ez;
-                double* f = ff + 11 * c;
-                float2 e = p ? treated(p, o, tc, ix, iz, any) : make_float2(0.f, 0.f);
-                double re = e.x, im = e.y;
+                e[c] = (valid && p) ? treated(p, o, tc, ix, iz, any) : make_float2(0.f, 0.f);
+                float lx = __shfl_up(e[c].x, 1, 64), ly = __shfl_up(e[c].y, 1, 64);
+                l[c] = (lane == 0) ? carry[c] : make_float2(lx, ly);
+                carry[c] = make_float2(__shfl(e[c].x, 63, 64), __shfl(e[c].y, 63, 64));
+            }
+            if (!valid) continue;
+            const double wAll = (ix == 0 || ix == nx - 1) ? 0.5 : 1.;
+            const double wDer = (ix == 1) ? 0.5 * wAll : wAll;
+            for (int c = 0; c < 2; c++) {
+                const float2* p = c == 0 ? S.ex : S.ez;
+                double f[11];
+                double re = e[c].x, im = e[c].y;
                 f[0] = re * re + im * im;
                 f[1] = x * f[0];
                 f[3] = z * f[0];
                 f[5] = in ? x * f[1] : 0.;
                 f[8] = in ? z * f[3] : 0.;
                 if (ix > 0) {
-                    double dRe = re - (double)prev[c].x, dIm = im - (double)prev[c].y, cr = dIm * re - dRe * im;
+                    double dRe = re - (double)l[c].x, dIm = im - (double)l[c].y, cr = dIm * re - dRe * im;
                     f[2] = cr + kx * x * f[0];
                     f[6] = in ? x * f[2] : 0.;
                     f[7] = in ? (dRe * dRe + dIm * dIm) + cr * kx * x + kx2 * x * x * f[0] : 0.;
@@ -197,22 +191,12 @@ __global__ void __launch_bounds__(SRWM_T) k_rad_moments(const srwm_mslice* __res
                     f[9] = in ? z * f[4] : 0.;
                     f[10] = in ? (dRe * dRe + dIm * dIm) + cr * kz * z + kz2 * z * z * f[0] : 0.;
                 } else { f[4] = 0.; f[9] = 0.; f[10] = 0.; }
-                prev[c] = e;
+                for (int k = 0; k < 11; k++) {
+                    double wx = (k == 2 || k == 6 || k == 7) ? wDer : wAll;
+                    double wz = (k == 4 || k == 9 || k == 10) ? rDer : rAll;
+                    acc[11 * c + k] += (wx * wz) * f[k];
+                }
             }
-            double wAll = (ix == 0 || ix == nx - 1) ? 0.5 : 1.;
-            double wDer = (ix == 1) ? 0.5 * wAll : wAll;
-            for (int k = 0; k < SRWM_NSUM; k++) {
-                int kk = k % 11;
-                bool der = (kk == 2 || kk == 6 || kk == 7);
-                row[k] += (der ? wDer : wAll) * ff[k];
-            }
-        }
-        double rAll = (iz == 0 || iz == nz - 1) ? 0.5 : 1.;
-        double rDer = (iz == 1) ? 0.5 * rAll : rAll;
-        for (int k = 0; k < SRWM_NSUM; k++) {
-            int kk = k % 11;
-            bool der = (kk == 4 || kk == 9 || kk == 10);
-            acc[k] += (der ? rDer : rAll) * row[k];
         }
     }
     // block reduction of the 22 sums

@@ -148,6 +148,18 @@ using namespace srwamd;
 
 extern "C" {
 
+// Resize instructions of one arProp entry (sroptcnt.cpp:183-211: [5..8] =
+// pxm, pxd, pzm, pzd, [9..11] centring); par for resize_plan().
+static bool prop_resize_par(const double* pp, int npp, double* par)
+{
+    if (pp == nullptr) return false;
+    par[0] = (npp > 4) ? pp[4] : 0.;
+    for (int k = 1; k <= 4; k++) par[k] = pp[4 + k];
+    par[5] = par[6] = 0.5;
+    if (npp > 9 && pp[9] != 0) par[5] = -1.;   // centre shifts are not supported
+    return !(par[1] == 1. && par[2] == 1. && par[3] == 1. && par[4] == 1.);
+}
+
 SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, char** arID, SRWLRadMesh* arIM,
                                     char** arI, double* arParGPU)
 {
@@ -157,23 +169,15 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
     if (pWfr->presCA != 0 || pWfr->presFT != 0) return SRWL_AMD_ERR_UNSUPPORTED;
     if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
     long nx = pWfr->mesh.nx, nz = pWfr->mesh.ny, ne = pWfr->mesh.ne;
-    int lx = log2_exact(nx), lz = log2_exact(nz);
-    if (lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_ERR_IN_FFT;
+    const long nx0 = nx, nz0 = nz;
+    if (nx < 2 || nz < 2 || ne < 1) return SRWL_INCORRECT_WFR_STRUCT;
+    auto propN = [&](int i) -> int { return (pOpt->arPropN != nullptr) ? (int)(unsigned char)pOpt->arPropN[i] : 9; };
     for (int i = 0; i < pOpt->nElem; i++) {
         if (std::strcmp(pOpt->arOptTypes[i], "drift") != 0) return SRWL_AMD_ERR_UNSUPPORTED;
         int mode, sub;
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
         int rc = drift_mode_from_prop(pp, mode, sub);
         if (rc) return rc;
-        if (pp && (std::fabs(pp[5] - 1) > 1e-4 || std::fabs(pp[6] - 1) > 1e-4 || std::fabs(pp[7] - 1) > 1e-4 ||
-                   std::fabs(pp[8] - 1) > 1e-4))
-            return SRWL_AMD_ERR_UNSUPPORTED;   // resize instructions
-    }
-    if (pOpt->nProp > pOpt->nElem) {
-        const double* pp = pOpt->arProp[pOpt->nElem];
-        if (pp && (std::fabs(pp[5] - 1) > 1e-4 || std::fabs(pp[6] - 1) > 1e-4 || std::fabs(pp[7] - 1) > 1e-4 ||
-                   std::fabs(pp[8] - 1) > 1e-4))
-            return SRWL_AMD_ERR_UNSUPPORTED;
     }
     int rc = fft_ready();
     if (rc) return rc;
@@ -181,16 +185,18 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
     long long nxy = (long long)nx * nz;
     long long n = nxy * ne;
     float2* dIn = (float2*)g_ctx.buf.W0.get(sizeof(float2) * 2 * n);
-    float2* dS = (float2*)g_ctx.buf.W1.get(sizeof(float2) * 2 * n);
+    DBuf* bufCur = &g_ctx.buf.W1;
+    DBuf* bufSpare = &g_ctx.buf.RsN;
+    float2* dS = (float2*)bufCur->get(sizeof(float2) * 2 * n);
     srwf_slice* dSp = (srwf_slice*)g_ctx.buf.W2.get(sizeof(srwf_slice) * 2 * ne);
     if (!dIn || !dS || !dSp) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     HIPCHK(hipMemcpyAsync(dIn, pWfr->arEx, sizeof(float2) * n, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dIn + n, pWfr->arEy, sizeof(float2) * n, hipMemcpyHostToDevice, st));
     // slice-major layout: slice (2*ie + comp) holds [iy][ix]
-    for (int c = 0; c < 2; c++) {
-        if (ne == 1) { HIPCHK(hipMemcpyAsync(dS + c * nxy, dIn + c * n, sizeof(float2) * nxy, hipMemcpyDeviceToDevice, st)); continue; }
-    }
-    if (ne > 1) {
+    if (ne == 1) {
+        for (int c = 0; c < 2; c++)
+            HIPCHK(hipMemcpyAsync(dS + c * nxy, dIn + c * n, sizeof(float2) * nxy, hipMemcpyDeviceToDevice, st));
+    } else {
         // gather each component into [ie][iy][ix], then interleave components per energy
         float2* tmp = (float2*)g_ctx.buf.W3.get(sizeof(float2) * 2 * n);
         if (!tmp) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
@@ -215,14 +221,42 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         if (!momZ) momZ = tmpMom.data() + 11 * ne;
     }
     struct MomCtx { SRWLWfr* w; const float2* d; long long nxy, nx; double *mx, *mz; } mctx;
-    mctx.w = pWfr; mctx.d = dS; mctx.nxy = nxy; mctx.nx = nx; mctx.mx = momX; mctx.mz = momZ;
+    mctx.w = pWfr; mctx.mx = momX; mctx.mz = momZ;
     auto recompute = [](void* c) -> int {
         MomCtx* m = (MomCtx*)c;   // slice-major device field: slice 2*ie + component, [iz][ix]
         return wfr_moments(*m->w, m->d, m->d + m->nxy, 2 * m->nxy, 1, m->nx, m->mx, m->mz);
     };
+    // resampling of the device field (sroptcnt.cpp:314-327 before an element, 408-423 after the last)
+    auto resize_device = [&](const double* pp, int npp) -> int {
+        double par[7];
+        if (!prop_resize_par(pp, npp, par)) return 0;
+        ResizePlan P;
+        int r = resize_plan(*pWfr, par, 7, P);
+        if (r == -1) return 0;
+        if (r) return r;
+        long long nxyN = (long long)P.x.n * P.z.n;
+        float2* dN = (float2*)bufSpare->get(sizeof(float2) * 2 * nxyN * ne);
+        if (!dN) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        srwr_layout lo = {1, nx, 2 * nxy}, ln = {1, P.x.n, 2 * nxyN};
+        r = device_resize(*pWfr, P, dS, dS + nxy, lo, dN, dN + nxyN, ln, st);
+        if (r) return r;
+        std::swap(bufCur, bufSpare);
+        dS = dN;
+        nx = P.x.n; nz = P.z.n; nxy = nxyN; n = nxy * ne;
+        pWfr->mesh.nx = nx; pWfr->mesh.ny = nz;
+        pWfr->mesh.xStart = P.x.start; pWfr->mesh.xFin = P.x.start + (nx - 1) * P.x.step;
+        pWfr->mesh.yStart = P.z.start; pWfr->mesh.yFin = P.z.start + (nz - 1) * P.z.step;
+        xStep = P.x.step; zStep = P.z.step;
+        return 0;
+    };
     for (int i = 0; i < pOpt->nElem; i++) {
         const SRWLOptD* d = (const SRWLOptD*)pOpt->arOpt[i];
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
+        rc = resize_device(pp, pp ? propN(i) : 0);
+        if (rc) return rc;
+        int lx = log2_exact(nx), lz = log2_exact(nz);
+        if (lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_ERR_IN_FFT;   // power-of-2 FFT sizes only
+        mctx.d = dS; mctx.nxy = nxy; mctx.nx = nx;
         int mode, sub;
         drift_mode_from_prop(pp, mode, sub);
         DriftGeom g;
@@ -267,11 +301,23 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         pWfr->Ry += d->L;
         if (pWfr->arElecPropMatr) drift_4x4(pWfr->arElecPropMatr, d->L);
     }
+    if (pOpt->arProp && pOpt->nProp > pOpt->nElem) {
+        rc = resize_device(pOpt->arProp[pOpt->nElem], propN(pOpt->nElem));
+        if (rc) return rc;
+    }
+    if (nx != nx0 || nz != nz0) {
+        // the caller's arrays are reallocated for the new mesh (srradstr.cpp:760-790)
+        if (g_wfrModif == nullptr) return SRWL_WFR_EXT_FUNC_NOT_DEFINED;
+        if ((*g_wfrModif)(2, pWfr, 0)) return SRWL_WFR_EXT_MODIF_FAILED;
+        if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_WFR_EXT_MODIF_FAILED;
+    }
     if (ne == 1) {
         HIPCHK(hipMemcpyAsync(pWfr->arEx, dS, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(pWfr->arEy, dS + nxy, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
     } else {
         float2* tmp = (float2*)g_ctx.buf.W3.get(sizeof(float2) * 2 * n);
+        dIn = (float2*)g_ctx.buf.W0.get(sizeof(float2) * 2 * n);
+        if (!tmp || !dIn) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
         for (long ie = 0; ie < ne; ie++)
             for (int c = 0; c < 2; c++)
                 HIPCHK(hipMemcpyAsync(tmp + c * n + ie * nxy, dS + (2 * ie + c) * nxy, sizeof(float2) * nxy,

@@ -1,0 +1,220 @@
+// resize.hip - resampling of wavefront slices onto a new transverse mesh
+// (the interpolation of SRW's RadResizeCore) for gfx950.
+//
+// Reference algorithms (paths relative to the reference source root):
+//   range-only enlargement   cpp/src/core/sroptelm.cpp:2850-2920
+//   cell interpolation       cpp/src/core/sroptelm.cpp:3038-3300, sroptelm.h:460-760
+//   cell constants           cpp/src/core/srstraux.h:1588-1683
+//   phase treatment          cpp/src/core/sroptelm.cpp:4774-4990
+//
+// One thread per new element (ix, iz, ie), energy fastest so that threads of
+// a wave read neighbouring words of the interleaved layout.  Each thread
+// loads the 4x4 cell of both components (64 floats, L1/L2 resident across
+// neighbouring threads), builds the bicubic forms of Re, Im and of the
+// intensity, and rescales the complex value to the interpolated intensity.
+// The cell sums keep the reference's float arithmetic (products and sums of
+// floats, then double), so values match to rounding.
+#include "srw_optics.h"
+#include "srw_resize.h"
+
+namespace {
+
+__global__ void k_phase(float2* ex, float2* ez, srwr_phase P)
+{
+    long long n = (long long)P.nx * P.nz * P.ne;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ie = (int)(i % P.ne);
+    long long p = i / P.ne;
+    int ix = (int)(p % P.nx), iz = (int)(p / P.nx);
+    double ePh = P.eStart + P.eStep * ie;
+    double ph = 0.;
+    if (P.fz) { double z = (P.zStart - P.zc) + iz * P.zStep; ph = P.sign * (SRW_QUAD_CONST / P.Rz) * ePh * z * z; }
+    if (P.fx) { double x = (P.xStart - P.xc) + ix * P.xStep; ph += P.sign * (SRW_QUAD_CONST / P.Rx) * ePh * x * x; }
+    float c, s;
+    cos_sin_optics(ph, c, s);
+    long long o = ie * P.l.perE + iz * P.l.perZ + ix * P.l.perX;
+    if (ex) ex[o] = rotate_f(ex[o], c, s);
+    if (ez) ez[o] = rotate_f(ez[o], c, s);
+}
+
+// Bicubic coefficients a[4*px + pz] of a 4x4 float cell F[4*iz + ix]
+// (sroptelm.h:526-546): sums in float, scaled in double.
+__device__ __forceinline__ void bicubic(const float* F, double* a)
+{
+#pragma clang fp contract(off)
+    const float f00 = F[0], f10 = F[1], f20 = F[2], f30 = F[3];
+    const float f01 = F[4], f11 = F[5], f21 = F[6], f31 = F[7];
+    const float f02 = F[8], f12 = F[9], f22 = F[10], f32 = F[11];
+    const float f03 = F[12], f13 = F[13], f23 = F[14], f33 = F[15];
+    const double c6 = 0.1666666667, c36 = 0.027777777778, c12 = 0.083333333333;
+    a[0] = f11;
+    a[1] = (-2 * f10 - 3 * f11 + 6 * f12 - f13) * c6;
+    a[2] = (f10 + f12 - 2 * f11) * 0.5;
+    a[3] = (f13 - f10 + 3 * (f11 - f12)) * c6;
+    a[4] = (-2 * f01 - 3 * f11 + 6 * f21 - f31) * c6;
+    a[5] = (4 * f00 + 6 * (f01 + f10 - f23 - f32) - 12 * (f02 + f20) + 2 * (f03 + f30) + 9 * f11 - 18 * (f12 + f21) +
+            3 * (f13 + f31) + 36 * f22 + f33) * c36;
+    a[6] = (-2 * (f00 + f02 - f31) + 4 * f01 - 3 * (f10 + f12) + 6 * (f11 + f20 + f22) - 12 * f21 - f30 - f32) * c12;
+    a[7] = (2 * (f00 - f03) + 6 * (-f01 + f02 - f20 + f23) + 3 * (f10 - f13 - f31 + f32) + 9 * (f12 - f11) +
+            18 * (f21 - f22) + f30 - f33) * c36;
+    a[8] = (f01 + f21 - 2 * f11) * 0.5;
+    a[9] = (2 * (-f00 + f13 - f20) - 3 * (f21 + f01) + 6 * (f02 + f11 + f22) + 4 * f10 - 12 * f12 - f23 - f03) * c12;
+    a[10] = (f00 + f02 + f22 + f20 - 2 * (f01 + f10 + f12 + f21) + 4 * f11) * 0.25;
+    a[11] = (-f00 + f03 - f20 + f23 + 3 * (f01 - f02 + f21 - f22) + 2 * (f10 - f13) + 6 * (f12 - f11)) * c12;
+    a[12] = (f31 - f01 + 3 * (f11 - f21)) * 0.16666666667;
+    a[13] = (2 * (f00 - f30) + 3 * (f01 - f13 + f23 - f31) + 6 * (-f02 - f10 + f20 + f32) + 9 * (f21 - f11) +
+             18 * (f12 - f22) + f03 - f33) * c36;
+    a[14] = (f30 + f32 - f00 - f02 + 2 * (f01 - f31) + 3 * (f10 + f12 - f20 - f22) + 6 * (f21 - f11)) * c12;
+    a[15] = (f00 - f03 - f30 + f33 + 3 * (-f01 + f02 - f10 + f13 + f20 - f23 + f31 - f32) +
+             9 * (f11 - f12 - f21 + f22)) * c36;
+}
+
+__device__ __forceinline__ double bicubic_eval(const double* a, double x, double z)
+{   // sroptelm.h:594-606 term order
+#pragma clang fp contract(off)
+    double xE2 = x * x, xz = x * z, zE2 = z * z;
+    double xE3 = xE2 * x, xE2z = xE2 * z, xzE2 = x * zE2, zE3 = zE2 * z, xE2zE2 = xE2 * zE2;
+    double xE3z = xE3 * z, xE3zE2 = xE3 * zE2, xE3zE3 = xE3 * zE3, xE2zE3 = xE2 * zE3, xzE3 = x * zE3;
+    return a[15] * xE3zE3 + a[14] * xE3zE2 + a[13] * xE3z + a[12] * xE3 + a[11] * xE2zE3 + a[10] * xE2zE2 +
+           a[9] * xE2z + a[8] * xE2 + a[7] * xzE3 + a[6] * xzE2 + a[5] * xz + a[4] * x + a[3] * zE3 + a[2] * zE2 +
+           a[1] * z + a[0];
+}
+
+__device__ __forceinline__ double bilinear_eval(const float* F, int ix, int iz, double x, double z)
+{   // sroptelm.h:563-569, 650-657
+#pragma clang fp contract(off)
+    float f00 = F[4 * iz + ix], f10 = F[4 * iz + ix + 1], f01 = F[4 * iz + 4 + ix], f11 = F[4 * iz + 4 + ix + 1];
+    double a0 = f00, a10 = 1. * (f10 - f00), a01 = 1. * (f01 - f00), a11 = 1. * (f00 - f01 - f10 + f11);
+    return a11 * (x * z) + a10 * x + a01 * z + a0;
+}
+
+// One component at relative cell position (xr, zr); re/im: the 16 cell values.
+__device__ __forceinline__ float2 resample_point(const float* re, const float* im, int ixRel, int izRel, double xr,
+                                                 double zr)
+{
+    float In[16];
+    for (int k = 0; k < 16; k++) In[k] = __fadd_rn(__fmul_rn(re[k], re[k]), __fmul_rn(im[k], im[k]));
+    float sum = In[0];
+    for (int k = 1; k < 16; k++) sum = __fadd_rn(sum, In[k]);
+    float avg = (float)(0.0625 * (double)sum);
+    float norm = 1.f;
+    if (fabsf(avg) > 1.f) {
+        float a = (float)(1. / avg);
+        for (int k = 0; k < 16; k++) In[k] = __fmul_rn(In[k], a);
+        norm = avg;
+    }
+    bool low = false;
+    for (int k = 0; k < 16; k++) low = low || (re[k] == 0.f) || (im[k] == 0.f);
+    float F0, F1, FI;
+    if (!low) {
+        double a[16];
+        bicubic(re, a); F0 = (float)bicubic_eval(a, xr, zr);
+        bicubic(im, a); F1 = (float)bicubic_eval(a, xr, zr);
+        bicubic(In, a);
+        double b = bicubic_eval(a, xr, zr);
+        FI = (float)((b > 0.) ? b : 0.);
+    } else {
+        int ix = ixRel < 0 ? 0 : (ixRel > 2 ? 2 : ixRel);
+        int iz = izRel < 0 ? 0 : (izRel > 2 ? 2 : izRel);
+        F0 = (float)bilinear_eval(re, ix, iz, xr, zr);
+        F1 = (float)bilinear_eval(im, ix, iz, xr, zr);
+        double b = bilinear_eval(In, ix, iz, xr, zr);
+        FI = (float)((b > 0.) ? b : 0.);
+    }
+    FI = __fmul_rn(FI, norm);
+    float appI = __fadd_rn(__fmul_rn(F0, F0), __fmul_rn(F1, F1));   // sroptelm.h:688-697
+    if (appI != 0.f) {
+        float f = (float)sqrt((double)__fdiv_rn(FI, appI));
+        F0 = __fmul_rn(F0, f); F1 = __fmul_rn(F1, f);
+    }
+    return make_float2(F0, F1);
+}
+
+__global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, const float2* __restrict__ oez,
+                                                float2* nex, float2* nez, srwr_params P)
+{
+    long long n = (long long)P.nxn * P.nzn * P.ne;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ie = (int)(i % P.ne);
+    long long p = i / P.ne;
+    int ix = (int)(p % P.nxn), iz = (int)(p / P.nxn);
+    long long on = ie * P.ln.perE + iz * P.ln.perZ + ix * P.ln.perX;
+    float2 vx = make_float2(0.f, 0.f), vz = make_float2(0.f, 0.f);
+    if (ix >= P.ix0 && ix <= P.ix1 && iz >= P.iz0 && iz <= P.iz1) {
+        double zAbs = P.zStartN + iz * P.zStepN, xAbs = P.xStartN + ix * P.xStepN;
+        if (P.onlyLarger) {
+            long izo = (long)((zAbs - P.zStartO) * (1. / P.zStepO) + 1.E-08);
+            long ixo = (long)((xAbs - P.xStartO) * (1. / P.xStepO) + 1.E-08);
+            izo = izo < 0 ? 0 : (izo > P.nzo - 1 ? P.nzo - 1 : izo);
+            ixo = ixo < 0 ? 0 : (ixo > P.nxo - 1 ? P.nxo - 1 : ixo);
+            long long oo = ie * P.lo.perE + izo * P.lo.perZ + ixo * P.lo.perX;
+            if (oex) vx = oex[oo];
+            if (oez) vz = oez[oo];
+        } else {
+            double izs = 1. / P.zStepO, ixs = 1. / P.xStepO;
+            int izc = (int)((zAbs - P.zStartO) * izs + 1.E-06);
+            double zRel = zAbs - (P.zStartO + izc * P.zStepO);
+            int izSt;
+            if (izc == P.nzo - 1) { izSt = izc - 3; zRel += 2. * P.zStepO; }
+            else if (izc == P.nzo - 2) { izSt = izc - 2; zRel += P.zStepO; }
+            else if (izc == 0) { izSt = izc; zRel -= P.zStepO; }
+            else izSt = izc - 1;
+            zRel *= izs;
+            int ixc = (int)((xAbs - P.xStartO) * ixs + 1.E-06);
+            double xRel = xAbs - (P.xStartO + ixc * P.xStepO);
+            int ixSt;
+            if (ixc == P.nxo - 1) { ixSt = ixc - 3; xRel += 2. * P.xStepO; }
+            else if (ixc == P.nxo - 2) { ixSt = ixc - 2; xRel += P.xStepO; }
+            else if (ixc == 0) { ixSt = ixc; xRel -= P.xStepO; }
+            else ixSt = ixc - 1;
+            xRel *= ixs;
+            // a full 4x4 cell inside the old mesh (the host requires nx, nz >= 4)
+            izSt = izSt < 0 ? 0 : (izSt > P.nzo - 4 ? P.nzo - 4 : izSt);
+            ixSt = ixSt < 0 ? 0 : (ixSt > P.nxo - 4 ? P.nxo - 4 : ixSt);
+            long long base = ie * P.lo.perE + izSt * P.lo.perZ + ixSt * P.lo.perX;
+            float re[16], im[16];
+            if (oex) {
+                for (int j = 0; j < 4; j++)
+                    for (int k = 0; k < 4; k++) {
+                        float2 v = oex[base + j * P.lo.perZ + k * P.lo.perX];
+                        re[4 * j + k] = v.x; im[4 * j + k] = v.y;
+                    }
+                vx = resample_point(re, im, ixc - ixSt, izc - izSt, xRel, zRel);
+            }
+            if (oez) {
+                for (int j = 0; j < 4; j++)
+                    for (int k = 0; k < 4; k++) {
+                        float2 v = oez[base + j * P.lo.perZ + k * P.lo.perX];
+                        re[4 * j + k] = v.x; im[4 * j + k] = v.y;
+                    }
+                vz = resample_point(re, im, ixc - ixSt, izc - izSt, xRel, zRel);
+            }
+        }
+    }
+    if (nex) nex[on] = vx;
+    if (nez) nez[on] = vz;
+}
+
+}  // namespace
+
+extern "C" {
+int srwr_launch_phase(float2* ex, float2* ez, const srwr_phase& P, hipStream_t st)
+{
+    long long n = (long long)P.nx * P.nz * P.ne;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ex, ez, P);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwr_launch_resize(const float2* oex, const float2* oez, float2* nex, float2* nez, const srwr_params& P,
+                       hipStream_t st)
+{
+    long long n = (long long)P.nxn * P.nzn * P.ne;
+    if (n <= 0) return 0;
+    if (!P.onlyLarger && (P.nxo < 4 || P.nzo < 4)) return -1;
+    hipLaunchKernelGGL(k_resize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, oex, oez, nex, nez, P);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+}

@@ -243,24 +243,45 @@ __global__ void __launch_bounds__(64) k_traj_rk4(const srwd_traj_job* __restrict
 // SRWD_PIC_MAXIT sweeps is recomputed serially by one thread, so every
 // trajectory ends with the serial answer or its converged equivalent.
 // ------------------------------------------------------------------------
-#define SRWD_PIC_THREADS 256
+#define SRWD_PIC_THREADS 512
 #define SRWD_PIC_MAXIT 48
 
+// rk_deriv with the field path fixed at compile time: ONE = 1 for a single
+// undulator with a single harmonic (constants in registers), 0 otherwise.
+template <int ONE>
+__device__ __forceinline__ void rk_deriv_t(const rk_ctx& c, const double* F, double* dF)
+{   // srgtrjdt.h:279-293
+    double xd = F[1], yd = F[3];
+    double zd = radical_one_plus_small(-(c.gEm2 + xd * xd + yd * yd));
+    double B[3];
+    if (ONE) {
+        B[0] = B[1] = B[2] = 0.;
+        double zr = F[4] - c.cz;
+        if (zr >= -c.halfTotTerm && zr <= c.halfTotTerm) harm_addB(c.h, c.halfTotTerm, F[2] - c.cy, zr, B);
+    } else field_B(*c.f, F[0], F[2], F[4], B);
+    dF[0] = xd;
+    dF[1] = c.mult * (yd * B[2] - zd * B[1]);
+    dF[2] = yd;
+    dF[3] = c.mult * (zd * B[0] - xd * B[2]);
+    dF[4] = zd;
+}
+
+template <int ONE>
 __device__ __forceinline__ void rk4_increment(const rk_ctx& c, const double* y, double h, double* D)
 {   // the update of rk4_step, returned instead of added
     double d[5], hh = 0.5 * h, h6 = h / 6., yt[5], dyt[5], dym[5];
-    rk_deriv(c, y, d);
+    rk_deriv_t<ONE>(c, y, d);
     for (int i = 0; i < 5; i++) yt[i] = y[i] + hh * d[i];
-    rk_deriv(c, yt, dyt);
+    rk_deriv_t<ONE>(c, yt, dyt);
     for (int i = 0; i < 5; i++) yt[i] = y[i] + hh * dyt[i];
-    rk_deriv(c, yt, dym);
+    rk_deriv_t<ONE>(c, yt, dym);
     for (int i = 0; i < 5; i++) { yt[i] = y[i] + h * dym[i]; dym[i] += dyt[i]; }
-    rk_deriv(c, yt, dyt);
+    rk_deriv_t<ONE>(c, yt, dyt);
     for (int i = 0; i < 5; i++) D[i] = h6 * (d[i] + dyt[i] + 2. * dym[i]);
 }
 
-// Exclusive block scan of 5 doubles per thread (256 threads = 4 waves).
-__device__ __forceinline__ void block_excl_scan5(const double* v, double* ex, double* lds /* [4][5] */)
+// Exclusive block scan of 5 doubles per thread (SRWD_PIC_THREADS / 64 waves).
+__device__ __forceinline__ void block_excl_scan5(const double* v, double* ex, double* lds /* [waves][5] */)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double inc[5];
@@ -285,7 +306,7 @@ __device__ __forceinline__ void block_excl_scan5(const double* v, double* ex, do
 
 // Block max-reduction of 10 non-negative doubles (5 changes, 5 magnitudes);
 // every thread receives the result.
-__device__ __forceinline__ void block_max10(double* v, double* lds /* [4][10] */)
+__device__ __forceinline__ void block_max10(double* v, double* lds /* [waves][10] */)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k = 0; k < 10; k++) {
@@ -303,33 +324,50 @@ __device__ __forceinline__ void block_max10(double* v, double* lds /* [4][10] */
     __syncthreads();
 }
 
+// Storage of a chain in an iterate buffer.  Step j of a chain with `per`
+// steps per thread is owned by thread j/per; its end point j+1 is kept at
+// (j % per)*T + j/per, so the threads of a wave touch consecutive words, and
+// point 0 at per*T.  A buffer component needs per*T + 1 doubles.
+__device__ __forceinline__ long long pic_at(long long j, long long per)
+{
+    if (j == 0) return per * SRWD_PIC_THREADS;
+    long long s = j - 1;
+    return (s % per) * SRWD_PIC_THREADS + s / per;
+}
+
 // Solves one chain of n points from y0 with step h (component k of point j at
-// buf[k*cap + j]) using the two iterate buffers Ya and Yb, and returns the one
-// that holds the result.  All threads of the block call it with the same
-// arguments.
-__device__ double* picard_chain(const rk_ctx& c, const double* y0, double h, long long n, double* Ya, double* Yb,
-                             long long cap, double* lds)
+// buf[k*cap + pic_at(j, per)]) using the two iterate buffers Ya and Yb, and
+// returns the one that holds the result.  All threads of the block call it
+// with the same arguments; n - 1 <= cap - 1 must hold.
+template <int ONE>
+__device__ __forceinline__ double* picard_chain(const rk_ctx& c, const double* y0, double h, long long n,
+                                                double* Ya, double* Yb, long long cap, double* lds,
+                                                long long& perOut)
 {
     const int t = threadIdx.x;
     const long long nSteps = n - 1;
+    const long long per = nSteps > 0 ? (nSteps + SRWD_PIC_THREADS - 1) / SRWD_PIC_THREADS : 1;
+    perOut = per;
     {   // initial iterate: straight line along the initial derivative
         double d0[5];
-        rk_deriv(c, y0, d0);
-        for (long long j = t; j < n; j += SRWD_PIC_THREADS)
-            for (int k = 0; k < 5; k++) Ya[k * cap + j] = (j == 0) ? y0[k] : y0[k] + (double)j * h * d0[k];
+        rk_deriv_t<ONE>(c, y0, d0);
+        for (long long j = t; j < n; j += SRWD_PIC_THREADS) {
+            long long a = pic_at(j, per);
+            for (int k = 0; k < 5; k++) Ya[k * cap + a] = (j == 0) ? y0[k] : y0[k] + (double)j * h * d0[k];
+        }
     }
     __syncthreads();
     if (nSteps <= 0) return Ya;
-    const long long per = (nSteps + SRWD_PIC_THREADS - 1) / SRWD_PIC_THREADS;
     const long long j0 = t * per, j1 = (j0 + per < nSteps) ? j0 + per : nSteps;
     bool done = false;
     for (int it = 0; it < SRWD_PIC_MAXIT && !done; it++) {
         double tot[5] = {0., 0., 0., 0., 0.};
         for (long long j = j0; j < j1; j++) {
             double y[5], D[5];
-            for (int k = 0; k < 5; k++) y[k] = Ya[k * cap + j];
-            rk4_increment(c, y, h, D);
-            for (int k = 0; k < 5; k++) { Yb[k * cap + j + 1] = D[k]; tot[k] += D[k]; }
+            long long a = pic_at(j, per), b = pic_at(j + 1, per);
+            for (int k = 0; k < 5; k++) y[k] = Ya[k * cap + a];
+            rk4_increment<ONE>(c, y, h, D);
+            for (int k = 0; k < 5; k++) { Yb[k * cap + b] = D[k]; tot[k] += D[k]; }
         }
         double ex[5];
         block_excl_scan5(tot, ex, lds);
@@ -337,15 +375,16 @@ __device__ double* picard_chain(const rk_ctx& c, const double* y0, double h, lon
         double acc[5];
         for (int k = 0; k < 5; k++) acc[k] = y0[k] + ex[k];
         for (long long j = j0; j < j1; j++) {
+            long long b = pic_at(j + 1, per);
             for (int k = 0; k < 5; k++) {
-                acc[k] += Yb[k * cap + j + 1];
-                m[k] = fmax(m[k], fabs(acc[k] - Ya[k * cap + j + 1]));
+                acc[k] += Yb[k * cap + b];
+                m[k] = fmax(m[k], fabs(acc[k] - Ya[k * cap + b]));
                 m[5 + k] = fmax(m[5 + k], fabs(acc[k]));
-                Yb[k * cap + j + 1] = acc[k];
+                Yb[k * cap + b] = acc[k];
             }
         }
-        if (t == 0) for (int k = 0; k < 5; k++) Yb[k * cap] = y0[k];
-        block_max10(m, lds + 20);
+        if (t == 0) for (int k = 0; k < 5; k++) Yb[k * cap + per * SRWD_PIC_THREADS] = y0[k];
+        block_max10(m, lds + 5 * (SRWD_PIC_THREADS / 64));
         double* sw = Ya; Ya = Yb; Yb = sw;
         done = true;
         for (int k = 0; k < 5; k++) done = done && (m[k] <= 1e-15 * m[5 + k]);
@@ -354,10 +393,11 @@ __device__ double* picard_chain(const rk_ctx& c, const double* y0, double h, lon
         // serial recurrence, one thread
         if (t == 0) {
             double y[5], D[5];
-            for (int k = 0; k < 5; k++) { y[k] = y0[k]; Ya[k * cap] = y[k]; }
+            for (int k = 0; k < 5; k++) { y[k] = y0[k]; Ya[k * cap + pic_at(0, per)] = y[k]; }
             for (long long j = 0; j < nSteps; j++) {
-                rk4_increment(c, y, h, D);
-                for (int k = 0; k < 5; k++) { y[k] += D[k]; Ya[k * cap + j + 1] = y[k]; }
+                rk4_increment<ONE>(c, y, h, D);
+                long long b = pic_at(j + 1, per);
+                for (int k = 0; k < 5; k++) { y[k] += D[k]; Ya[k * cap + b] = y[k]; }
             }
         }
         __syncthreads();
@@ -380,52 +420,54 @@ __device__ __forceinline__ void rk_ctx_setup(rk_ctx& c, const srwd_traj_job& J, 
 }
 
 // Stores chain point j of buffer R at output index base + dir*j.
-__device__ __forceinline__ void picard_store(const double* R, long long cap, long long n, long long base, int dir,
-                                             double* oX, double* oXp, double* oY, double* oYp, double* oZ)
+__device__ __forceinline__ void picard_store(const double* R, long long cap, long long per, long long n,
+                                             long long base, int dir, double* oX, double* oXp, double* oY,
+                                             double* oYp, double* oZ)
 {
     for (long long j = threadIdx.x; j < n; j += SRWD_PIC_THREADS) {
-        long long o = base + dir * j;
-        oX[o] = R[j]; oXp[o] = R[cap + j]; oY[o] = R[2 * cap + j]; oYp[o] = R[3 * cap + j];
-        if (oZ) oZ[o] = R[4 * cap + j];
+        long long o = base + dir * j, a = pic_at(j, per);
+        oX[o] = R[a]; oXp[o] = R[cap + a]; oY[o] = R[2 * cap + a]; oYp[o] = R[3 * cap + a];
+        if (oZ) oZ[o] = R[4 * cap + a];
     }
 }
 
 // Advances y by a chain of n points with step h (the reference's "arrive"
 // integrations), in pieces that fit the scratch buffers.
+template <int ONE>
 __device__ void picard_arrive(const rk_ctx& c, double* y, double h, long long n, double* Ya, double* Yb,
                               long long cap, double* lds)
 {
     long long left = n;
     if (cap < 2) return;
     while (left > 1) {
-        long long m = left < cap ? left : cap;
-        double* R = picard_chain(c, y, h, m, Ya, Yb, cap, lds);
+        long long m = left < cap ? left : cap, per;
+        double* R = picard_chain<ONE>(c, y, h, m, Ya, Yb, cap, lds, per);
         __syncthreads();
-        for (int k = 0; k < 5; k++) y[k] = R[k * cap + m - 1];
+        for (int k = 0; k < 5; k++) y[k] = R[k * cap + pic_at(m - 1, per)];
         __syncthreads();
         left -= m - 1;
     }
 }
 
 // Same trajectory as k_traj_rk4 (srgtrjdt.cpp:90-340), one workgroup per
-// trajectory, each chain solved by picard_chain.  scratch holds 10*ld doubles
-// per trajectory.
+// trajectory, each chain solved by picard_chain.  scratch holds 10*cap
+// doubles per trajectory, cap = ceil((ld - 1)/T)*T + 1.
+template <int ONE>
 __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_traj_job* __restrict__ jobs,
                                                                    const srwd_field* __restrict__ fields,
                                                                    double* X, double* Xp, double* Y,
                                                                    double* Yp, double* Z, long long ld,
-                                                                   double* scratch)
+                                                                   double* scratch, long long cap)
 {
-    __shared__ double lds[64];
+    __shared__ double lds[15 * (SRWD_PIC_THREADS / 64)];
     const int tr = blockIdx.x;
     const srwd_traj_job J = jobs[tr];
     rk_ctx c;
     rk_ctx_setup(c, J, fields);
     double* oX = X + tr * ld; double* oXp = Xp + tr * ld; double* oY = Y + tr * ld;
     double* oYp = Yp + tr * ld; double* oZ = Z ? Z + tr * ld : nullptr;
-    double* Ya = scratch + (size_t)tr * 10 * ld;
-    double* Yb = Ya + 5 * ld;
-    const long long cap = ld;
+    double* Ya = scratch + (size_t)tr * 10 * cap;
+    double* Yb = Ya + 5 * cap;
     const long long ns = J.np;
     const double sMin = J.ctStart, sMax = J.ctEnd;
     const double sStep = (ns <= 1) ? 0 : (sMax - sMin) / (ns - 1);
@@ -439,16 +481,17 @@ __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_tra
         const double sNear = below ? sMax : sMin, sFar = below ? sMin : sMax;
         long long auxNp = (long long)(int)fabs(sNear / sStep) + 1;
         if (auxNp <= 1) auxNp = 2;
-        picard_arrive(c, ic, sNear / (double)(auxNp - 1), auxNp, Ya, Yb, cap, lds);
+        picard_arrive<ONE>(c, ic, sNear / (double)(auxNp - 1), auxNp, Ya, Yb, cap, lds);
         if (ns <= 1) {
             if (threadIdx.x == 0) {
                 oX[0] = ic[0]; oXp[0] = ic[1]; oY[0] = ic[2]; oYp[0] = ic[3]; if (oZ) oZ[0] = ic[4];
             }
             return;
         }
-        double* R = picard_chain(c, ic, (sFar - sNear) / (double)(ns - 1), ns, Ya, Yb, cap, lds);
-        if (below) picard_store(R, cap, ns, ns - 1, -1, oX, oXp, oY, oYp, oZ);
-        else picard_store(R, cap, ns, 0, 1, oX, oXp, oY, oYp, oZ);
+        long long per;
+        double* R = picard_chain<ONE>(c, ic, (sFar - sNear) / (double)(ns - 1), ns, Ya, Yb, cap, lds, per);
+        if (below) picard_store(R, cap, per, ns, ns - 1, -1, oX, oXp, oY, oYp, oZ);
+        else picard_store(R, cap, per, ns, 0, 1, oX, oXp, oY, oYp, oZ);
         return;
     }
     long long is0 = 0;
@@ -459,7 +502,7 @@ __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_tra
         s0Act = sMin + is0 * sStep;
         if (s0Act < -tol) {   // one RK4 step from 0 to s0Act, computed by every thread alike
             double D[5];
-            rk4_increment(c, ic, s0Act, D);
+            rk4_increment<ONE>(c, ic, s0Act, D);
             for (int k = 0; k < 5; k++) ic[k] += D[k];
         }
     }
@@ -470,8 +513,9 @@ __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_tra
                 oX[0] = ic[0]; oXp[0] = ic[1]; oY[0] = ic[2]; oYp[0] = ic[3]; if (oZ) oZ[0] = ic[4];
             }
         } else {
-            double* R = picard_chain(c, ic, (sMin - s0Act) / (double)(nsL - 1), nsL, Ya, Yb, cap, lds);
-            picard_store(R, cap, nsL, nsL - 1, -1, oX, oXp, oY, oYp, oZ);
+            long long per;
+            double* R = picard_chain<ONE>(c, ic, (sMin - s0Act) / (double)(nsL - 1), nsL, Ya, Yb, cap, lds, per);
+            picard_store(R, cap, per, nsL, nsL - 1, -1, oX, oXp, oY, oYp, oZ);
         }
         __syncthreads();
     }
@@ -483,8 +527,9 @@ __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_tra
                 oX[is0] = ic[0]; oXp[is0] = ic[1]; oY[is0] = ic[2]; oYp[is0] = ic[3]; if (oZ) oZ[is0] = ic[4];
             }
         } else {
-            double* R = picard_chain(c, ic, (sMax - s0Act) / (double)(nsR - 1), nsR, Ya, Yb, cap, lds);
-            picard_store(R, cap, nsR, is0, 1, oX, oXp, oY, oYp, oZ);
+            long long per;
+            double* R = picard_chain<ONE>(c, ic, (sMax - s0Act) / (double)(nsR - 1), nsR, Ya, Yb, cap, lds, per);
+            picard_store(R, cap, per, nsR, is0, 1, oX, oXp, oY, oYp, oZ);
         }
     }
 }
@@ -915,11 +960,15 @@ extern "C" {
 
 int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int nTraj, double* dX,
                      double* dXp, double* dY, double* dYp, double* dZ, long long ld, double* dScratch,
-                     hipStream_t st)
+                     int singleHarmonic, hipStream_t st)
 {
     if (dScratch && ld >= 16) {   // picard_arrive needs buffers of at least 2 points
-        hipLaunchKernelGGL(k_traj_picard, dim3(nTraj), dim3(SRWD_PIC_THREADS), 0, st, dJobs, dFields, dX, dXp, dY,
-                           dYp, dZ, ld, dScratch);
+        if (singleHarmonic)
+            hipLaunchKernelGGL(k_traj_picard<1>, dim3(nTraj), dim3(SRWD_PIC_THREADS), 0, st, dJobs, dFields, dX, dXp,
+                               dY, dYp, dZ, ld, dScratch, srwd_pic_cap(ld));
+        else
+            hipLaunchKernelGGL(k_traj_picard<0>, dim3(nTraj), dim3(SRWD_PIC_THREADS), 0, st, dJobs, dFields, dX, dXp,
+                               dY, dYp, dZ, ld, dScratch, srwd_pic_cap(ld));
     } else {
         int threads = 2 * nTraj;
         hipLaunchKernelGGL(k_traj_rk4, dim3((threads + 63) / 64), dim3(64), 0, st, dJobs, dFields, nTraj,

@@ -4,6 +4,7 @@
 #include "../../include/srwl_amd_ext.h"
 #include "srw_dev.h"
 #include "srw_moments.h"
+#include "srw_resize.h"
 #include <vector>
 
 namespace srwamd {
@@ -23,7 +24,8 @@ struct Buffers {
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
     DBuf Stokes, Draws, Wt, Net, Grad;
     DBuf Pic;                     // parallel-in-s RK4 iterates
-    DBuf MomSl, MomWork, MomSums, MomLims;   // wavefront moments
+    DBuf MomSl, MomWork, MomSums, MomLims, MomG;   // wavefront moments
+    DBuf RsO, RsN;                // resampling
 };
 
 struct Context {
@@ -88,5 +90,21 @@ int drift_radius(const SRWLWfr& w, int sub, double L, double* momX, double* momZ
                  void* ctx, double& Rx, double& Rz);
 void drift_moments(const SRWLWfr& w, double L, double* mom);
 void drift_4x4(double* P, double L);
+
+// Resampling (resize_api.cpp).
+struct ResizeAxis {
+    long n = 0, i0 = 0, i1 = 0;     // new size and the index range filled from the old mesh
+    double start = 0, step = 0;
+};
+struct ResizePlan {
+    double pxm = 1, pxd = 1, pzm = 1, pzd = 1;
+    ResizeAxis x, z;
+    int onlyLarger = 0, fx = 0, fz = 0;
+};
+long next_fft_size(long n);
+int resize_plan(const SRWLWfr& w, const double* par, int npar, ResizePlan& P);
+int device_resize(const SRWLWfr& w, const ResizePlan& P, float2* oX, float2* oZ, const srwr_layout& lo, float2* nX,
+                  float2* nZ, const srwr_layout& ln, hipStream_t st);
+int realloc_wfr_arrays(SRWLWfr& w, const ResizePlan& P);
 
 }  // namespace srwamd

@@ -97,11 +97,16 @@ struct srwd_item {
 
 extern "C" {
 // Launch wrappers implemented in sr.hip; all asynchronous on stream st.
-// dScratch (10*ld doubles per trajectory) selects the parallel-in-s solver;
-// nullptr runs the serial two-thread-per-trajectory kernel.
+// Iterate-buffer length per component of the parallel-in-s RK4 for
+// trajectories of up to ld points (512 threads per trajectory).
+inline long long srwd_pic_cap(long long ld) { return ((ld - 1 + 511) / 512) * 512 + 1; }
+
+// dScratch (10*srwd_pic_cap(ld) doubles per trajectory) selects the
+// parallel-in-s solver; nullptr runs the serial kernel.
+// singleHarmonic: every field of dFields is one undulator with one harmonic.
 int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int nTraj, double* dX,
                      double* dXp, double* dY, double* dYp, double* dZ, long long ld, double* dScratch,
-                     hipStream_t st);
+                     int singleHarmonic, hipStream_t st);
 int srwd_launch_nz(srwd_trj_info* dInfo, int nTraj, const double* dX, const double* dXp, const double* dY,
                    const double* dYp, long long ldT, hipStream_t st);
 int srwd_launch_tables(const srwd_trj_info* dInfo, int nTraj, long long maxInt, const double* dX,

@@ -9,6 +9,7 @@
 //   srwlCalcPartTraj       cpp/src/lib/srwlib.cpp:328-373
 //   srwlUtiGetErrText etc. cpp/src/lib/srwlib.cpp:78-140
 #include "srw_api_internal.h"
+#include "srw_fft.h"
 
 #include <algorithm>
 #include <cmath>
@@ -127,6 +128,19 @@ static inline double gamma_internal(const SRWLParticle& p)
 
 #define HIPCHK(x) do { if ((x) != hipSuccess) return SRWL_AMD_ERR_GPU; } while (0)
 
+// Moments of a device field in SRW's [iy][ix][ie] order: several energies are
+// first gathered to one contiguous [iy][ix] slice per energy.
+static int field_moments(SRWLWfr& w, const float2* dEx, const float2* dEz)
+{
+    long long ne = w.mesh.ne, nxy = (long long)w.mesh.nx * w.mesh.ny;
+    if (ne == 1) return wfr_moments(w, dEx, dEz, 0, 1, w.mesh.nx, w.arMomX, w.arMomY);
+    float2* G = (float2*)g_ctx.buf.MomG.get(sizeof(float2) * 2 * (size_t)(nxy * ne));
+    if (!G) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    if (srwf_launch_gather(dEx, G, (int)ne, nxy, 0, g_ctx.stream)) return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_gather(dEz, G + nxy * ne, (int)ne, nxy, 0, g_ctx.stream)) return SRWL_AMD_ERR_GPU;
+    return wfr_moments(w, G, G + nxy * ne, nxy, 1, w.mesh.nx, w.arMomX, w.arMomY);
+}
+
 // Runs the single-electron SR computation for a set of requests.  Outputs
 // stay on the device: request j writes ne*nx*ny complex values of Ex and Ez
 // starting at float2 offset outOff[j] of dEx / dEz.
@@ -214,10 +228,12 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         // user-supplied trajectories are re-uploaded after the RK pass overwrote them
         double* dPic = nullptr;
         if (!g_ctx.trajSerial) {
-            dPic = (double*)B.Pic.get(sizeof(double) * 10 * (size_t)ldT * nT);
+            dPic = (double*)B.Pic.get(sizeof(double) * 10 * (size_t)srwd_pic_cap(ldT) * nT);
             if (!dPic) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
         }
-        if (srwd_launch_traj(dJobs, dFields, nT, dX, dXp, dY, dYp, nullptr, ldT, dPic, st)) return SRWL_AMD_ERR_GPU;
+        int one = 1;
+        for (auto& f : fields) one = one && (f.n == 1 && f.u[0].nh == 1);
+        if (srwd_launch_traj(dJobs, dFields, nT, dX, dXp, dY, dYp, nullptr, ldT, dPic, one, st)) return SRWL_AMD_ERR_GPU;
         for (int j = 0; j < nT; j++) {
             if (!jobs[j].userTrj) continue;
             const SRWLPrtTrj* u = jobs[j].userTrj;
@@ -508,10 +524,11 @@ SRWL_EXPORT int srwlCalcPartTraj(SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld, double*
     HIPCHK(hipMemcpyAsync(dF, &f, sizeof(f), hipMemcpyHostToDevice, st));
     double* dPic = nullptr;
     if (!g_ctx.trajSerial) {
-        dPic = (double*)B.Pic.get(sizeof(double) * 10 * (size_t)pTrj->np);
+        dPic = (double*)B.Pic.get(sizeof(double) * 10 * (size_t)srwd_pic_cap(pTrj->np));
         if (!dPic) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     }
-    if (srwd_launch_traj(dJ, dF, 1, dX, dXp, dY, dYp, dZ, pTrj->np, dPic, st)) return SRWL_AMD_ERR_GPU;
+    int one = (f.n == 1 && f.u[0].nh == 1);
+    if (srwd_launch_traj(dJ, dF, 1, dX, dXp, dY, dYp, dZ, pTrj->np, dPic, one, st)) return SRWL_AMD_ERR_GPU;
     HIPCHK(hipMemcpyAsync(pTrj->arX, dX, b, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(pTrj->arXp, dXp, b, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(pTrj->arY, dY, b, hipMemcpyDeviceToHost, st));
@@ -582,9 +599,7 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     pWfr->xc = res[0].xc; pWfr->yc = res[0].zc;
     // srradint.cpp:3570-3574: statistical moments of every energy slice, then the mean energy
     if (pWfr->arMomX || pWfr->arMomY) {
-        long long ne = J.mesh.ne;
-        rc = wfr_moments(*pWfr, (const float2*)dEx, (const float2*)dEz, 1, ne, ne * J.mesh.nx, pWfr->arMomX,
-                         pWfr->arMomY);
+        rc = field_moments(*pWfr, (const float2*)dEx, (const float2*)dEz);
         if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
@@ -633,9 +648,7 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
         w->Rx = w->Ry = res[i].Robs; w->dRx = w->dRy = res[i].RobsErr;
         w->xc = res[i].xc; w->yc = res[i].zc;
         if (w->arMomX || w->arMomY) {
-            long long ne = w->mesh.ne;
-            rc = wfr_moments(*w, (const float2*)dEx + off[i], (const float2*)dEz + off[i], 1, ne, ne * w->mesh.nx,
-                             w->arMomX, w->arMomY);
+            rc = field_moments(*w, (const float2*)dEx + off[i], (const float2*)dEz + off[i]);
             if (rc) return rc;
         }
         w->avgPhotEn = 0.5 * (w->mesh.eStart + w->mesh.eFin);

@@ -41,6 +41,7 @@ def _load():
     if hasattr(lib, "srwlEmitPropMultiE"):
         lib.srwlEmitPropMultiE.argtypes = [P(S.SRWLStokes), P(S.SRWLWfr), P(S.SRWLMagFldC), P(S.SRWLOptC),
                                            P(C.c_double), C.c_int, C.c_longlong, P(C.c_double), P(C.c_double)]
+    lib.srwlResizeElecField.argtypes = [P(S.SRWLWfr), C.c_char, P(C.c_double)]
     lib.srwlAmdGetStats.argtypes = [P(C.c_double), C.c_int]
     lib.srwlAmdSetTiming.argtypes = [C.c_int]
     lib.srwlAmdSetKeepOnDevice.argtypes = [C.c_int]
@@ -50,6 +51,43 @@ def _load():
 
 lib = _load()
 LIB_PATH = _LIB_PATH
+
+# Wavefront arrays are reallocated by the library through this callback when
+# a call changes the mesh size, as the reference binding's ModifySRWLWfr does
+# (srwlpy.cpp:4141-4420, registered at module init :5934-5941).  The C
+# structure passed to a call is mapped back to its Python wavefront here.
+_WFR_MODIF = C.CFUNCTYPE(C.c_int, C.c_int, C.POINTER(S.SRWLWfr), C.c_char)
+_active_wfr = {}
+
+
+def _modify_wfr(action, pw, pol):
+    try:
+        w = _active_wfr.get(C.addressof(pw.contents))
+        if w is None:
+            return -1
+        c = pw.contents
+        if action in (2, 12):
+            from array import array
+            n = 2 * c.mesh.ne * c.mesh.nx * c.mesh.ny
+            p = pol if isinstance(pol, int) else (ord(pol) if pol else 0)
+            if p in (0, ord('x'), ord('X')):
+                w.arEx = array('f', bytes(4 * n))
+                c.arEx = M._addr(w.arEx)
+            if p in (0, ord('y'), ord('Y'), ord('z'), ord('Z')):
+                w.arEy = array('f', bytes(4 * n))
+                c.arEy = M._addr(w.arEy)
+            M.mesh_back(c.mesh, w.mesh)
+            return 0
+        if action == 0:
+            return 0
+        return -1
+    except Exception:
+        return -1
+
+
+_wfr_modif_cb = _WFR_MODIF(_modify_wfr)
+lib.srwlUtiSetWfrModifFunc.argtypes = [_WFR_MODIF]
+lib.srwlUtiSetWfrModifFunc(_wfr_modif_cb)
 
 
 def err_text(code):
@@ -127,6 +165,24 @@ def set_traj_serial(on=True):
     lib.srwlAmdSetTrajSerial(1 if on else 0)
 
 
+def ResizeElecField(_wfr, _type, _par):
+    """Resampling of the wavefront mesh (reference srwlpy.cpp ResizeElecField,
+    srwlib.cpp:860-900); _par = [0, pxm, pxd, pzm, pzd, 0.5, 0.5]."""
+    keep = M.Keep()
+    cw = M.wfr(_wfr, keep)
+    par = list(_par) + [0.5] * max(0, 7 - len(_par))
+    cp = M.dvec(par, keep)
+    t = _type.encode() if isinstance(_type, str) else _type
+    _active_wfr[C.addressof(cw)] = _wfr
+    try:
+        rc = lib.srwlResizeElecField(C.byref(cw), C.c_char(t), cp)
+    finally:
+        _active_wfr.pop(C.addressof(cw), None)
+    _check(rc)
+    M.wfr_back(cw, _wfr)
+    return _wfr
+
+
 def PropagElecField(_wfr, _opt_cnt, _int_list=None, _gpu_idx=None):
     """Propagation through a container of drift spaces (reference srwlpy.cpp:5214-5320)."""
     if _int_list is not None:
@@ -134,7 +190,12 @@ def PropagElecField(_wfr, _opt_cnt, _int_list=None, _gpu_idx=None):
     keep = M.Keep()
     cw = M.wfr(_wfr, keep)
     co = M.opt_cnt(_opt_cnt, keep)
-    _check(lib.srwlPropagElecField(C.byref(cw), C.byref(co), 0, None, None, None, None))
+    _active_wfr[C.addressof(cw)] = _wfr     # resize instructions may reallocate the arrays
+    try:
+        rc = lib.srwlPropagElecField(C.byref(cw), C.byref(co), 0, None, None, None, None)
+    finally:
+        _active_wfr.pop(C.addressof(cw), None)
+    _check(rc)
     M.wfr_back(cw, _wfr)
     return _wfr
 

@@ -29,6 +29,11 @@ def load():
     lib.orc_field_B.argtypes = [P(S.SRWLMagFldC), C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.orc_trj_interp.argtypes = [P(S.SRWLPrtTrj), P(C.c_double), C.c_longlong, P(C.c_double)]
     lib.orc_struct_sizes.argtypes = [P(C.c_long)]
+    lib.orc_next_fft_size.argtypes = [C.c_long]
+    lib.orc_next_fft_size.restype = C.c_long
+    lib.orc_resize_mesh.argtypes = [P(S.SRWLWfr), P(C.c_double), C.c_int, P(C.c_double)]
+    lib.orc_resize_fill.argtypes = [P(S.SRWLWfr), P(S.SRWLWfr), P(C.c_double), P(C.c_double)]
+    lib.orc_compute_rad_moments.argtypes = [P(S.SRWLWfr), C.c_void_p, C.c_void_p]
     lib.orc_emit_prop_multi_e.argtypes = [P(S.SRWLStokes), P(S.SRWLWfr), P(S.SRWLMagFldC), P(S.SRWLOptC),
                                           P(C.c_double), C.c_int, C.c_longlong, C.c_ulonglong, C.c_longlong,
                                           C.c_int, P(C.c_double)]
@@ -107,3 +112,25 @@ def multi_e(stokes, w0, mag, optc, prec, n_macro, seed=1, first=0, nthreads=8, s
     if sums is not None:
         sums.extend(list(out2))
     return rc
+
+
+def resize(w, par):
+    """srwlResizeElecField(w, 'c', par) restated: returns the new wavefront
+    (w's arrays are consumed by the in-place phase treatment)."""
+    import copy
+    from array import array
+    keep = M.Keep()
+    cw = M.wfr(w, keep)
+    cp = M.dvec(par, keep)
+    out = (C.c_double * 10)()
+    rc = lib.orc_resize_mesh(C.byref(cw), cp, len(par), out)
+    if rc:
+        return rc, None
+    nx, ny = int(out[0]), int(out[1])
+    wn = copy.deepcopy(w)
+    wn.allocate(w.mesh.ne, nx, ny)
+    wn.mesh.nx, wn.mesh.ny = nx, ny
+    wn.mesh.xStart, wn.mesh.xFin, wn.mesh.yStart, wn.mesh.yFin = out[2], out[3], out[4], out[5]
+    cn = M.wfr(wn, keep)
+    rc = lib.orc_resize_fill(C.byref(cw), C.byref(cn), cp, out)
+    return rc, wn
