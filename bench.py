@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--multi-e", type=int, default=1, help="also measure the multi-electron time to 1%% rel-L2")
     ap.add_argument("--me-n", type=int, default=512, help="multi-electron emission mesh")
     ap.add_argument("--me-max", type=int, default=4096, help="electron cap per multi-electron run")
+    ap.add_argument("--uneven", type=int, default=2048, help="requests of the uneven batch (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -72,14 +73,50 @@ def multi_e_runs(srwlpy, W, n, nmax):
         st.mesh.xStart, st.mesh.xFin = -2.5e-3, 2.5e-3
         st.mesh.yStart, st.mesh.yFin = -2.5e-3, 2.5e-3
         info = srwlpy.EmitPropMultiE(st, w0, mag, opt, prec, nmax, _seed=20251202, _batch=256, _proposal=prop,
-                                     _target=0.01, _train=4)
+                                     _target=0.01, _train=4, _ideal_gain=(prop == 0))
         res[name] = {"seconds": info["seconds"], "electrons": int(info["electrons"]),
                      "rel_l2_est": info["rel_l2_est"], "ess": info["ess"]}
+        if prop == 0:
+            # variance ratio of plain MC to the variance-optimal proposal q* ~ p ||I||:
+            # the most any importance-sampling proposal can save on this beam
+            res[name]["ideal_proposal_gain"] = info["ideal_gain"]
     p, q = res["plain_mc"], res["learned"]
     res["speedup_learned_vs_plain"] = p["seconds"] / q["seconds"] if q["seconds"] > 0 else None
     res["workload"] = ("Gaussian beam 4096-electron cap, %dx%d mesh at 20 m, 30 m drift (analytic quadratic phase), "
                        "intensity re-gridded to %dx%d over +/-2.5 mm" % (n, n, n, n))
     return res
+
+
+def sr_traffic(n, ne):
+    """HBM bytes per k_sr_integral launch from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy), when it was
+    measured for this workload; None otherwise."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_sr_integral_%dx%dx%d_traffic.md" % (n, n, ne))
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def uneven_batch(srwlpy, W, n):
+    """n independent requests (meshes 128^2..1024^2, harmonics 1-5, K in
+    [0.5, 2.5]) computed in one device pass; fields left in HBM."""
+    wfrs, mags, prec = W.uneven_requests(n=n, seed=7, alloc=False)
+    for w in wfrs:
+        w.arMomX = w.arMomY = None      # no moments: field computation only
+    pts = sum(w.mesh.nx * w.mesh.ny for w in wfrs)
+    srwlpy.set_keep_on_device(True)
+    srwlpy.reset_stats()
+    t0 = time.perf_counter()
+    srwlpy.CalcElecFieldSRBatch(wfrs, mags, prec)
+    dt = time.perf_counter() - t0
+    st = srwlpy.stats()
+    return {"requests": n, "points": pts, "seconds": dt, "points_per_s": pts / dt,
+            "sr_kernel_ms": st["sr_ms"], "sr_evals": st["sr_evals"],
+            "sr_kernel_tflops": FLOPS_PER_EVAL * st["sr_evals"] / (st["sr_ms"] * 1e-3) / 1e12 if st["sr_ms"] else None,
+            "workload": "%d requests, meshes 128^2-1024^2 over +/-1 mm at 20 m, harmonic 1-5 of K in [0.5, 2.5], "
+                        "one device pass, fields left in HBM" % n}
 
 
 def cpu_baseline(n, ne_sample, threads):
@@ -131,6 +168,7 @@ def main():
     srwlpy.CalcElecFieldSR(w, 0, mag, prec)
     pcie_rate = pts / (time.perf_counter() - t1)
     me = multi_e_runs(srwlpy, W, a.me_n, a.me_max) if a.multi_e else None
+    ub = uneven_batch(srwlpy, W, a.uneven) if a.uneven > 0 else None
     cpu = None
     if not a.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -146,18 +184,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 trajectory and phase reference)",
+        "dtype": "f32",
+        "precision": "FP32 integrand and sums (compensated); FP64 trajectory, tables and per-chunk phase reference",
         "data": "synthetic",
         "config": {"workload": "planar undulator 3 GeV, lambda_u 20 mm x 100, K 1.5; %dx%d mesh at 20 m, %d photon "
                                "energies +/-5%% around the 1st harmonic; 10001-point trajectory, method 0"
                                % (a.n, a.n, a.ne),
                    "mesh": [a.n, a.n], "energies": a.ne, "trajectory_points": 10001, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": sr_traffic(a.n, a.ne),
                      "kernel": "k_sr_integral", "kernel_ms": k_ms, "evals_per_launch": evals,
                      "flops_per_eval": FLOPS_PER_EVAL},
         "pcie_inclusive_points_per_s": pcie_rate,
         "multi_electron": me,
+        "uneven_batch": ub,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -246,10 +246,12 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
  * averaged into pStokes->arS0 (float32, nx*ny).
  * arPar: [0] seed, [1] electrons per device batch (0: auto), [2] proposal
  *   mode (0 plain Gaussian MC, 1 learned mixture proposal), [3] target
- *   rel-L2 for early stop (0: none), [4] training steps per batch.
- * arOut (optional, >= 8 doubles): [0] electrons used, [1] effective sample
- *   size, [2] sum of weights, [3] device seconds, [4] rel-L2 error estimate,
- *   [5] batch size used, [6] last training loss, [7] training steps taken. */
+ *   rel-L2 for early stop (0: none), [4] training steps per batch, [5] (plain
+ *   mode) > 0: estimate the variance gain of the optimal proposal (arOut[8]).
+ * arOut (optional, >= 8 doubles, >= 9 with arPar[5] > 0): [0] electrons
+ *   used, [1] effective sample size, [2] sum of weights, [3] device seconds,
+ *   [4] rel-L2 error estimate, [5] batch size used, [6] last training loss,
+ *   [7] training steps taken, [8] variance ratio plain / optimal proposal. */
 SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagFldC* pMagFld,
                                    SRWLOptC* pOpt, double* precPar, int nPrecPar,
                                    long long nMacroElec, double* arPar, double* arOut);

@@ -1523,6 +1523,8 @@ ORC_API int orc_emit_prop_multi_e(SRWLStokes* st, SRWLWfr* w0, SRWLMagFldC* mag,
             srw_normals6(seed, (unsigned long long)(first + i), g);
             SRWLWfr w = *w0;
             w.arEx = (char*)ex; w.arEy = (char*)ez;
+            w.arMomX = w.arMomY = 0;       /* per-electron state stays private to the thread */
+            w.arElecPropMatr = 0;
             SRWLParticle* p = &w.partBeam.partStatMom1;
             /* srwlib.py:9793-9803 */
             double pxp = SigQX * g[0], px = SigPX * g[1] + AX * pxp / GX;
@@ -2083,4 +2085,25 @@ ORC_API int orc_resize_fill(SRWLWfr* wOld, SRWLWfr* wNew, const double* par, con
         orc_treat_quad_phase(wNew, 1, fx, fz);
     }
     return 0;
+}
+
+/* Test hook: electron i of the sampler mapped to the beam (srwlib.py:9000-9029,
+ * 9793-9803), as the multi-electron restatement draws it. */
+ORC_API void orc_beam_sample(const SRWLPartBeam* b, unsigned long long seed, long long i, SRWLParticle* out)
+{
+    const double* m = b->arStatMom2;
+    double dx = m[0] * m[2] - m[1] * m[1], dy = m[3] * m[5] - m[4] * m[4];
+    double mx = 0.5 / dx, my = 0.5 / dy;
+    double BX = m[0] * mx, GX = m[2] * mx, AX = m[1] * mx;
+    double BY = m[3] * my, GY = m[5] * my, AY = m[4] * my;
+    double SigPX = 1. / sqrt(2. * GX), SigQX = sqrt(GX / (2. * (BX * GX - AX * AX)));
+    double SigPY = 1. / sqrt(2. * GY), SigQY = sqrt(GY / (2. * (BY * GY - AY * AY)));
+    double relE = sqrt(m[10]);
+    double g[6];
+    srw_normals6(seed, (unsigned long long)i, g);
+    *out = b->partStatMom1;
+    double pxp = SigQX * g[0], px = SigPX * g[1] + AX * pxp / GX;
+    double pyp = SigQY * g[2], py = SigPY * g[3] + AY * pyp / GY;
+    out->x += px; out->xp += pxp; out->y += py; out->yp += pyp;
+    out->gamma *= (1. + relE * g[4]);
 }

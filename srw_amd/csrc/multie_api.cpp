@@ -248,6 +248,10 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     if (!dF || !dI || !dS || !dEm || !dW || !dSp || !dN2) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     HIPCHK(hipMemsetAsync(dS, 0, sizeof(double) * 2 * nr, st));
     int useQ = (arPar && arPar[2] > 0) ? 1 : 0;
+    // arPar[5] > 0: also estimate the gain of the variance-optimal proposal
+    // q* ~ p ||I|| for this estimator from the electrons drawn (plain runs)
+    int diag = (arPar && arPar[5] > 0 && !useQ) ? 1 : 0;
+    double sumN2 = 0., sumN = 0.;
     int trainSteps = (arPar && arPar[4] > 0) ? (int)arPar[4] : 4;
     Proposal Q;
     std::vector<float> hOut(SRWL_OUT);
@@ -281,7 +285,7 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     auto t0 = std::chrono::steady_clock::now();
     long long done = 0;
     double sumW = 0, sumW2 = 0;
-    double errEst = -1;
+    double errEst = -1, lastA1 = 0;
     std::vector<double> S1h, S2h;
     while (done < nMacroElec) {
         long long nb = std::min<long long>(B, nMacroElec - done);
@@ -429,9 +433,15 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         }
         HIPCHK(hipMemcpyAsync(dEm, em.data(), sizeof(srwm_emesh) * nb, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dW, w.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st));
-        if (useQ) HIPCHK(hipMemsetAsync(dN2, 0, sizeof(double) * nb, st));
-        if (srwm_launch_accum(dI, nxy, dEm, dW, (int)nb, rm, dS, dS + nr, useQ ? dN2 : nullptr, st))
+        if (useQ || diag) HIPCHK(hipMemsetAsync(dN2, 0, sizeof(double) * nb, st));
+        if (srwm_launch_accum(dI, nxy, dEm, dW, (int)nb, rm, dS, dS + nr, (useQ || diag) ? dN2 : nullptr, st))
             return SRWL_AMD_ERR_GPU;
+        if (diag) {
+            std::vector<double> n2(nb);
+            HIPCHK(hipMemcpyAsync(n2.data(), dN2, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (long long j = 0; j < nb; j++) { sumN2 += n2[j]; sumN += std::sqrt(std::max(n2[j], 0.)); }
+        }
         if (useQ) {
             // online training on this batch: target q* ~ p * ||I||
             std::vector<double> n2(nb);
@@ -467,6 +477,7 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
             for (long long r = 0; r < nr; r++) { a1 += S1h[r] * S1h[r]; a2 += S2h[r]; }
             double N = (double)done;
             errEst = (done > 1 && a1 > 0) ? std::sqrt(std::max(0., N * (a2 - a1 / N) / ((N - 1.) * a1))) : -1;
+            lastA1 = a1;
             if (target > 0 && errEst > 0 && errEst <= target) break;
         }
     }
@@ -484,6 +495,12 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         arOut[5] = (double)B;
         arOut[6] = lastLoss;
         arOut[7] = useQ ? Q.step : 0;
+        if (diag) {
+            // N Var = E||I||^2 - ||mu||^2 for p; (E||I||)^2 - ||mu||^2 for q* ~ p ||I||
+            double N = (double)done, m2 = lastA1 / (N * N);
+            double vp = sumN2 / N - m2, vq = (sumN / N) * (sumN / N) - m2;
+            arOut[8] = (vq > 0) ? vp / vq : 0.;
+        }
     }
     return 0;
 }

@@ -643,8 +643,12 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
     for (int i = 0; i < nWfr; i++) {
         SRWLWfr* w = arWfr[i];
         long long n = (long long)w->mesh.ne * w->mesh.nx * w->mesh.ny;
-        HIPCHK(hipMemcpyAsync(w->arEx, dEx + 2 * off[i], sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
-        HIPCHK(hipMemcpyAsync(w->arEy, dEz + 2 * off[i], sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
+        if (!g_ctx.keepOnDevice) {
+            HIPCHK(hipMemcpyAsync(w->arEx, dEx + 2 * off[i], sizeof(float) * 2 * n, hipMemcpyDeviceToHost,
+                                  g_ctx.stream));
+            HIPCHK(hipMemcpyAsync(w->arEy, dEz + 2 * off[i], sizeof(float) * 2 * n, hipMemcpyDeviceToHost,
+                                  g_ctx.stream));
+        }
         w->Rx = w->Ry = res[i].Robs; w->dRx = w->dRy = res[i].RobsErr;
         w->xc = res[i].xc; w->yc = res[i].zc;
         if (w->arMomX || w->arMomY) {

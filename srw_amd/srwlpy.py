@@ -221,7 +221,7 @@ def UtiFFT(_data, _mesh, _dir):
 
 
 def EmitPropMultiE(_stokes, _wfr0, _mag, _opt_cnt, _prec_par, _n_macro, _seed=1, _batch=0, _proposal=0,
-                   _target=0., _train=0):
+                   _target=0., _train=0, _ideal_gain=0):
     """Partially coherent intensity averaged over _n_macro electrons drawn from _wfr0.partBeam
     (replaces the per-electron loop of srwl_wfr_emit_prop_multi_e, reference srwlib.py:8506).
     _stokes.arS (float, nx*ny of _stokes.mesh) receives the averaged total intensity.
@@ -232,12 +232,13 @@ def EmitPropMultiE(_stokes, _wfr0, _mag, _opt_cnt, _prec_par, _n_macro, _seed=1,
     cm = M.mag_cnt(_mag, keep)
     co = M.opt_cnt(_opt_cnt, keep) if _opt_cnt is not None else None
     prec = M.dvec(_prec_par, keep)
-    par = M.dvec([_seed, _batch, _proposal, _target, _train, 0, 0, 0], keep)
+    par = M.dvec([_seed, _batch, _proposal, _target, _train, 1 if _ideal_gain else 0, 0, 0], keep)
     out = (C.c_double * 16)()
     _check(lib.srwlEmitPropMultiE(C.byref(cs), C.byref(cw), C.byref(cm), C.byref(co) if co is not None else None,
                                   prec, len(_prec_par), int(_n_macro), par, out))
-    keys = ("electrons", "ess", "sum_w", "seconds", "rel_l2_est", "batch", "train_loss", "reserved")
-    return dict(zip(keys, list(out)[:8]))
+    keys = ("electrons", "ess", "sum_w", "seconds", "rel_l2_est", "batch", "train_loss", "train_steps",
+            "ideal_gain")
+    return dict(zip(keys, list(out)[:9]))
 
 
 def normals(seed, i):

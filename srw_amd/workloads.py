@@ -93,6 +93,40 @@ def stokes_like(w):
     return s
 
 
+def uneven_requests(n=2048, seed=7, meshes=(128, 256, 512, 1024), half=1e-3, z=20., np_traj=10001,
+                    per=0.02, nper=100, E_GeV=3.0, alloc=True):
+    """Independent single-electron requests with mixed meshes, harmonics and
+    undulator strengths: mesh N uniform over `meshes`, K ~ U[0.5, 2.5],
+    photon energy = h * E1(K) with h uniform over 1..5; zero-emittance 3 GeV
+    electron, +/-half at z.  Returns (wavefronts, fields, precPar); with
+    alloc=False the field arrays are one-element placeholders (for runs that
+    leave the fields in device memory)."""
+    import random
+    rng = random.Random(seed)
+    wfrs, mags = [], []
+    for _ in range(n):
+        N = rng.choice(meshes)
+        K = rng.uniform(0.5, 2.5)
+        h = rng.randint(1, 5)
+        beam = electron_beam(E_GeV)
+        e = h * first_harmonic_eV(beam.partStatMom1.gamma, K, per)
+        if alloc:
+            w = wavefront(beam, 1, e, e, N, N, half, half, z)
+        else:
+            w = SRWLWfr()
+            w.mesh.ne, w.mesh.nx, w.mesh.ny = 1, N, N
+            w.mesh.zStart = z
+            w.mesh.eStart = w.mesh.eFin = e
+            w.mesh.xStart, w.mesh.xFin = -half, half
+            w.mesh.yStart, w.mesh.yFin = -half, half
+            w.partBeam = beam
+            w.arEx = array('f', [0., 0.])
+            w.arEy = array('f', [0., 0.])
+        wfrs.append(w)
+        mags.append(planar_undulator(K, per, nper))
+    return wfrs, mags, traj_prec(np_traj, per=per, nper=nper)
+
+
 def beam_for_multi_e(E_GeV=3.0, Iavg=0.5):
     return gaussian_beam_moments(electron_beam(E_GeV, Iavg))
 

@@ -29,6 +29,7 @@ def load():
     lib.orc_field_B.argtypes = [P(S.SRWLMagFldC), C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.orc_trj_interp.argtypes = [P(S.SRWLPrtTrj), P(C.c_double), C.c_longlong, P(C.c_double)]
     lib.orc_struct_sizes.argtypes = [P(C.c_long)]
+    lib.orc_beam_sample.argtypes = [P(S.SRWLPartBeam), C.c_ulonglong, C.c_longlong, P(S.SRWLParticle)]
     lib.orc_next_fft_size.argtypes = [C.c_long]
     lib.orc_next_fft_size.restype = C.c_long
     lib.orc_resize_mesh.argtypes = [P(S.SRWLWfr), P(C.c_double), C.c_int, P(C.c_double)]

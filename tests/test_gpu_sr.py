@@ -98,3 +98,16 @@ def test_termin_modes_and_step():
         prec[5] = termin
         prec[1] = prec[1] * 1.7   # integration step not equal to the trajectory step
         _check(*_run_pair(w, mag, prec))
+
+
+def test_uneven_batch_matches_single_requests():
+    # different meshes, harmonics and undulator strengths in one device pass
+    from srw_amd import srwlpy
+    print("case: uneven batch of 6 requests", flush=True)
+    wfrs, mags, prec = W.uneven_requests(n=6, seed=3, meshes=(8, 12, 16))
+    wg = [clone_wfr(w) for w in wfrs]
+    srwlpy.CalcElecFieldSRBatch(wg, mags, prec)
+    for w, g, m in zip(wfrs, wg, mags):
+        wo = clone_wfr(w)
+        assert oracle.calc_field(wo, m, prec) == 0
+        _check(g, wo)

@@ -267,3 +267,29 @@ def test_drift_analytic_quadratic_phase_matches_gaussian_beam():
     Ex = cfield(w.arEx, w)[:, :, 0]
     assert rel_l2(Ex, _gauss(X, Y, z0 + L, w0, lam)) < 1e-4
     assert np.sum(np.abs(Ex) ** 2) * dx1 * dx1 == pytest.approx(P0, rel=1e-5)
+
+
+def test_beam_sampler_reproduces_second_moments():
+    # 6-D Gaussian beam with x-x' correlation: sample moments of the mapped
+    # electrons match arStatMom2 within their statistical errors
+    beam = W.gaussian_beam_moments(W.electron_beam(), sx=30e-6, sxp=5e-6, sy=5e-6, syp=2e-6, se=1e-3)
+    beam.arStatMom2[1] = -0.4 * 30e-6 * 5e-6          # <x x'>
+    cb = M.part_beam(beam)
+    p = S.SRWLParticle()
+    n = 40000
+    a = np.empty((n, 5))
+    for i in range(n):
+        oracle.lib.orc_beam_sample(C.byref(cb), 20251202, i, C.byref(p))
+        a[i] = (p.x, p.xp, p.y, p.yp, p.gamma)
+    g0 = beam.partStatMom1.gamma
+    m = beam.arStatMom2
+    cov = np.cov(a[:, :4].T)
+    rel = 4 * math.sqrt(2. / n)                       # 4 sigma of a sample variance
+    assert cov[0, 0] == pytest.approx(m[0], rel=rel)
+    assert cov[1, 1] == pytest.approx(m[2], rel=rel)
+    assert cov[2, 2] == pytest.approx(m[3], rel=rel)
+    assert cov[3, 3] == pytest.approx(m[5], rel=rel)
+    assert cov[0, 1] == pytest.approx(m[1], abs=4 * math.sqrt(m[0] * m[2] / n))
+    assert abs(cov[0, 2]) < 4 * math.sqrt(m[0] * m[3] / n)
+    assert np.var(a[:, 4] / g0 - 1) == pytest.approx(m[10], rel=rel)
+    assert abs(np.mean(a[:, 0])) < 4 * math.sqrt(m[0] / n)
