@@ -23,6 +23,10 @@ HBM_PEAK_GBS = 8000.0
 FLOPS_PER_EVAL = 64                 # algorithmic FP32-equivalent flops per integrand evaluation
 
 
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,12 +171,18 @@ def main():
     t1 = time.perf_counter()
     srwlpy.CalcElecFieldSR(w, 0, mag, prec)
     pcie_rate = pts / (time.perf_counter() - t1)
+    log("single-electron field: %.4g points/s" % value)
     me = multi_e_runs(srwlpy, W, a.me_n, a.me_max) if a.multi_e else None
+    if me:
+        log("multi-electron runs done")
     ub = uneven_batch(srwlpy, W, a.uneven) if a.uneven > 0 else None
+    if ub:
+        log("uneven batch done")
     cpu = None
     if not a.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(a.n, a.cpu_energies, threads)
+        log("cpu baseline done")
     out = {
         "metric": "single-e field pts/s",
         "value": value,

@@ -766,19 +766,11 @@ __global__ void __launch_bounds__(256) k_phase_table(const double* __restrict__ 
 // Block = 256 observation points of one item; blockIdx.y = split of the
 // integration range.  Per-point data of the trajectory are uniform across the
 // block and are read with scalar loads.
-struct kahan4 { float s[4], c[4]; };
+// The phase of point (x_o, z_o) at s is Tc + A[s] + alpha dInv + beta dU +
+// gam dV (revolutions), with Tc the point's FP64 phase at the chunk reference
+// reduced to [-1/2, 1/2]; the loop leaves Tc out and each chunk sum is turned
+// by exp(2 pi i Tc) once.
 #define SRWD_BLOCK 32
-#ifndef SRWD_REDUCE_PHASE
-#define SRWD_REDUCE_PHASE 0
-#endif
-
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x)
-{
-    float y = x - c;
-    float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
 
 __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict__ items,
                                                      const int* __restrict__ blockItem,
@@ -807,39 +799,39 @@ __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict
     const srwd_rec* R = rec + (long long)IT.traj * ldI;
     const srwd_chunk* CH = chunks + (long long)IT.traj * ldC;
     const float* Ar = A + (long long)IT.arow * ldI;
-    kahan4 acc;
-    for (int k = 0; k < 4; k++) { acc.s[k] = 0.f; acc.c[k] = 0.f; }
+    double ox = 0., oxi = 0., oz = 0., ozi = 0.;
     for (long long c = cBeg; c < cEnd; c++) {
         const srwd_chunk C = CH[c];
         double tc = kph * (r2 * C.inv - 2. * xo * C.U - 2. * zo * C.V);
         float Tc = (float)(tc - rint(tc));
         long long i0 = c * SRWD_CHUNK, i1 = i0 + SRWD_CHUNK;
         if (i1 > nInt) i1 = nInt;
+        // chunk sums of b (P - r_o) exp(2 pi i (t - Tc)): 32-point FP32 block
+        // sums added in FP32, then turned by exp(2 pi i Tc) into FP64
+        float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
         for (long long b0 = i0; b0 < i1; b0 += SRWD_BLOCK) {
             float bx = 0.f, bxi = 0.f, bz = 0.f, bzi = 0.f;
             long long b1 = b0 + SRWD_BLOCK < i1 ? b0 + SRWD_BLOCK : i1;
 #pragma unroll 4
             for (long long i = b0; i < b1; i++) {
                 const srwd_rec r = R[i];
-                float t = Ar[i] + Tc;
-                t = fmaf(alpha, r.dInv, t);
+                float t = fmaf(alpha, r.dInv, Ar[i]);
                 t = fmaf(beta, r.dU, t);
                 t = fmaf(gam, r.dV, t);
-#if SRWD_REDUCE_PHASE
-                t = t - rintf(t);
-#endif
                 float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
                 float wax = (r.Px - fxo) * r.b, waz = (r.Pz - fzo) * r.b;
                 bx = fmaf(wax, cs, bx); bxi = fmaf(wax, sn, bxi);
                 bz = fmaf(waz, cs, bz); bzi = fmaf(waz, sn, bzi);
             }
-            kahan_add(acc.s[0], acc.c[0], bx); kahan_add(acc.s[1], acc.c[1], bxi);
-            kahan_add(acc.s[2], acc.c[2], bz); kahan_add(acc.s[3], acc.c[3], bzi);
+            q0 += bx; q1 += bxi; q2 += bz; q3 += bzi;
         }
+        double cT = __builtin_amdgcn_cosf(Tc), sT = __builtin_amdgcn_sinf(Tc);
+        ox += q0 * cT - q1 * sT; oxi += q0 * sT + q1 * cT;
+        oz += q2 * cT - q3 * sT; ozi += q2 * sT + q3 * cT;
     }
     if (!valid) return;
     double* o = part + ((long long)blockIdx.y * nPointsPad + flat) * 4;
-    for (int k = 0; k < 4; k++) o[k] = (double)acc.s[k] - (double)acc.c[k];
+    o[0] = ox; o[1] = oxi; o[2] = oz; o[3] = ozi;
 }
 
 // ------------------------------------------------------------------------
