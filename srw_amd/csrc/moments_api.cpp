@@ -17,7 +17,8 @@
 
 namespace srwamd {
 
-int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums, std::vector<int>& lims)
+int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums, std::vector<int>& lims,
+                   bool lanesOverSlices)
 {
     int n = (int)sl.size();
     sums.assign((size_t)n * 24, 0.);
@@ -26,17 +27,18 @@ int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums
     int rc = ensure_device();
     if (rc) return rc;
     hipStream_t st = g_ctx.stream;
-    long long ws = 0;
-    for (auto& s : sl) ws = std::max<long long>(ws, (long long)s.nx + s.nz);
+    int nxMax = 1, nzMax = 1;
+    for (auto& s : sl) { nxMax = std::max(nxMax, s.nx); nzMax = std::max(nzMax, s.nz); }
     Buffers& B = g_ctx.buf;
     srwm_mslice* dSl = (srwm_mslice*)B.MomSl.get(sizeof(srwm_mslice) * n);
-    double* dWork = (double*)B.MomWork.get(sizeof(double) * (size_t)ws * n);
+    void* dWork = B.MomWork.get((size_t)srwm_work_bytes(n, nxMax, nzMax, lanesOverSlices ? 1 : 0));
     double* dSums = (double*)B.MomSums.get(sizeof(double) * 24 * (size_t)n);
     int* dLims = (int*)B.MomLims.get(sizeof(int) * 4 * (size_t)n);
     if (!dSl || !dWork || !dSums || !dLims) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     if (hipMemcpyAsync(dSl, sl.data(), sizeof(srwm_mslice) * n, hipMemcpyHostToDevice, st) != hipSuccess)
         return SRWL_AMD_ERR_GPU;
-    if (srwm_launch_moments(dSl, n, dWork, ws, dSums, dLims, st)) return SRWL_AMD_ERR_GPU;
+    if (srwm_launch_moments(dSl, n, nxMax, nzMax, lanesOverSlices ? 1 : 0, dWork, dSums, dLims, st))
+        return SRWL_AMD_ERR_GPU;
     if (hipMemcpyAsync(sums.data(), dSums, sizeof(double) * 24 * n, hipMemcpyDeviceToHost, st) != hipSuccess)
         return SRWL_AMD_ERR_GPU;
     if (hipMemcpyAsync(lims.data(), dLims, sizeof(int) * 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -102,7 +104,8 @@ int wfr_moments(const SRWLWfr& w, const float2* dEx, const float2* dEz, long lon
     for (long ie = 0; ie < w.mesh.ne; ie++) sl.push_back(wfr_slice(w, dEx, dEz, perE, perX, perZ, ie));
     std::vector<double> sums;
     std::vector<int> lims;
-    int rc = device_moments(sl, sums, lims);
+    // energy-fastest order: the slices' data are adjacent, one lane per slice
+    int rc = device_moments(sl, sums, lims, perE == 1 && w.mesh.ne > 1);
     if (rc) return rc;
     for (size_t i = 0; i < sl.size(); i++) {
         if (momX && dEx) moments_from_sums(&sums[24 * i], sl[i], momX + 11 * i);

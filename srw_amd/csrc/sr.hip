@@ -449,9 +449,13 @@ __device__ void picard_arrive(const rk_ctx& c, double* y, double h, long long n,
     }
 }
 
-// Same trajectory as k_traj_rk4 (srgtrjdt.cpp:90-340), one workgroup per
-// trajectory, each chain solved by picard_chain.  scratch holds 10*cap
-// doubles per trajectory, cap = ceil((ld - 1)/T)*T + 1.
+// Same trajectory as k_traj_rk4 (srgtrjdt.cpp:90-340), each chain solved by
+// picard_chain.  Two workgroups per trajectory (blockIdx.y): when the range
+// spans ct = 0, the upstream and downstream chains are independent and
+// workgroup 0 integrates the upstream one while workgroup 1 integrates the
+// downstream one; otherwise workgroup 0 does all the work.  scratch holds
+// 20*cap doubles per trajectory (10*cap per workgroup), cap =
+// ceil((ld - 1)/T)*T + 1.
 template <int ONE>
 __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_traj_job* __restrict__ jobs,
                                                                    const srwd_field* __restrict__ fields,
@@ -466,7 +470,8 @@ __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_tra
     rk_ctx_setup(c, J, fields);
     double* oX = X + tr * ld; double* oXp = Xp + tr * ld; double* oY = Y + tr * ld;
     double* oYp = Yp + tr * ld; double* oZ = Z ? Z + tr * ld : nullptr;
-    double* Ya = scratch + (size_t)tr * 10 * cap;
+    const int side = blockIdx.y;
+    double* Ya = scratch + ((size_t)tr * 2 + side) * 10 * cap;
     double* Yb = Ya + 5 * cap;
     const long long ns = J.np;
     const double sMin = J.ctStart, sMax = J.ctEnd;
@@ -477,6 +482,7 @@ __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_tra
     for (int k = 0; k < 5; k++) ic[k] = J.init[k];
     if ((left && sMax < -tol) || (right && sMin > tol)) {
         // the whole range lies on one side of ct = 0
+        if (side) return;
         const bool below = left && sMax < -tol;
         const double sNear = below ? sMax : sMin, sFar = below ? sMin : sMax;
         long long auxNp = (long long)(int)fabs(sNear / sStep) + 1;
@@ -506,7 +512,7 @@ __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_tra
             for (int k = 0; k < 5; k++) ic[k] += D[k];
         }
     }
-    if (left) {
+    if (left && side == 0) {
         long long nsL = is0 + 1;
         if (nsL <= 1) {
             if (threadIdx.x == 0) {
@@ -517,9 +523,9 @@ __global__ void __launch_bounds__(SRWD_PIC_THREADS) k_traj_picard(const srwd_tra
             double* R = picard_chain<ONE>(c, ic, (sMin - s0Act) / (double)(nsL - 1), nsL, Ya, Yb, cap, lds, per);
             picard_store(R, cap, per, nsL, nsL - 1, -1, oX, oXp, oY, oYp, oZ);
         }
-        __syncthreads();
     }
-    if (right) {
+    // the downstream chain: workgroup 1, or workgroup 0 when there is no upstream one
+    if (right && side == (left ? 1 : 0)) {
         long long nsR = ns - is0;
         if (nsR < 1) nsR = 1;
         if (nsR <= 1) {
@@ -956,10 +962,10 @@ int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int 
 {
     if (dScratch && ld >= 16) {   // picard_arrive needs buffers of at least 2 points
         if (singleHarmonic)
-            hipLaunchKernelGGL(k_traj_picard<1>, dim3(nTraj), dim3(SRWD_PIC_THREADS), 0, st, dJobs, dFields, dX, dXp,
+            hipLaunchKernelGGL(k_traj_picard<1>, dim3(nTraj, 2), dim3(SRWD_PIC_THREADS), 0, st, dJobs, dFields, dX, dXp,
                                dY, dYp, dZ, ld, dScratch, srwd_pic_cap(ld));
         else
-            hipLaunchKernelGGL(k_traj_picard<0>, dim3(nTraj), dim3(SRWD_PIC_THREADS), 0, st, dJobs, dFields, dX, dXp,
+            hipLaunchKernelGGL(k_traj_picard<0>, dim3(nTraj, 2), dim3(SRWD_PIC_THREADS), 0, st, dJobs, dFields, dX, dXp,
                                dY, dYp, dZ, ld, dScratch, srwd_pic_cap(ld));
     } else {
         int threads = 2 * nTraj;
@@ -1058,7 +1064,9 @@ __global__ void __launch_bounds__(256) k_rx_samples(const srwd_trj_info* __restr
 // Robs, RobsAbsErr, xc, zc.  One block per trajectory: each of the four
 // misfit criteria keeps the first sample (scanning from the downstream end,
 // then from the upstream end) whose misfit changes sign relative to the end
-// sample; the per-criterion minimum index is found with LDS atomics.
+// sample.  Each thread visits its samples in increasing order and keeps its
+// first hit per criterion; the block minimum follows from a wave reduction
+// and one LDS exchange.
 __global__ void __launch_bounds__(256) k_find_rx(const srwd_trj_info* __restrict__ info, const double* samp,
                                                  long long ldS, const double* mesh, double* out)
 {
@@ -1070,9 +1078,8 @@ __global__ void __launch_bounds__(256) k_find_rx(const srwd_trj_info* __restrict
     const long long N = I.np;
     const double sStart = I.sStart, sRange = (N - 1) * I.sStep, sEnd = sStart + sRange;
     const long long L = N - 1;
-    __shared__ unsigned long long first[8];
-    if (threadIdx.x < 8) first[threadIdx.x] = ~0ull;
-    __syncthreads();
+    const unsigned long long NONE = ~0ull;
+    __shared__ unsigned long long first[4][8];
     const double R0b = I.yObs - sEnd, R0f = I.yObs - sStart;
     double crd[4] = {xS, xE, zS, zE};
     double ref[8];
@@ -1081,24 +1088,39 @@ __global__ void __launch_bounds__(256) k_find_rx(const srwd_trj_info* __restrict
         ref[k] = (crd[k] - (zp ? za[L] : xa[L])) * (1. / R0b) - (zp ? bz[L] : bx[L]);
         ref[4 + k] = (crd[k] - (zp ? za[0] : xa[0])) * (1. / R0f) - (zp ? bz[0] : bx[0]);
     }
+    unsigned long long mine[8];
+    for (int k = 0; k < 8; k++) mine[k] = NONE;
     for (long long is = 1 + threadIdx.x; is < N; is += blockDim.x) {
         double Rb = R0b + (double)is * I.sStep, Rf = R0f - (double)is * I.sStep;
         long long jb = L - is, jf = is;
         for (int k = 0; k < 4; k++) {
             bool zp = k >= 2;
             double mb = (crd[k] - (zp ? za[jb] : xa[jb])) * (1. / Rb) - (zp ? bz[jb] : bx[jb]);
-            if (mb * ref[k] < 0.) atomicMin(&first[k], (unsigned long long)is);
+            if (mb * ref[k] < 0. && mine[k] == NONE) mine[k] = (unsigned long long)is;
             double mf = (crd[k] - (zp ? za[jf] : xa[jf])) * (1. / Rf) - (zp ? bz[jf] : bx[jf]);
-            if (mf * ref[4 + k] < 0.) atomicMin(&first[4 + k], (unsigned long long)is);
+            if (mf * ref[4 + k] < 0. && mine[4 + k] == NONE) mine[4 + k] = (unsigned long long)is;
         }
     }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = 0; k < 8; k++) {
+        unsigned long long v = mine[k];
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(v, off, 64);
+            v = o < v ? o : v;
+        }
+        if (lane == 0) first[w][k] = v;
+    }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 8; k++)
+            for (int q = 1; q < 4; q++) first[0][k] = first[q][k] < first[0][k] ? first[q][k] : first[0][k];
+    }
     if (threadIdx.x != 0) return;
     const double VL = 1.E+23;
     double Rv[8];
     for (int k = 0; k < 4; k++) {
-        Rv[k] = first[k] == ~0ull ? VL : R0b + (double)first[k] * I.sStep;
-        Rv[4 + k] = first[4 + k] == ~0ull ? VL : R0f - (double)first[4 + k] * I.sStep;
+        Rv[k] = first[0][k] == NONE ? VL : R0b + (double)first[0][k] * I.sStep;
+        Rv[4 + k] = first[0][4 + k] == NONE ? VL : R0f - (double)first[0][4 + k] * I.sStep;
     }
     double minRx = Rv[0] < Rv[1] ? Rv[0] : Rv[1], minRz = Rv[2] < Rv[3] ? Rv[2] : Rv[3];
     double minR = minRx < minRz ? minRx : minRz;

@@ -24,7 +24,7 @@ struct Buffers {
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
     DBuf Stokes, Draws, Wt, Net, Grad;
     DBuf Pic;                     // parallel-in-s RK4 iterates
-    DBuf MomSl, MomWork, MomSums, MomLims, MomG;   // wavefront moments
+    DBuf MomSl, MomWork, MomSums, MomLims;   // wavefront moments
     DBuf RsO, RsN;                // resampling
 };
 
@@ -82,7 +82,8 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
 // device wavefront is at base[ie*perE + iz*perZ + ix*perX] (float2 units).
 int wfr_moments(const SRWLWfr& w, const float2* dEx, const float2* dEz, long long perE, long long perX,
                 long long perZ, double* momX, double* momZ);
-int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums, std::vector<int>& lims);
+int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums, std::vector<int>& lims,
+                   bool lanesOverSlices = false);
 void moments_from_sums(const double* S, const srwm_mslice& s, double* mom);
 srwm_mslice wfr_slice(const SRWLWfr& w, const float2* dEx, const float2* dEz, long long perE, long long perX,
                       long long perZ, long ie);

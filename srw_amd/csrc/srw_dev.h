@@ -101,7 +101,7 @@ extern "C" {
 // trajectories of up to ld points (512 threads per trajectory).
 inline long long srwd_pic_cap(long long ld) { return ((ld - 1 + 511) / 512) * 512 + 1; }
 
-// dScratch (10*srwd_pic_cap(ld) doubles per trajectory) selects the
+// dScratch (20*srwd_pic_cap(ld) doubles per trajectory) selects the
 // parallel-in-s solver; nullptr runs the serial kernel.
 // singleHarmonic: every field of dFields is one undulator with one harmonic.
 int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int nTraj, double* dX,

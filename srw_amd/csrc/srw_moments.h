@@ -17,9 +17,13 @@ struct srwm_mslice {
 #define SRWM_NSUM 22
 
 extern "C" {
+// Device work space (bytes) of srwm_launch_moments for these sizes.
+long long srwm_work_bytes(int nSlices, int nxMax, int nzMax, int lanesOverSlices);
 // For each slice: the 22 moment sums (sums[s*24 + k]) and the index limits
-// of the 90 % power rectangle (lims[s*4 + k]).  work holds nx + nz doubles
-// per slice (workStride apart).
-int srwm_launch_moments(const srwm_mslice* dSlices, int nSlices, double* dWork, long long workStride,
-                        double* dSums, int* dLims, hipStream_t st);
+// of the 90 % power rectangle (lims[s*4 + k]).  nxMax / nzMax bound every
+// slice's nx / nz.  lanesOverSlices selects the mapping for slices s whose
+// data lie at base + s (SRW's energy-fastest order); otherwise each slice's
+// rows should be contiguous in x.
+int srwm_launch_moments(const srwm_mslice* dSlices, int nSlices, int nxMax, int nzMax, int lanesOverSlices,
+                        void* dWork, double* dSums, int* dLims, hipStream_t st);
 }

@@ -128,17 +128,12 @@ static inline double gamma_internal(const SRWLParticle& p)
 
 #define HIPCHK(x) do { if ((x) != hipSuccess) return SRWL_AMD_ERR_GPU; } while (0)
 
-// Moments of a device field in SRW's [iy][ix][ie] order: several energies are
-// first gathered to one contiguous [iy][ix] slice per energy.
+// Moments of a device field in SRW's [iy][ix][ie] order: energy ie, point
+// (ix, iy) at ie + ne*(ix + nx*iy).
 static int field_moments(SRWLWfr& w, const float2* dEx, const float2* dEz)
 {
-    long long ne = w.mesh.ne, nxy = (long long)w.mesh.nx * w.mesh.ny;
-    if (ne == 1) return wfr_moments(w, dEx, dEz, 0, 1, w.mesh.nx, w.arMomX, w.arMomY);
-    float2* G = (float2*)g_ctx.buf.MomG.get(sizeof(float2) * 2 * (size_t)(nxy * ne));
-    if (!G) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    if (srwf_launch_gather(dEx, G, (int)ne, nxy, 0, g_ctx.stream)) return SRWL_AMD_ERR_GPU;
-    if (srwf_launch_gather(dEz, G + nxy * ne, (int)ne, nxy, 0, g_ctx.stream)) return SRWL_AMD_ERR_GPU;
-    return wfr_moments(w, G, G + nxy * ne, nxy, 1, w.mesh.nx, w.arMomX, w.arMomY);
+    long long ne = w.mesh.ne, nx = w.mesh.nx;
+    return wfr_moments(w, dEx, dEz, 1, ne, nx * ne, w.arMomX, w.arMomY);
 }
 
 // Runs the single-electron SR computation for a set of requests.  Outputs
@@ -228,7 +223,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         // user-supplied trajectories are re-uploaded after the RK pass overwrote them
         double* dPic = nullptr;
         if (!g_ctx.trajSerial) {
-            dPic = (double*)B.Pic.get(sizeof(double) * 10 * (size_t)srwd_pic_cap(ldT) * nT);
+            dPic = (double*)B.Pic.get(sizeof(double) * 20 * (size_t)srwd_pic_cap(ldT) * nT);
             if (!dPic) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
         }
         int one = 1;
@@ -524,7 +519,7 @@ SRWL_EXPORT int srwlCalcPartTraj(SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld, double*
     HIPCHK(hipMemcpyAsync(dF, &f, sizeof(f), hipMemcpyHostToDevice, st));
     double* dPic = nullptr;
     if (!g_ctx.trajSerial) {
-        dPic = (double*)B.Pic.get(sizeof(double) * 10 * (size_t)srwd_pic_cap(pTrj->np));
+        dPic = (double*)B.Pic.get(sizeof(double) * 20 * (size_t)srwd_pic_cap(pTrj->np));
         if (!dPic) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     }
     int one = (f.n == 1 && f.u[0].nh == 1);
@@ -640,6 +635,9 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
     rc = sr_run(jobs, fields, dEx, dEz, off, res, termin);
     if (rc) return rc;
     int warn = 0;
+    std::vector<srwm_mslice> momSl;
+    std::vector<std::pair<int, int>> momOwner;   // (request, energy) of each slice
+    long long nAdj = 0;
     for (int i = 0; i < nWfr; i++) {
         SRWLWfr* w = arWfr[i];
         long long n = (long long)w->mesh.ne * w->mesh.nx * w->mesh.ny;
@@ -651,13 +649,31 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
         }
         w->Rx = w->Ry = res[i].Robs; w->dRx = w->dRy = res[i].RobsErr;
         w->xc = res[i].xc; w->yc = res[i].zc;
-        if (w->arMomX || w->arMomY) {
-            rc = field_moments(*w, (const float2*)dEx + off[i], (const float2*)dEz + off[i]);
-            if (rc) return rc;
+        if ((w->arMomX || w->arMomY) && (w->mesh.nx > 1 || w->mesh.ny > 1)) {
+            long long ne = w->mesh.ne, nx = w->mesh.nx;
+            for (long ie = 0; ie < ne; ie++) {
+                momSl.push_back(wfr_slice(*w, (const float2*)dEx + off[i], (const float2*)dEz + off[i], 1, ne, nx * ne,
+                                          ie));
+                momOwner.push_back({i, (int)ie});
+                if (ne > 1) nAdj++;
+            }
         }
         w->avgPhotEn = 0.5 * (w->mesh.eStart + w->mesh.eFin);
         w->presCA = 0; w->presFT = 0;
         if (res[i].warn) warn = res[i].warn;
+    }
+    if (!momSl.empty()) {
+        // all requests' slices in one device pass
+        std::vector<double> sums;
+        std::vector<int> lims;
+        rc = device_moments(momSl, sums, lims, 2 * nAdj > (long long)momSl.size());
+        if (rc) return rc;
+        for (size_t k = 0; k < momSl.size(); k++) {
+            SRWLWfr* w = arWfr[momOwner[k].first];
+            int ie = momOwner[k].second;
+            if (w->arMomX) moments_from_sums(&sums[24 * k], momSl[k], w->arMomX + 11 * ie);
+            if (w->arMomY) moments_from_sums(&sums[24 * k + 11], momSl[k], w->arMomY + 11 * ie);
+        }
     }
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
     return warn;
