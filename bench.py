@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--me-n", type=int, default=512, help="multi-electron emission mesh")
     ap.add_argument("--me-max", type=int, default=4096, help="electron cap per multi-electron run")
     ap.add_argument("--uneven", type=int, default=2048, help="requests of the uneven batch (0: skip)")
+    ap.add_argument("--prop", type=int, default=1, help="also run the resample + drift leg on the SR wavefront")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -52,10 +53,16 @@ def copy_calibration(mib):
     n = mib * (1 << 20) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    b.copy_(a)
+    e0.record()
     for _ in range(5):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}))
+    ms = e0.elapsed_time(e1) / 5
+    print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 6, "ms_per_copy": ms,
+                      "read_plus_write_GBps": 2 * n * 4 / (ms * 1e-3) / 1e9}))
 
 
 def multi_e_runs(srwlpy, W, n, nmax):
@@ -103,6 +110,17 @@ def sr_traffic(n, ne):
     return float(m.group(1)) if m else None
 
 
+def fft_traffic(n, slices):
+    """HBM bytes of one drift (its three FFT passes) from the committed PMC
+    summary for this size, when measured; None otherwise."""
+    import re
+    path = os.path.join(ROOT, "profiles", "fft_drift_%dx%dx%d_traffic.md" % (n, n, slices))
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_drift\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def uneven_batch(srwlpy, W, n):
     """n independent requests (meshes 128^2..1024^2, harmonics 1-5, K in
     [0.5, 2.5]) computed in one device pass; fields left in HBM."""
@@ -121,6 +139,42 @@ def uneven_batch(srwlpy, W, n):
             "sr_kernel_tflops": FLOPS_PER_EVAL * st["sr_evals"] / (st["sr_ms"] * 1e-3) / 1e12 if st["sr_ms"] else None,
             "workload": "%d requests, meshes 128^2-1024^2 over +/-1 mm at 20 m, harmonic 1-5 of K in [0.5, 2.5], "
                         "one device pass, fields left in HBM" % n}
+
+
+def c2_propagation(srwlpy, w, n_out=1024, L=20.):
+    """Propagation leg of the 201-energy workload: the SR wavefront (already in
+    host memory) resampled from n x n to n_out x n_out (range magnified, step
+    kept) and drifted L metres with the quadratic phase treated analytically
+    (arProp[3] = 1, radius from the wavefront moments)."""
+    from srw_amd.srwlib import SRWLOptC, SRWLOptD
+    n, ne = w.mesh.nx, w.mesh.ne
+    pm = (n_out - 3) / float(n)       # rounds to a size whose next good FFT size is n_out
+    opt = SRWLOptC([SRWLOptD(L)], [[0, 0, 1.0, 1, 0, pm, 1.0, pm, 1.0]])
+    srwlpy.reset_stats()
+    t0 = time.perf_counter()
+    srwlpy.PropagElecField(w, opt)
+    wall = time.perf_counter() - t0
+    st = srwlpy.stats()
+    if (w.mesh.nx, w.mesh.ny) != (n_out, n_out):
+        raise RuntimeError("propagation produced a %dx%d mesh" % (w.mesh.nx, w.mesh.ny))
+    slices = 2 * ne
+    alg = 64. * n_out * n_out * slices        # 128 N^2 bytes per energy (2 components x 2 FFTs x 2 sweeps x 16 B)
+    fft_ms = st["fft_ms"]
+    return {
+        "workload": "%d energies, %dx%d resampled to %dx%d (range x%.3f), %g m drift, arProp[3] = 1"
+                    % (ne, n, n, n_out, n_out, pm, L),
+        "seconds_wall_incl_host_copies": wall,
+        "device_ms": st["prop_device_ms"],
+        "resize_ms": st["resize_ms"],
+        "resize_gbs": st["resize_bytes"] / (st["resize_ms"] * 1e-3) / 1e9 if st["resize_ms"] else None,
+        "drift_ms": fft_ms,
+        "roofline_fft": {"bound": "hbm", "achieved": alg / (fft_ms * 1e-3) / 1e9 if fft_ms else None,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": alg / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if fft_ms else None,
+                         "traffic": fft_traffic(n_out, slices), "kernels": "k_rows_fwd + k_cols + k_rows_inv",
+                         "algorithmic_bytes": alg,
+                         "design_pass_bytes": st["fft_bytes"]},
+    }
 
 
 def cpu_baseline(n, ne_sample, threads):
@@ -172,6 +226,10 @@ def main():
     srwlpy.CalcElecFieldSR(w, 0, mag, prec)
     pcie_rate = pts / (time.perf_counter() - t1)
     log("single-electron field: %.4g points/s" % value)
+    prop = None
+    if a.prop:
+        prop = c2_propagation(srwlpy, w)
+        log("propagation leg done")
     me = multi_e_runs(srwlpy, W, a.me_n, a.me_max) if a.multi_e else None
     if me:
         log("multi-electron runs done")
@@ -206,6 +264,7 @@ def main():
                      "kernel": "k_sr_integral", "kernel_ms": k_ms, "evals_per_launch": evals,
                      "flops_per_eval": FLOPS_PER_EVAL},
         "pcie_inclusive_points_per_s": pcie_rate,
+        "propagation": prop,
         "multi_electron": me,
         "uneven_batch": ub,
         "cpu_baseline": cpu,

@@ -25,6 +25,9 @@
 
 namespace {
 
+// Global loads issued back to back per thread before their values are used.
+#define SRWF_BATCH 8
+
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -36,9 +39,26 @@ __device__ __forceinline__ float2 cis_rev(double rev)
     return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
 }
 
+// LDS position of element i of a line: one pad slot after every 8 elements,
+// so the strided accesses of the radix-8 rounds (strides 1, 2, 8 elements)
+// fall on distinct banks.  A line of N elements spans line_pitch(N) slots.
+__device__ __forceinline__ int pad_idx(int i) { return i + (i >> 3); }
+__host__ __device__ __forceinline__ int line_pitch(int n) { return n + n / 8 + 1; }
+
 __device__ __forceinline__ int bitrev(int v, int logn)
 {
     return (int)(__builtin_bitreverse32((unsigned)v) >> (32 - logn));
+}
+
+// LDS twiddle table: tw[k] = exp(-i 2pi k / N) for k < N/2, then for each
+// H = 2^lH <= N/8 the H twiddles w_{2H}^t (t < H) stored contiguously at
+// N/2 + H - 1 + t.  The lanes of a stage read consecutive or repeated
+// entries instead of entries N/2H apart, which would fall on one LDS bank.
+__host__ __device__ __forceinline__ int tw_slots(int n) { return n / 2 + n / 4 - 1; }
+
+__device__ __forceinline__ int tw_index(int logn, int lH, int t)
+{
+    return (lH >= logn - 2) ? (t << (logn - 1 - lH)) : (1 << (logn - 1)) + (1 << lH) - 1 + t;
 }
 
 // Twiddle w_{2H}^t = exp(-+i 2pi t / 2H) = tw[t * (N / 2H)] (tw[k] = exp(-i 2pi k / N)).
@@ -50,74 +70,72 @@ __device__ __forceinline__ float2 twid(const float2* tw, int k)
     return w;
 }
 
-// One register round of DIF stages with half sizes H = b*2^(K-1) ... b on
-// groups of 2^K elements spaced b apart.  Lines of N elements with pitch P.
+// One register round of DIF stages with half sizes H = b*2^(K-1) ... b
+// (b = 2^lb) on groups of 2^K elements spaced b apart.  Lines of N = 2^logn
+// elements with pitch P.  All sizes are powers of two, so the index
+// arithmetic is shifts and masks.
 template <int K, bool INV>
-__device__ __forceinline__ void dif_round(float2* s, const float2* tw, int logn, int b, int nlines, int P, int tid,
+__device__ __forceinline__ void dif_round(float2* s, const float2* tw, int logn, int lb, int nlines, int P, int tid,
                                           int nthr)
 {
-    const int N = 1 << logn;
     const int R = 1 << K;
-    const int groupsPerLine = N >> K;
-    const int hTop = b << (K - 1);
-    for (int G = tid; G < nlines * groupsPerLine; G += nthr) {
-        int line = G / groupsPerLine, g = G - line * groupsPerLine;
-        int j = (g / b) * (2 * hTop) + (g % b);
+    const int b = 1 << lb;
+    const int lg = logn - K;   // log2 of the groups per line
+    for (int G = tid; G < (nlines << lg); G += nthr) {
+        int line = G >> lg, g = G & ((1 << lg) - 1);
+        int j = ((g >> lb) << (lb + K)) + (g & (b - 1));
         float2* L = s + line * P;
         float2 x[R];
 #pragma unroll
-        for (int m = 0; m < R; m++) x[m] = L[j + m * b];
+        for (int m = 0; m < R; m++) x[m] = L[pad_idx(j + m * b)];
 #pragma unroll
         for (int r = K - 1; r >= 0; r--) {
-            int H = b << r;
-            int tstep = N / (2 * H);
+            const int lH = lb + r;   // H = 2^lH
 #pragma unroll
             for (int m = 0; m < R; m++) {
                 if (m & (1 << r)) continue;
                 int p = j + m * b;
                 float2 a = x[m], c = x[m + (1 << r)];
-                float2 w = twid<INV>(tw, (p & (H - 1)) * tstep);
+                float2 w = twid<INV>(tw, tw_index(logn, lH, p & ((1 << lH) - 1)));
                 x[m] = make_float2(a.x + c.x, a.y + c.y);
                 x[m + (1 << r)] = cmulf(make_float2(a.x - c.x, a.y - c.y), w);
             }
         }
 #pragma unroll
-        for (int m = 0; m < R; m++) L[j + m * b] = x[m];
+        for (int m = 0; m < R; m++) L[pad_idx(j + m * b)] = x[m];
     }
 }
 
 // One register round of DIT (inverse) stages with half sizes b ... b*2^(K-1).
 template <int K>
-__device__ __forceinline__ void dit_round(float2* s, const float2* tw, int logn, int b, int nlines, int P, int tid,
+__device__ __forceinline__ void dit_round(float2* s, const float2* tw, int logn, int lb, int nlines, int P, int tid,
                                           int nthr)
 {
-    const int N = 1 << logn;
     const int R = 1 << K;
-    const int groupsPerLine = N >> K;
-    const int hTop = b << (K - 1);
-    for (int G = tid; G < nlines * groupsPerLine; G += nthr) {
-        int line = G / groupsPerLine, g = G - line * groupsPerLine;
-        int j = (g / b) * (2 * hTop) + (g % b);
+    const int b = 1 << lb;
+    const int lg = logn - K;
+    for (int G = tid; G < (nlines << lg); G += nthr) {
+        int line = G >> lg, g = G & ((1 << lg) - 1);
+        int j = ((g >> lb) << (lb + K)) + (g & (b - 1));
         float2* L = s + line * P;
         float2 x[R];
 #pragma unroll
-        for (int m = 0; m < R; m++) x[m] = L[j + m * b];
+        for (int m = 0; m < R; m++) x[m] = L[pad_idx(j + m * b)];
 #pragma unroll
         for (int r = 0; r < K; r++) {
-            int H = b << r;
-            int tstep = N / (2 * H);
+            const int lH = lb + r;
 #pragma unroll
             for (int m = 0; m < R; m++) {
                 if (m & (1 << r)) continue;
                 int p = j + m * b;
-                float2 w = twid<true>(tw, (p & (H - 1)) * tstep);
+                float2 w = twid<true>(tw, tw_index(logn, lH, p & ((1 << lH) - 1)));
                 float2 a = x[m], c = cmulf(x[m + (1 << r)], w);
                 x[m] = make_float2(a.x + c.x, a.y + c.y);
                 x[m + (1 << r)] = make_float2(a.x - c.x, a.y - c.y);
             }
         }
 #pragma unroll
-        for (int m = 0; m < R; m++) L[j + m * b] = x[m];
+        for (int m = 0; m < R; m++) L[pad_idx(j + m * b)] = x[m];
     }
 }
 
@@ -129,11 +147,11 @@ __device__ void lds_dif(float2* s, const float2* tw, int logn, int nlines, int P
     while (done < logn) {
         int rem = logn - done;
         int K = rem >= 3 ? 3 : rem;
-        int b = 1 << (logn - done - K);   // smallest half size of this round
+        int lb = logn - done - K;   // log2 of the smallest half size of this round
         __syncthreads();
-        if (K == 3) dif_round<3, INV>(s, tw, logn, b, nlines, P, tid, nthr);
-        else if (K == 2) dif_round<2, INV>(s, tw, logn, b, nlines, P, tid, nthr);
-        else dif_round<1, INV>(s, tw, logn, b, nlines, P, tid, nthr);
+        if (K == 3) dif_round<3, INV>(s, tw, logn, lb, nlines, P, tid, nthr);
+        else if (K == 2) dif_round<2, INV>(s, tw, logn, lb, nlines, P, tid, nthr);
+        else dif_round<1, INV>(s, tw, logn, lb, nlines, P, tid, nthr);
         done += K;
     }
     __syncthreads();
@@ -146,11 +164,11 @@ __device__ void lds_dit(float2* s, const float2* tw, int logn, int nlines, int P
     while (done < logn) {
         int rem = logn - done;
         int K = rem >= 3 ? 3 : rem;
-        int b = 1 << done;
+        int lb = done;
         __syncthreads();
-        if (K == 3) dit_round<3>(s, tw, logn, b, nlines, P, tid, nthr);
-        else if (K == 2) dit_round<2>(s, tw, logn, b, nlines, P, tid, nthr);
-        else dit_round<1>(s, tw, logn, b, nlines, P, tid, nthr);
+        if (K == 3) dit_round<3>(s, tw, logn, lb, nlines, P, tid, nthr);
+        else if (K == 2) dit_round<2>(s, tw, logn, lb, nlines, P, tid, nthr);
+        else dit_round<1>(s, tw, logn, lb, nlines, P, tid, nthr);
         done += K;
     }
     __syncthreads();
@@ -159,43 +177,61 @@ __device__ void lds_dit(float2* s, const float2* tw, int logn, int nlines, int P
 __device__ __forceinline__ void load_twiddles(float2* stw, const float2* tw, int n)
 {
     for (int i = threadIdx.x; i < n / 2; i += blockDim.x) stw[i] = tw[i];
+    // the per-stage tables for H = 1, 2, ..., n/8
+    const int logn = 31 - __clz(n);
+    for (int i = threadIdx.x; i < n / 4 - 1; i += blockDim.x) {
+        int lH = 31 - __clz(i + 1), t = i + 1 - (1 << lH);
+        stw[n / 2 + i] = tw[t << (logn - 1 - lH)];
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Pass A: rows, pre-multiply, forward DIF along x.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_rows_fwd(float2* data, long long sliceStride, const srwf_slice* __restrict__ sp,
+__global__ void __launch_bounds__(1024) k_rows_fwd(float2* data, long long sliceStride, const srwf_slice* __restrict__ sp,
                                                   const float2* __restrict__ tw, int logn, int ny, int rowsPerWG,
                                                   int preOp, int inv)
 {
     extern __shared__ float2 smem[];
     const int N = 1 << logn;
-    const int P = N + 1;
+    const int P = line_pitch(N);
     float2* stw = smem;
-    float2* s = smem + N / 2;
+    float2* s = smem + tw_slots(N);
     load_twiddles(stw, tw, N);
     const int slice = blockIdx.y;
     const srwf_slice& S = sp[slice];
     float2* base = data + slice * sliceStride;
     const int row0 = blockIdx.x * rowsPerWG;
     const int nrows = min(rowsPerWG, ny - row0);
-    for (int e = threadIdx.x; e < nrows * N; e += blockDim.x) {
-        int r = e / N, ix = e - r * N, iy = row0 + r;
-        float2 v = base[(long long)iy * N + ix];
-        if (preOp == 1) {   // quadratic phase removal, sroptdrf.h:812-821
-            double rx = S.xStart + ix * S.xStep - S.xc, rz = S.zStart + iy * S.zStep - S.zc;
-            v = cmulf(v, cis_rev(S.pre_rev * (S.invRx * rx * rx + S.invRz * rz * rz)));
-        } else if (preOp == 2) {   // linear-phase pre-shift of a non-centred input (gmfft.h:250-276)
-            v = cmulf(v, cis_rev(S.sh0x * ((ix - (N >> 1)) * S.xStep) + S.sh0y * ((iy - (ny >> 1)) * S.zStep)));
+    const int total = nrows * N;
+    for (int e0 = threadIdx.x; e0 < total; e0 += SRWF_BATCH * blockDim.x) {
+        float2 vb[SRWF_BATCH];   // all loads of a batch in flight before any is used
+#pragma unroll
+        for (int q = 0; q < SRWF_BATCH; q++) {
+            int e = e0 + q * blockDim.x;
+            if (e < total) vb[q] = base[(long long)row0 * N + e];
         }
-        if ((ix + iy) & 1) { v.x = -v.x; v.y = -v.y; }
-        s[r * P + ix] = v;
+#pragma unroll
+        for (int q = 0; q < SRWF_BATCH; q++) {
+            int e = e0 + q * blockDim.x;
+            if (e >= total) break;
+            int r = e >> logn, ix = e & (N - 1), iy = row0 + r;
+            float2 v = vb[q];
+            if (preOp == 1) {   // quadratic phase removal, sroptdrf.h:812-821
+                double rx = S.xStart + ix * S.xStep - S.xc, rz = S.zStart + iy * S.zStep - S.zc;
+                v = cmulf(v, cis_rev(S.pre_rev * (S.invRx * rx * rx + S.invRz * rz * rz)));
+            } else if (preOp == 2) {   // linear-phase pre-shift of a non-centred input (gmfft.h:250-276)
+                v = cmulf(v, cis_rev(S.sh0x * ((ix - (N >> 1)) * S.xStep) + S.sh0y * ((iy - (ny >> 1)) * S.zStep)));
+            }
+            if ((ix + iy) & 1) { v.x = -v.x; v.y = -v.y; }
+            s[r * P + pad_idx(ix)] = v;
+        }
     }
     if (inv) lds_dif<true>(s, stw, logn, nrows, P, threadIdx.x, blockDim.x);
     else lds_dif<false>(s, stw, logn, nrows, P, threadIdx.x, blockDim.x);
     for (int e = threadIdx.x; e < nrows * N; e += blockDim.x) {
-        int r = e / N, ix = e - r * N;
-        base[(long long)(row0 + r) * N + ix] = s[r * P + ix];
+        int r = e >> logn, ix = e & (N - 1);
+        base[(long long)(row0 + r) * N + ix] = s[r * P + pad_idx(ix)];
     }
 }
 
@@ -205,29 +241,45 @@ __global__ void __launch_bounds__(256) k_rows_fwd(float2* data, long long sliceS
 // centred-output post-multiply and natural-order output (standalone FFT).
 // Column positions hold bit-reversed x frequencies.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_cols(float2* data, long long sliceStride, const srwf_slice* __restrict__ sp,
+__global__ void __launch_bounds__(1024) k_cols(float2* data, long long sliceStride, const srwf_slice* __restrict__ sp,
                                               const float2* __restrict__ twy, int logny, int lognx, int colsPerWG,
                                               int op, int inv, float2* out)
 {
     extern __shared__ float2 smem[];
     const int NY = 1 << logny, NX = 1 << lognx;
-    const int P = NY + 1;
+    const int lc = 31 - __clz(colsPerWG);   // colsPerWG is a power of two
+    const int P = line_pitch(NY);
     float2* stw = smem;
-    float2* s = smem + NY / 2;
+    float2* s = smem + tw_slots(NY);
     load_twiddles(stw, twy, NY);
     const int slice = blockIdx.y;
     const srwf_slice& S = sp[slice];
     float2* base = data + slice * sliceStride;
     const int c0 = blockIdx.x * colsPerWG;
-    for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
-        int r = e / colsPerWG, c = e - r * colsPerWG;
-        s[c * P + r] = base[(long long)r * NX + c0 + c];
+    const int total = NY * colsPerWG;
+    for (int e0 = threadIdx.x; e0 < total; e0 += SRWF_BATCH * blockDim.x) {
+        float2 vb[SRWF_BATCH];
+#pragma unroll
+        for (int q = 0; q < SRWF_BATCH; q++) {
+            int e = e0 + q * blockDim.x;
+            if (e < total) {
+                int r = e >> lc, c = e & (colsPerWG - 1);
+                vb[q] = base[(long long)r * NX + c0 + c];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < SRWF_BATCH; q++) {
+            int e = e0 + q * blockDim.x;
+            if (e >= total) break;
+            int r = e >> lc, c = e & (colsPerWG - 1);
+            s[c * P + pad_idx(r)] = vb[q];
+        }
     }
     if (op == 2 && inv) lds_dif<true>(s, stw, logny, colsPerWG, P, threadIdx.x, blockDim.x);
     else lds_dif<false>(s, stw, logny, colsPerWG, P, threadIdx.x, blockDim.x);
     if (op == 0 || op == 1) {
         for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
-            int c = e / NY, r = e - c * NY;
+            int c = e >> logny, r = e & (NY - 1);
             int kx = bitrev(c0 + c, lognx), ky = bitrev(r, logny);
             double qx = (kx - (NX >> 1)) * S.qStepX, qz = (ky - (NY >> 1)) * S.qStepZ;
             double rev;
@@ -239,19 +291,19 @@ __global__ void __launch_bounds__(256) k_cols(float2* data, long long sliceStrid
             }
             float2 w = cis_rev(rev);
             w.x *= S.midScale; w.y *= S.midScale;
-            s[c * P + r] = cmulf(s[c * P + r], w);
+            s[c * P + pad_idx(r)] = cmulf(s[c * P + pad_idx(r)], w);
         }
         lds_dit(s, stw, logny, colsPerWG, P, threadIdx.x, blockDim.x);
         for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
-            int r = e / colsPerWG, c = e - r * colsPerWG;
-            base[(long long)r * NX + c0 + c] = s[c * P + r];
+            int r = e >> lc, c = e & (colsPerWG - 1);
+            base[(long long)r * NX + c0 + c] = s[c * P + pad_idx(r)];
         }
     } else {
         // standalone transform: (-1)^(kx+ky+NX/2+NY/2) * scale * post-shift, natural order
         for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
-            int r = e / colsPerWG, c = e - r * colsPerWG;
+            int r = e >> lc, c = e & (colsPerWG - 1);
             int ky = r, kx = bitrev(c0 + c, lognx);
-            float2 v = s[c * P + bitrev(ky, logny)];
+            float2 v = s[c * P + pad_idx(bitrev(ky, logny))];
             float sg = ((kx + ky + (NX >> 1) + (NY >> 1)) & 1) ? -S.midScale : S.midScale;
             v.x *= sg; v.y *= sg;
             if (S.postShift) {
@@ -268,30 +320,45 @@ __global__ void __launch_bounds__(256) k_cols(float2* data, long long sliceStrid
 // 1: sign and new quadratic phase (mode 3, sroptdrf.h:846-860);
 // 2: as 1 but writes only |Ex|^2+|Ez|^2 of slice pairs (2k, 2k+1).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_rows_inv(float2* data, long long sliceStride, const srwf_slice* __restrict__ sp,
+__global__ void __launch_bounds__(1024) k_rows_inv(float2* data, long long sliceStride, const srwf_slice* __restrict__ sp,
                                                   const float2* __restrict__ tw, int logn, int ny, int rowsPerWG,
                                                   int postOp, float* inten, long long intenStride)
 {
     extern __shared__ float2 smem[];
     const int N = 1 << logn;
-    const int P = N + 1;
+    const int P = line_pitch(N);
     float2* stw = smem;
-    float2* s = smem + N / 2;
+    float2* s = smem + tw_slots(N);
     load_twiddles(stw, tw, N);
     const int pair = (postOp == 2);
     const int slice0 = blockIdx.y << pair;
     const int nsl = 1 + pair;
     const int row0 = blockIdx.x * rowsPerWG;
     const int nrows = min(rowsPerWG, ny - row0);
-    for (int e = threadIdx.x; e < nsl * nrows * N; e += blockDim.x) {
-        int l = e / N, ix = e - l * N;
-        int k = l / nrows, r = l - k * nrows;
-        s[l * P + ix] = data[(slice0 + k) * sliceStride + (long long)(row0 + r) * N + ix];
+    const int total = nsl * nrows * N;
+    for (int e0 = threadIdx.x; e0 < total; e0 += SRWF_BATCH * blockDim.x) {
+        float2 vb[SRWF_BATCH];
+#pragma unroll
+        for (int q = 0; q < SRWF_BATCH; q++) {
+            int e = e0 + q * blockDim.x;
+            if (e < total) {
+                int l = e >> logn, ix = e & (N - 1);
+                int k = l / nrows, r = l - k * nrows;
+                vb[q] = data[(slice0 + k) * sliceStride + (long long)(row0 + r) * N + ix];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < SRWF_BATCH; q++) {
+            int e = e0 + q * blockDim.x;
+            if (e >= total) break;
+            int l = e >> logn, ix = e & (N - 1);
+            s[l * P + pad_idx(ix)] = vb[q];
+        }
     }
     lds_dit(s, stw, logn, nsl * nrows, P, threadIdx.x, blockDim.x);
     const srwf_slice& S = sp[slice0];
     for (int e = threadIdx.x; e < nrows * N; e += blockDim.x) {
-        int r = e / N, ix = e - r * N, iy = row0 + r;
+        int r = e >> logn, ix = e & (N - 1), iy = row0 + r;
         float2 m = make_float2(((ix + iy) & 1) ? -1.f : 1.f, 0.f);
         if (postOp >= 1) {
             double rx = S.nxStart + ix * S.nxStep - S.xc, rz = S.nzStart + iy * S.nzStep - S.zc;
@@ -299,12 +366,12 @@ __global__ void __launch_bounds__(256) k_rows_inv(float2* data, long long sliceS
             m = make_float2(m.x * w.x, m.x * w.y);
         }
         if (postOp == 2) {
-            float2 a = s[r * P + ix], b = s[(nrows + r) * P + ix];
+            float2 a = s[r * P + pad_idx(ix)], b = s[(nrows + r) * P + pad_idx(ix)];
             // the unit-modulus phase does not change the intensity
             inten[blockIdx.y * intenStride + (long long)iy * N + ix] = a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y;
         } else {
             for (int k = 0; k < nsl; k++) {
-                float2 v = cmulf(s[(k * nrows + r) * P + ix], m);
+                float2 v = cmulf(s[(k * nrows + r) * P + pad_idx(ix)], m);
                 data[(slice0 + k) * sliceStride + (long long)iy * N + ix] = v;
             }
         }
@@ -329,7 +396,53 @@ __global__ void k_scatter(const float2* __restrict__ src, float2* dst, int ne, l
     dst[p * ne + ie] = src[i];
 }
 
-int lds_bytes(int n, int lines) { return (int)(sizeof(float2) * ((n / 2) + (size_t)lines * (n + 1))); }
+// Both components between SRW's [iy][ix][ie] order (ex, ez) and the
+// slice-major order d[(2*ie + c)*nxy + p], p = iy*nx + ix, through 64 x 64
+// LDS tiles: the energy-fastest side is read (or written) along ie, the
+// slice side along p, both coalesced.  Grid (ceil(nxy/64), ceil(ne/64), 2).
+__global__ void __launch_bounds__(256) k_to_slices(const float2* __restrict__ ex, const float2* __restrict__ ez,
+                                                   float2* d, int ne, long long nxy)
+{
+    __shared__ float2 tile[64][65];
+    const int c = blockIdx.z, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const float2* src = c ? ez : ex;
+    const long long p0 = (long long)blockIdx.x * 64;
+    const int e0 = blockIdx.y * 64;
+    for (int r = ty; r < 64; r += 4) {
+        long long p = p0 + r;
+        int e = e0 + tx;
+        if (p < nxy && e < ne) tile[r][tx] = src[p * ne + e];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        int e = e0 + r;
+        long long p = p0 + tx;
+        if (p < nxy && e < ne) d[(2LL * e + c) * nxy + p] = tile[tx][r];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_from_slices(const float2* __restrict__ d, float2* ex, float2* ez, int ne,
+                                                     long long nxy)
+{
+    __shared__ float2 tile[64][65];
+    const int c = blockIdx.z, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    float2* dst = c ? ez : ex;
+    const long long p0 = (long long)blockIdx.x * 64;
+    const int e0 = blockIdx.y * 64;
+    for (int r = ty; r < 64; r += 4) {
+        int e = e0 + r;
+        long long p = p0 + tx;
+        if (p < nxy && e < ne) tile[tx][r] = d[(2LL * e + c) * nxy + p];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        long long p = p0 + r;
+        int e = e0 + tx;
+        if (p < nxy && e < ne) dst[p * ne + e] = tile[r][tx];
+    }
+}
+
+int lds_bytes(int n, int lines) { return (int)(sizeof(float2) * (tw_slots(n) + (size_t)lines * line_pitch(n))); }
 
 }  // namespace
 
@@ -344,15 +457,27 @@ int srwf_init_attributes()
     return 0;
 }
 
-int srwf_rows_per_wg(int logn) { int n = 1 << logn; return n >= 2048 ? 1 : (2048 / n < 16 ? 2048 / n : 16); }
-int srwf_cols_per_wg(int logn) { int n = 1 << logn; return n >= 2048 ? 4 : (n >= 1024 ? 8 : 16); }
+// Tile shapes (measured on the 1024 x 1024 x 402-slice drift):
+// * rows: 2048 elements (two rows of 1024) per 256-thread workgroup;
+// * columns: 8 columns per 1024-thread workgroup from 1024 rows up (80 KB of
+//   LDS, two workgroups per CU overlap loading and transforming), 16 columns
+//   (128-byte row segments) below.
+#define SRWF_ROW_ELEMS 2048
+#define SRWF_ROWS_THREADS 256
+#define SRWF_COLS_BIG_THREADS 1024
+int srwf_rows_per_wg(int logn)
+{
+    int n = 1 << logn, r = SRWF_ROW_ELEMS / n;
+    return r < 1 ? 1 : (r > 16 ? 16 : r);
+}
+int srwf_cols_per_wg(int logn) { int n = 1 << logn; return n >= 1024 ? 8 : 16; }
 
 int srwf_launch_rows_fwd(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* tw,
                          int logn, int ny, int preOp, int inv, hipStream_t st)
 {
     int rpw = srwf_rows_per_wg(logn);
     dim3 grid((ny + rpw - 1) / rpw, nSlices);
-    hipLaunchKernelGGL(k_rows_fwd, grid, dim3(256), lds_bytes(1 << logn, rpw), st, data, sliceStride, dSp, tw, logn,
+    hipLaunchKernelGGL(k_rows_fwd, grid, dim3(SRWF_ROWS_THREADS), lds_bytes(1 << logn, rpw), st, data, sliceStride, dSp, tw, logn,
                        ny, rpw, preOp, inv);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -364,7 +489,8 @@ int srwf_launch_cols(float2* data, long long sliceStride, int nSlices, const srw
     int nx = 1 << lognx;
     if (cpw > nx) cpw = nx;
     dim3 grid(nx / cpw, nSlices);
-    hipLaunchKernelGGL(k_cols, grid, dim3(256), lds_bytes(1 << logny, cpw), st, data, sliceStride, dSp, twy, logny,
+    const int thr = ((1 << logny) * cpw >= 8192) ? SRWF_COLS_BIG_THREADS : 256;
+    hipLaunchKernelGGL(k_cols, grid, dim3(thr), lds_bytes(1 << logny, cpw), st, data, sliceStride, dSp, twy, logny,
                        lognx, cpw, op, inv, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -376,8 +502,16 @@ int srwf_launch_rows_inv(float2* data, long long sliceStride, int nSlices, const
     if (postOp == 2) rpw = rpw > 1 ? rpw / 2 : 1;
     int nl = (postOp == 2 ? 2 : 1) * rpw;
     dim3 grid((ny + rpw - 1) / rpw, postOp == 2 ? nSlices / 2 : nSlices);
-    hipLaunchKernelGGL(k_rows_inv, grid, dim3(256), lds_bytes(1 << logn, nl), st, data, sliceStride, dSp, tw, logn,
+    hipLaunchKernelGGL(k_rows_inv, grid, dim3(SRWF_ROWS_THREADS), lds_bytes(1 << logn, nl), st, data, sliceStride, dSp, tw, logn,
                        ny, rpw, postOp, inten, intenStride);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwf_launch_slices(float2* ex, float2* ez, float2* d, int ne, long long nxy, int toSlices, hipStream_t st)
+{
+    dim3 grid((unsigned)((nxy + 63) / 64), (unsigned)((ne + 63) / 64), 2);
+    if (toSlices) hipLaunchKernelGGL(k_to_slices, grid, dim3(256), 0, st, ex, ez, d, ne, nxy);
+    else hipLaunchKernelGGL(k_from_slices, grid, dim3(256), 0, st, d, ex, ez, ne, nxy);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

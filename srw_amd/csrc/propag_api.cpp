@@ -197,16 +197,9 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         for (int c = 0; c < 2; c++)
             HIPCHK(hipMemcpyAsync(dS + c * nxy, dIn + c * n, sizeof(float2) * nxy, hipMemcpyDeviceToDevice, st));
     } else {
-        // gather each component into [ie][iy][ix], then interleave components per energy
-        float2* tmp = (float2*)g_ctx.buf.W3.get(sizeof(float2) * 2 * n);
-        if (!tmp) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-        for (int c = 0; c < 2; c++)
-            if (srwf_launch_gather(dIn + c * n, tmp + c * n, (int)ne, nxy, 0, st)) return SRWL_AMD_ERR_GPU;
-        for (long ie = 0; ie < ne; ie++)
-            for (int c = 0; c < 2; c++)
-                HIPCHK(hipMemcpyAsync(dS + (2 * ie + c) * nxy, tmp + c * n + ie * nxy, sizeof(float2) * nxy,
-                                      hipMemcpyDeviceToDevice, st));
+        if (srwf_launch_slices(dIn, dIn + n, dS, (int)ne, nxy, 1, st)) return SRWL_AMD_ERR_GPU;
     }
+    if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev2, st));
     double xStep = (pWfr->mesh.xFin - pWfr->mesh.xStart) / (nx - 1);
     double zStep = (pWfr->mesh.yFin - pWfr->mesh.yStart) / (nz - 1);
     double eStep = ne > 1 ? (pWfr->mesh.eFin - pWfr->mesh.eStart) / (ne - 1) : 0;
@@ -238,8 +231,18 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         float2* dN = (float2*)bufSpare->get(sizeof(float2) * 2 * nxyN * ne);
         if (!dN) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
         srwr_layout lo = {1, nx, 2 * nxy}, ln = {1, P.x.n, 2 * nxyN};
+        if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
         r = device_resize(*pWfr, P, dS, dS + nxy, lo, dN, dN + nxyN, ln, st);
         if (r) return r;
+        if (g_ctx.timing) {
+            HIPCHK(hipEventRecord(g_ctx.ev1, st));
+            HIPCHK(hipEventSynchronize(g_ctx.ev1));
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
+            g_ctx.lastRsMs = ms;
+            // phase removal (read + write old), resampling (read old, write new), phase (read + write new)
+            g_ctx.lastRsBytes = 8. * 2. * ne * (2. * nxy + (double)nxy + (double)nxyN + 2. * (double)nxyN);
+        }
         std::swap(bufCur, bufSpare);
         dS = dN;
         nx = P.x.n; nz = P.z.n; nxy = nxyN; n = nxy * ne;
@@ -275,6 +278,8 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
             sp[2 * ie + 1] = sp[2 * ie];
         }
         HIPCHK(hipMemcpyAsync(dSp, sp.data(), sizeof(srwf_slice) * 2 * ne, hipMemcpyHostToDevice, st));
+        // twiddle tables are built and uploaded on first use of a size: outside the timed passes
+        if (!twiddles(lx) || !twiddles(lz)) return SRWL_AMD_ERR_GPU;
         if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
         rc = drift_passes(dS, nxy, (int)(2 * ne), dSp, lx, lz, mode, 1, nullptr, 0, st);
         if (rc) return rc;
@@ -305,6 +310,12 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         rc = resize_device(pOpt->arProp[pOpt->nElem], propN(pOpt->nElem));
         if (rc) return rc;
     }
+    if (ne > 1) {
+        dIn = (float2*)g_ctx.buf.W0.get(sizeof(float2) * 2 * n);
+        if (!dIn) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        if (srwf_launch_slices(dIn, dIn + n, dS, (int)ne, nxy, 0, st)) return SRWL_AMD_ERR_GPU;
+    }
+    if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev3, st));
     if (nx != nx0 || nz != nz0) {
         // the caller's arrays are reallocated for the new mesh (srradstr.cpp:760-790)
         if (g_wfrModif == nullptr) return SRWL_WFR_EXT_FUNC_NOT_DEFINED;
@@ -315,19 +326,15 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         HIPCHK(hipMemcpyAsync(pWfr->arEx, dS, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(pWfr->arEy, dS + nxy, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
     } else {
-        float2* tmp = (float2*)g_ctx.buf.W3.get(sizeof(float2) * 2 * n);
-        dIn = (float2*)g_ctx.buf.W0.get(sizeof(float2) * 2 * n);
-        if (!tmp || !dIn) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-        for (long ie = 0; ie < ne; ie++)
-            for (int c = 0; c < 2; c++)
-                HIPCHK(hipMemcpyAsync(tmp + c * n + ie * nxy, dS + (2 * ie + c) * nxy, sizeof(float2) * nxy,
-                                      hipMemcpyDeviceToDevice, st));
-        for (int c = 0; c < 2; c++)
-            if (srwf_launch_gather(tmp + c * n, dIn + c * n, (int)ne, nxy, 1, st)) return SRWL_AMD_ERR_GPU;
         HIPCHK(hipMemcpyAsync(pWfr->arEx, dIn, sizeof(float2) * n, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(pWfr->arEy, dIn + n, sizeof(float2) * n, hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
+    if (g_ctx.timing) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, g_ctx.ev2, g_ctx.ev3);
+        g_ctx.lastPropDevMs = ms;
+    }
     return 0;
 }
 

@@ -19,20 +19,43 @@
 
 namespace {
 
+// Element order of a launch over (ix, iz, ie): x fastest when the layout's
+// x stride is 1 (slice-major fields), energy fastest otherwise (SRW's
+// order), so that consecutive threads touch consecutive addresses.
+__device__ __forceinline__ void split_index(long long i, const srwr_layout& l, int nx, int nz, int ne, int& ix,
+                                            int& iz, int& ie)
+{
+    if (l.perX == 1) {
+        ix = (int)(i % nx);
+        long long q = i / nx;
+        iz = (int)(q % nz);
+        ie = (int)(q / nz);
+    } else {
+        ie = (int)(i % ne);
+        long long p = i / ne;
+        ix = (int)(p % nx);
+        iz = (int)(p / nx);
+    }
+}
+
+__device__ __forceinline__ void phase_cs(const srwr_phase& P, int ix, int iz, int ie, float& c, float& s)
+{
+    double ePh = P.eStart + P.eStep * ie;
+    double ph = 0.;
+    if (P.fz) { double z = (P.zStart - P.zc) + iz * P.zStep; ph = P.sign * (SRW_QUAD_CONST / P.Rz) * ePh * z * z; }
+    if (P.fx) { double x = (P.xStart - P.xc) + ix * P.xStep; ph += P.sign * (SRW_QUAD_CONST / P.Rx) * ePh * x * x; }
+    cos_sin_optics(ph, c, s);
+}
+
 __global__ void k_phase(float2* ex, float2* ez, srwr_phase P)
 {
     long long n = (long long)P.nx * P.nz * P.ne;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int ie = (int)(i % P.ne);
-    long long p = i / P.ne;
-    int ix = (int)(p % P.nx), iz = (int)(p / P.nx);
-    double ePh = P.eStart + P.eStep * ie;
-    double ph = 0.;
-    if (P.fz) { double z = (P.zStart - P.zc) + iz * P.zStep; ph = P.sign * (SRW_QUAD_CONST / P.Rz) * ePh * z * z; }
-    if (P.fx) { double x = (P.xStart - P.xc) + ix * P.xStep; ph += P.sign * (SRW_QUAD_CONST / P.Rx) * ePh * x * x; }
+    int ix, iz, ie;
+    split_index(i, P.l, P.nx, P.nz, P.ne, ix, iz, ie);
     float c, s;
-    cos_sin_optics(ph, c, s);
+    phase_cs(P, ix, iz, ie, c, s);
     long long o = ie * P.l.perE + iz * P.l.perZ + ix * P.l.perX;
     if (ex) ex[o] = rotate_f(ex[o], c, s);
     if (ez) ez[o] = rotate_f(ez[o], c, s);
@@ -137,9 +160,8 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
     long long n = (long long)P.nxn * P.nzn * P.ne;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int ie = (int)(i % P.ne);
-    long long p = i / P.ne;
-    int ix = (int)(p % P.nxn), iz = (int)(p / P.nxn);
+    int ix, iz, ie;
+    split_index(i, P.ln, P.nxn, P.nzn, P.ne, ix, iz, ie);
     long long on = ie * P.ln.perE + iz * P.ln.perZ + ix * P.ln.perX;
     float2 vx = make_float2(0.f, 0.f), vz = make_float2(0.f, 0.f);
     if (ix >= P.ix0 && ix <= P.ix1 && iz >= P.iz0 && iz <= P.iz1) {
@@ -192,6 +214,12 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
                 vz = resample_point(re, im, ixc - ixSt, izc - izSt, xRel, zRel);
             }
         }
+    }
+    if (P.hasPost) {   // the same rounding as k_phase applied to the stored values
+        float c, s;
+        phase_cs(P.post, ix, iz, ie, c, s);
+        vx = rotate_f(vx, c, s);
+        vz = rotate_f(vz, c, s);
     }
     if (nex) nex[on] = vx;
     if (nez) nez[on] = vz;

@@ -128,11 +128,10 @@ int device_resize(const SRWLWfr& w, const ResizePlan& P, float2* oX, float2* oZ,
     R.eStep = w.mesh.ne > 1 ? (w.mesh.eFin - w.mesh.eStart) / (w.mesh.ne - 1) : 0.;
     R.ix0 = (int)P.x.i0; R.ix1 = (int)P.x.i1; R.iz0 = (int)P.z.i0; R.iz1 = (int)P.z.i1;
     R.onlyLarger = P.onlyLarger;
+    // the phase is added back to each new value before it is stored
+    R.hasPost = (P.fx || P.fz) ? 1 : 0;
+    if (R.hasPost) R.post = phase_desc(w, P.x.n, P.z.n, P.x.start, P.x.step, P.z.start, P.z.step, ln, P.fx, P.fz, 1);
     if (srwr_launch_resize(oX, oZ, nX, nZ, R, st)) return SRWL_AMD_ERR_GPU;
-    if (P.fx || P.fz) {
-        srwr_phase ph = phase_desc(w, P.x.n, P.z.n, P.x.start, P.x.step, P.z.start, P.z.step, ln, P.fx, P.fz, 1);
-        if (srwr_launch_phase(nX, nZ, ph, st)) return SRWL_AMD_ERR_GPU;
-    }
     return 0;
 }
 

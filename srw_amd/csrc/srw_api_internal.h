@@ -46,6 +46,8 @@ struct Context {
     double lastFftMs = 0, lastFftBytes = 0;
     double sumFftMs = 0, sumFftBytes = 0;
     int nFftCalls = 0;
+    double lastRsMs = 0, lastRsBytes = 0;   // last resampling of a propagation
+    double lastPropDevMs = 0;               // device part of the last propagation (after upload, before download)
 };
 
 extern Context g_ctx;

@@ -30,5 +30,9 @@ int srwf_launch_cols(float2* data, long long sliceStride, int nSlices, const srw
 int srwf_launch_rows_inv(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* tw,
                          int logn, int ny, int postOp, float* inten, long long intenStride, hipStream_t st);
 int srwf_launch_gather(const float2* src, float2* dst, int ne, long long nxy, int scatter, hipStream_t st);
+// Both field components between SRW's [iy][ix][ie] order (ex, ez) and the
+// slice-major d[(2*ie + c)*nxy + p]: toSlices = 1 reads ex/ez and writes d,
+// 0 reads d and writes ex/ez.
+int srwf_launch_slices(float2* ex, float2* ez, float2* d, int ne, long long nxy, int toSlices, hipStream_t st);
 int srwf_init_attributes();
 }

@@ -8,16 +8,6 @@ struct srwr_layout {
     long long perX, perZ, perE;
 };
 
-struct srwr_params {
-    srwr_layout lo, ln;                     // old / new layouts
-    int nxo, nzo, nxn, nzn, ne;
-    double xStartO, xStepO, zStartO, zStepO;
-    double xStartN, xStepN, zStartN, zStepN;
-    double eStart, eStep;
-    int ix0, ix1, iz0, iz1;                 // filled range of the new mesh
-    int onlyLarger;                         // copy without interpolation
-};
-
 // Quadratic phase exp(sign * i pi/lambda ((x-xc)^2/Rx + (z-zc)^2/Rz)) applied
 // in place with float storage (sign -1 removes, +1 adds).
 struct srwr_phase {
@@ -26,6 +16,18 @@ struct srwr_phase {
     double xStart, xStep, zStart, zStep, eStart, eStep;
     double Rx, Rz, xc, zc;
     int fx, fz, sign;
+};
+
+struct srwr_params {
+    srwr_layout lo, ln;                     // old / new layouts
+    int nxo, nzo, nxn, nzn, ne;
+    double xStartO, xStepO, zStartO, zStepO;
+    double xStartN, xStepN, zStartN, zStepN;
+    double eStart, eStep;
+    int ix0, ix1, iz0, iz1;                 // filled range of the new mesh
+    int onlyLarger;                         // copy without interpolation
+    int hasPost;                            // apply `post` to each new value before it is stored
+    srwr_phase post;                        // on the new mesh and layout
 };
 
 extern "C" {

@@ -687,6 +687,8 @@ SRWL_EXPORT int srwlAmdSetTiming(int on)
     if (on && !g_ctx.timing) {
         HIPCHK(hipEventCreate(&g_ctx.ev0));
         HIPCHK(hipEventCreate(&g_ctx.ev1));
+        HIPCHK(hipEventCreate(&g_ctx.ev2));
+        HIPCHK(hipEventCreate(&g_ctx.ev3));
     }
     g_ctx.timing = on != 0;
     return 0;
@@ -695,10 +697,11 @@ SRWL_EXPORT int srwlAmdSetTiming(int on)
 SRWL_EXPORT int srwlAmdGetStats(double* out, int n)
 {
     if (out == nullptr) return 0;
-    double v[12] = {g_ctx.lastSrMs, (double)g_ctx.lastSrEvals, (double)g_ctx.lastSrBlocks, (double)g_ctx.lastSrSplit,
+    double v[15] = {g_ctx.lastSrMs, (double)g_ctx.lastSrEvals, (double)g_ctx.lastSrBlocks, (double)g_ctx.lastSrSplit,
                     g_ctx.lastFftMs, g_ctx.lastFftBytes, g_ctx.sumSrMs, g_ctx.sumSrEvals, (double)g_ctx.nSrCalls,
-                    g_ctx.sumFftMs, g_ctx.sumFftBytes, (double)g_ctx.nFftCalls};
-    for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
+                    g_ctx.sumFftMs, g_ctx.sumFftBytes, (double)g_ctx.nFftCalls,
+                    g_ctx.lastRsMs, g_ctx.lastRsBytes, g_ctx.lastPropDevMs};
+    for (int i = 0; i < n && i < 15; i++) out[i] = v[i];
     return 0;
 }
 

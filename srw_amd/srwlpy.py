@@ -142,10 +142,10 @@ def CalcElecFieldSRBatch(_arWfr, _arMag, _prec_par):
 
 
 def stats():
-    out = (C.c_double * 12)()
-    lib.srwlAmdGetStats(out, 12)
+    out = (C.c_double * 15)()
+    lib.srwlAmdGetStats(out, 15)
     keys = ("sr_ms", "sr_evals", "sr_blocks", "sr_split", "fft_ms", "fft_bytes", "sum_sr_ms", "sum_sr_evals",
-            "n_sr", "sum_fft_ms", "sum_fft_bytes", "n_fft")
+            "n_sr", "sum_fft_ms", "sum_fft_bytes", "n_fft", "resize_ms", "resize_bytes", "prop_device_ms")
     return dict(zip(keys, list(out)))
 
 

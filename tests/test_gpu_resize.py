@@ -63,3 +63,25 @@ def test_propagation_with_resize_before_drift():
     assert rc == 0 and wo2.mesh.nx == 512
     assert oracle.propagate(wo2, W.drift_container(20., 1)) == 0
     _compare(wg, wo2, tol=2e-5)
+
+
+def test_propagation_multi_energy_resize_per_energy():
+    # the 201-energy workload in small: 5 energies of a 41x41 SR field,
+    # resampled to 128x128 (range x3.1, step kept) and drifted 20 m with the
+    # moments-based radius; the error is reported per photon energy
+    from srw_amd import srwlpy
+    print("case: 5 energies, 41^2 -> 128^2 resize + 20 m drift", flush=True)
+    w = _sr(41, ne=5, de=0.03)
+    pm = 127. / 41.
+    wg, wo = clone_wfr(w), clone_wfr(w)
+    opt = SRWLOptC([SRWLOptD(20.)], [[0, 0, 1., 1, 0, pm, 1., pm, 1.]])
+    srwlpy.PropagElecField(wg, opt)
+    rc, wo2 = oracle.resize(wo, [0, pm, 1., pm, 1., 0.5, 0.5])
+    assert rc == 0 and wo2.mesh.nx == 128
+    assert oracle.propagate(wo2, W.drift_container(20., 1)) == 0
+    _compare(wg, wo2, tol=2e-5)
+    exg, exo = cfield(wg.arEx, wg), cfield(wo2.arEx, wo2)
+    for ie in range(w.mesh.ne):
+        e = rel_l2(exg[:, :, ie], exo[:, :, ie])
+        print("energy %d: rel-L2 Ex %.3e" % (ie, e), flush=True)
+        assert e < 2e-5
