@@ -21,6 +21,7 @@ sys.path.insert(0, ROOT)
 FP32_VALU_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_EVAL = 64                 # algorithmic FP32-equivalent flops per integrand evaluation
+VALU_LANE_ISSUE_PER_S = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes per cycle x clock
 
 
 def log(msg):
@@ -121,6 +122,18 @@ def fft_traffic(n, slices):
     return float(m.group(1)) if m else None
 
 
+def sr_valu_per_eval(n, ne):
+    """Vector instructions per integrand evaluation of k_sr_integral from the
+    committed counter summary (SQ_INSTS_VALU x 64 / evaluations); None when
+    not measured for this workload."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_sr_integral_%dx%dx%d_sq_counters.md" % (n, n, ne))
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"valu_instr_per_eval\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def uneven_batch(srwlpy, W, n):
     """n independent requests (meshes 128^2..1024^2, harmonics 1-5, K in
     [0.5, 2.5]) computed in one device pass; fields left in HBM."""
@@ -143,13 +156,13 @@ def uneven_batch(srwlpy, W, n):
 
 def c2_propagation(srwlpy, w, n_out=1024, L=20.):
     """Propagation leg of the 201-energy workload: the SR wavefront (already in
-    host memory) resampled from n x n to n_out x n_out (range magnified, step
-    kept) and drifted L metres with the quadratic phase treated analytically
-    (arProp[3] = 1, radius from the wavefront moments)."""
+    host memory) resampled from n x n to n_out x n_out (range x2, i.e. +/-2 mm,
+    resolution x n_out/(2n)) and drifted L metres with the quadratic phase
+    treated analytically (arProp[3] = 1, radius from the wavefront moments)."""
     from srw_amd.srwlib import SRWLOptC, SRWLOptD
     n, ne = w.mesh.nx, w.mesh.ne
-    pm = (n_out - 3) / float(n)       # rounds to a size whose next good FFT size is n_out
-    opt = SRWLOptC([SRWLOptD(L)], [[0, 0, 1.0, 1, 0, pm, 1.0, pm, 1.0]])
+    pm, pd = 2.0, n_out / (2.0 * n)
+    opt = SRWLOptC([SRWLOptD(L)], [[0, 0, 1.0, 1, 0, pm, pd, pm, pd]])
     srwlpy.reset_stats()
     t0 = time.perf_counter()
     srwlpy.PropagElecField(w, opt)
@@ -161,8 +174,8 @@ def c2_propagation(srwlpy, w, n_out=1024, L=20.):
     alg = 64. * n_out * n_out * slices        # 128 N^2 bytes per energy (2 components x 2 FFTs x 2 sweeps x 16 B)
     fft_ms = st["fft_ms"]
     return {
-        "workload": "%d energies, %dx%d resampled to %dx%d (range x%.3f), %g m drift, arProp[3] = 1"
-                    % (ne, n, n, n_out, n_out, pm, L),
+        "workload": "%d energies, %dx%d resampled to %dx%d (range x%g, resolution x%.3f), %g m drift, arProp[3] = 1"
+                    % (ne, n, n, n_out, n_out, pm, pd, L),
         "seconds_wall_incl_host_copies": wall,
         "device_ms": st["prop_device_ms"],
         "resize_ms": st["resize_ms"],
@@ -220,6 +233,8 @@ def main():
     k_ms = st["sum_sr_ms"] / max(st["n_sr"], 1)
     evals = st["sum_sr_evals"] / max(st["n_sr"], 1)
     achieved = FLOPS_PER_EVAL * evals / (k_ms * 1e-3) / 1e12
+    vpe = sr_valu_per_eval(a.n, a.ne)
+    issue = vpe * evals / (k_ms * 1e-3) / VALU_LANE_ISSUE_PER_S if vpe else None
     # PCIe-inclusive rate (results copied to host buffers), reported aside
     srwlpy.set_keep_on_device(False)
     t1 = time.perf_counter()
@@ -262,7 +277,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": sr_traffic(a.n, a.ne),
                      "kernel": "k_sr_integral", "kernel_ms": k_ms, "evals_per_launch": evals,
-                     "flops_per_eval": FLOPS_PER_EVAL},
+                     "flops_per_eval": FLOPS_PER_EVAL, "valu_instr_per_eval": vpe, "valu_issue_frac": issue},
         "pcie_inclusive_points_per_s": pcie_rate,
         "propagation": prop,
         "multi_electron": me,

@@ -715,16 +715,18 @@ __global__ void __launch_bounds__(256) k_sr_tables(const srwd_trj_info* __restri
     }
     // composite-rule weight in units of h/15 (srradint.cpp:1576-1588)
     double w = (gi == 0 || gi == I.nInt - 1) ? 7. : ((gi & 1) ? 16. : 14.);
-    // w*A_x = w*inv^2*(Px - xo) with the projected position Px = X + Btx*(yObs - s)
+    // w*A_x = b*(Px - xo) with b = w*inv^2 and the projected position
+    // Px = X + Btx*(yObs - s); the record keeps b and b*Px (products in FP64)
     srwd_rec R;
     R.dInv = (float)(inv - invC);
     R.dU = (float)(U - UC);
     R.dV = (float)(V - VC);
-    R.Px = (float)(Xv + Btx * (I.yObs - s));
-    R.Pz = (float)(Zv + Btz * (I.yObs - s));
-    R.b = (float)(w * inv * inv);
+    double bd = w * inv * inv, Pxd = Xv + Btx * (I.yObs - s), Pzd = Zv + Btz * (I.yObs - s);
     R.pad0 = 0.f;
     R.pad1 = 0.f;
+    R.b = (float)bd;
+    R.bPx = (float)(bd * Pxd);
+    R.bPz = (float)(bd * Pzd);
     rec[tr * ldI + gi] = R;
     Qout[tr * ldI + gi] = Q;
     if (c0 == gi) {
@@ -777,6 +779,7 @@ __global__ void __launch_bounds__(256) k_phase_table(const double* __restrict__ 
 // reduced to [-1/2, 1/2]; the loop leaves Tc out and each chunk sum is turned
 // by exp(2 pi i Tc) once.
 #define SRWD_BLOCK 32
+static_assert(SRWD_CHUNK == 256, "k_sr_integral stages one chunk per 256-thread block");
 
 __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict__ items,
                                                      const int* __restrict__ blockItem,
@@ -798,7 +801,7 @@ __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict
     double kph = SRWD_KPH(IT.eph) * (0.5 / M_PI);
     double r2 = xo * xo + zo * zo;
     float alpha = (float)(kph * r2), beta = (float)(-2. * kph * xo), gam = (float)(-2. * kph * zo);
-    float fxo = (float)xo, fzo = (float)zo;
+    const float fxo = (float)xo, fzo = (float)zo;
     long long nInt = I.nInt;
     long long nChunk = (nInt + SRWD_CHUNK - 1) / SRWD_CHUNK;
     long long cBeg = (nChunk * blockIdx.y) / nSplit, cEnd = (nChunk * (blockIdx.y + 1)) / nSplit;
@@ -806,26 +809,39 @@ __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict
     const srwd_chunk* CH = chunks + (long long)IT.traj * ldC;
     const float* Ar = A + (long long)IT.arow * ldI;
     double ox = 0., oxi = 0., oz = 0., ozi = 0.;
+    // per chunk: {A, b, b Px, b Pz} of its points, staged once by the block
+    // (blockDim.x == SRWD_CHUNK) and read by every lane with the same address
+    __shared__ float4 sA[SRWD_CHUNK];
+    const float nfxo = -fxo, nfzo = -fzo;
     for (long long c = cBeg; c < cEnd; c++) {
         const srwd_chunk C = CH[c];
         double tc = kph * (r2 * C.inv - 2. * xo * C.U - 2. * zo * C.V);
         float Tc = (float)(tc - rint(tc));
         long long i0 = c * SRWD_CHUNK, i1 = i0 + SRWD_CHUNK;
         if (i1 > nInt) i1 = nInt;
+        const int n = (int)(i1 - i0);
+        const srwd_rec* Rc = R + i0;
+        __syncthreads();
+        if ((int)threadIdx.x < n) {
+            const srwd_rec& q = Rc[threadIdx.x];
+            sA[threadIdx.x] = make_float4(Ar[i0 + threadIdx.x], q.b, q.bPx, q.bPz);
+        }
+        __syncthreads();
         // chunk sums of b (P - r_o) exp(2 pi i (t - Tc)): 32-point FP32 block
         // sums added in FP32, then turned by exp(2 pi i Tc) into FP64
         float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
-        for (long long b0 = i0; b0 < i1; b0 += SRWD_BLOCK) {
+        for (int b0 = 0; b0 < n; b0 += SRWD_BLOCK) {
             float bx = 0.f, bxi = 0.f, bz = 0.f, bzi = 0.f;
-            long long b1 = b0 + SRWD_BLOCK < i1 ? b0 + SRWD_BLOCK : i1;
+            const int b1 = b0 + SRWD_BLOCK < n ? b0 + SRWD_BLOCK : n;
 #pragma unroll 4
-            for (long long i = b0; i < b1; i++) {
-                const srwd_rec r = R[i];
-                float t = fmaf(alpha, r.dInv, Ar[i]);
+            for (int k = b0; k < b1; k++) {
+                const srwd_rec r = Rc[k];        // scalar loads: dInv, dU, dV
+                const float4 L4 = sA[k];
+                float t = fmaf(alpha, r.dInv, L4.x);
                 t = fmaf(beta, r.dU, t);
                 t = fmaf(gam, r.dV, t);
                 float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
-                float wax = (r.Px - fxo) * r.b, waz = (r.Pz - fzo) * r.b;
+                float wax = fmaf(L4.y, nfxo, L4.z), waz = fmaf(L4.y, nfzo, L4.w);   // b (P - r_o)
                 bx = fmaf(wax, cs, bx); bxi = fmaf(wax, sn, bxi);
                 bz = fmaf(waz, cs, bz); bzi = fmaf(waz, sn, bzi);
             }
