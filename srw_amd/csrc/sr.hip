@@ -779,9 +779,10 @@ __global__ void __launch_bounds__(256) k_phase_table(const double* __restrict__ 
 // reduced to [-1/2, 1/2]; the loop leaves Tc out and each chunk sum is turned
 // by exp(2 pi i Tc) once.
 #define SRWD_BLOCK 32
+#define SRWD_ROWS 4
 static_assert(SRWD_CHUNK == 256, "k_sr_integral stages one chunk per 256-thread block");
 
-__global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict__ items,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_sr_integral(const srwd_item* __restrict__ items,
                                                      const int* __restrict__ blockItem,
                                                      const srwd_trj_info* __restrict__ info,
                                                      const srwd_rec* __restrict__ rec,
@@ -809,9 +810,18 @@ __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict
     const srwd_chunk* CH = chunks + (long long)IT.traj * ldC;
     const float* Ar = A + (long long)IT.arow * ldI;
     double ox = 0., oxi = 0., oz = 0., ozi = 0.;
-    // per chunk: {A, b, b Px, b Pz} of its points, staged once by the block
-    // (blockDim.x == SRWD_CHUNK) and read by every lane with the same address
-    __shared__ float4 sA[SRWD_CHUNK];
+    // Per chunk the block stages in LDS, for each of its points s, the data
+    // its lanes combine with their own x_o, z_o (blockDim.x == SRWD_CHUNK):
+    // * when the block's points lie on at most SRWD_ROWS mesh rows, one float4
+    //   per (row, s) = {A[s] + gam_row dV[s], b, b Px, b Pz}: the row's part of
+    //   the phase is formed once per block instead of once per lane;
+    // * otherwise one float4 per s = {A[s], b, b Px, b Pz}.
+    __shared__ float4 sA[SRWD_ROWS][SRWD_CHUNK];
+    const int first = (blockIdx.x - IT.blockBase) * 256;
+    const int last = min(first + 255, IT.npts - 1);
+    const int rLo = first / IT.nxc, nRows = last / IT.nxc - rLo + 1;
+    const bool byRow = nRows <= SRWD_ROWS;
+    const int rs = byRow && valid ? iz - rLo : 0;
     const float nfxo = -fxo, nfzo = -fzo;
     for (long long c = cBeg; c < cEnd; c++) {
         const srwd_chunk C = CH[c];
@@ -824,32 +834,57 @@ __global__ void __launch_bounds__(256) k_sr_integral(const srwd_item* __restrict
         __syncthreads();
         if ((int)threadIdx.x < n) {
             const srwd_rec& q = Rc[threadIdx.x];
-            sA[threadIdx.x] = make_float4(Ar[i0 + threadIdx.x], q.b, q.bPx, q.bPz);
+            const float a = Ar[i0 + threadIdx.x];
+            if (byRow) {
+                for (int rr = 0; rr < nRows; rr++) {
+                    // the same float as the lane's gam for that row
+                    float gr = (float)(-2. * kph * (IT.zStart + (rLo + rr) * IT.zStep));
+                    sA[rr][threadIdx.x] = make_float4(fmaf(gr, q.dV, a), q.b, q.bPx, q.bPz);
+                }
+            } else {
+                sA[0][threadIdx.x] = make_float4(a, q.b, q.bPx, q.bPz);
+            }
         }
         __syncthreads();
         // chunk sums of b (P - r_o) exp(2 pi i (t - Tc)): 32-point FP32 block
         // sums added in FP32, then turned by exp(2 pi i Tc) into FP64
         float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+        const float4* S = sA[rs];
         for (int b0 = 0; b0 < n; b0 += SRWD_BLOCK) {
             float bx = 0.f, bxi = 0.f, bz = 0.f, bzi = 0.f;
             const int b1 = b0 + SRWD_BLOCK < n ? b0 + SRWD_BLOCK : n;
+            if (byRow) {
 #pragma unroll 4
-            for (int k = b0; k < b1; k++) {
-                const srwd_rec r = Rc[k];        // scalar loads: dInv, dU, dV
-                const float4 L4 = sA[k];
-                float t = fmaf(alpha, r.dInv, L4.x);
-                t = fmaf(beta, r.dU, t);
-                t = fmaf(gam, r.dV, t);
-                float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
-                float wax = fmaf(L4.y, nfxo, L4.z), waz = fmaf(L4.y, nfzo, L4.w);   // b (P - r_o)
-                bx = fmaf(wax, cs, bx); bxi = fmaf(wax, sn, bxi);
-                bz = fmaf(waz, cs, bz); bzi = fmaf(waz, sn, bzi);
+                for (int k = b0; k < b1; k++) {
+                    const srwd_rec r = Rc[k];        // scalar loads: dInv, dU
+                    const float4 L4 = S[k];
+                    float t = fmaf(alpha, r.dInv, L4.x);
+                    t = fmaf(beta, r.dU, t);
+                    float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
+                    float wax = fmaf(L4.y, nfxo, L4.z), waz = fmaf(L4.y, nfzo, L4.w);   // b (P - r_o)
+                    bx = fmaf(wax, cs, bx); bxi = fmaf(wax, sn, bxi);
+                    bz = fmaf(waz, cs, bz); bzi = fmaf(waz, sn, bzi);
+                }
+            } else {
+#pragma unroll 4
+                for (int k = b0; k < b1; k++) {
+                    const srwd_rec r = Rc[k];        // scalar loads: dInv, dU, dV
+                    const float4 L4 = S[k];
+                    float t = fmaf(alpha, r.dInv, L4.x);
+                    t = fmaf(beta, r.dU, t);
+                    t = fmaf(gam, r.dV, t);
+                    float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
+                    float wax = fmaf(L4.y, nfxo, L4.z), waz = fmaf(L4.y, nfzo, L4.w);
+                    bx = fmaf(wax, cs, bx); bxi = fmaf(wax, sn, bxi);
+                    bz = fmaf(waz, cs, bz); bzi = fmaf(waz, sn, bzi);
+                }
             }
             q0 += bx; q1 += bxi; q2 += bz; q3 += bzi;
         }
+        double rxr = q0, rxi = q1, rzr = q2, rzi = q3;
         double cT = __builtin_amdgcn_cosf(Tc), sT = __builtin_amdgcn_sinf(Tc);
-        ox += q0 * cT - q1 * sT; oxi += q0 * sT + q1 * cT;
-        oz += q2 * cT - q3 * sT; ozi += q2 * sT + q3 * cT;
+        ox += rxr * cT - rxi * sT; oxi += rxr * sT + rxi * cT;
+        oz += rzr * cT - rzi * sT; ozi += rzr * sT + rzi * cT;
     }
     if (!valid) return;
     double* o = part + ((long long)blockIdx.y * nPointsPad + flat) * 4;

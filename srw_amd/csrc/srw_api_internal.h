@@ -48,6 +48,8 @@ struct Context {
     int nFftCalls = 0;
     double lastRsMs = 0, lastRsBytes = 0;   // last resampling of a propagation
     double lastPropDevMs = 0;               // device part of the last propagation (after upload, before download)
+    int* warnHost = nullptr;                // pinned: termination warning of the last SR pass
+    long long pendingEvals = 0;             // integrand evaluations of the last SR pass
 };
 
 extern Context g_ctx;
@@ -78,7 +80,10 @@ int field_from_c(const SRWLMagFldC* c, srwd_field& f, double& gsStart, double& g
 int setup_job_from_field(SrJob& J, const SRWLParticle& p, const SRWLMagFldC* mag, const double* precPar,
                          std::vector<srwd_field>& fields);
 int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
-           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin);
+           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin,
+           bool defer = false);
+// Completes a deferred sr_run once the stream has been synchronised.
+void sr_complete(std::vector<SrResult>& res);
 
 // Wavefront moments (moments_api.cpp).  Component element (ix, iz, ie) of a
 // device wavefront is at base[ie*perE + iz*perZ + ix*perX] (float2 units).

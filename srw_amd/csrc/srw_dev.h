@@ -60,10 +60,10 @@ struct srwd_trj_info {
 };
 
 // Per integration point, energy independent (FP32), 32 bytes: phase increments
-// relative to the chunk reference (read with scalar loads), then the weighted
+// relative to the chunk reference (read with scalar loads), the weighted
 // 1/R^2 b and b times the projected positions (staged in LDS per chunk).
 struct srwd_rec {
-    float dInv, dU, dV, pad0, b, bPx, bPz, pad1;
+    float dInv, dU, dV, b, bPx, bPz, pad0, pad1;
 };
 
 // Per chunk reference (FP64).

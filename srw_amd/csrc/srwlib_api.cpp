@@ -35,6 +35,8 @@ int ensure_device()
     if (dev >= n) return SRWL_AMD_ERR_GPU;
     if (hipSetDevice(dev) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    if (hipHostMalloc((void**)&g_ctx.warnHost, sizeof(int), hipHostMallocDefault) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    *g_ctx.warnHost = 0;
     g_ctx.device = dev;
     g_ctx.ready = true;
     return 0;
@@ -140,7 +142,7 @@ static int field_moments(SRWLWfr& w, const float2* dEx, const float2* dEz)
 // stay on the device: request j writes ne*nx*ny complex values of Ex and Ez
 // starting at float2 offset outOff[j] of dEx / dEz.
 int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
-           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin)
+           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin, bool defer)
 {
     int rc = ensure_device();
     if (rc) return rc;
@@ -362,23 +364,32 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     if (srwd_launch_finalize(dItems, dBlockItem, nBlocks, dInfo, dEdge, termin, dPart, nSplit, nPointsPad, dEx, dEz,
                              dWarn, st))
         return SRWL_AMD_ERR_GPU;
-    int warn = 0;
-    HIPCHK(hipMemcpyAsync(&warn, dWarn, sizeof(int), hipMemcpyDeviceToHost, st));
+    // the warning flag is read back with the caller's final synchronisation
+    HIPCHK(hipMemcpyAsync(g_ctx.warnHost, dWarn, sizeof(int), hipMemcpyDeviceToHost, st));
+    long long ev = 0;
+    for (auto& it : items) ev += (long long)it.npts * info[it.traj].nInt;
+    g_ctx.pendingEvals = ev;
+    if (defer) return 0;
     HIPCHK(hipStreamSynchronize(st));
+    sr_complete(res);
+    return 0;
+}
+
+// After the stream has been synchronised: timing statistics and the
+// warning of the last sr_run.
+void sr_complete(std::vector<SrResult>& res)
+{
     if (g_ctx.timing) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
         g_ctx.lastSrMs = ms;
-        long long ev = 0;
-        for (auto& it : items) ev += (long long)it.npts * info[it.traj].nInt;
-        g_ctx.lastSrEvals = ev;
+        g_ctx.lastSrEvals = g_ctx.pendingEvals;
         g_ctx.sumSrMs += ms;
-        g_ctx.sumSrEvals += (double)ev;
+        g_ctx.sumSrEvals += (double)g_ctx.pendingEvals;
         g_ctx.nSrCalls += 1;
     }
-    if (warn)
+    if (*g_ctx.warnHost)
         for (auto& r : res) if (!r.warn) r.warn = SRWL_WARN_BAD_RAD_INT_TERMINATIONS;
-    return 0;
 }
 
 // srwlib.cpp:403-452 + 328-373: trajectory range for a request without a
@@ -583,7 +594,7 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     if (!dEx || !dEz) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     std::vector<long long> off(1, 0);
     std::vector<SrResult> res;
-    rc = sr_run(jobs, fields, dEx, dEz, off, res, termin);
+    rc = sr_run(jobs, fields, dEx, dEz, off, res, termin, true);
     if (rc) return rc;
     if (!g_ctx.keepOnDevice) {
         HIPCHK(hipMemcpyAsync(pWfr->arEx, dEx, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
@@ -598,6 +609,7 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
         if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    sr_complete(res);
     pWfr->avgPhotEn = 0.5 * (pWfr->mesh.eStart + pWfr->mesh.eFin);
     pWfr->presCA = 0; pWfr->presFT = 0;
     return res[0].warn;
@@ -632,7 +644,7 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
     float* dEz = (float*)g_ctx.buf.Ez.get(sizeof(float) * 2 * tot);
     if (!dEx || !dEz) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     std::vector<SrResult> res;
-    rc = sr_run(jobs, fields, dEx, dEz, off, res, termin);
+    rc = sr_run(jobs, fields, dEx, dEz, off, res, termin, true);
     if (rc) return rc;
     int warn = 0;
     std::vector<srwm_mslice> momSl;
@@ -660,7 +672,6 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
         }
         w->avgPhotEn = 0.5 * (w->mesh.eStart + w->mesh.eFin);
         w->presCA = 0; w->presFT = 0;
-        if (res[i].warn) warn = res[i].warn;
     }
     if (!momSl.empty()) {
         // all requests' slices in one device pass
@@ -676,6 +687,8 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
         }
     }
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    sr_complete(res);
+    for (auto& r : res) if (r.warn) warn = r.warn;
     return warn;
 }
 
