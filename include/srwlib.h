@@ -165,6 +165,7 @@ typedef struct SRWLStructOpticsContainer {
 #define SRWL_INCORRECT_PARAM_FOR_WFR_PROP             (171 + SRWL_FIRST_ERR)
 #define SRWL_WFR_EXT_FUNC_NOT_DEFINED                 (172 + SRWL_FIRST_ERR)
 #define SRWL_INCORRECT_PARAM_FOR_FFT                  (180 + SRWL_FIRST_ERR)
+#define SRWL_RESIZE_NOT_SUPPORTED_WITH_SUCH_PARAMS    (193 + SRWL_FIRST_ERR)
 /* Codes of this build (no counterpart in the reference): */
 #define SRWL_AMD_ERR_GPU                              (300 + SRWL_FIRST_ERR)
 #define SRWL_AMD_ERR_UNSUPPORTED                      (301 + SRWL_FIRST_ERR)
@@ -217,6 +218,17 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
  * the field arrays through the callback of srwlUtiSetWfrModifFunc (action 2);
  * without one the call returns SRWL_WFR_EXT_FUNC_NOT_DEFINED. */
 SRWL_EXPORT int srwlResizeElecField(SRWLWfr* pWfr, char type, double* par);
+
+/* reference srwlib.h:787 (srwlib.cpp:914-931): resampling onto the
+ * transverse mesh pMesh (nx, ny, xStart, xFin, yStart, yFin; the photon-energy
+ * mesh must be the wavefront's own).  par = [0 (no FFT-side resizing),
+ * allow the quadratic-phase treatment (1/0), allow the Re/Im correction from
+ * the interpolated intensity (1/0)]; the reference's Python binding passes
+ * {0, 1, 1} by default.  New points outside the old mesh are zero.  Arrays
+ * are reallocated as for srwlResizeElecField.  Resampling in photon energy
+ * returns SRWL_AMD_ERR_UNSUPPORTED; both at once, as in the reference,
+ * SRWL_RESIZE_NOT_SUPPORTED_WITH_SUCH_PARAMS. */
+SRWL_EXPORT int srwlResizeElecFieldMesh(SRWLWfr* pWfr, SRWLRadMesh* pMesh, double* par);
 
 /* reference srwlib.h:871 (srwlib.cpp:1090-1204): SRW-convention 2D FFT of
  * float complex data; arMesh = [xStart, xStep, nx, yStart, yStep, ny]. */

@@ -1971,7 +1971,7 @@ static void orc_bilinear(const float* F, int ix, int iz, double* a)
  * and of the intensity (normalised by its cell average when above 1), the
  * complex value then rescaled to the interpolated intensity; bilinear
  * interpolation inside the sub-cell when the cell holds an exact zero. */
-static void orc_resample_point(const float* cell, int ixRel, int izRel, double xr, double zr, float* out)
+static void orc_resample_point(const float* cell, int ixRel, int izRel, double xr, double zr, int improve, float* out)
 {
     float In[16];   /* intensity cell, sroptelm.h:487-507 */
     for (int k = 0; k < 16; k++) In[k] = cell[k] * cell[k] + cell[16 + k] * cell[16 + k];
@@ -2007,6 +2007,7 @@ static void orc_resample_point(const float* cell, int ixRel, int izRel, double x
         double b = a[3] * xz + a[1] * xr + a[2] * zr + a[0];
         FI = (float)((b > 0.) ? b : 0.);
     }
+    if (!improve) { out[0] = F[0]; out[1] = F[1]; return; }
     FI *= norm;
     /* sroptelm.h:688-697 */
     float appI = F[0] * F[0] + F[1] * F[1];
@@ -2015,6 +2016,77 @@ static void orc_resample_point(const float* cell, int ixRel, int izRel, double x
         F[0] *= f; F[1] *= f;
     }
     out[0] = F[0]; out[1] = F[1];
+}
+
+/* Interpolation of wOld onto the points [ix0, ix1] x [iz0, iz1] of wNew (the
+ * rest of wNew stays as the caller prepared it, zero), shared by
+ * srwlResizeElecField (sroptelm.cpp:3038-3339) and srwlResizeElecFieldMesh
+ * (srradstr.cpp:4504-4904).  fx, fz: the quadratic phase is removed from
+ * wOld in place before and added to wNew after; improve: Re and Im rescaled
+ * to the interpolated intensity; zeroOutside: new points outside the old
+ * mesh (1e-6 of the new step) stay zero (srradstr.cpp:4726, 4761). */
+static int orc_resize_core(SRWLWfr* wOld, SRWLWfr* wNew, long ix0, long ix1, long iz0, long iz1, int fx, int fz,
+                           int improve, int zeroOutside)
+{
+    orc_mesh3 mo = orc_mesh_of(wOld), mn = orc_mesh_of(wNew);
+    if (mo.nx < 4 || mo.nz < 4) return SRWL_INCORRECT_PARAM_FOR_RESIZE;
+    long long perX = mo.ne * 2, perZo = perX * mo.nx, perZn = perX * mn.nx;
+    const float* oX = (const float*)wOld->arEx;
+    const float* oZ = (const float*)wOld->arEy;
+    float* nX = (float*)wNew->arEx;
+    float* nZ = (float*)wNew->arEy;
+    int treated = fx || fz;
+    if (treated) orc_treat_quad_phase(wOld, 0, fx, fz);
+    double ixs = 1. / mo.xStep, izs = 1. / mo.zStep;
+    double xMin = mo.xStart, xMax = mo.xStart + (mo.nx - 1) * mo.xStep;
+    double zMin = mo.zStart, zMax = mo.zStart + (mo.nz - 1) * mo.zStep;
+    double xTol = 1.e-06 * mn.xStep, zTol = 1.e-06 * mn.zStep;
+    for (long ie = 0; ie < mn.ne; ie++)
+        for (long iz = iz0; iz <= iz1; iz++) {
+            double zAbs = mn.zStart + iz * mn.zStep;
+            int zOut = zeroOutside && (zAbs < zMin - zTol || zAbs > zMax + zTol);
+            int izc = (int)((zAbs - mo.zStart) * izs + 1.E-06);
+            double zRel = zAbs - (mo.zStart + izc * mo.zStep);
+            int izSt;
+            if (izc == mo.nz - 1) { izSt = izc - 3; zRel += 2. * mo.zStep; }
+            else if (izc == mo.nz - 2) { izSt = izc - 2; zRel += mo.zStep; }
+            else if (izc == 0) { izSt = izc; zRel -= mo.zStep; }
+            else izSt = izc - 1;
+            zRel *= izs;
+            for (long ix = ix0; ix <= ix1; ix++) {
+                double xAbs = mn.xStart + ix * mn.xStep;
+                long long on = iz * perZn + ix * perX + 2 * ie;
+                if (zOut || (zeroOutside && (xAbs < xMin - xTol || xAbs > xMax + xTol))) {
+                    if (nX) { nX[on] = 0.f; nX[on + 1] = 0.f; }
+                    if (nZ) { nZ[on] = 0.f; nZ[on + 1] = 0.f; }
+                    continue;
+                }
+                int ixc = (int)((xAbs - mo.xStart) * ixs + 1.E-06);
+                double xRel = xAbs - (mo.xStart + ixc * mo.xStep);
+                int ixSt;
+                if (ixc == mo.nx - 1) { ixSt = ixc - 3; xRel += 2. * mo.xStep; }
+                else if (ixc == mo.nx - 2) { ixSt = ixc - 2; xRel += mo.xStep; }
+                else if (ixc == 0) { ixSt = ixc; xRel -= mo.xStep; }
+                else ixSt = ixc - 1;
+                xRel *= ixs;
+                long long oo = izSt * perZo + ixSt * perX + 2 * ie;
+                float cell[32];
+                if (nX) {
+                    orc_cell(oX + oo, perX, perZo, cell);
+                    orc_resample_point(cell, ixc - ixSt, izc - izSt, xRel, zRel, improve, nX + on);
+                }
+                if (nZ) {
+                    orc_cell(oZ + oo, perX, perZo, cell);
+                    orc_resample_point(cell, ixc - ixSt, izc - izSt, xRel, zRel, improve, nZ + on);
+                }
+            }
+        }
+    if (treated) {
+        wNew->Rx = wOld->Rx; wNew->Ry = wOld->Ry; wNew->dRx = wOld->dRx; wNew->dRy = wOld->dRy;
+        wNew->xc = wOld->xc; wNew->yc = wOld->yc;
+        orc_treat_quad_phase(wNew, 1, fx, fz);
+    }
+    return 0;
 }
 
 /* Fills the resampled wavefront wNew (mesh and zeroed arrays prepared by the
@@ -2048,43 +2120,46 @@ ORC_API int orc_resize_fill(SRWLWfr* wOld, SRWLWfr* wNew, const double* par, con
             }
         return 0;
     }
+    /* sroptelm.cpp:3051-3060: radius error below 40 % */
     int fx = fabs(wOld->dRx) < 0.4 * fabs(wOld->Rx), fz = fabs(wOld->dRy) < 0.4 * fabs(wOld->Ry);
-    int treated = fx || fz;
-    if (treated) orc_treat_quad_phase(wOld, 0, fx, fz);
-    double ixs = 1. / mo.xStep, izs = 1. / mo.zStep;
-    for (long ie = 0; ie < mn.ne; ie++)
-        for (long iz = iz0; iz <= iz1; iz++) {
-            double zAbs = mn.zStart + iz * mn.zStep;
-            int izc = (int)((zAbs - mo.zStart) * izs + 1.E-06);
-            double zRel = zAbs - (mo.zStart + izc * mo.zStep);
-            int izSt;
-            if (izc == mo.nz - 1) { izSt = izc - 3; zRel += 2. * mo.zStep; }
-            else if (izc == mo.nz - 2) { izSt = izc - 2; zRel += mo.zStep; }
-            else if (izc == 0) { izSt = izc; zRel -= mo.zStep; }
-            else izSt = izc - 1;
-            zRel *= izs;
-            for (long ix = ix0; ix <= ix1; ix++) {
-                double xAbs = mn.xStart + ix * mn.xStep;
-                int ixc = (int)((xAbs - mo.xStart) * ixs + 1.E-06);
-                double xRel = xAbs - (mo.xStart + ixc * mo.xStep);
-                int ixSt;
-                if (ixc == mo.nx - 1) { ixSt = ixc - 3; xRel += 2. * mo.xStep; }
-                else if (ixc == mo.nx - 2) { ixSt = ixc - 2; xRel += mo.xStep; }
-                else if (ixc == 0) { ixSt = ixc; xRel -= mo.xStep; }
-                else ixSt = ixc - 1;
-                xRel *= ixs;
-                long long oo = izSt * perZo + ixSt * perX + 2 * ie, on = iz * perZn + ix * perX + 2 * ie;
-                float cell[32];
-                if (nX) { orc_cell(oX + oo, perX, perZo, cell); orc_resample_point(cell, ixc - ixSt, izc - izSt, xRel, zRel, nX + on); }
-                if (nZ) { orc_cell(oZ + oo, perX, perZo, cell); orc_resample_point(cell, ixc - ixSt, izc - izSt, xRel, zRel, nZ + on); }
-            }
-        }
-    if (treated) {
-        wNew->Rx = wOld->Rx; wNew->Ry = wOld->Ry; wNew->dRx = wOld->dRx; wNew->dRy = wOld->dRy;
-        wNew->xc = wOld->xc; wNew->yc = wOld->yc;
-        orc_treat_quad_phase(wNew, 1, fx, fz);
+    return orc_resize_core(wOld, wNew, ix0, ix1, iz0, iz1, fx, fz, 1, 0);
+}
+
+/* srwlResizeElecFieldMesh(wOld, mesh of wNew, par) (srradstr.cpp:4218-4502,
+ * 4504-4904) for a transverse resize: wNew carries the new mesh and zeroed
+ * arrays; par = [FFT side (0), allow quadratic-phase treatment, allow the
+ * Re/Im correction].  Returns 0 or an error code. */
+ORC_API int orc_resize_fill_mesh(SRWLWfr* wOld, SRWLWfr* wNew, const double* par)
+{
+    orc_mesh3 mo = orc_mesh_of(wOld), mn = orc_mesh_of(wNew);
+    if (par[0] != 0.) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (mn.ne != mo.ne) return SRWL_AMD_ERR_UNSUPPORTED;
+    double xFinO = mo.xStart + mo.xStep * (mo.nx - 1), zFinO = mo.zStart + mo.zStep * (mo.nz - 1);
+    double xFinN = mn.xStart + mn.xStep * (mn.nx - 1), zFinN = mn.zStart + mn.zStep * (mn.nz - 1);
+    /* srradstr.cpp:4581-4608: new points inside the old mesh */
+    long ix0 = 0, ix1 = mn.nx - 1, iz0 = 0, iz1 = mn.nz - 1;
+    if (mn.xStart < mo.xStart) {
+        ix0 = (long)((mo.xStart - mn.xStart) / mn.xStep - 1.e-13);
+        if (mn.xStart + (ix0 + 0.1) * mn.xStep < mo.xStart) ix0++;
+        if (ix0 < 0) ix0 = 0;
     }
-    return 0;
+    if (xFinO < xFinN) {
+        ix1 = (long)((xFinO - mn.xStart) / mn.xStep + 1.e-13);
+        if (ix1 >= mn.nx) ix1 = mn.nx - 1;
+    }
+    if (mn.zStart < mo.zStart) {
+        iz0 = (long)((mo.zStart - mn.zStart) / mn.zStep - 1.e-13);
+        if (mn.zStart + (iz0 + 0.1) * mn.zStep < mo.zStart) iz0++;
+        if (iz0 < 0) iz0 = 0;
+    }
+    if (zFinO < zFinN) {
+        iz1 = (long)((zFinO - mn.zStart) / mn.zStep + 1.e-13);
+        if (iz1 >= mn.nz) iz1 = mn.nz - 1;
+    }
+    /* srradstr.h:786-811: radius error below 20 %, if the caller allows it */
+    int allow = par[1] != 0.;
+    int fx = allow && fabs(wOld->dRx) < 0.2 * fabs(wOld->Rx), fz = allow && fabs(wOld->dRy) < 0.2 * fabs(wOld->Ry);
+    return orc_resize_core(wOld, wNew, ix0, ix1, iz0, iz1, fx, fz, par[2] != 0., 1);
 }
 
 /* Test hook: electron i of the sampler mapped to the beam (srwlib.py:9000-9029,

@@ -114,7 +114,7 @@ __device__ __forceinline__ double bilinear_eval(const float* F, int ix, int iz, 
 
 // One component at relative cell position (xr, zr); re/im: the 16 cell values.
 __device__ __forceinline__ float2 resample_point(const float* re, const float* im, int ixRel, int izRel, double xr,
-                                                 double zr)
+                                                 double zr, bool improve)
 {
     float In[16];
     for (int k = 0; k < 16; k++) In[k] = __fadd_rn(__fmul_rn(re[k], re[k]), __fmul_rn(im[k], im[k]));
@@ -145,6 +145,7 @@ __device__ __forceinline__ float2 resample_point(const float* re, const float* i
         double b = bilinear_eval(In, ix, iz, xr, zr);
         FI = (float)((b > 0.) ? b : 0.);
     }
+    if (!improve) return make_float2(F0, F1);
     FI = __fmul_rn(FI, norm);
     float appI = __fadd_rn(__fmul_rn(F0, F0), __fmul_rn(F1, F1));   // sroptelm.h:688-697
     if (appI != 0.f) {
@@ -166,7 +167,10 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
     float2 vx = make_float2(0.f, 0.f), vz = make_float2(0.f, 0.f);
     if (ix >= P.ix0 && ix <= P.ix1 && iz >= P.iz0 && iz <= P.iz1) {
         double zAbs = P.zStartN + iz * P.zStepN, xAbs = P.xStartN + ix * P.xStepN;
-        if (P.onlyLarger) {
+        if (P.zeroOutside && (zAbs < P.zMin - P.zTol || zAbs > P.zMax + P.zTol || xAbs < P.xMin - P.xTol ||
+                              xAbs > P.xMax + P.xTol)) {
+            // outside the old wavefront: stays zero (srradstr.cpp:4726, 4761-4775)
+        } else if (P.onlyLarger) {
             long izo = (long)((zAbs - P.zStartO) * (1. / P.zStepO) + 1.E-08);
             long ixo = (long)((xAbs - P.xStartO) * (1. / P.xStepO) + 1.E-08);
             izo = izo < 0 ? 0 : (izo > P.nzo - 1 ? P.nzo - 1 : izo);
@@ -203,7 +207,7 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
                         float2 v = oex[base + j * P.lo.perZ + k * P.lo.perX];
                         re[4 * j + k] = v.x; im[4 * j + k] = v.y;
                     }
-                vx = resample_point(re, im, ixc - ixSt, izc - izSt, xRel, zRel);
+                vx = resample_point(re, im, ixc - ixSt, izc - izSt, xRel, zRel, P.improve != 0);
             }
             if (oez) {
                 for (int j = 0; j < 4; j++)
@@ -211,7 +215,7 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
                         float2 v = oez[base + j * P.lo.perZ + k * P.lo.perX];
                         re[4 * j + k] = v.x; im[4 * j + k] = v.y;
                     }
-                vz = resample_point(re, im, ixc - ixSt, izc - izSt, xRel, zRel);
+                vz = resample_point(re, im, ixc - ixSt, izc - izSt, xRel, zRel, P.improve != 0);
             }
         }
     }

@@ -89,6 +89,8 @@ int resize_plan(const SRWLWfr& w, const double* par, int npar, ResizePlan& P)
     P.fx = std::fabs(w.dRx) < 0.4 * std::fabs(w.Rx);
     P.fz = std::fabs(w.dRy) < 0.4 * std::fabs(w.Ry);
     if (P.onlyLarger) P.fx = P.fz = 0;
+    P.improve = 1;
+    P.zeroOutside = 0;
     return 0;
 }
 
@@ -128,6 +130,10 @@ int device_resize(const SRWLWfr& w, const ResizePlan& P, float2* oX, float2* oZ,
     R.eStep = w.mesh.ne > 1 ? (w.mesh.eFin - w.mesh.eStart) / (w.mesh.ne - 1) : 0.;
     R.ix0 = (int)P.x.i0; R.ix1 = (int)P.x.i1; R.iz0 = (int)P.z.i0; R.iz1 = (int)P.z.i1;
     R.onlyLarger = P.onlyLarger;
+    R.improve = P.improve;
+    R.zeroOutside = P.zeroOutside;
+    R.xMin = w.mesh.xStart; R.xMax = w.mesh.xFin; R.zMin = w.mesh.yStart; R.zMax = w.mesh.yFin;
+    R.xTol = 1.e-06 * P.x.step; R.zTol = 1.e-06 * P.z.step;
     // the phase is added back to each new value before it is stored
     R.hasPost = (P.fx || P.fz) ? 1 : 0;
     if (R.hasPost) R.post = phase_desc(w, P.x.n, P.z.n, P.x.start, P.x.step, P.z.start, P.z.step, ln, P.fx, P.fz, 1);
@@ -182,6 +188,97 @@ SRWL_EXPORT int srwlResizeElecField(SRWLWfr* pWfr, char type, double* par)
     HIPCHK(hipMemcpyAsync(dO + nOld, pWfr->arEy, sizeof(float2) * nOld, hipMemcpyHostToDevice, st));
     srwr_layout lo = {ne, ne * pWfr->mesh.nx, 1};
     srwr_layout ln = {ne, ne * P.x.n, 1};
+    rc = device_resize(*pWfr, P, dO, dO + nOld, lo, dN, dN + nNew, ln, st);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    rc = realloc_wfr_arrays(*pWfr, P);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(pWfr->arEx, dN, sizeof(float2) * nNew, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(pWfr->arEy, dN + nNew, sizeof(float2) * nNew, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// reference srwlib.h:787 (srwlib.cpp:914-931, srradstr.cpp:4218-4502 and
+// ResizeCoreXZ :4504-4904): resampling onto the given mesh.  par = [FFT
+// side, allow the quadratic-phase treatment, allow the Re/Im correction from
+// the intensity], as the reference's Python binding fills it ({0, 1, 1}).
+SRWL_EXPORT int srwlResizeElecFieldMesh(SRWLWfr* pWfr, SRWLRadMesh* pMesh, double* par)
+{
+    if (pWfr == nullptr || pMesh == nullptr || par == nullptr) return SRWL_INCORRECT_PARAM_FOR_RESIZE;
+    if (pWfr->presCA != 0 || pWfr->presFT != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    const SRWLRadMesh& mo = pWfr->mesh;
+    const SRWLRadMesh& mn = *pMesh;
+    long ne = mo.ne, nx = mo.nx, nz = mo.ny;
+    double eStep = ne > 1 ? (mo.eFin - mo.eStart) / (ne - 1) : 0.;
+    double xStep = nx > 1 ? (mo.xFin - mo.xStart) / (nx - 1) : 0.;
+    double zStep = nz > 1 ? (mo.yFin - mo.yStart) / (nz - 1) : 0.;
+    double eStepN = mn.ne <= 1 ? 0. : (mn.eFin - mn.eStart) / (mn.ne - 1);
+    double xStepN = mn.nx <= 1 ? 0. : (mn.xFin - mn.xStart) / (mn.nx - 1);
+    double zStepN = mn.ny <= 1 ? 0. : (mn.yFin - mn.yStart) / (mn.ny - 1);
+    const double relTol = 1.e-06;
+    bool reqE = !(((mn.ne == ne) && std::fabs(mn.eStart - mo.eStart) <= relTol * eStepN &&
+                   std::fabs(eStepN - eStep) <= relTol * eStepN) || ne <= 1);
+    bool reqX = !(((mn.nx == nx) && std::fabs(mn.xStart - mo.xStart) <= relTol * xStepN &&
+                   std::fabs(xStepN - xStep) <= relTol * xStepN) || nx <= 1);
+    bool reqZ = !(((mn.ny == nz) && std::fabs(mn.yStart - mo.yStart) <= relTol * zStepN &&
+                   std::fabs(zStepN - zStep) <= relTol * zStepN) || nz <= 1);
+    if (!(reqE || reqX || reqZ)) return 0;
+    if (reqE && (reqX || reqZ)) return SRWL_RESIZE_NOT_SUPPORTED_WITH_SUCH_PARAMS;
+    if (reqE) return SRWL_AMD_ERR_UNSUPPORTED;      // resampling vs photon energy
+    if (par[0] != 0.) return SRWL_AMD_ERR_UNSUPPORTED;   // resampling on the FFT side
+    if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
+    if (nx < 4 || nz < 4) return SRWL_INCORRECT_PARAM_FOR_RESIZE;   // a 4x4 cell per new point
+    if (mn.nx < 1 || mn.ny < 1) return SRWL_INCORRECT_PARAM_FOR_RESIZE;
+    ResizePlan P;
+    P.x.n = mn.nx; P.x.start = mn.xStart; P.x.step = xStepN;
+    P.z.n = mn.ny; P.z.start = mn.yStart; P.z.step = zStepN;
+    // filled index range: the new points inside the old mesh (srradstr.cpp:4581-4608)
+    double xFinO = mo.xStart + xStep * (nx - 1), zFinO = mo.yStart + zStep * (nz - 1);
+    long ix0 = 0, ix1 = mn.nx - 1, iz0 = 0, iz1 = mn.ny - 1;
+    if (mn.xStart < mo.xStart) {
+        ix0 = (long)((mo.xStart - mn.xStart) / xStepN - 1.e-13);
+        if (mn.xStart + (ix0 + 0.1) * xStepN < mo.xStart) ix0++;
+        if (ix0 < 0) ix0 = 0;
+    }
+    if (xFinO < mn.xFin) {
+        ix1 = (long)((xFinO - mn.xStart) / xStepN + 1.e-13);
+        if (ix1 >= mn.nx) ix1 = mn.nx - 1;
+    }
+    if (mn.yStart < mo.yStart) {
+        iz0 = (long)((mo.yStart - mn.yStart) / zStepN - 1.e-13);
+        if (mn.yStart + (iz0 + 0.1) * zStepN < mo.yStart) iz0++;
+        if (iz0 < 0) iz0 = 0;
+    }
+    if (zFinO < mn.yFin) {
+        iz1 = (long)((zFinO - mn.yStart) / zStepN + 1.e-13);
+        if (iz1 >= mn.ny) iz1 = mn.ny - 1;
+    }
+    // meshes that do not intersect give a zero field (srradstr.cpp:4382-4425, 4610-4631)
+    auto outside = [](double a0, double a1, double b0, double b1) {
+        return (a0 < b0 && a1 < b0 && a0 < b1 && a1 < b1) || (a0 > b0 && a1 > b0 && a0 > b1 && a1 > b1);
+    };
+    bool disjoint = outside(mn.xStart, mn.xFin, mo.xStart, xFinO) || outside(mn.yStart, mn.yFin, mo.yStart, zFinO);
+    if (disjoint) { ix0 = 1; ix1 = 0; iz0 = 1; iz1 = 0; }   // empty range: all zero
+    P.x.i0 = ix0; P.x.i1 = ix1; P.z.i0 = iz0; P.z.i1 = iz1;
+    P.onlyLarger = 0;
+    P.improve = par[2] != 0. ? 1 : 0;
+    P.zeroOutside = 1;
+    // srradstr.h:786-811 (coordinate representation): radius error below 20 %
+    bool allowQuad = par[1] != 0.;
+    P.fx = allowQuad && std::fabs(pWfr->dRx) < 0.2 * std::fabs(pWfr->Rx);
+    P.fz = allowQuad && std::fabs(pWfr->dRy) < 0.2 * std::fabs(pWfr->Ry);
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t st = g_ctx.stream;
+    long long nOld = (long long)ne * nx * nz, nNew = (long long)ne * mn.nx * mn.ny;
+    float2* dO = (float2*)g_ctx.buf.RsO.get(sizeof(float2) * 2 * nOld);
+    float2* dN = (float2*)g_ctx.buf.RsN.get(sizeof(float2) * 2 * nNew);
+    if (!dO || !dN) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    HIPCHK(hipMemcpyAsync(dO, pWfr->arEx, sizeof(float2) * nOld, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dO + nOld, pWfr->arEy, sizeof(float2) * nOld, hipMemcpyHostToDevice, st));
+    srwr_layout lo = {ne, ne * nx, 1};
+    srwr_layout ln = {ne, ne * (long long)mn.nx, 1};
     rc = device_resize(*pWfr, P, dO, dO + nOld, lo, dN, dN + nNew, ln, st);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(st));

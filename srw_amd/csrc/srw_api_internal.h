@@ -108,6 +108,7 @@ struct ResizePlan {
     double pxm = 1, pxd = 1, pzm = 1, pzd = 1;
     ResizeAxis x, z;
     int onlyLarger = 0, fx = 0, fz = 0;
+    int improve = 1, zeroOutside = 0;
 };
 long next_fft_size(long n);
 int resize_plan(const SRWLWfr& w, const double* par, int npar, ResizePlan& P);

@@ -28,6 +28,9 @@ struct srwr_params {
     int onlyLarger;                         // copy without interpolation
     int hasPost;                            // apply `post` to each new value before it is stored
     srwr_phase post;                        // on the new mesh and layout
+    int improve;                            // rescale Re, Im to the interpolated intensity
+    int zeroOutside;                        // zero new points outside [min - tol, max + tol] of the old mesh
+    double xMin, xMax, zMin, zMax, xTol, zTol;
 };
 
 extern "C" {

@@ -469,6 +469,8 @@ SRWL_EXPORT int srwlUtiGetErrText(char* t, int erNo)
     case SRWL_INCORRECT_WFR_STRUCT: s = "Incorrect wavefront structure"; break;
     case SRWL_WFR_EXT_MODIF_FAILED: s = "Wavefront modification by the calling application failed"; break;
     case SRWL_INCORRECT_PARAM_FOR_RESIZE: s = "Incorrect parameters for wavefront resizing"; break;
+    case SRWL_RESIZE_NOT_SUPPORTED_WITH_SUCH_PARAMS:
+        s = "Resizing of the wavefront is not supported with these parameters"; break;
     case SRWL_INCORRECT_PARAM_FOR_WFR_PROP: s = "Incorrect parameters for wavefront propagation"; break;
     case SRWL_WFR_EXT_FUNC_NOT_DEFINED: s = "Wavefront modification function of the calling application is not defined"; break;
     case SRWL_INCORRECT_PARAM_FOR_FFT: s = "Incorrect parameters for FFT"; break;

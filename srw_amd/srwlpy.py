@@ -42,6 +42,7 @@ def _load():
         lib.srwlEmitPropMultiE.argtypes = [P(S.SRWLStokes), P(S.SRWLWfr), P(S.SRWLMagFldC), P(S.SRWLOptC),
                                            P(C.c_double), C.c_int, C.c_longlong, P(C.c_double), P(C.c_double)]
     lib.srwlResizeElecField.argtypes = [P(S.SRWLWfr), C.c_char, P(C.c_double)]
+    lib.srwlResizeElecFieldMesh.argtypes = [P(S.SRWLWfr), P(S.SRWLRadMesh), P(C.c_double)]
     lib.srwlAmdGetStats.argtypes = [P(C.c_double), C.c_int]
     lib.srwlAmdSetTiming.argtypes = [C.c_int]
     lib.srwlAmdSetKeepOnDevice.argtypes = [C.c_int]
@@ -176,6 +177,28 @@ def ResizeElecField(_wfr, _type, _par):
     _active_wfr[C.addressof(cw)] = _wfr
     try:
         rc = lib.srwlResizeElecField(C.byref(cw), C.c_char(t), cp)
+    finally:
+        _active_wfr.pop(C.addressof(cw), None)
+    _check(rc)
+    M.wfr_back(cw, _wfr)
+    return _wfr
+
+
+def ResizeElecFieldMesh(_wfr, _mesh, _par):
+    """Resampling of the wavefront onto the mesh _mesh (reference srwlpy.cpp
+    ResizeElecFieldMesh, srwlib.cpp:914-931); _par = [FFT side (0), allow the
+    quadratic-phase treatment, allow the Re/Im correction], defaults {0, 1, 1}
+    for the elements not given, as in the reference binding."""
+    keep = M.Keep()
+    cw = M.wfr(_wfr, keep)
+    cm = M.mesh(_mesh)
+    par = [0., 1., 1.]
+    for i, v in enumerate(list(_par)[:3]):
+        par[i] = float(v)
+    cp = M.dvec(par, keep)
+    _active_wfr[C.addressof(cw)] = _wfr
+    try:
+        rc = lib.srwlResizeElecFieldMesh(C.byref(cw), C.byref(cm), cp)
     finally:
         _active_wfr.pop(C.addressof(cw), None)
     _check(rc)

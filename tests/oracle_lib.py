@@ -34,6 +34,7 @@ def load():
     lib.orc_next_fft_size.restype = C.c_long
     lib.orc_resize_mesh.argtypes = [P(S.SRWLWfr), P(C.c_double), C.c_int, P(C.c_double)]
     lib.orc_resize_fill.argtypes = [P(S.SRWLWfr), P(S.SRWLWfr), P(C.c_double), P(C.c_double)]
+    lib.orc_resize_fill_mesh.argtypes = [P(S.SRWLWfr), P(S.SRWLWfr), P(C.c_double)]
     lib.orc_compute_rad_moments.argtypes = [P(S.SRWLWfr), C.c_void_p, C.c_void_p]
     lib.orc_emit_prop_multi_e.argtypes = [P(S.SRWLStokes), P(S.SRWLWfr), P(S.SRWLMagFldC), P(S.SRWLOptC),
                                           P(C.c_double), C.c_int, C.c_longlong, C.c_ulonglong, C.c_longlong,
@@ -134,4 +135,21 @@ def resize(w, par):
     wn.mesh.xStart, wn.mesh.xFin, wn.mesh.yStart, wn.mesh.yFin = out[2], out[3], out[4], out[5]
     cn = M.wfr(wn, keep)
     rc = lib.orc_resize_fill(C.byref(cw), C.byref(cn), cp, out)
+    return rc, wn
+
+
+def resize_to_mesh(w, mesh, par=(0, 1, 1)):
+    """srwlResizeElecFieldMesh(w, mesh, par) restated for a transverse mesh:
+    returns (rc, new wavefront); w's arrays are consumed by the phase
+    treatment."""
+    import copy
+    keep = M.Keep()
+    cw = M.wfr(w, keep)
+    wn = copy.deepcopy(w)
+    wn.allocate(w.mesh.ne, mesh.nx, mesh.ny)
+    wn.mesh.nx, wn.mesh.ny = mesh.nx, mesh.ny
+    wn.mesh.xStart, wn.mesh.xFin, wn.mesh.yStart, wn.mesh.yFin = mesh.xStart, mesh.xFin, mesh.yStart, mesh.yFin
+    cn = M.wfr(wn, keep)
+    cp = M.dvec(list(par), keep)
+    rc = lib.orc_resize_fill_mesh(C.byref(cw), C.byref(cn), cp)
     return rc, wn

@@ -85,3 +85,30 @@ def test_propagation_multi_energy_resize_per_energy():
         e = rel_l2(exg[:, :, ie], exo[:, :, ie])
         print("energy %d: rel-L2 Ex %.3e" % (ie, e), flush=True)
         assert e < 2e-5
+
+
+def _mesh(ne, e, nx, x0, x1, ny, y0, y1):
+    from srw_amd.srwlib import SRWLRadMesh
+    m = SRWLRadMesh()
+    m.ne, m.nx, m.ny = ne, nx, ny
+    m.eStart, m.eFin = e
+    m.xStart, m.xFin, m.yStart, m.yFin = x0, x1, y0, y1
+    return m
+
+
+@pytest.mark.parametrize("n,ne,box,par", [
+    (64, 1, (100, -8e-4, 8e-4, 90, -7e-4, 9e-4), [0, 1, 1]),     # finer, shifted
+    (64, 1, (128, -2e-3, 1e-3, 128, -1.5e-3, 2e-3), [0, 1, 1]),  # larger: zeros outside
+    (64, 1, (48, -5e-4, 5e-4, 48, -5e-4, 5e-4), [0, 0, 1]),      # coarser, no phase treatment
+    (48, 3, (96, -9e-4, 9e-4, 80, -9e-4, 9e-4), [0, 1, 0]),      # no Re/Im correction, 3 energies
+])
+def test_resize_to_mesh_matches_oracle(n, ne, box, par):
+    from srw_amd import srwlpy
+    print("case: resize to mesh n=%d ne=%d box=%s par=%s" % (n, ne, box, par), flush=True)
+    w = _sr(n, ne, 0.02 if ne > 1 else 0.)
+    m = _mesh(ne, (w.mesh.eStart, w.mesh.eFin), *box)
+    wg, wo = clone_wfr(w), clone_wfr(w)
+    srwlpy.ResizeElecFieldMesh(wg, m, par)
+    rc, wo2 = oracle.resize_to_mesh(wo, m, par)
+    assert rc == 0
+    _compare(wg, wo2)
