@@ -123,15 +123,18 @@ def fft_traffic(n, slices):
 
 
 def sr_valu_per_eval(n, ne):
-    """Vector instructions per integrand evaluation of k_sr_integral from the
-    committed counter summary (SQ_INSTS_VALU x 64 / evaluations); None when
-    not measured for this workload."""
+    """(vector, transcendental) instructions per integrand evaluation of
+    k_sr_integral from the committed counter summary (SQ_INSTS_VALU and
+    SQ_INSTS_VALU_TRANS_F32 per 64 evaluations); (None, None) when not
+    measured for this workload."""
     import re
     path = os.path.join(ROOT, "profiles", "k_sr_integral_%dx%dx%d_sq_counters.md" % (n, n, ne))
     if not os.path.exists(path):
-        return None
-    m = re.search(r"valu_instr_per_eval\s*=\s*([0-9.eE+]+)", open(path).read())
-    return float(m.group(1)) if m else None
+        return None, None
+    txt = open(path).read()
+    m = re.search(r"valu_instr_per_eval\s*=\s*([0-9.eE+]+)", txt)
+    t = re.search(r"valu_trans_per_eval\s*=\s*([0-9.eE+]+)", txt)
+    return (float(m.group(1)) if m else None), (float(t.group(1)) if t else None)
 
 
 def uneven_batch(srwlpy, W, n):
@@ -233,8 +236,10 @@ def main():
     k_ms = st["sum_sr_ms"] / max(st["n_sr"], 1)
     evals = st["sum_sr_evals"] / max(st["n_sr"], 1)
     achieved = FLOPS_PER_EVAL * evals / (k_ms * 1e-3) / 1e12
-    vpe = sr_valu_per_eval(a.n, a.ne)
-    issue = vpe * evals / (k_ms * 1e-3) / VALU_LANE_ISSUE_PER_S if vpe else None
+    vpe, tpe = sr_valu_per_eval(a.n, a.ne)
+    # issue rate of the non-transcendental vector instructions (the transcendental
+    # ones overlap them) over one wave64 instruction per 4 cycles per SIMD
+    issue = (vpe - tpe) * evals / (k_ms * 1e-3) / VALU_LANE_ISSUE_PER_S if vpe and tpe is not None else None
     # PCIe-inclusive rate (results copied to host buffers), reported aside
     srwlpy.set_keep_on_device(False)
     t1 = time.perf_counter()
@@ -277,7 +282,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": sr_traffic(a.n, a.ne),
                      "kernel": "k_sr_integral", "kernel_ms": k_ms, "evals_per_launch": evals,
-                     "flops_per_eval": FLOPS_PER_EVAL, "valu_instr_per_eval": vpe, "valu_issue_frac": issue},
+                     "flops_per_eval": FLOPS_PER_EVAL, "valu_instr_per_eval": vpe, "valu_trans_per_eval": tpe,
+                     "valu_issue_frac": issue},
         "pcie_inclusive_points_per_s": pcie_rate,
         "propagation": prop,
         "multi_electron": me,
