@@ -42,6 +42,12 @@ SRWL_EXPORT int srwlAmdSetSplit(int nSplit);
  * recurrence, the switch exists to compare them. */
 SRWL_EXPORT int srwlAmdSetTrajSerial(int on);
 
+/* Final level (1..20) of every computed point of the last method-1 field
+ * computation (precPar[0] = 1), in the reference's point order: rows, then
+ * columns, then photon energies of the computed region, request by request.
+ * Copies up to cap values into out (when given); returns the count. */
+SRWL_EXPORT long long srwlAmdLastAutoLevels(int* out, long long cap);
+
 /* Six standard normals of electron i of the counter-based sampler used by
  * srwlEmitPropMultiE (Philox4x32-10 + Box-Muller, see srw_rng.h). */
 SRWL_EXPORT int srwlAmdNormals(unsigned long long seed, long long i, double* g6);

@@ -151,6 +151,7 @@ typedef struct SRWLStructOpticsContainer {
 #define SRWL_ERR_S0_OUT_OF_FIELD_LIMITS               (28 + SRWL_FIRST_ERR)
 #define SRWL_ERR_WAVELENGTH_COMPARABLE_WITH_DISTANCE  (29 + SRWL_FIRST_ERR)
 #define SRWL_ERR_IN_FFT                               (40 + SRWL_FIRST_ERR)
+#define SRWL_ERR_CAN_NOT_COMPUTE_RADIATION_INTEGRAL   (56 + SRWL_FIRST_ERR)
 #define SRWL_ERR_INCORRECT_MAG_FIELD_PER              (124 + SRWL_FIRST_ERR)
 #define SRWL_ERR_INCORRECT_PER_MAG_FIELD_LENGTH       (125 + SRWL_FIRST_ERR)
 #define SRWL_NO_FUNC_ARG_DATA                         (160 + SRWL_FIRST_ERR)

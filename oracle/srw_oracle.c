@@ -564,6 +564,15 @@ ORC_API int orc_trj_interp(const SRWLPrtTrj* trj, const double* s, long long n, 
 /* ------------------------------------------------------------------ */
 /* SR integral (srradint.cpp, srradint.h)                               */
 /* ------------------------------------------------------------------ */
+/* Deepest level of the adaptive method evaluated here.  The reference keeps
+ * doubling until a level holds more than 8e8 points (level 29,
+ * srradint.cpp:1734, 2025-2031); this restatement and the GPU path stop after
+ * level 20 (2^21 new points per observation point) with the same error. */
+#define ORC_AUTO_MAX_LEVEL 20
+/* srradint.h:1092-1098 with MaxNumPoToSave = 10000: levels up to 11 use
+ * tabulated trajectory data, deeper ones evaluate it per point. */
+#define ORC_AUTO_MAX_TAB_LEVEL 11
+
 typedef struct {
     const orc_trj* T;
     double sIntegStart, sIntegFin, sIntegStep;
@@ -576,6 +585,13 @@ typedef struct {
     /* edge derivatives that persist across points (srradint.h:53) */
     double complex initDer[2], finDer[2];
     int warn;
+    /* method 1 (srradint.h:66-70, srradint.cpp:67-70, 293): level tables,
+     * relative precision and the running maximum of |E|^2 that turns the
+     * convergence test absolute after the first point of a call */
+    double relPrec, maxFlux, absPrec;
+    int sameLoop, nLvl;
+    double* lvl[ORC_AUTO_MAX_LEVEL + 1];   /* 6 arrays of n: Btx, X, IBx, Btz, Z, IBz */
+    long long lvlN[ORC_AUTO_MAX_LEVEL + 1];
 } orc_sr;
 
 /* srradint.h:867-880 with srradint.cpp:48-60 constants */
@@ -727,6 +743,142 @@ static void orc_manual(orc_sr* S, double xo, double yo, double zo, double eph, d
     }
 }
 
+/* srradint.h:1016-1036 FillNextLevel: trajectory data at np points from sSt
+ * to sEn, by the tabulating evaluation (srtrjdat.cpp:1185-1248). */
+static int orc_lvl_fill(orc_sr* S, int L, double sSt, double sEn, long long np)
+{
+    double* b = (double*)malloc(sizeof(double) * 6 * np);
+    if (!b) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    double s = sSt, st = (np > 1) ? (sEn - sSt) / (np - 1) : 0.;
+    for (long long i = 0; i < np; i++) {
+        orc_trj_eval(S->T, s, 1, b + i, b + np + i, b + 2 * np + i, b + 3 * np + i, b + 4 * np + i, b + 5 * np + i,
+                     0, 0);
+        s += st;
+    }
+    S->lvl[L] = b;
+    S->lvlN[L] = np;
+    S->nLvl++;
+    return 0;
+}
+
+static void orc_lvl_free(orc_sr* S)
+{
+    for (int L = 0; L < S->nLvl; L++) free(S->lvl[L]);
+    S->nLvl = 0;
+}
+
+/* Integrand of the near-field radiation integral at s from trajectory values
+ * (srradint.cpp:1771-1786 and 1901-1914): phase and the two amplitudes. */
+static void orc_auto_term(double s, double Btx, double X, double IBx, double Btz, double Z, double IBz,
+                          double xo, double yo, double zo, double kph, double gEm2, double* Ph, double* Ax,
+                          double* Az)
+{
+    double inv = 1. / (yo - s);
+    double dx = xo - X, dz = zo - Z;
+    double Nx = dx * inv, Nz = dz * inv;
+    double a0 = (IBx + IBz) + dx * Nx + dz * Nz;
+    *Ph = kph * (s * gEm2 + a0);
+    *Ax = (Btx - Nx) * inv;
+    *Az = (Btz - Nz) * inv;
+}
+
+/* srradint.cpp:1731-2046 RadIntegrationAuto1, coordinate representation.
+ * Level 0 is the 5-point composite rule over [sStart, sEnd]; each next level
+ * adds the midpoints, halving the step, and reuses the previous sums.  The
+ * loop ends at the first level whose phase advances by at most pi between
+ * its new points and whose |E|^2 changed by at most relPrec*|E|^2 (first
+ * point of a call) or relPrec*max|E|^2 of the points already done.  *lvlOut
+ * receives the final level. */
+static int orc_auto1(orc_sr* S, double xo, double yo, double zo, double eph, double complex* out, int* lvlOut)
+{
+    const double wfe = 7. / 15., wf1 = 16. / 15., wf2 = 14. / 15., wd = 1. / 15.;
+    double actNorm = S->normConst * eph * 0.80654658E-03;
+    double kph = kPI_SR * 1.E+06 * 0.80654658 * eph, gEm2 = S->T->gammaEm2;
+    double sStart = S->sIntegStart, sEnd = S->sIntegFin;
+    long long npl = 5;
+    int res;
+    if (S->nLvl == 0 && (res = orc_lvl_fill(S, 0, sStart, sEnd, npl))) return res;
+    double step = (sEnd - sStart) / (npl - 1);
+    /* level 0: ends into wF, odd points into s1, even interior points into s2 */
+    double wF[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+    double PhInit = 0.;
+    {
+        const double* t = S->lvl[0];
+        long long n = S->lvlN[0];
+        double s = sStart;
+        for (long long k = 0; k < npl; k++) {
+            double Ph, Ax, Az, c, sn;
+            orc_auto_term(s, t[k], t[n + k], t[2 * n + k], t[3 * n + k], t[4 * n + k], t[5 * n + k], xo, yo, zo, kph,
+                          gEm2, &Ph, &Ax, &Az);
+            orc_cos_sin(Ph, &c, &sn);
+            double* acc = (k == 0 || k == npl - 1) ? wF : ((k & 1) ? s1 : s2);
+            acc[0] += Ax * c; acc[1] += Ax * sn; acc[2] += Az * c; acc[3] += Az * sn;
+            if (k == 0) PhInit = Ph;
+            s += step;
+        }
+        for (int q = 0; q < 4; q++) wF[q] *= wfe;
+    }
+    double complex dX = S->initDer[0] - S->finDer[0], dZ = S->initDer[1] - S->finDer[1];
+    double wDD[4] = {wd * creal(dX), wd * cimag(dX), wd * creal(dZ), wd * cimag(dZ)};
+    double o0[4] = {creal(out[0]), cimag(out[0]), creal(out[1]), cimag(out[1])};
+    double Int[4], sq = 0.;
+    for (int q = 0; q < 4; q++) {
+        Int[q] = o0[q] + actNorm * step * (wF[q] + wf1 * s1[q] + wf2 * s2[q] + step * wDD[q]);
+        sq += Int[q] * Int[q];
+    }
+    npl--;
+    int L = 0;
+    for (;;) {
+        for (int q = 0; q < 4; q++) { s2[q] += s1[q]; s1[q] = 0.; }
+        int mayBeLast = 1;
+        double PhPrev = PhInit;
+        L++;
+        if (L > ORC_AUTO_MAX_LEVEL) return SRWL_ERR_CAN_NOT_COMPUTE_RADIATION_INTEGRAL;
+        double half = 0.5 * step, s = sStart + half;
+        const double* t = 0;
+        long long n = 0;
+        if (L <= ORC_AUTO_MAX_TAB_LEVEL) {
+            if (S->nLvl <= L && (res = orc_lvl_fill(S, L, s, sEnd - half, npl))) return res;
+            t = S->lvl[L];
+            n = S->lvlN[L];
+        }
+        for (long long i = 0; i < npl; i++) {
+            double Btx, X, IBx, Btz, Z, IBz, Ph, Ax, Az, c, sn;
+            if (t) { Btx = t[i]; X = t[n + i]; IBx = t[2 * n + i]; Btz = t[3 * n + i]; Z = t[4 * n + i]; IBz = t[5 * n + i]; }
+            else orc_trj_eval(S->T, s, 0, &Btx, &X, &IBx, &Btz, &Z, &IBz, 0, 0);
+            orc_auto_term(s, Btx, X, IBx, Btz, Z, IBz, xo, yo, zo, kph, gEm2, &Ph, &Ax, &Az);
+            orc_cos_sin(Ph, &c, &sn);
+            s1[0] += Ax * c; s1[1] += Ax * sn; s1[2] += Az * c; s1[3] += Az * sn;
+            s += step;
+            if (Ph - PhPrev > kPI_SR) mayBeLast = 0;
+            PhPrev = Ph;
+        }
+        double loc[4], locSq = 0.;
+        for (int q = 0; q < 4; q++) {
+            loc[q] = o0[q] + actNorm * half * (wF[q] + wf1 * s1[q] + wf2 * s2[q] + half * wDD[q]);
+            locSq += loc[q] * loc[q];
+        }
+        int done = 0;
+        if (mayBeLast) {
+            double test = fabs(locSq - sq);
+            int notYet = (S->sameLoop && S->maxFlux > 0.) ? (test > S->absPrec) : (test > S->relPrec * locSq);
+            done = !notYet;
+        }
+        for (int q = 0; q < 4; q++) Int[q] = loc[q];
+        sq = locSq;
+        step = half;
+        npl *= 2;
+        if (done) break;
+    }
+    out[0] = Int[0] + I * Int[1];
+    out[1] = Int[2] + I * Int[3];
+    if ((S->sameLoop && S->maxFlux < sq) || !S->sameLoop) {
+        S->maxFlux = sq; S->absPrec = S->relPrec * S->maxFlux; S->sameLoop = 1;
+    }
+    if (lvlOut) *lvlOut = L;
+    return 0;
+}
+
 /* srradstr.cpp:2310-2501 FindAverageDistanceToSource */
 static void orc_find_rx(const orc_trj* T, double yObs, double xS, double xE, double zS, double zE,
                         const double* precPar, double* Robs, double* RobsErr, double* xc, double* zc)
@@ -825,16 +977,22 @@ static void orc_free_trj_arrays(SRWLPrtTrj* t)
 }
 
 /* srwlib.cpp:456-540 + srradint.cpp:289-394, 3231-3350, 3551-3614.
- * Method 0 (manual) only; nthreads > 1 parallelizes over mesh rows (the
- * edge-derivative state is then per row, which only matters when the
- * asymptotic expansion is rejected). */
-ORC_API int orc_srwlCalcElecFieldSR_mt(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
-                                      double* precPar, int nPrecPar, int nthreads)
+ * Methods 0 (manual) and 1 (adaptive, srradint.cpp:1731-2046).  For method 0,
+ * nthreads > 1 parallelizes over mesh rows (the edge-derivative state is
+ * then per row, which only matters when the asymptotic expansion is
+ * rejected).  Method 1 always runs in the reference's point order, since its
+ * convergence threshold depends on the points already done.  levels, when
+ * given, receives the final level of each computed point in that order
+ * (rows, columns, energies of the computed region). */
+ORC_API int orc_srwlCalcElecFieldSR_ex(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
+                                      double* precPar, int nPrecPar, int nthreads, int* levels)
 {
     if (pWfr == 0 || precPar == 0) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
     int trjDef = !(pTrj == 0 || ((!pTrj->arX || !pTrj->arXp) && (!pTrj->arY || !pTrj->arYp)) || pTrj->np <= 0);
     if (!trjDef && (pMagFld == 0)) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
-    if ((int)precPar[0] != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    int meth = (int)precPar[0];
+    if (meth != 0 && meth != 1) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (meth == 1 && !(precPar[1] > 0.)) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
     if (nPrecPar > 6 && precPar[6] > 0) return SRWL_AMD_ERR_UNSUPPORTED;
     SRWLPrtTrj own;
     int res = 0;
@@ -879,8 +1037,16 @@ ORC_API int orc_srwlCalcElecFieldSR_mt(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
         if (T.s0 < T.sStart || T.s0 > T.sStart + T.sStep * (T.np - 1)) { res = SRWL_ERR_S0_OUT_OF_FIELD_LIMITS; goto done1; }
         if (yObs < S.sIntegFin) { res = SRWL_ERR_OBS_WITHIN_INTEGR_LIMITS; goto done1; }
         if ((yObs - S.sIntegFin) < 3. * 1.239842E-06 / m->eStart) { res = SRWL_ERR_WAVELENGTH_COMPARABLE_WITH_DISTANCE; goto done1; }
+        if (meth == 1) {   /* srradint.cpp:3590-3594, 293; srradint.h:1027-1030 */
+            S.relPrec = precPar[1];
+            S.sameLoop = 1;
+            S.maxFlux = S.absPrec = 0.;
+            S.sIntegStep = (S.sIntegFin - S.sIntegStart) / 4.;
+            S.slower = 1;   /* no manual tables */
+            nthreads = 1;
+        }
         /* srradint.h:1043-1058 + 814-830 */
-        {
+        if (meth == 0) {
             int b0 = (int)((S.sIntegFin - S.sIntegStart) / S.sIntegStep) >> 1;
             long long npl = ((long long)b0 << 1) + 1;
             S.slower = (10000 <= npl);
@@ -924,7 +1090,40 @@ ORC_API int orc_srwlCalcElecFieldSR_mt(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
             for (long ix = 0; ix < nx; ix++) { if (symX && (x - xc0) > xTol) { nxc = ix; break; } x += xStep; }
         }
         if (nthreads < 1) nthreads = 1;
-        int warnAll = 0;
+        int warnAll = 0, errAll = 0;
+        if (meth == 1) {
+            /* sequential, in the reference's order: one state for all points */
+            long long k = 0;
+            double z = zS;
+            for (long iz = 0; iz < nzc && !errAll; iz++) {
+                double x = xS;
+                for (long ix = 0; ix < nxc && !errAll; ix++) {
+                    double e = m->eStart;
+                    for (long ie = 0; ie < ne; ie++) {
+                        double complex v[2] = {0, 0};
+                        if (S.termin > 0) {
+                            double complex lr[2] = {0, 0}, rr[2] = {0, 0};
+                            if (S.termin == 1 || S.termin == 2) orc_residual(&S, S.sIntegStart, 1, x, yObs, z, e, lr);
+                            if (S.termin == 1 || S.termin == 3) orc_residual(&S, S.sIntegFin, 2, x, yObs, z, e, rr);
+                            v[0] = rr[0] - lr[0]; v[1] = rr[1] - lr[1];
+                        }
+                        int lv = 0;
+                        int r1 = orc_auto1(&S, x, yObs, z, e, v, &lv);
+                        if (r1) { errAll = r1; break; }
+                        if (levels) levels[k] = lv;
+                        k++;
+                        long long o = iz * perZ + ix * perX + ie * 2;
+                        ex[o] = (float)creal(v[0]); ex[o + 1] = (float)cimag(v[0]);
+                        ez[o] = (float)creal(v[1]); ez[o + 1] = (float)cimag(v[1]);
+                        e += eStep;
+                    }
+                    x += xStep;
+                }
+                z += zStep;
+            }
+            orc_lvl_free(&S);
+            if (errAll) { res = errAll; goto done1; }
+        } else
 #pragma omp parallel num_threads(nthreads) reduction(|:warnAll)
         {
             orc_sr L = S;
@@ -987,6 +1186,12 @@ done1:
 done0:
     if (!trjDef) orc_free_trj_arrays(&own);
     return res;
+}
+
+ORC_API int orc_srwlCalcElecFieldSR_mt(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
+                                      double* precPar, int nPrecPar, int nthreads)
+{
+    return orc_srwlCalcElecFieldSR_ex(pWfr, pTrj, pMagFld, precPar, nPrecPar, nthreads, 0);
 }
 
 ORC_API int orc_srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,

@@ -26,6 +26,7 @@ struct Buffers {
     DBuf Pic;                     // parallel-in-s RK4 iterates
     DBuf MomSl, MomWork, MomSums, MomLims;   // wavefront moments
     DBuf RsO, RsN;                // resampling
+    DBuf AutoTab, AutoFlags, AutoS, AutoF, AutoWork;   // adaptive SR integral (method 1)
 };
 
 struct Context {
@@ -50,6 +51,8 @@ struct Context {
     double lastPropDevMs = 0;               // device part of the last propagation (after upload, before download)
     int* warnHost = nullptr;                // pinned: termination warning of the last SR pass
     long long pendingEvals = 0;             // integrand evaluations of the last SR pass
+    std::vector<int> autoLevels;            // final levels of the last method-1 pass, in point order
+    int autoSweeps = 0;                     // fixed-point sweeps of the last method-1 pass (largest chunk)
 };
 
 extern Context g_ctx;
@@ -69,7 +72,17 @@ struct SrJob {
     const SRWLPrtTrj* userTrj = nullptr;
     SRWLRadMesh mesh;
     double step = 0, sLim0 = 0, sLim1 = 0;
+    int meth = 0;                 // precPar[0]: 0 manual, 1 adaptive
+    double relPrec = 0;           // precPar[1] of method 1
 };
+
+// Computed region of a request's mesh: the whole mesh, or the half / quarter
+// that the symmetry shortcut computes (srradint.cpp:319-345, 3231-3260).
+struct SrRegion {
+    double xStep = 0, zStep = 0;
+    int symX = 0, symZ = 0, nxc = 0, nzc = 0;
+};
+SrRegion sr_region(const SRWLRadMesh& m, const srwd_trj_info& I);
 
 struct SrResult {
     double Robs = 0, RobsErr = 0, xc = 0, zc = 0;
@@ -84,6 +97,12 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
            bool defer = false);
 // Completes a deferred sr_run once the stream has been synchronised.
 void sr_complete(std::vector<SrResult>& res);
+// Method 1 for every job, after sr_run has set up trajectories, edges and
+// wavefront radii (auto_api.cpp); synchronous.
+int sr_auto_run(const std::vector<SrJob>& jobs, const std::vector<srwd_trj_info>& info, const srwd_trj_info* dInfo,
+                const double* dX, const double* dY, const double* dIBx, const double* dIBz, long long ldT,
+                const srwd_edge* dEdge, int termin, float* dEx, float* dEz, const std::vector<long long>& outOff,
+                std::vector<SrResult>& res);
 
 // Wavefront moments (moments_api.cpp).  Component element (ix, iz, ie) of a
 // device wavefront is at base[ie*perE + iz*perZ + ix*perX] (float2 units).

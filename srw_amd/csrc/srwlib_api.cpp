@@ -182,11 +182,18 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             if (J.sLim0 > I.sIntegStart && J.sLim0 < I.sIntegFin) I.sIntegStart = J.sLim0;
             if (J.sLim1 > I.sIntegStart && J.sLim1 < I.sIntegFin) I.sIntegFin = J.sLim1;
         }
-        // srradint.h:1043-1058, 814-830
-        if (!(J.step > 0)) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
-        int b0 = (int)((I.sIntegFin - I.sIntegStart) / J.step) >> 1;
-        I.nInt = ((long long)b0 << 1) + 1;
-        if (I.nInt < 3) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
+        if (J.meth == 1) {
+            // adaptive method: no manual integration points (the manual
+            // tables below get a 3-point placeholder)
+            if (!(J.relPrec > 0)) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
+            I.nInt = 3;
+        } else {
+            // srradint.h:1043-1058, 814-830
+            if (!(J.step > 0)) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
+            int b0 = (int)((I.sIntegFin - I.sIntegStart) / J.step) >> 1;
+            I.nInt = ((long long)b0 << 1) + 1;
+            if (I.nInt < 3) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
+        }
         I.h = (I.sIntegFin - I.sIntegStart) / (I.nInt - 1);
         I.yObs = J.mesh.zStart;
         // srradint.cpp:92-123
@@ -271,6 +278,25 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     HIPCHK(hipMemcpyAsync(rx.data(), dRx, sizeof(double) * 4 * nT, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(info.data(), dInfo, sizeof(srwd_trj_info) * nT, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    for (int j = 0; j < nT; j++) {
+        res[j].Robs = rx[4 * j]; res[j].RobsErr = rx[4 * j + 1];
+        res[j].xc = rx[4 * j + 2]; res[j].zc = rx[4 * j + 3];
+    }
+    if (jobs[0].meth == 1) {
+        for (auto& J : jobs) if (J.meth != 1) return SRWL_AMD_ERR_BATCH;
+        if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
+        rc = sr_auto_run(jobs, info, dInfo, dX, dY, dIBx, dIBz, ldT, dEdge, termin, dEx, dEz, outOff, res);
+        if (rc) return rc;
+        if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev1, st));
+        long long ev = 0;
+        for (int L : g_ctx.autoLevels) ev += 4LL * (1LL << L) + 1;
+        g_ctx.pendingEvals = ev;
+        *g_ctx.warnHost = 0;
+        if (defer) return 0;
+        HIPCHK(hipStreamSynchronize(st));
+        sr_complete(res);
+        return 0;
+    }
     // --- items -------------------------------------------------------------
     std::vector<srwd_item> items;
     std::vector<int> blockItem;
@@ -281,32 +307,13 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     for (int j = 0; j < nT; j++) {
         SrJob& J = jobs[j];
         const srwd_trj_info& I = info[j];
-        res[j].Robs = rx[4 * j]; res[j].RobsErr = rx[4 * j + 1];
-        res[j].xc = rx[4 * j + 2]; res[j].zc = rx[4 * j + 3];
         const SRWLRadMesh& m = J.mesh;
         int nx = (int)m.nx, nz = (int)m.ny, ne = (int)m.ne;
-        double xStep = nx > 1 ? (m.xFin - m.xStart) / (nx - 1) : 0;
-        double zStep = nz > 1 ? (m.yFin - m.yStart) / (nz - 1) : 0;
+        SrRegion G = sr_region(m, I);
+        double xStep = G.xStep, zStep = G.zStep;
         double eStep = ne > 1 ? (m.eFin - m.eStart) / (ne - 1) : 0;
-        double xS = m.xStart, xE = m.xStart + (nx - 1) * xStep;
-        double zS = m.yStart, zE = m.yStart + (nz - 1) * zStep;
-        // srradint.cpp:3231-3260
-        int symX = 0, symZ = 0;
-        if (!I.verNZ && nx > 1) {
-            double tol = xStep * 0.01;
-            symX = (std::fabs(I.dxds0 * (I.yObs - I.s0)) < tol) && (std::fabs(0.5 * (xS + xE) - I.x0) < tol);
-        }
-        if (!I.horNZ && nz > 1) {
-            double tol = zStep * 0.01;
-            symZ = (std::fabs(I.dzds0 * (I.yObs - I.s0)) < tol) && (std::fabs(0.5 * (zS + zE) - I.z0) < tol);
-        }
-        int nzc = nz, nxc = nx;
-        {   // srradint.cpp:331-345: rows / columns beyond the centre are mirrored
-            double z = zS;
-            for (int iz = 0; iz < nz; iz++) { if (symZ && (z - I.z0) > zStep * 0.001) { nzc = iz; break; } z += zStep; }
-            double x = xS;
-            for (int ix = 0; ix < nx; ix++) { if (symX && (x - I.x0) > xStep * 0.001) { nxc = ix; break; } x += xStep; }
-        }
+        double xS = m.xStart, zS = m.yStart;
+        int symX = G.symX, symZ = G.symZ, nzc = G.nzc, nxc = G.nxc;
         double normConst = std::sqrt(normBase * J.Iavg * 1.E+09 / 1.602189246E-19);
         for (int ie = 0; ie < ne; ie++) {
             srwd_item it;
@@ -373,6 +380,32 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     HIPCHK(hipStreamSynchronize(st));
     sr_complete(res);
     return 0;
+}
+
+SrRegion sr_region(const SRWLRadMesh& m, const srwd_trj_info& I)
+{
+    SrRegion G;
+    int nx = (int)m.nx, nz = (int)m.ny;
+    G.xStep = nx > 1 ? (m.xFin - m.xStart) / (nx - 1) : 0;
+    G.zStep = nz > 1 ? (m.yFin - m.yStart) / (nz - 1) : 0;
+    double xS = m.xStart, xE = m.xStart + (nx - 1) * G.xStep;
+    double zS = m.yStart, zE = m.yStart + (nz - 1) * G.zStep;
+    // srradint.cpp:3231-3260
+    if (!I.verNZ && nx > 1) {
+        double tol = G.xStep * 0.01;
+        G.symX = (std::fabs(I.dxds0 * (I.yObs - I.s0)) < tol) && (std::fabs(0.5 * (xS + xE) - I.x0) < tol);
+    }
+    if (!I.horNZ && nz > 1) {
+        double tol = G.zStep * 0.01;
+        G.symZ = (std::fabs(I.dzds0 * (I.yObs - I.s0)) < tol) && (std::fabs(0.5 * (zS + zE) - I.z0) < tol);
+    }
+    G.nzc = nz; G.nxc = nx;
+    // srradint.cpp:331-345: rows / columns beyond the centre are mirrored
+    double z = zS;
+    for (int iz = 0; iz < nz; iz++) { if (G.symZ && (z - I.z0) > G.zStep * 0.001) { G.nzc = iz; break; } z += G.zStep; }
+    double x = xS;
+    for (int ix = 0; ix < nx; ix++) { if (G.symX && (x - I.x0) > G.xStep * 0.001) { G.nxc = ix; break; } x += G.xStep; }
+    return G;
 }
 
 // After the stream has been synchronised: timing statistics and the
@@ -564,7 +597,8 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     bool fldDef = !(pMagFld == nullptr || pMagFld->arMagFld == nullptr || pMagFld->arMagFldTypes == nullptr ||
                     pMagFld->nElem <= 0);
     if (!trjDef && !fldDef) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
-    if ((int)precPar[0] != 0) return SRWL_AMD_ERR_UNSUPPORTED;      // methods 1, 2: not in this build
+    int meth = (int)precPar[0];
+    if (meth != 0 && meth != 1) return SRWL_AMD_ERR_UNSUPPORTED;      // method 2: not in this build
     if (nPrecPar > 6 && precPar[6] > 0) return SRWL_AMD_ERR_UNSUPPORTED;  // automatic mesh
     if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
     if (pWfr->mesh.nx <= 0 || pWfr->mesh.ny <= 0 || pWfr->mesh.ne <= 0) return SRWL_INCORRECT_WFR_STRUCT;
@@ -585,7 +619,9 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     }
     J.Iavg = pWfr->partBeam.Iavg;
     J.mesh = pWfr->mesh;
+    J.meth = meth;
     J.step = precPar[1];
+    J.relPrec = precPar[1];
     J.sLim0 = precPar[2]; J.sLim1 = precPar[3];
     int termin = (nPrecPar <= 0 || nPrecPar > 5) ? (int)(char)precPar[5] : 1;
     long long n = (long long)J.mesh.ne * J.mesh.nx * J.mesh.ny;
@@ -621,7 +657,8 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
                                          int nPrecPar)
 {
     if (arWfr == nullptr || arMagFld == nullptr || precPar == nullptr || nWfr <= 0) return SRWL_AMD_ERR_BATCH;
-    if ((int)precPar[0] != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    int meth = (int)precPar[0];
+    if (meth != 0 && meth != 1) return SRWL_AMD_ERR_UNSUPPORTED;
     if (nPrecPar > 6 && precPar[6] > 0) return SRWL_AMD_ERR_UNSUPPORTED;
     std::vector<SrJob> jobs(nWfr);
     std::vector<srwd_field> fields;
@@ -634,7 +671,9 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
         if (rc) return rc;
         jobs[i].Iavg = w->partBeam.Iavg;
         jobs[i].mesh = w->mesh;
+        jobs[i].meth = meth;
         jobs[i].step = precPar[1];
+        jobs[i].relPrec = precPar[1];
         jobs[i].sLim0 = precPar[2]; jobs[i].sLim1 = precPar[3];
         off[i] = tot;
         tot += (long long)w->mesh.ne * w->mesh.nx * w->mesh.ny;

@@ -47,6 +47,8 @@ def _load():
     lib.srwlAmdSetTiming.argtypes = [C.c_int]
     lib.srwlAmdSetKeepOnDevice.argtypes = [C.c_int]
     lib.srwlAmdNormals.argtypes = [C.c_ulonglong, C.c_longlong, P(C.c_double)]
+    lib.srwlAmdLastAutoLevels.argtypes = [P(C.c_int), C.c_longlong]
+    lib.srwlAmdLastAutoLevels.restype = C.c_longlong
     return lib
 
 
@@ -164,6 +166,16 @@ def set_keep_on_device(on=True):
 
 def set_traj_serial(on=True):
     lib.srwlAmdSetTrajSerial(1 if on else 0)
+
+
+def last_auto_levels():
+    """Final levels of the points of the last method-1 field computation, in
+    the reference's point order (rows, columns, energies of the computed region)."""
+    import numpy as np
+    n = lib.srwlAmdLastAutoLevels(None, 0)
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    lib.srwlAmdLastAutoLevels(out.ctypes.data_as(C.POINTER(C.c_int)), n)
+    return out[:n]
 
 
 def ResizeElecField(_wfr, _type, _par):

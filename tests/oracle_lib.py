@@ -22,6 +22,8 @@ def load():
     P = C.POINTER
     lib.orc_srwlCalcElecFieldSR_mt.argtypes = [P(S.SRWLWfr), P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double),
                                                C.c_int, C.c_int]
+    lib.orc_srwlCalcElecFieldSR_ex.argtypes = [P(S.SRWLWfr), P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double),
+                                               C.c_int, C.c_int, P(C.c_int)]
     lib.orc_srwlCalcPartTraj.argtypes = [P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double)]
     lib.orc_srwlPropagElecField.argtypes = [P(S.SRWLWfr), P(S.SRWLOptC)]
     lib.orc_srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_byte, C.c_byte, C.c_byte, C.c_double]
@@ -55,6 +57,37 @@ def calc_field(w, mag, prec, trj=None, nthreads=8):
                                         C.byref(cm) if cm is not None else None, cp, len(prec), nthreads)
     M.wfr_back(cw, w)
     return rc
+
+
+def calc_field_levels(w, mag, prec, trj=None):
+    """Method-1 field with the final level of every computed point, in the
+    reference's order (rows, columns, energies of the computed region)."""
+    import numpy as np
+    keep = M.Keep()
+    cw = M.wfr(w, keep)
+    cm = M.mag_cnt(mag, keep) if mag is not None else None
+    ct = M.prt_trj(trj, keep) if trj is not None else None
+    cp = M.dvec(prec, keep)
+    lv = np.full(w.mesh.nx * w.mesh.ny * w.mesh.ne, -1, dtype=np.int32)
+    rc = lib.orc_srwlCalcElecFieldSR_ex(C.byref(cw), C.byref(ct) if ct is not None else None,
+                                        C.byref(cm) if cm is not None else None, cp, len(prec), 1,
+                                        lv.ctypes.data_as(C.POINTER(C.c_int)))
+    M.wfr_back(cw, w)
+    return rc, lv[lv >= 0]
+
+
+def trj_interp(trj, s):
+    """Interpolated trajectory data at the positions s: rows of (Btx, X,
+    IBx, Btz, Z, IBz, Bx, Bz)."""
+    import numpy as np
+    keep = M.Keep()
+    ct = M.prt_trj(trj, keep)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    out = np.zeros((len(s), 8))
+    rc = lib.orc_trj_interp(C.byref(ct), s.ctypes.data_as(C.POINTER(C.c_double)), len(s),
+                            out.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc == 0
+    return out
 
 
 def calc_traj(trj, mag):
