@@ -48,6 +48,26 @@ SRWL_EXPORT int srwlAmdSetTrajSerial(int on);
  * Copies up to cap values into out (when given); returns the count. */
 SRWL_EXPORT long long srwlAmdLastAutoLevels(int* out, long long cap);
 
+/* on != 0: resample wavefronts with one thread per new point (each point
+ * builds its own interpolation forms) instead of one thread per old-mesh cell
+ * (forms built once per cell); both give identical values, the switch exists
+ * to compare them. */
+SRWL_EXPORT int srwlAmdSetResizePerPoint(int on);
+
+/* FFT tile shapes (rows per workgroup and threads of the forward row pass,
+ * columns per workgroup and threads of the column pass, rows per workgroup
+ * and threads of the inverse row pass).  srwlAmdSetFftTiles(t6) makes every
+ * pass use t6 (a shape that does not fit falls back to the default),
+ * srwlAmdSetFftTiles(NULL) returns to the autotuner; srwlAmdSetTuning(0)
+ * selects the fixed default shapes instead of the autotuner.
+ * srwlAmdGetFftTiles copies the shapes the autotuner chose for a drift of
+ * 2^lognx x 2^lognz slices (mode 0 or 3, postOp 0 field, 1 field with the new
+ * quadratic phase, 2 intensity) and returns 1, or returns 0 when that shape
+ * has not been tuned yet.  Every shape gives the same values. */
+SRWL_EXPORT int srwlAmdSetFftTiles(const int* t6);
+SRWL_EXPORT int srwlAmdSetTuning(int on);
+SRWL_EXPORT int srwlAmdGetFftTiles(int lognx, int lognz, int mode, int postOp, int* out6);
+
 /* Six standard normals of electron i of the counter-based sampler used by
  * srwlEmitPropMultiE (Philox4x32-10 + Box-Muller, see srw_rng.h). */
 SRWL_EXPORT int srwlAmdNormals(unsigned long long seed, long long i, double* g6);

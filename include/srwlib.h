@@ -194,9 +194,13 @@ SRWL_EXPORT int srwlUtiGetErrText(char* t, int erNo);
 SRWL_EXPORT int srwlCalcPartTraj(SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld, double* precPar);
 
 /* reference srwlib.h:639 (srwlib.cpp:456-540): near-field SR electric field.
- * precPar: [0] method (0 manual; 1, 2 not in this build), [1] step [m],
- * [2], [3] integration limits, [4] trajectory points, [5] terminating terms,
- * [6] sampling factor (>0 not in this build). */
+ * precPar: [0] method (0 manual, 1 adaptive; 2 not in this build), [1] step
+ * [m] (method 0) or relative precision (method 1), [2], [3] integration
+ * limits, [4] trajectory points, [5] terminating terms, [6] sampling factor:
+ * > 0 sets the mesh sizes from the wavefront radius (srradstr.cpp:1063-1119)
+ * and reallocates the field arrays through the callback of
+ * srwlUtiSetWfrModifFunc (action 2) when they change; without one the call
+ * returns SRWL_WFR_EXT_FUNC_NOT_DEFINED. */
 SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
                                     double* precPar, int nPrecPar);
 

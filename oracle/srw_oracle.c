@@ -984,8 +984,28 @@ static void orc_free_trj_arrays(SRWLPrtTrj* t)
  * convergence threshold depends on the points already done.  levels, when
  * given, receives the final level of each computed point in that order
  * (rows, columns, energies of the computed region). */
+ORC_API long orc_next_fft_size(long n);
+
+/* srradstr.cpp:1063-1119 CheckNxNzForSR, coordinate representation: the mesh
+ * step that samples the wavefront curvature phase pi(x-xc)^2/(lambda R) at the
+ * mesh edge with at most pi/(1.2 f) per step, rounded up to an even size with
+ * a fast FFT and at least 8 points. */
+static long orc_auto_mesh_axis(double start, double fin, long n, double c, double R, double lambda, double f)
+{
+    double step = n > 1 ? (fin - start) / (n - 1) : 0.;
+    double halfLamR = 0.5 * lambda * R;
+    double sRel = start - c, eRel = start + (n - 1) * step - c;
+    double d0 = fabs(halfLamR / sRel), d1 = fabs(halfLamR / eRel);
+    double d = ((d0 < d1) ? d0 : d1) / (f * 1.2);
+    long m = (long)(fabs(eRel - sRel) / d) + 1;
+    if (((m >> 1) << 1) != m) m++;
+    m = orc_next_fft_size(m);
+    if (m < 8) m = 8;
+    return m;
+}
+
 ORC_API int orc_srwlCalcElecFieldSR_ex(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
-                                      double* precPar, int nPrecPar, int nthreads, int* levels)
+                                      double* precPar, int nPrecPar, int nthreads, int* levels, long* nxnzOut)
 {
     if (pWfr == 0 || precPar == 0) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
     int trjDef = !(pTrj == 0 || ((!pTrj->arX || !pTrj->arXp) && (!pTrj->arY || !pTrj->arYp)) || pTrj->np <= 0);
@@ -993,7 +1013,7 @@ ORC_API int orc_srwlCalcElecFieldSR_ex(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
     int meth = (int)precPar[0];
     if (meth != 0 && meth != 1) return SRWL_AMD_ERR_UNSUPPORTED;
     if (meth == 1 && !(precPar[1] > 0.)) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
-    if (nPrecPar > 6 && precPar[6] > 0) return SRWL_AMD_ERR_UNSUPPORTED;
+    double sampFact = (nPrecPar > 6) ? precPar[6] : 0.;
     SRWLPrtTrj own;
     int res = 0;
     if (!trjDef) {
@@ -1019,6 +1039,15 @@ ORC_API int orc_srwlCalcElecFieldSR_ex(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
         orc_find_rx(&T, yObs, xS, xE, zS, zE, precPar, &Robs, &RobsErr, &xcS, &zcS);
         pWfr->Rx = pWfr->Ry = Robs; pWfr->dRx = pWfr->dRy = RobsErr;
         pWfr->xc = xcS; pWfr->yc = zcS;
+        if (sampFact > 0.) {
+            /* srradstr.cpp:353-371: a new mesh size needs the caller's
+             * reallocation, which this restatement does not have */
+            double lambda = 1.239842E-06 / m->eStart;
+            long nxN = orc_auto_mesh_axis(m->xStart, m->xFin, nx, xcS, Robs, lambda, sampFact);
+            long nzN = orc_auto_mesh_axis(m->yStart, m->yFin, nz, zcS, Robs, lambda, sampFact);
+            if (nxnzOut) { nxnzOut[0] = nxN; nxnzOut[1] = nzN; }
+            if (nxN != nx || nzN != nz) { res = SRWL_WFR_EXT_FUNC_NOT_DEFINED; goto done1; }
+        }
         /* precision parameters (srradint.cpp:3579-3614) */
         orc_sr S;
         memset(&S, 0, sizeof(S));
@@ -1191,7 +1220,7 @@ done0:
 ORC_API int orc_srwlCalcElecFieldSR_mt(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
                                       double* precPar, int nPrecPar, int nthreads)
 {
-    return orc_srwlCalcElecFieldSR_ex(pWfr, pTrj, pMagFld, precPar, nPrecPar, nthreads, 0);
+    return orc_srwlCalcElecFieldSR_ex(pWfr, pTrj, pMagFld, precPar, nPrecPar, nthreads, 0, 0);
 }
 
 ORC_API int orc_srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,

@@ -472,37 +472,72 @@ int srwf_rows_per_wg(int logn)
 }
 int srwf_cols_per_wg(int logn) { int n = 1 << logn; return n >= 1024 ? 8 : 16; }
 
-int srwf_launch_rows_fwd(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* tw,
-                         int logn, int ny, int preOp, int inv, hipStream_t st)
+void srwf_default_tiles(int lognx, int logny, int postOp, srwf_tiles* t)
 {
-    int rpw = srwf_rows_per_wg(logn);
+    t->rpwF = srwf_rows_per_wg(lognx);
+    t->thrF = SRWF_ROWS_THREADS;
+    int cpw = srwf_cols_per_wg(logny), nx = 1 << lognx;
+    t->cpw = cpw > nx ? nx : cpw;
+    t->thrC = ((1 << logny) * t->cpw >= 8192) ? SRWF_COLS_BIG_THREADS : 256;
+    int rpw = srwf_rows_per_wg(lognx);
+    if (postOp == 2) rpw = rpw > 1 ? rpw / 2 : 1;
+    t->rpwI = rpw;
+    t->thrI = SRWF_ROWS_THREADS;
+}
+
+int srwf_tiles_fit(int lognx, int logny, int postOp, const srwf_tiles* t)
+{
+    const int lim = 160 * 1024, nx = 1 << lognx, ny = 1 << logny;
+    auto okThr = [](int v) { return v == 256 || v == 512 || v == 1024; };
+    if (!okThr(t->thrF) || !okThr(t->thrC) || !okThr(t->thrI)) return 0;
+    if (t->rpwF < 1 || t->rpwF > ny || lds_bytes(nx, t->rpwF) > lim) return 0;
+    if (t->rpwI < 1 || t->rpwI > ny || lds_bytes(nx, (postOp == 2 ? 2 : 1) * t->rpwI) > lim) return 0;
+    if (t->cpw < 1 || t->cpw > nx || (t->cpw & (t->cpw - 1)) != 0 || lds_bytes(ny, t->cpw) > lim) return 0;
+    return 1;
+}
+
+static srwf_tiles tiles_or_default(int lognx, int logny, int postOp, const srwf_tiles* t)
+{
+    srwf_tiles d;
+    if (t && srwf_tiles_fit(lognx, logny, postOp, t)) return *t;
+    srwf_default_tiles(lognx, logny, postOp, &d);
+    return d;
+}
+
+int srwf_launch_rows_fwd(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* tw,
+                         int logn, int ny, int preOp, int inv, const srwf_tiles* tiles, hipStream_t st)
+{
+    int logny = 31 - __builtin_clz((unsigned)ny);
+    srwf_tiles T = tiles_or_default(logn, logny, 0, tiles);
+    int rpw = T.rpwF;
     dim3 grid((ny + rpw - 1) / rpw, nSlices);
-    hipLaunchKernelGGL(k_rows_fwd, grid, dim3(SRWF_ROWS_THREADS), lds_bytes(1 << logn, rpw), st, data, sliceStride, dSp, tw, logn,
+    hipLaunchKernelGGL(k_rows_fwd, grid, dim3(T.thrF), lds_bytes(1 << logn, rpw), st, data, sliceStride, dSp, tw, logn,
                        ny, rpw, preOp, inv);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int srwf_launch_cols(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* twy,
-                     int logny, int lognx, int op, int inv, float2* out, hipStream_t st)
+                     int logny, int lognx, int op, int inv, float2* out, const srwf_tiles* tiles, hipStream_t st)
 {
-    int cpw = srwf_cols_per_wg(logny);
+    srwf_tiles T = tiles_or_default(lognx, logny, 0, tiles);
+    int cpw = T.cpw;
     int nx = 1 << lognx;
-    if (cpw > nx) cpw = nx;
     dim3 grid(nx / cpw, nSlices);
-    const int thr = ((1 << logny) * cpw >= 8192) ? SRWF_COLS_BIG_THREADS : 256;
-    hipLaunchKernelGGL(k_cols, grid, dim3(thr), lds_bytes(1 << logny, cpw), st, data, sliceStride, dSp, twy, logny,
+    hipLaunchKernelGGL(k_cols, grid, dim3(T.thrC), lds_bytes(1 << logny, cpw), st, data, sliceStride, dSp, twy, logny,
                        lognx, cpw, op, inv, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int srwf_launch_rows_inv(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* tw,
-                         int logn, int ny, int postOp, float* inten, long long intenStride, hipStream_t st)
+                         int logn, int ny, int postOp, float* inten, long long intenStride, const srwf_tiles* tiles,
+                         hipStream_t st)
 {
-    int rpw = srwf_rows_per_wg(logn);
-    if (postOp == 2) rpw = rpw > 1 ? rpw / 2 : 1;
+    int logny = 31 - __builtin_clz((unsigned)ny);
+    srwf_tiles T = tiles_or_default(logn, logny, postOp, tiles);
+    int rpw = T.rpwI;
     int nl = (postOp == 2 ? 2 : 1) * rpw;
     dim3 grid((ny + rpw - 1) / rpw, postOp == 2 ? nSlices / 2 : nSlices);
-    hipLaunchKernelGGL(k_rows_inv, grid, dim3(SRWF_ROWS_THREADS), lds_bytes(1 << logn, nl), st, data, sliceStride, dSp, tw, logn,
+    hipLaunchKernelGGL(k_rows_inv, grid, dim3(T.thrI), lds_bytes(1 << logn, nl), st, data, sliceStride, dSp, tw, logn,
                        ny, rpw, postOp, inten, intenStride);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -63,14 +63,15 @@ namespace {
 // contribute zero).  One thread per result point; loops over the batch.
 __global__ void __launch_bounds__(256) k_accum_interp(const float* __restrict__ I, long long iStride,
                                                       const srwm_emesh* __restrict__ em, const float* __restrict__ w,
-                                                      int nb, srwm_rmesh rm, double* S1, double* S2, double* norm2)
+                                                      int nb, srwm_rmesh rm, double* S1, double* S2, double* norm2,
+                                                      double* Srep, int perRep)
 {
     long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long nr = (long long)rm.nx * rm.nz;
     bool valid = r < nr;
     int ix = valid ? (int)(r % rm.nx) : 0, iz = valid ? (int)(r / rm.nx) : 0;
     double xr = rm.xStart + ix * rm.xStep, zr = rm.zStart + iz * rm.zStep;
-    double s1 = 0., s2 = 0.;
+    double s1 = 0., s2 = 0., sg = 0.;
     for (int j = 0; j < nb; j++) {
         const srwm_emesh E = em[j];
         float f = 0.f;
@@ -87,6 +88,13 @@ __global__ void __launch_bounds__(256) k_accum_interp(const float* __restrict__ 
         double v = (double)w[j] * (double)f;
         s1 += v;
         s2 += v * v;
+        if (Srep) {
+            sg += v;
+            if ((j + 1) % perRep == 0) {
+                if (valid) Srep[(long long)(j / perRep) * nr + r] += sg;
+                sg = 0.;
+            }
+        }
         if (norm2) {
             // per-electron squared L2 norm of the re-gridded intensity (training signal)
             double q = (double)f * (double)f;
@@ -109,11 +117,13 @@ __global__ void k_stokes_out(const double* __restrict__ S1, double scale, long l
 
 extern "C" {
 int srwm_launch_accum(const float* I, long long iStride, const srwm_emesh* em, const float* w, int nb,
-                      srwm_rmesh rm, double* S1, double* S2, double* norm2, hipStream_t st)
+                      srwm_rmesh rm, double* S1, double* S2, double* norm2, double* Srep, int perRep,
+                      hipStream_t st)
 {
     long long nr = (long long)rm.nx * rm.nz;
+    if (Srep && (perRep <= 0 || nb % perRep != 0)) return -1;
     hipLaunchKernelGGL(k_accum_interp, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, I, iStride, em, w, nb,
-                       rm, S1, S2, norm2);
+                       rm, S1, S2, norm2, Srep, perRep);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

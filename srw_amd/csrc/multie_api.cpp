@@ -70,8 +70,9 @@ struct Proposal {
     double logit[SRWL_K], mu[SRWL_K][SRWL_D], ls[SRWL_K][SRWL_D];
 };
 
-// Initial parameters: scaled normal hidden weights, zero output weights and a
-// fixed spread of component means (so the components are distinguishable).
+// Initial parameters: scaled normal hidden weights and small output weights,
+// output biases zero, so that every component starts as N(0, I) and the
+// initial proposal is p itself; training then moves it toward q*.
 void proposal_init_params(std::vector<float>& P, unsigned long long seed)
 {
     P.assign(SRWL_NPARAM, 0.f);
@@ -85,7 +86,6 @@ void proposal_init_params(std::vector<float>& P, unsigned long long seed)
     fill(SRWL_OFF_W1, SRWL_H * SRWL_IN, 1. / std::sqrt((double)SRWL_IN), 1);
     fill(SRWL_OFF_W2, SRWL_H * SRWL_H, 1. / std::sqrt((double)SRWL_H), 2);
     fill(SRWL_OFF_W3, SRWL_OUT * SRWL_H, 0.01 / std::sqrt((double)SRWL_H), 3);
-    fill(SRWL_OFF_B3 + SRWL_K, SRWL_K * SRWL_D, 0.6, 4);   // component means
 }
 
 void proposal_set(Proposal& Q, const float* o)
@@ -248,20 +248,47 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     if (!dF || !dI || !dS || !dEm || !dW || !dSp || !dN2) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     HIPCHK(hipMemsetAsync(dS, 0, sizeof(double) * 2 * nr, st));
     int useQ = (arPar && arPar[2] > 0) ? 1 : 0;
+    // arPar[6] > 0: randomized quasi-Monte Carlo draws (digitally shifted
+    // Sobol' points, arPar[7] independent replicates, 16 by default) instead
+    // of pseudo-random ones; the error estimate is the spread of the
+    // replicate averages.  Batches hold whole groups of one point per
+    // replicate.
+    const int rqmc = (arPar && arPar[6] > 0) ? 1 : 0;
+    const int R = rqmc ? ((arPar && arPar[7] > 1) ? (int)arPar[7] : 16) : 1;
+    if (rqmc && useQ) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (rqmc) {
+        if (nMacroElec < R) return SRWL_AMD_ERR_BATCH;
+        B = std::max<long long>(R, (B / R) * R);
+    }
+    uint32_t sobolV[SRW_SOBOL_DIMS][32];
+    if (rqmc) srw_sobol_directions(sobolV);
+    double* dSrep = nullptr;
+    if (rqmc) {
+        dSrep = (double*)Bf.Rep.get(sizeof(double) * R * nr);
+        if (!dSrep) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        HIPCHK(hipMemsetAsync(dSrep, 0, sizeof(double) * R * nr, st));
+    }
+    std::vector<double> Srh;
     // arPar[5] > 0: also estimate the gain of the variance-optimal proposal
     // q* ~ p ||I|| for this estimator from the electrons drawn (plain runs)
     int diag = (arPar && arPar[5] > 0 && !useQ) ? 1 : 0;
     double sumN2 = 0., sumN = 0.;
-    int trainSteps = (arPar && arPar[4] > 0) ? (int)arPar[4] : 4;
+    int trainSteps = (arPar && arPar[4] > 0) ? (int)arPar[4] : 32;
+    // training set: every electron drawn so far (up to kReplay, oldest
+    // dropped first) with its standardized phase-space point, its weight p/q
+    // at the time it was drawn and its intensity norm
+    const long long kReplay = 8192;
+    std::vector<float> repG;
+    std::vector<double> repW, repC;
     Proposal Q;
     std::vector<float> hOut(SRWL_OUT);
     double lastLoss = 0;
     if (useQ) {
-        float* base = (float*)Bf.Net.get(sizeof(float) * (4 * SRWL_NPARAM + SRWL_NACT + SRWL_OUT + 8 + 7 * B + 64));
+        float* base = (float*)Bf.Net.get(sizeof(float) * (4 * SRWL_NPARAM + SRWL_NACT + SRWL_OUT + 8 + 7 * kReplay + 64));
         if (!base) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
         Q.P = base; Q.m = Q.P + SRWL_NPARAM; Q.v = Q.m + SRWL_NPARAM; Q.G = Q.v + SRWL_NPARAM;
         Q.act = Q.G + SRWL_NPARAM; Q.gout = Q.act + SRWL_NACT; Q.loss = Q.gout + SRWL_OUT; Q.in = Q.loss + 2;
-        Q.gs = Q.in + 8; Q.wh = Q.gs + 6 * B;
+        Q.gs = Q.in + 8; Q.wh = Q.gs + 6 * kReplay;
         std::vector<float> P0;
         proposal_init_params(P0, 0x6c65617264ull);
         HIPCHK(hipMemcpyAsync(Q.P, P0.data(), sizeof(float) * SRWL_NPARAM, hipMemcpyHostToDevice, st));
@@ -289,6 +316,9 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     std::vector<double> S1h, S2h;
     while (done < nMacroElec) {
         long long nb = std::min<long long>(B, nMacroElec - done);
+        if (rqmc) nb = (nb / R) * R;
+        if (nb <= 0) break;
+        const long long perRep = nb / R;
         std::vector<SrJob> jobs(nb);
         std::vector<long long> off(nb);
         std::vector<float> w(nb, 1.f);
@@ -298,6 +328,8 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
             HIPCHK(hipMemcpyAsync(hOut.data(), Q.act + SRWL_ACT_OUT, sizeof(float) * SRWL_OUT, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             proposal_set(Q, hOut.data());
+            // the first batch, before any training, is drawn from p itself
+            Q.alpha = (Q.step == 0) ? 1.0 : 0.25;
         }
         for (long long j = 0; j < nb; j++) {
             double g[6];
@@ -307,6 +339,9 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
                 proposal_logpq(Q, g, lp, lq);
                 w[j] = (float)std::exp(lp - lq);
                 for (int d = 0; d < 6; d++) gsamp[6 * j + d] = (float)g[d];
+            } else if (rqmc) {
+                // slot j: replicate j / perRep, its point done / R + j % perRep
+                srw_rqmc_normals6(sobolV, seed, (unsigned)(j / perRep), (unsigned long long)(done / R + j % perRep), g);
             } else srw_normals6(seed, (unsigned long long)(done + j), g);
             SRWLParticle p = pWfr0->partBeam.partStatMom1;
             bs.map(g, p);
@@ -434,7 +469,8 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         HIPCHK(hipMemcpyAsync(dEm, em.data(), sizeof(srwm_emesh) * nb, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dW, w.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st));
         if (useQ || diag) HIPCHK(hipMemsetAsync(dN2, 0, sizeof(double) * nb, st));
-        if (srwm_launch_accum(dI, nxy, dEm, dW, (int)nb, rm, dS, dS + nr, (useQ || diag) ? dN2 : nullptr, st))
+        if (srwm_launch_accum(dI, nxy, dEm, dW, (int)nb, rm, dS, dS + nr, (useQ || diag) ? dN2 : nullptr, dSrep,
+                              (int)perRep, st))
             return SRWL_AMD_ERR_GPU;
         if (diag) {
             std::vector<double> n2(nb);
@@ -447,16 +483,31 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
             std::vector<double> n2(nb);
             HIPCHK(hipMemcpyAsync(n2.data(), dN2, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
-            std::vector<float> wh(nb);
+            for (long long j = 0; j < nb; j++) {
+                repG.insert(repG.end(), gsamp.begin() + 6 * j, gsamp.begin() + 6 * j + 6);
+                repW.push_back(w[j]);
+                repC.push_back(std::sqrt(std::max(n2[j], 0.)));
+            }
+            if ((long long)repW.size() > kReplay) {
+                size_t drop = repW.size() - (size_t)kReplay;
+                repG.erase(repG.begin(), repG.begin() + 6 * drop);
+                repW.erase(repW.begin(), repW.begin() + drop);
+                repC.erase(repC.begin(), repC.begin() + drop);
+            }
+            // weighted maximum likelihood toward q* ~ p ||I||: weight of a
+            // sample drawn from q_j is p/q_j * ||I|| (normalized)
+            int nr = (int)repW.size();
+            std::vector<float> wh(nr);
             double tot = 0;
-            for (long long j = 0; j < nb; j++) { double c = w[j] * std::sqrt(std::max(n2[j], 0.)); wh[j] = (float)c; tot += c; }
-            for (long long j = 0; j < nb; j++) wh[j] = tot > 0 ? (float)(wh[j] / tot) : 0.f;
-            HIPCHK(hipMemcpyAsync(Q.gs, gsamp.data(), sizeof(float) * 6 * nb, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(Q.wh, wh.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st));
+            for (int j = 0; j < nr; j++) tot += repW[j] * repC[j];
+            for (int j = 0; j < nr; j++) wh[j] = tot > 0 ? (float)(repW[j] * repC[j] / tot) : 0.f;
+            HIPCHK(hipMemcpyAsync(Q.gs, repG.data(), sizeof(float) * 6 * nr, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(Q.wh, wh.data(), sizeof(float) * nr, hipMemcpyHostToDevice, st));
+            const float trainAlpha = 0.25f;
             for (int e = 0; e < trainSteps; e++) {
                 if (srwl_launch_mlp_forward(Q.P, Q.in, Q.act, st)) return SRWL_AMD_ERR_GPU;
                 HIPCHK(hipMemsetAsync(Q.gout, 0, sizeof(float) * (SRWL_OUT + 2), st));
-                if (srwl_launch_gmm_grad(Q.act, Q.gs, Q.wh, (int)nb, (float)Q.alpha, Q.gout, Q.loss, st))
+                if (srwl_launch_gmm_grad(Q.act, Q.gs, Q.wh, nr, trainAlpha, Q.gout, Q.loss, st))
                     return SRWL_AMD_ERR_GPU;
                 if (srwl_launch_mlp_backward(Q.P, Q.act, Q.gout, Q.G, st)) return SRWL_AMD_ERR_GPU;
                 Q.step++;
@@ -477,6 +528,19 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
             for (long long r = 0; r < nr; r++) { a1 += S1h[r] * S1h[r]; a2 += S2h[r]; }
             double N = (double)done;
             errEst = (done > 1 && a1 > 0) ? std::sqrt(std::max(0., N * (a2 - a1 / N) / ((N - 1.) * a1))) : -1;
+            if (rqmc) {
+                // variance of the mean from the R replicate means (each of done / R points)
+                Srh.resize((size_t)R * nr);
+                HIPCHK(hipMemcpyAsync(Srh.data(), dSrep, sizeof(double) * R * nr, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                double nPer = N / R, v = 0.;
+                for (long long q = 0; q < nr; q++) {
+                    double m = S1h[q] / N, d2 = 0.;
+                    for (int k = 0; k < R; k++) { double d = Srh[(size_t)k * nr + q] / nPer - m; d2 += d * d; }
+                    v += d2 / ((double)R * (R - 1));
+                }
+                errEst = a1 > 0 ? std::sqrt(v * N * N / a1) : -1;
+            }
             lastA1 = a1;
             if (target > 0 && errEst > 0 && errEst <= target) break;
         }
@@ -495,6 +559,7 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         arOut[5] = (double)B;
         arOut[6] = lastLoss;
         arOut[7] = useQ ? Q.step : 0;
+        arOut[9] = R;
         if (diag) {
             // N Var = E||I||^2 - ||mu||^2 for p; (E||I||)^2 - ||mu||^2 for q* ~ p ||I||
             double N = (double)done, m2 = lastA1 / (N * N);

@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <vector>
 
@@ -121,6 +122,91 @@ void drift_slice(srwf_slice& S, int mode, const DriftGeom& g, double L, int trea
     }
 }
 
+// Autotuner of the FFT tile shapes.  For each (nx, nz, pass operations) the
+// first call times every shape that fits, pass by pass, on scratch slices of
+// the same size (at most 16, with the caller's slice parameters) and keeps the
+// fastest; later calls reuse it.  Shapes change only the work assignment, so
+// the values computed do not depend on the choice.
+struct TileKey {
+    int lx, lz, preOp, colOp, inv, postOp;
+    bool operator<(const TileKey& o) const
+    {
+        int a[6] = {lx, lz, preOp, colOp, inv, postOp}, b[6] = {o.lx, o.lz, o.preOp, o.colOp, o.inv, o.postOp};
+        for (int i = 0; i < 6; i++) if (a[i] != b[i]) return a[i] < b[i];
+        return false;
+    }
+};
+static std::map<TileKey, srwf_tiles> g_tiles;
+
+static int tune_time(hipStream_t st, hipEvent_t e0, hipEvent_t e1, float& ms, const std::function<int()>& run)
+{
+    if (run()) return -1;                       // first launch: code load, caches
+    if (hipEventRecord(e0, st) != hipSuccess) return -1;
+    for (int k = 0; k < 3; k++) if (run()) return -1;
+    if (hipEventRecord(e1, st) != hipSuccess) return -1;
+    if (hipEventSynchronize(e1) != hipSuccess) return -1;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1;
+    return 0;
+}
+
+const srwf_tiles* fft_tiles(int lognx, int lognz, int preOp, int colOp, int inv, int postOp, const srwf_slice* dSp,
+                            int nSlices, hipStream_t st)
+{
+    if (g_ctx.tilesForced) return &g_ctx.forcedTiles;
+    if (!g_ctx.tuneTiles) return nullptr;
+    TileKey key = {lognx, lognz, preOp, colOp, inv, postOp};
+    auto it = g_tiles.find(key);
+    if (it != g_tiles.end()) return &it->second;
+    srwf_tiles best;
+    srwf_default_tiles(lognx, lognz, postOp, &best);
+    const long long nxy = 1LL << (lognx + lognz);
+    int S = nSlices < 16 ? nSlices : 16;
+    if (postOp == 2) S &= ~1;
+    const float2* twx = twiddles(lognx);
+    const float2* twz = twiddles(lognz);
+    // S slices, S output slices of a standalone transform, S/2 intensity slices
+    const size_t nScratch = (size_t)3 * S * nxy;
+    float2* d = (float2*)g_ctx.buf.Tune.get(sizeof(float2) * nScratch);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = S > 0 && d && twx && twz && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+              hipMemsetAsync(d, 0, sizeof(float2) * nScratch, st) == hipSuccess;
+    float2* out = d + (size_t)S * nxy;
+    float* inten = (float*)(d + (size_t)2 * S * nxy);
+    const int nz = 1 << lognz;
+    const int thrs[3] = {256, 512, 1024};
+    auto tryPass = [&](int pass) {
+        float bestMs = 1e30f;
+        srwf_tiles keep = best;
+        for (int a = 1; a <= (pass == 1 ? 64 : 32); a <<= 1)
+            for (int ti = 0; ti < 3; ti++) {
+                srwf_tiles c = best;
+                if (pass == 0) { c.rpwF = a; c.thrF = thrs[ti]; }
+                else if (pass == 1) { c.cpw = a; c.thrC = thrs[ti]; }
+                else { c.rpwI = a; c.thrI = thrs[ti]; }
+                if (!srwf_tiles_fit(lognx, lognz, postOp, &c)) continue;
+                float ms = 0;
+                int rc = tune_time(st, e0, e1, ms, [&]() -> int {
+                    if (pass == 0) return srwf_launch_rows_fwd(d, nxy, S, dSp, twx, lognx, nz, preOp, inv, &c, st);
+                    if (pass == 1) return srwf_launch_cols(d, nxy, S, dSp, twz, lognz, lognx, colOp, inv, out, &c, st);
+                    return srwf_launch_rows_inv(d, nxy, S, dSp, twx, lognx, nz, postOp, inten, nxy, &c, st);
+                });
+                if (rc) { ok = false; return; }
+                if (ms < bestMs) { bestMs = ms; keep = c; }
+            }
+        best = keep;
+    };
+    if (ok) tryPass(0);
+    if (ok) tryPass(1);
+    if (ok && postOp >= 0) tryPass(2);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (!ok) {
+        (void)hipGetLastError();
+        srwf_default_tiles(lognx, lognz, postOp, &best);
+    }
+    return &(g_tiles[key] = best);
+}
+
 // Runs passes A-C over nSlices N x N slices in place.
 int drift_passes(float2* d, long long sliceStride, int nSlices, const srwf_slice* dSp, int lognx, int lognz,
                  int mode, int postOp, float* inten, long long intenStride, hipStream_t st)
@@ -129,13 +215,15 @@ int drift_passes(float2* d, long long sliceStride, int nSlices, const srwf_slice
     const float2* twz = twiddles(lognz);
     if (!twx || !twz) return SRWL_AMD_ERR_GPU;
     int nz = 1 << lognz;
-    if (srwf_launch_rows_fwd(d, sliceStride, nSlices, dSp, twx, lognx, nz, mode == 3 ? 1 : 0, 0, st))
-        return SRWL_AMD_ERR_GPU;
-    if (srwf_launch_cols(d, sliceStride, nSlices, dSp, twz, lognz, lognx, mode == 3 ? 1 : 0, 0, nullptr, st))
-        return SRWL_AMD_ERR_GPU;
     int pop = postOp;
     if (pop == 1 && mode != 3) pop = 0;
-    if (srwf_launch_rows_inv(d, sliceStride, nSlices, dSp, twx, lognx, nz, pop, inten, intenStride, st))
+    const int pre = mode == 3 ? 1 : 0, col = mode == 3 ? 1 : 0;
+    const srwf_tiles* T = fft_tiles(lognx, lognz, pre, col, 0, pop, dSp, nSlices, st);
+    if (srwf_launch_rows_fwd(d, sliceStride, nSlices, dSp, twx, lognx, nz, pre, 0, T, st))
+        return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_cols(d, sliceStride, nSlices, dSp, twz, lognz, lognx, col, 0, nullptr, T, st))
+        return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_rows_inv(d, sliceStride, nSlices, dSp, twx, lognx, nz, pop, inten, intenStride, T, st))
         return SRWL_AMD_ERR_GPU;
     return 0;
 }
@@ -377,12 +465,40 @@ SRWL_EXPORT int srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMes
     const float2* twz = twiddles(lz);
     if (!twx || !twz) return SRWL_AMD_ERR_GPU;
     int inv = dir > 0 ? 0 : 1;
-    if (srwf_launch_rows_fwd(d, nxy, 1, dSp, twx, lx, (int)nz, 0, inv, st)) return SRWL_AMD_ERR_GPU;
-    if (srwf_launch_cols(d, nxy, 1, dSp, twz, lz, lx, 2, inv, o, st)) return SRWL_AMD_ERR_GPU;
+    const srwf_tiles* T = fft_tiles(lx, lz, 0, 2, inv, -1, dSp, 1, st);
+    if (srwf_launch_rows_fwd(d, nxy, 1, dSp, twx, lx, (int)nz, 0, inv, T, st)) return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_cols(d, nxy, 1, dSp, twz, lz, lx, 2, inv, o, T, st)) return SRWL_AMD_ERR_GPU;
     HIPCHK(hipMemcpyAsync(pcData, o, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     arMesh[0] = xStartTr; arMesh[1] = xStepTr; arMesh[3] = zStartTr; arMesh[4] = zStepTr;
     return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetFftTiles(const int* t6)
+{
+    g_ctx.tilesForced = t6 != nullptr;
+    if (t6) {
+        srwf_tiles& T = g_ctx.forcedTiles;
+        T.rpwF = t6[0]; T.thrF = t6[1]; T.cpw = t6[2]; T.thrC = t6[3]; T.rpwI = t6[4]; T.thrI = t6[5];
+    }
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetTuning(int on)
+{
+    g_ctx.tuneTiles = on != 0;
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdGetFftTiles(int lognx, int lognz, int mode, int postOp, int* out6)
+{
+    int pre = mode == 3 ? 1 : 0;
+    if (postOp == 1 && mode != 3) postOp = 0;
+    auto it = g_tiles.find(TileKey{lognx, lognz, pre, pre, 0, postOp});
+    if (it == g_tiles.end()) return 0;
+    const srwf_tiles& T = it->second;
+    if (out6) { out6[0] = T.rpwF; out6[1] = T.thrF; out6[2] = T.cpw; out6[3] = T.thrC; out6[4] = T.rpwI; out6[5] = T.thrI; }
+    return 1;
 }
 
 SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, char intType, char depType, double e,

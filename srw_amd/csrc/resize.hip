@@ -112,47 +112,103 @@ __device__ __forceinline__ double bilinear_eval(const float* F, int ix, int iz, 
     return a11 * (x * z) + a10 * x + a01 * z + a0;
 }
 
-// One component at relative cell position (xr, zr); re/im: the 16 cell values.
-__device__ __forceinline__ float2 resample_point(const float* re, const float* im, int ixRel, int izRel, double xr,
-                                                 double zr, bool improve)
+// Intensity of the 16 cell values, scaled to an average of at most 1, and the
+// scale (sroptelm.cpp:3150-3200); low: the cell holds an exact zero, which
+// selects the bilinear form (sroptelm.h:650-657).
+__device__ __forceinline__ void cell_intensity(const float* re, const float* im, float* In, float& norm, bool& low)
 {
-    float In[16];
     for (int k = 0; k < 16; k++) In[k] = __fadd_rn(__fmul_rn(re[k], re[k]), __fmul_rn(im[k], im[k]));
     float sum = In[0];
     for (int k = 1; k < 16; k++) sum = __fadd_rn(sum, In[k]);
     float avg = (float)(0.0625 * (double)sum);
-    float norm = 1.f;
+    norm = 1.f;
     if (fabsf(avg) > 1.f) {
         float a = (float)(1. / avg);
         for (int k = 0; k < 16; k++) In[k] = __fmul_rn(In[k], a);
         norm = avg;
     }
-    bool low = false;
+    low = false;
     for (int k = 0; k < 16; k++) low = low || (re[k] == 0.f) || (im[k] == 0.f);
-    float F0, F1, FI;
-    if (!low) {
-        double a[16];
-        bicubic(re, a); F0 = (float)bicubic_eval(a, xr, zr);
-        bicubic(im, a); F1 = (float)bicubic_eval(a, xr, zr);
-        bicubic(In, a);
-        double b = bicubic_eval(a, xr, zr);
-        FI = (float)((b > 0.) ? b : 0.);
-    } else {
-        int ix = ixRel < 0 ? 0 : (ixRel > 2 ? 2 : ixRel);
-        int iz = izRel < 0 ? 0 : (izRel > 2 ? 2 : izRel);
-        F0 = (float)bilinear_eval(re, ix, iz, xr, zr);
-        F1 = (float)bilinear_eval(im, ix, iz, xr, zr);
-        double b = bilinear_eval(In, ix, iz, xr, zr);
-        FI = (float)((b > 0.) ? b : 0.);
-    }
-    if (!improve) return make_float2(F0, F1);
+}
+
+__device__ __forceinline__ float2 bilinear_point(const float* re, const float* im, const float* In, int ixRel,
+                                                 int izRel, double xr, double zr, float& FI)
+{
+    int ix = ixRel < 0 ? 0 : (ixRel > 2 ? 2 : ixRel);
+    int iz = izRel < 0 ? 0 : (izRel > 2 ? 2 : izRel);
+    float F0 = (float)bilinear_eval(re, ix, iz, xr, zr);
+    float F1 = (float)bilinear_eval(im, ix, iz, xr, zr);
+    double b = bilinear_eval(In, ix, iz, xr, zr);
+    FI = (float)((b > 0.) ? b : 0.);
+    return make_float2(F0, F1);
+}
+
+__device__ __forceinline__ float2 bicubic_point(const double* aRe, const double* aIm, const double* aIn, double xr,
+                                                double zr, float& FI)
+{
+    float F0 = (float)bicubic_eval(aRe, xr, zr);
+    float F1 = (float)bicubic_eval(aIm, xr, zr);
+    double b = bicubic_eval(aIn, xr, zr);
+    FI = (float)((b > 0.) ? b : 0.);
+    return make_float2(F0, F1);
+}
+
+// Rescales the interpolated complex value to the interpolated intensity
+// (sroptelm.h:688-697).
+__device__ __forceinline__ float2 finish_point(float2 F, float FI, float norm, bool improve)
+{
+    float F0 = F.x, F1 = F.y;
+    if (!improve) return F;
     FI = __fmul_rn(FI, norm);
-    float appI = __fadd_rn(__fmul_rn(F0, F0), __fmul_rn(F1, F1));   // sroptelm.h:688-697
+    float appI = __fadd_rn(__fmul_rn(F0, F0), __fmul_rn(F1, F1));
     if (appI != 0.f) {
         float f = (float)sqrt((double)__fdiv_rn(FI, appI));
         F0 = __fmul_rn(F0, f); F1 = __fmul_rn(F1, f);
     }
     return make_float2(F0, F1);
+}
+
+// One component at relative cell position (xr, zr); re/im: the 16 cell values.
+__device__ __forceinline__ float2 resample_point(const float* re, const float* im, int ixRel, int izRel, double xr,
+                                                 double zr, bool improve)
+{
+    float In[16], norm, FI;
+    bool low;
+    cell_intensity(re, im, In, norm, low);
+    float2 F;
+    if (!low) {
+        double aRe[16], aIm[16], aIn[16];
+        bicubic(re, aRe); bicubic(im, aIm); bicubic(In, aIn);
+        F = bicubic_point(aRe, aIm, aIn, xr, zr, FI);
+    } else F = bilinear_point(re, im, In, ixRel, izRel, xr, zr, FI);
+    return finish_point(F, FI, norm, improve);
+}
+
+// Interval of the old mesh that holds abs, the first node of its 4-point
+// stencil and the relative coordinate in steps (sroptelm.cpp:3094-3140).
+__device__ __forceinline__ void cell_of(double abs, double start, double step, int n, int& c, int& st, double& rel)
+{
+    double is = 1. / step;
+    c = (int)((abs - start) * is + 1.E-06);
+    rel = abs - (start + c * step);
+    if (c == n - 1) { st = c - 3; rel += 2. * step; }
+    else if (c == n - 2) { st = c - 2; rel += step; }
+    else if (c == 0) { st = c; rel -= step; }
+    else st = c - 1;
+    rel *= is;
+    // a full 4x4 cell inside the old mesh (the host requires nx, nz >= 4)
+    st = st < 0 ? 0 : (st > n - 4 ? n - 4 : st);
+}
+
+// Whether new point (ix, iz) at (xAbs, zAbs) takes a value from the old mesh
+// (otherwise it stays zero, srradstr.cpp:4726, 4761-4775).
+__device__ __forceinline__ bool filled_point(const srwr_params& P, int ix, int iz, double xAbs, double zAbs)
+{
+    if (ix < P.ix0 || ix > P.ix1 || iz < P.iz0 || iz > P.iz1) return false;
+    if (P.zeroOutside && (zAbs < P.zMin - P.zTol || zAbs > P.zMax + P.zTol || xAbs < P.xMin - P.xTol ||
+                          xAbs > P.xMax + P.xTol))
+        return false;
+    return true;
 }
 
 __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, const float2* __restrict__ oez,
@@ -165,12 +221,9 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
     split_index(i, P.ln, P.nxn, P.nzn, P.ne, ix, iz, ie);
     long long on = ie * P.ln.perE + iz * P.ln.perZ + ix * P.ln.perX;
     float2 vx = make_float2(0.f, 0.f), vz = make_float2(0.f, 0.f);
-    if (ix >= P.ix0 && ix <= P.ix1 && iz >= P.iz0 && iz <= P.iz1) {
-        double zAbs = P.zStartN + iz * P.zStepN, xAbs = P.xStartN + ix * P.xStepN;
-        if (P.zeroOutside && (zAbs < P.zMin - P.zTol || zAbs > P.zMax + P.zTol || xAbs < P.xMin - P.xTol ||
-                              xAbs > P.xMax + P.xTol)) {
-            // outside the old wavefront: stays zero (srradstr.cpp:4726, 4761-4775)
-        } else if (P.onlyLarger) {
+    double zAbs = P.zStartN + iz * P.zStepN, xAbs = P.xStartN + ix * P.xStepN;
+    if (filled_point(P, ix, iz, xAbs, zAbs)) {
+        if (P.onlyLarger) {
             long izo = (long)((zAbs - P.zStartO) * (1. / P.zStepO) + 1.E-08);
             long ixo = (long)((xAbs - P.xStartO) * (1. / P.xStepO) + 1.E-08);
             izo = izo < 0 ? 0 : (izo > P.nzo - 1 ? P.nzo - 1 : izo);
@@ -179,26 +232,11 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
             if (oex) vx = oex[oo];
             if (oez) vz = oez[oo];
         } else {
-            double izs = 1. / P.zStepO, ixs = 1. / P.xStepO;
-            int izc = (int)((zAbs - P.zStartO) * izs + 1.E-06);
-            double zRel = zAbs - (P.zStartO + izc * P.zStepO);
-            int izSt;
-            if (izc == P.nzo - 1) { izSt = izc - 3; zRel += 2. * P.zStepO; }
-            else if (izc == P.nzo - 2) { izSt = izc - 2; zRel += P.zStepO; }
-            else if (izc == 0) { izSt = izc; zRel -= P.zStepO; }
-            else izSt = izc - 1;
-            zRel *= izs;
-            int ixc = (int)((xAbs - P.xStartO) * ixs + 1.E-06);
-            double xRel = xAbs - (P.xStartO + ixc * P.xStepO);
-            int ixSt;
-            if (ixc == P.nxo - 1) { ixSt = ixc - 3; xRel += 2. * P.xStepO; }
-            else if (ixc == P.nxo - 2) { ixSt = ixc - 2; xRel += P.xStepO; }
-            else if (ixc == 0) { ixSt = ixc; xRel -= P.xStepO; }
-            else ixSt = ixc - 1;
-            xRel *= ixs;
-            // a full 4x4 cell inside the old mesh (the host requires nx, nz >= 4)
-            izSt = izSt < 0 ? 0 : (izSt > P.nzo - 4 ? P.nzo - 4 : izSt);
-            ixSt = ixSt < 0 ? 0 : (ixSt > P.nxo - 4 ? P.nxo - 4 : ixSt);
+            if (P.cellMode) return;   // written by k_resize_cells
+            int izc, izSt, ixc, ixSt;
+            double zRel, xRel;
+            cell_of(zAbs, P.zStartO, P.zStepO, P.nzo, izc, izSt, zRel);
+            cell_of(xAbs, P.xStartO, P.xStepO, P.nxo, ixc, ixSt, xRel);
             long long base = ie * P.lo.perE + izSt * P.lo.perZ + ixSt * P.lo.perX;
             float re[16], im[16];
             if (oex) {
@@ -219,7 +257,8 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
             }
         }
     }
-    if (P.hasPost) {   // the same rounding as k_phase applied to the stored values
+    bool zero = vx.x == 0.f && vx.y == 0.f && vz.x == 0.f && vz.y == 0.f;
+    if (P.hasPost && !zero) {   // the same rounding as k_phase applied to the stored values
         float c, s;
         phase_cs(P.post, ix, iz, ie, c, s);
         vx = rotate_f(vx, c, s);
@@ -227,6 +266,87 @@ __global__ void __launch_bounds__(256) k_resize(const float2* __restrict__ oex, 
     }
     if (nex) nex[on] = vx;
     if (nez) nez[on] = vz;
+}
+
+// Interpolated points, one thread per (old interval cx, old interval cz,
+// energy): the bicubic forms of the cell are built once and evaluated at every
+// new point whose interval is (cx, cz), with the per-point arithmetic of
+// k_resize (which writes all other points).  Candidate new indices come from
+// inverting the interval rule with one point of margin; each is kept only
+// when its own interval computation gives (cx, cz).
+__global__ void __launch_bounds__(256) k_resize_cells(const float2* __restrict__ oex, const float2* __restrict__ oez,
+                                                      float2* nex, float2* nez, srwr_params P, int cx0, int ncx,
+                                                      int cz0, int ncz)
+{
+    long long n = (long long)ncx * ncz * P.ne;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int cx, cz, ie;
+    if (P.lo.perX == 1) { cx = (int)(i % ncx); long long q = i / ncx; cz = (int)(q % ncz); ie = (int)(q / ncz); }
+    else { ie = (int)(i % P.ne); long long q = i / P.ne; cx = (int)(q % ncx); cz = (int)(q / ncx); }
+    cx += cx0; cz += cz0;
+    auto cand = [](int c, double startO, double stepO, double startN, double stepN, int lo, int hi, int& a, int& b) {
+        double l = startO + c * stepO;
+        a = (int)floor((l - startN) / stepN) - 1;
+        b = (int)ceil((l + stepO - startN) / stepN) + 1;
+        a = a < lo ? lo : a;
+        b = b > hi ? hi : b;
+    };
+    int ixA, ixB, izA, izB;
+    cand(cx, P.xStartO, P.xStepO, P.xStartN, P.xStepN, P.ix0, P.ix1, ixA, ixB);
+    cand(cz, P.zStartO, P.zStepO, P.zStartN, P.zStepN, P.iz0, P.iz1, izA, izB);
+    if (ixA > ixB || izA > izB) return;
+    // the new indices of this interval pair (contiguous: the interval is
+    // monotone in the index) and the cell they share
+    int ixSt = -1, izSt = -1, ixF = 0, ixL = -1, izF = 0, izL = -1;
+    for (int iz = izA; iz <= izB; iz++) {
+        int c, st; double r;
+        cell_of(P.zStartN + iz * P.zStepN, P.zStartO, P.zStepO, P.nzo, c, st, r);
+        if (c == cz) { if (izSt < 0) { izSt = st; izF = iz; } izL = iz; }
+    }
+    for (int ix = ixA; ix <= ixB; ix++) {
+        int c, st; double r;
+        cell_of(P.xStartN + ix * P.xStepN, P.xStartO, P.xStepO, P.nxo, c, st, r);
+        if (c == cx) { if (ixSt < 0) { ixSt = st; ixF = ix; } ixL = ix; }
+    }
+    if (ixSt < 0 || izSt < 0) return;
+    const long long base = ie * P.lo.perE + izSt * P.lo.perZ + ixSt * P.lo.perX;
+    for (int comp = 0; comp < 2; comp++) {
+        const float2* o = comp == 0 ? oex : oez;
+        float2* d = comp == 0 ? nex : nez;
+        if (o == nullptr || d == nullptr) continue;
+        float re[16], im[16], In[16], norm;
+        bool low;
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 4; k++) {
+                float2 v = o[base + j * P.lo.perZ + k * P.lo.perX];
+                re[4 * j + k] = v.x; im[4 * j + k] = v.y;
+            }
+        cell_intensity(re, im, In, norm, low);
+        double aRe[16], aIm[16], aIn[16];
+        if (!low) { bicubic(re, aRe); bicubic(im, aIm); bicubic(In, aIn); }
+        for (int iz = izF; iz <= izL; iz++) {
+            double zAbs = P.zStartN + iz * P.zStepN;
+            int izc, st; double zRel;
+            cell_of(zAbs, P.zStartO, P.zStepO, P.nzo, izc, st, zRel);
+            for (int ix = ixF; ix <= ixL; ix++) {
+                double xAbs = P.xStartN + ix * P.xStepN;
+                int ixc, st2; double xRel;
+                cell_of(xAbs, P.xStartO, P.xStepO, P.nxo, ixc, st2, xRel);
+                if (!filled_point(P, ix, iz, xAbs, zAbs)) continue;
+                float FI;
+                float2 F = low ? bilinear_point(re, im, In, ixc - ixSt, izc - izSt, xRel, zRel, FI)
+                               : bicubic_point(aRe, aIm, aIn, xRel, zRel, FI);
+                F = finish_point(F, FI, norm, P.improve != 0);
+                if (P.hasPost) {
+                    float c, s;
+                    phase_cs(P.post, ix, iz, ie, c, s);
+                    F = rotate_f(F, c, s);
+                }
+                d[ie * P.ln.perE + iz * P.ln.perZ + ix * P.ln.perX] = F;
+            }
+        }
+    }
 }
 
 }  // namespace
@@ -247,6 +367,20 @@ int srwr_launch_resize(const float2* oex, const float2* oez, float2* nex, float2
     if (n <= 0) return 0;
     if (!P.onlyLarger && (P.nxo < 4 || P.nzo < 4)) return -1;
     hipLaunchKernelGGL(k_resize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, oex, oez, nex, nez, P);
+    if (P.cellMode && !P.onlyLarger) {
+        // interval range of the filled new points (monotone in the index), one interval of margin
+        auto interval = [](double abs, double start, double step) { return (int)((abs - start) * (1. / step) + 1.E-06); };
+        int cx0 = interval(P.xStartN + P.ix0 * P.xStepN, P.xStartO, P.xStepO) - 1;
+        int cx1 = interval(P.xStartN + P.ix1 * P.xStepN, P.xStartO, P.xStepO) + 1;
+        int cz0 = interval(P.zStartN + P.iz0 * P.zStepN, P.zStartO, P.zStepO) - 1;
+        int cz1 = interval(P.zStartN + P.iz1 * P.zStepN, P.zStartO, P.zStepO) + 1;
+        if (cx1 >= cx0 && cz1 >= cz0) {
+            int ncx = cx1 - cx0 + 1, ncz = cz1 - cz0 + 1;
+            long long m = (long long)ncx * ncz * P.ne;
+            hipLaunchKernelGGL(k_resize_cells, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, oex, oez, nex, nez,
+                               P, cx0, ncx, cz0, ncz);
+        }
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 }

@@ -137,6 +137,7 @@ int device_resize(const SRWLWfr& w, const ResizePlan& P, float2* oX, float2* oZ,
     // the phase is added back to each new value before it is stored
     R.hasPost = (P.fx || P.fz) ? 1 : 0;
     if (R.hasPost) R.post = phase_desc(w, P.x.n, P.z.n, P.x.start, P.x.step, P.z.start, P.z.step, ln, P.fx, P.fz, 1);
+    R.cellMode = g_ctx.resizePerPoint ? 0 : 1;
     if (srwr_launch_resize(oX, oZ, nX, nZ, R, st)) return SRWL_AMD_ERR_GPU;
     return 0;
 }

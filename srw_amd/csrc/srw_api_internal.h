@@ -5,6 +5,7 @@
 #include "srw_dev.h"
 #include "srw_moments.h"
 #include "srw_resize.h"
+#include "srw_fft.h"
 #include <vector>
 
 namespace srwamd {
@@ -22,11 +23,12 @@ struct Buffers {
     DBuf Rec, Chunk, Q, Edge, Mesh, Rx, Items, BlockItem, RowTraj, RowE, RowN, A, Part, Warn;
     DBuf Ex, Ez;                  // field outputs
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
-    DBuf Stokes, Draws, Wt, Net, Grad;
+    DBuf Stokes, Draws, Wt, Net, Grad, Rep;
     DBuf Pic;                     // parallel-in-s RK4 iterates
     DBuf MomSl, MomWork, MomSums, MomLims;   // wavefront moments
     DBuf RsO, RsN;                // resampling
     DBuf AutoTab, AutoFlags, AutoS, AutoF, AutoWork;   // adaptive SR integral (method 1)
+    DBuf Tune;                    // scratch slices of the FFT tile autotuner
 };
 
 struct Context {
@@ -42,6 +44,10 @@ struct Context {
     int forceSplit = 0;
     bool trajSerial = false;      // test switch: serial RK4 kernel instead of the parallel one
     bool keepOnDevice = false;    // benchmark switch: leave results in HBM
+    bool resizePerPoint = false;  // test switch: resample per new point instead of per old cell
+    bool tuneTiles = true;        // FFT tile shapes from the autotuner (else the fixed defaults)
+    bool tilesForced = false;     // test switch: every FFT pass uses forcedTiles
+    srwf_tiles forcedTiles = {0, 0, 0, 0, 0, 0};
     double sumSrMs = 0, sumSrEvals = 0;
     int nSrCalls = 0;
     double lastFftMs = 0, lastFftBytes = 0;
@@ -94,7 +100,9 @@ int setup_job_from_field(SrJob& J, const SRWLParticle& p, const SRWLMagFldC* mag
                          std::vector<srwd_field>& fields);
 int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
            const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin,
-           bool defer = false);
+           bool defer = false, bool geomOnly = false);
+// Mesh size of one axis for the automatic mesh of precPar[6] > 0.
+long sr_auto_mesh_size(double start, double fin, long n, double c, double R, double lambda, double f);
 // Completes a deferred sr_run once the stream has been synchronised.
 void sr_complete(std::vector<SrResult>& res);
 // Method 1 for every job, after sr_run has set up trajectories, edges and

@@ -20,15 +20,29 @@ struct srwf_slice {
     int pad;
 };
 
+// Tile shapes of the three passes: rows per workgroup and threads of the
+// forward row pass, columns per workgroup and threads of the column pass,
+// rows per workgroup and threads of the inverse row pass.  Any shape that
+// fits (srwf_tiles_fit) gives the same values; only the work assignment
+// changes.
+struct srwf_tiles {
+    int rpwF, thrF, cpw, thrC, rpwI, thrI;
+};
+
 extern "C" {
 int srwf_rows_per_wg(int logn);
 int srwf_cols_per_wg(int logn);
+void srwf_default_tiles(int lognx, int logny, int postOp, srwf_tiles* t);
+// 1 when the shape is valid for nx = 2^lognx, ny = 2^logny (LDS, thread counts).
+int srwf_tiles_fit(int lognx, int logny, int postOp, const srwf_tiles* t);
+// tiles: nullptr (or a shape that does not fit) selects the default shape.
 int srwf_launch_rows_fwd(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* tw,
-                         int logn, int ny, int preOp, int inv, hipStream_t st);
+                         int logn, int ny, int preOp, int inv, const srwf_tiles* tiles, hipStream_t st);
 int srwf_launch_cols(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* twy,
-                     int logny, int lognx, int op, int inv, float2* out, hipStream_t st);
+                     int logny, int lognx, int op, int inv, float2* out, const srwf_tiles* tiles, hipStream_t st);
 int srwf_launch_rows_inv(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* tw,
-                         int logn, int ny, int postOp, float* inten, long long intenStride, hipStream_t st);
+                         int logn, int ny, int postOp, float* inten, long long intenStride, const srwf_tiles* tiles,
+                         hipStream_t st);
 int srwf_launch_gather(const float2* src, float2* dst, int ne, long long nxy, int scatter, hipStream_t st);
 // Both field components between SRW's [iy][ix][ie] order (ex, ez) and the
 // slice-major d[(2*ie + c)*nxy + p]: toSlices = 1 reads ex/ez and writes d,

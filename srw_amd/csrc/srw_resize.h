@@ -31,6 +31,7 @@ struct srwr_params {
     int improve;                            // rescale Re, Im to the interpolated intensity
     int zeroOutside;                        // zero new points outside [min - tol, max + tol] of the old mesh
     double xMin, xMax, zMin, zMax, xTol, zTol;
+    int cellMode;                           // interpolate per old cell (k_resize_cells) instead of per new point
 };
 
 extern "C" {

@@ -142,7 +142,7 @@ static int field_moments(SRWLWfr& w, const float2* dEx, const float2* dEz)
 // stay on the device: request j writes ne*nx*ny complex values of Ex and Ez
 // starting at float2 offset outOff[j] of dEx / dEz.
 int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
-           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin, bool defer)
+           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin, bool defer, bool geomOnly)
 {
     int rc = ensure_device();
     if (rc) return rc;
@@ -282,6 +282,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         res[j].Robs = rx[4 * j]; res[j].RobsErr = rx[4 * j + 1];
         res[j].xc = rx[4 * j + 2]; res[j].zc = rx[4 * j + 3];
     }
+    if (geomOnly) return 0;
     if (jobs[0].meth == 1) {
         for (auto& J : jobs) if (J.meth != 1) return SRWL_AMD_ERR_BATCH;
         if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
@@ -406,6 +407,23 @@ SrRegion sr_region(const SRWLRadMesh& m, const srwd_trj_info& I)
     double x = xS;
     for (int ix = 0; ix < nx; ix++) { if (G.symX && (x - I.x0) > G.xStep * 0.001) { G.nxc = ix; break; } x += G.xStep; }
     return G;
+}
+
+// srradstr.cpp:1063-1119 CheckNxNzForSR, coordinate representation: the
+// number of points along one mesh axis [start, fin] that samples the
+// wavefront's curvature phase pi (x - c)^2 / (lambda R) at the mesh edges with
+// at most pi / (1.2 f) per step; even, a fast FFT size, at least 8.
+long sr_auto_mesh_size(double start, double fin, long n, double c, double R, double lambda, double f)
+{
+    double step = n > 1 ? (fin - start) / (n - 1) : 0.;
+    double halfLamR = 0.5 * lambda * R;
+    double sRel = start - c, eRel = start + (n - 1) * step - c;
+    double d0 = std::fabs(halfLamR / sRel), d1 = std::fabs(halfLamR / eRel);
+    double d = std::min(d0, d1) / (f * 1.2);
+    long m = (long)(std::fabs(eRel - sRel) / d) + 1;
+    if (((m >> 1) << 1) != m) m++;
+    m = next_fft_size(m);
+    return std::max(m, 8L);
 }
 
 // After the stream has been synchronised: timing statistics and the
@@ -599,7 +617,7 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     if (!trjDef && !fldDef) return SRWL_INCORRECT_PARAM_FOR_SR_COMP;
     int meth = (int)precPar[0];
     if (meth != 0 && meth != 1) return SRWL_AMD_ERR_UNSUPPORTED;      // method 2: not in this build
-    if (nPrecPar > 6 && precPar[6] > 0) return SRWL_AMD_ERR_UNSUPPORTED;  // automatic mesh
+    double sampFact = nPrecPar > 6 ? precPar[6] : 0.;
     if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
     if (pWfr->mesh.nx <= 0 || pWfr->mesh.ny <= 0 || pWfr->mesh.ne <= 0) return SRWL_INCORRECT_WFR_STRUCT;
     std::vector<SrJob> jobs(1);
@@ -624,14 +642,31 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     J.relPrec = precPar[1];
     J.sLim0 = precPar[2]; J.sLim1 = precPar[3];
     int termin = (nPrecPar <= 0 || nPrecPar > 5) ? (int)(char)precPar[5] : 1;
-    long long n = (long long)J.mesh.ne * J.mesh.nx * J.mesh.ny;
     int rc = ensure_device();
     if (rc) return rc;
+    std::vector<long long> off(1, 0);
+    std::vector<SrResult> res;
+    if (sampFact > 0.) {
+        // srradstr.cpp:324-375: mesh sizes from the wavefront radius and
+        // centre, the wavefront reallocated by the caller when they change
+        rc = sr_run(jobs, fields, nullptr, nullptr, off, res, termin, false, true);
+        if (rc) return rc;
+        const SRWLRadMesh& m = pWfr->mesh;
+        double lambda = 1.239842E-06 / m.eStart;
+        long nxN = sr_auto_mesh_size(m.xStart, m.xFin, m.nx, res[0].xc, res[0].Robs, lambda, sampFact);
+        long nzN = sr_auto_mesh_size(m.yStart, m.yFin, m.ny, res[0].zc, res[0].Robs, lambda, sampFact);
+        if (nxN != m.nx || nzN != m.ny) {
+            pWfr->mesh.nx = nxN; pWfr->mesh.ny = nzN;
+            if (g_wfrModif == nullptr) return SRWL_WFR_EXT_FUNC_NOT_DEFINED;
+            if ((*g_wfrModif)(2, pWfr, 0)) return SRWL_WFR_EXT_MODIF_FAILED;
+            if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_WFR_EXT_MODIF_FAILED;
+            J.mesh = pWfr->mesh;
+        }
+    }
+    long long n = (long long)J.mesh.ne * J.mesh.nx * J.mesh.ny;
     float* dEx = (float*)g_ctx.buf.Ex.get(sizeof(float) * 2 * n);
     float* dEz = (float*)g_ctx.buf.Ez.get(sizeof(float) * 2 * n);
     if (!dEx || !dEz) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    std::vector<long long> off(1, 0);
-    std::vector<SrResult> res;
     rc = sr_run(jobs, fields, dEx, dEz, off, res, termin, true);
     if (rc) return rc;
     if (!g_ctx.keepOnDevice) {
@@ -781,6 +816,12 @@ SRWL_EXPORT int srwlAmdSetSplit(int nSplit)
 SRWL_EXPORT int srwlAmdSetTrajSerial(int on)
 {
     g_ctx.trajSerial = on != 0;
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetResizePerPoint(int on)
+{
+    g_ctx.resizePerPoint = on != 0;
     return 0;
 }
 

@@ -123,7 +123,12 @@ def CalcElecFieldSR(_wfr, _trj, _mag, _prec_par):
         ct = C.byref(M.prt_trj(_trj, keep))
     cm = C.byref(M.mag_cnt(_mag, keep)) if (_mag is not None and _mag != 0) else None
     prec = M.dvec(_prec_par, keep)
-    rc = lib.srwlCalcElecFieldSR(C.byref(cw), ct, cm, prec, len(_prec_par))
+    # precPar[6] > 0 may resize the mesh through the modification callback
+    _active_wfr[C.addressof(cw)] = _wfr
+    try:
+        rc = lib.srwlCalcElecFieldSR(C.byref(cw), ct, cm, prec, len(_prec_par))
+    finally:
+        _active_wfr.pop(C.addressof(cw), None)
     _check(rc)
     M.wfr_back(cw, _wfr)
     return _wfr
@@ -166,6 +171,30 @@ def set_keep_on_device(on=True):
 
 def set_traj_serial(on=True):
     lib.srwlAmdSetTrajSerial(1 if on else 0)
+
+
+def set_fft_tiles(t6=None):
+    """Forces the FFT tile shapes (6 ints) of every pass; None returns to the autotuner."""
+    if t6 is None:
+        lib.srwlAmdSetFftTiles(None)
+    else:
+        lib.srwlAmdSetFftTiles((C.c_int * 6)(*[int(v) for v in t6]))
+
+
+def set_tuning(on=True):
+    lib.srwlAmdSetTuning(1 if on else 0)
+
+
+def fft_tiles(lognx, lognz, mode, post_op):
+    """Tile shapes the autotuner chose for a drift shape, or None."""
+    out = (C.c_int * 6)()
+    if lib.srwlAmdGetFftTiles(int(lognx), int(lognz), int(mode), int(post_op), out):
+        return list(out)
+    return None
+
+
+def set_resize_per_point(on=True):
+    lib.srwlAmdSetResizePerPoint(1 if on else 0)
 
 
 def last_auto_levels():
@@ -256,10 +285,12 @@ def UtiFFT(_data, _mesh, _dir):
 
 
 def EmitPropMultiE(_stokes, _wfr0, _mag, _opt_cnt, _prec_par, _n_macro, _seed=1, _batch=0, _proposal=0,
-                   _target=0., _train=0, _ideal_gain=0):
+                   _target=0., _train=0, _ideal_gain=0, _rqmc=0, _replicates=16):
     """Partially coherent intensity averaged over _n_macro electrons drawn from _wfr0.partBeam
     (replaces the per-electron loop of srwl_wfr_emit_prop_multi_e, reference srwlib.py:8506).
     _stokes.arS (float, nx*ny of _stokes.mesh) receives the averaged total intensity.
+    _proposal=1: learned importance-sampling proposal; _rqmc=1: randomized
+    quasi-Monte Carlo draws in _replicates independently shifted replicates.
     Returns a dict of run information."""
     keep = M.Keep()
     cs = M.stokes(_stokes, keep)
@@ -267,13 +298,14 @@ def EmitPropMultiE(_stokes, _wfr0, _mag, _opt_cnt, _prec_par, _n_macro, _seed=1,
     cm = M.mag_cnt(_mag, keep)
     co = M.opt_cnt(_opt_cnt, keep) if _opt_cnt is not None else None
     prec = M.dvec(_prec_par, keep)
-    par = M.dvec([_seed, _batch, _proposal, _target, _train, 1 if _ideal_gain else 0, 0, 0], keep)
+    par = M.dvec([_seed, _batch, _proposal, _target, _train, 1 if _ideal_gain else 0, 1 if _rqmc else 0,
+                  _replicates], keep)
     out = (C.c_double * 16)()
     _check(lib.srwlEmitPropMultiE(C.byref(cs), C.byref(cw), C.byref(cm), C.byref(co) if co is not None else None,
                                   prec, len(_prec_par), int(_n_macro), par, out))
     keys = ("electrons", "ess", "sum_w", "seconds", "rel_l2_est", "batch", "train_loss", "train_steps",
-            "ideal_gain")
-    return dict(zip(keys, list(out)[:9]))
+            "ideal_gain", "replicates")
+    return dict(zip(keys, list(out)[:10]))
 
 
 def normals(seed, i):

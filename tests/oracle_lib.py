@@ -23,7 +23,7 @@ def load():
     lib.orc_srwlCalcElecFieldSR_mt.argtypes = [P(S.SRWLWfr), P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double),
                                                C.c_int, C.c_int]
     lib.orc_srwlCalcElecFieldSR_ex.argtypes = [P(S.SRWLWfr), P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double),
-                                               C.c_int, C.c_int, P(C.c_int)]
+                                               C.c_int, C.c_int, P(C.c_int), P(C.c_long)]
     lib.orc_srwlCalcPartTraj.argtypes = [P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double)]
     lib.orc_srwlPropagElecField.argtypes = [P(S.SRWLWfr), P(S.SRWLOptC)]
     lib.orc_srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_byte, C.c_byte, C.c_byte, C.c_double]
@@ -71,9 +71,33 @@ def calc_field_levels(w, mag, prec, trj=None):
     lv = np.full(w.mesh.nx * w.mesh.ny * w.mesh.ne, -1, dtype=np.int32)
     rc = lib.orc_srwlCalcElecFieldSR_ex(C.byref(cw), C.byref(ct) if ct is not None else None,
                                         C.byref(cm) if cm is not None else None, cp, len(prec), 1,
-                                        lv.ctypes.data_as(C.POINTER(C.c_int)))
+                                        lv.ctypes.data_as(C.POINTER(C.c_int)), None)
     M.wfr_back(cw, w)
     return rc, lv[lv >= 0]
+
+
+WFR_EXT_FUNC_NOT_DEFINED = 23172
+
+
+def calc_field_auto_mesh(w, mag, prec, nthreads=8):
+    """Field with the automatic mesh of precPar[6] > 0: the oracle reports the
+    sizes it needs, the wavefront is reallocated here (the caller's part in
+    the reference) and the computation repeated on the new mesh."""
+    sizes = (C.c_long * 2)(0, 0)
+
+    def run():
+        keep = M.Keep()
+        cw = M.wfr(w, keep)
+        cm = M.mag_cnt(mag, keep)
+        cp = M.dvec(prec, keep)
+        rc = lib.orc_srwlCalcElecFieldSR_ex(C.byref(cw), None, C.byref(cm), cp, len(prec), nthreads, None, sizes)
+        M.wfr_back(cw, w)
+        return rc
+    rc = run()
+    if rc == WFR_EXT_FUNC_NOT_DEFINED:
+        w.allocate(w.mesh.ne, sizes[0], sizes[1])
+        rc = run()
+    return rc, (sizes[0], sizes[1])
 
 
 def trj_interp(trj, s):

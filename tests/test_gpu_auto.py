@@ -98,3 +98,26 @@ def test_auto_rejects_zero_precision():
     w = _mesh(3, 3, 1, -1e-3, 1e-3, -1e-3, 1e-3)
     with pytest.raises(RuntimeError):
         srwlpy.CalcElecFieldSR(w, 0, W.planar_undulator(), array('d', [1, 0., 0, 0, 10001, 1, 0]))
+
+
+@pytest.mark.parametrize("meth,f", [(0, 1.0), (1, 0.5)])
+def test_auto_mesh_matches_oracle(meth, f):
+    """precPar[6] > 0: the library resizes the mesh from the wavefront radius
+    and reallocates the Python wavefront through its callback."""
+    from srw_amd import srwlpy
+    w, mag, prec = W.single_electron(n=15)
+    p = array('d', list(prec))
+    p[0] = meth
+    if meth == 1:
+        p[1] = 0.01
+    p[6] = f
+    wg, wo = clone_wfr(w), clone_wfr(w)
+    print("case: auto mesh, method %d, sampling factor %g" % (meth, f), flush=True)
+    srwlpy.CalcElecFieldSR(wg, 0, mag, p)
+    rc, sz = oracle.calc_field_auto_mesh(wo, mag, p)
+    assert rc == 0
+    assert (wg.mesh.nx, wg.mesh.ny) == sz == (wo.mesh.nx, wo.mesh.ny)
+    assert len(wg.arEx) == 2 * sz[0] * sz[1] * w.mesh.ne
+    e = rel_l2(cfield(wg.arEx, wg), cfield(wo.arEx, wo))
+    print("mesh %dx%d rel-L2 Ex %.2e" % (sz[0], sz[1], e), flush=True)
+    assert e < TOL

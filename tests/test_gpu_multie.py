@@ -47,3 +47,21 @@ def test_multi_e_matches_oracle(mode, batch):
     print("rel-L2 %.3e electrons %d" % (err, info["electrons"]), flush=True)
     assert info["electrons"] == 16
     assert err < 2e-4
+
+
+def test_rqmc_estimate_agrees_with_plain_mc():
+    """Randomized quasi-Monte Carlo draws estimate the same intensity: the
+    difference from a plain run is within the combined error estimates."""
+    from srw_amd import srwlpy
+    print("case: rqmc vs plain, 64x64, 256 electrons", flush=True)
+    st_p, w0, mag, opt, prec = _setup()
+    st_q, _, _, _, _ = _setup()
+    ip = srwlpy.EmitPropMultiE(st_p, w0, mag, opt, prec, 512, _seed=5, _batch=128)
+    iq = srwlpy.EmitPropMultiE(st_q, w0, mag, opt, prec, 256, _seed=6, _batch=128, _rqmc=1, _replicates=16)
+    a, b = np.array(st_p.arS[:48 * 48]), np.array(st_q.arS[:48 * 48])
+    diff = rel_l2(b, a)
+    ep, eq = ip["rel_l2_est"], iq["rel_l2_est"]
+    print("plain err %.4f (512) rqmc err %.4f (256, %d replicates) diff %.4f" %
+          (ep, eq, iq["replicates"], diff), flush=True)
+    assert iq["electrons"] == 256 and iq["replicates"] == 16
+    assert 0 < eq and diff < 4 * np.hypot(ep, eq)

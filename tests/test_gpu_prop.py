@@ -113,3 +113,34 @@ def test_intensity(pol):
     assert oracle.intensity(a2, w, pol, 0, 3, e) == 0
     x, y = np.array(a1), np.array(a2)
     assert rel_l2(x, y) < 1e-6
+
+
+@pytest.mark.parametrize("n,mode,tiles", [
+    (256, 2, (1, 1024, 4, 256, 4, 512)),
+    (256, 0, (8, 256, 16, 1024, 2, 256)),
+    (128, 2, (16, 512, 32, 1024, 16, 1024)),
+])
+def test_drift_tile_shapes_give_same_values(n, mode, tiles):
+    """Any tile shape of the FFT passes (the autotuner's search space) gives
+    the same bits as the default shape, and the autotuned shape as well."""
+    from srw_amd import srwlpy
+    print("case: drift tiles n=%d mode=%d tiles=%s" % (n, mode, tiles), flush=True)
+    w = _sr_wfr(n)
+    opt = W.drift_container(20., mode)
+    res = []
+    for setting in ("default", "forced", "tuned"):
+        wg = clone_wfr(w)
+        if setting == "default":
+            srwlpy.set_tuning(False)
+        elif setting == "forced":
+            srwlpy.set_fft_tiles(tiles)
+        try:
+            srwlpy.PropagElecField(wg, opt)
+        finally:
+            srwlpy.set_tuning(True)
+            srwlpy.set_fft_tiles(None)
+        res.append(np.frombuffer(wg.arEx, np.float32).copy())
+    lg = int(np.log2(n))
+    print("tuned tiles", srwlpy.fft_tiles(lg, lg, 3 if mode else 0, 1 if mode else 0), flush=True)
+    assert np.array_equal(res[0], res[1])
+    assert np.array_equal(res[0], res[2])

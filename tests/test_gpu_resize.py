@@ -112,3 +112,26 @@ def test_resize_to_mesh_matches_oracle(n, ne, box, par):
     rc, wo2 = oracle.resize_to_mesh(wo, m, par)
     assert rc == 0
     _compare(wg, wo2)
+
+
+@pytest.mark.parametrize("n,ne,par", [
+    (64, 1, [0, 2., 2.5, 2., 2.5]),
+    (48, 3, [0, 1.5, 2., 1.2, 1.5]),
+    (64, 1, [0, 0.5, 1., 0.5, 1.]),
+])
+def test_resize_per_cell_equals_per_point(n, ne, par):
+    """The per-cell kernel (forms built once per old cell) gives the same bits
+    as the per-point kernel."""
+    from srw_amd import srwlpy
+    print("case: resize modes n=%d ne=%d par=%s" % (n, ne, par), flush=True)
+    w = _sr(n, ne, 0.02 if ne > 1 else 0.)
+    wa, wb = clone_wfr(w), clone_wfr(w)
+    srwlpy.ResizeElecField(wa, 'c', par)
+    srwlpy.set_resize_per_point(True)
+    try:
+        srwlpy.ResizeElecField(wb, 'c', par)
+    finally:
+        srwlpy.set_resize_per_point(False)
+    assert (wa.mesh.nx, wa.mesh.ny) == (wb.mesh.nx, wb.mesh.ny)
+    assert np.array_equal(np.frombuffer(wa.arEx, np.float32), np.frombuffer(wb.arEx, np.float32))
+    assert np.array_equal(np.frombuffer(wa.arEy, np.float32), np.frombuffer(wb.arEy, np.float32))

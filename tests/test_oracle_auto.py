@@ -127,3 +127,28 @@ def test_converges_to_trajectory_step_quadrature():
 def test_bad_relative_precision_is_rejected():
     w, mag, p = _point((0.,), 0., rel=0.)
     assert oracle.calc_field_levels(w, mag, p)[0] > 0
+
+
+@pytest.mark.parametrize("f", [0.5, 1.0, 2.0])
+def test_auto_mesh_samples_the_curvature_phase(f):
+    """precPar[6] = f: the new mesh has at least as many intervals as the
+    curvature phase pi (x - xc)^2 / (lambda R) needs to advance by at most
+    pi/(1.2 f) per step at the mesh edges (the count rounded down, as the
+    reference counts them), and the next smaller fast-FFT size has fewer."""
+    w, mag, prec = W.single_electron(n=21)
+    p = array('d', list(prec))
+    p[6] = f
+    rc, (nx, ny) = oracle.calc_field_auto_mesh(w, mag, p)
+    assert rc == 0 and (w.mesh.nx, w.mesh.ny) == (nx, ny)
+    lam = 1.239842e-6 / w.mesh.eStart
+    for n, a0, a1, c in ((nx, w.mesh.xStart, w.mesh.xFin, w.xc), (ny, w.mesh.yStart, w.mesh.yFin, w.yc)):
+        edge = max(abs(a0 - c), abs(a1 - c))
+        d_max = lam * w.Rx / (2 * edge) / (1.2 * f)      # largest step meeting the condition
+        need = int((a1 - a0) / d_max)                     # intervals
+        assert n - 1 >= need
+        smaller = max(m for m in range(8, n) if oracle.lib.orc_next_fft_size(m) == m)
+        assert smaller - 1 < need
+    # the field on the new mesh is the plain computation on that mesh
+    w2 = clone_wfr(w)
+    assert oracle.calc_field(w2, mag, prec) == 0
+    assert rel_l2(cfield(w.arEx, w), cfield(w2.arEx, w2)) == 0.

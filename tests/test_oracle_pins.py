@@ -293,3 +293,47 @@ def test_beam_sampler_reproduces_second_moments():
     assert abs(cov[0, 2]) < 4 * math.sqrt(m[0] * m[3] / n)
     assert np.var(a[:, 4] / g0 - 1) == pytest.approx(m[10], rel=rel)
     assert abs(np.mean(a[:, 0])) < 4 * math.sqrt(m[0] / n)
+
+
+def test_sobol_points_are_nets_and_inverse_normal_is_exact():
+    """Sobol' points of the RQMC sampler: every dimension alone is a
+    (0,1)-sequence (each of the 2^m intervals of width 2^-m holds one of the
+    first 2^m points), dimensions 1-2 form a (0,2)-sequence (each elementary
+    box of area 2^-8 holds one of the first 256 points); the inverse normal
+    inverts Phi = erfc(-g/sqrt 2)/2."""
+    import ctypes as C
+    import math
+    import os
+    import subprocess
+    import tempfile
+    import numpy as np
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = r'''
+#include <stdio.h>
+#include "srw_rng.h"
+int main(void) {
+    uint32_t V[SRW_SOBOL_DIMS][32];
+    srw_sobol_directions(V);
+    for (unsigned k = 0; k < 1024; k++)
+        for (int d = 0; d < SRW_SOBOL_DIMS; d++) printf("%u\n", srw_sobol_word(V, d, k));
+    double us[] = {1e-9, 1e-4, 0.02, 0.31, 0.5, 0.77, 0.999, 1 - 1e-9};
+    for (int i = 0; i < 8; i++) printf("%.17g %.17g\n", us[i], srw_norm_inv(us[i]));
+    return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as td:
+        c, exe = os.path.join(td, "s.c"), os.path.join(td, "s")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-O2", "-I", os.path.join(root, "include"), c, "-o", exe, "-lm"], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    w = np.array([int(x) for x in out[:1024 * 5]], dtype=np.uint64).reshape(1024, 5)
+    for m in (4, 7, 10):
+        for d in range(5):
+            cells = w[:2 ** m, d] >> (32 - m)
+            assert len(set(cells.tolist())) == 2 ** m
+    for j in range(9):
+        box = (w[:256, 0] >> (32 - j)) * (1 << (8 - j)) + (w[:256, 1] >> (32 - (8 - j)) if j < 8 else 0)
+        assert len(set(box.tolist())) == 256
+    for line in out[1024 * 5:1024 * 5 + 8]:
+        u, g = (float(x) for x in line.split())
+        assert 0.5 * math.erfc(-g / math.sqrt(2)) == pytest.approx(u, rel=1e-13)
