@@ -66,18 +66,22 @@ def copy_calibration(mib):
                       "read_plus_write_GBps": 2 * n * 4 / (ms * 1e-3) / 1e9}))
 
 
-def multi_e_runs(srwlpy, W, n, nmax):
+def multi_e_runs(srwlpy, W, n, nmax, validate=True):
     """Time to 1 % rel-L2 of the partially coherent intensity (Gaussian 6-D beam,
     sigma_x 30 um, sigma_x' 5 urad, sigma_y 5 um, sigma_y' 2 urad, sigma_E/E 1e-3;
     n x n mesh at 20 m, 30 m drift), plain Gaussian MC vs the learned proposal.
     The rel-L2 error is the estimator's own standard error (per-pixel sample
-    variance of w*I), the stop criterion of the run."""
+    variance of w*I; for randomized quasi-Monte Carlo, the spread of 16
+    independently shifted replicates), the stop criterion of the run; with
+    validate, each run's true error against a 16384-electron RQMC reference
+    is reported beside it."""
     from srw_amd.srwlib import SRWLStokes
     beam = W.beam_for_multi_e()
     e1 = W.first_harmonic_eV(beam.partStatMom1.gamma, 1.5, 0.02)
     w0 = W.wavefront(beam, 1, e1, e1, n, n, 1e-3, 1e-3, 20.)
     mag, prec, opt = W.planar_undulator(), W.traj_prec(10001), W.drift_container(30., 2)
-    res = {}
+    import numpy as np
+    res, outs = {}, {}
     for name, prop, rqmc in (("plain_mc", 0, 0), ("learned", 1, 0), ("rqmc", 0, 1)):
         st = SRWLStokes()
         st.allocate(1, n, n, _n_comp=1)
@@ -95,7 +99,22 @@ def multi_e_runs(srwlpy, W, n, nmax):
             res[name]["ideal_proposal_gain"] = info["ideal_gain"]
         if rqmc:
             res[name]["replicates"] = int(info["replicates"])
+        outs[name] = np.array(st.arS[:n * n], dtype=np.float64)
         log("multi-electron %s: %.3f s, %d electrons" % (name, info["seconds"], info["electrons"]))
+    if validate:
+        # true errors of the three runs against a 16384-electron RQMC reference
+        st = SRWLStokes()
+        st.allocate(1, n, n, _n_comp=1)
+        st.mesh.eStart = st.mesh.eFin = e1
+        st.mesh.xStart, st.mesh.xFin = -2.5e-3, 2.5e-3
+        st.mesh.yStart, st.mesh.yFin = -2.5e-3, 2.5e-3
+        info = srwlpy.EmitPropMultiE(st, w0, mag, opt, prec, 16384, _seed=31, _batch=1024, _rqmc=1, _replicates=16)
+        ref = np.array(st.arS[:n * n], dtype=np.float64)
+        for name in outs:
+            res[name]["rel_l2_true"] = float(np.linalg.norm(outs[name] - ref) / np.linalg.norm(ref))
+        res["reference"] = {"electrons": 16384, "sampler": "rqmc, 16 replicates", "rel_l2_est": info["rel_l2_est"],
+                            "seconds": info["seconds"]}
+        log("multi-electron reference done")
     p, q, r = res["plain_mc"], res["learned"], res["rqmc"]
     res["speedup_learned_vs_plain"] = p["seconds"] / q["seconds"] if q["seconds"] > 0 else None
     res["speedup_rqmc_vs_plain"] = p["seconds"] / r["seconds"] if r["seconds"] > 0 else None

@@ -213,7 +213,10 @@ SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, ch
 
 /* reference srwlib.h:821 (srwlib.cpp:1023-1074): propagation through a
  * container of drift spaces ("drift"); arProp[3] = 0 (angular representation)
- * or 2 (quadratic phase treated analytically with the wavefront radius). */
+ * or 1, 2 (quadratic phase treated analytically with the wavefront radius
+ * from the moments or from Rx, Ry); arProp[5..8] resize the mesh first.
+ * Mesh sizes: even, 8..2048, with factors 2, 3, 5, 7, 11, 13 (SRW's own
+ * fast-FFT sizes); others return SRWL_ERR_IN_FFT. */
 SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, char** arID,
                                     SRWLRadMesh* arIM, char** arI, double* arParGPU);
 
@@ -236,7 +239,9 @@ SRWL_EXPORT int srwlResizeElecField(SRWLWfr* pWfr, char type, double* par);
 SRWL_EXPORT int srwlResizeElecFieldMesh(SRWLWfr* pWfr, SRWLRadMesh* pMesh, double* par);
 
 /* reference srwlib.h:871 (srwlib.cpp:1090-1204): SRW-convention 2D FFT of
- * float complex data; arMesh = [xStart, xStep, nx, yStart, yStep, ny]. */
+ * float complex data; arMesh = [xStart, xStep, nx, yStart, yStep, ny].
+ * nx, ny even, 8..2048, with factors 2, 3, 5, 7, 11, 13; other even sizes
+ * return SRWL_AMD_ERR_UNSUPPORTED. */
 SRWL_EXPORT int srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMesh, int dir,
                            double* arParGPU);
 
@@ -251,24 +256,31 @@ SRWL_EXPORT int srwlUtiGPUProc(int op, double* arParGPU);
 
 /* Computes nWfr independent single-electron fields in one device pass.
  * Each request has its own mesh, electron and magnetic field; precPar is
- * shared (method 0).  Equivalent to calling srwlCalcElecFieldSR per item. */
+ * shared (method 0 or 1; precPar[6] > 0 is not supported here).  Equivalent
+ * to calling srwlCalcElecFieldSR per item. */
 SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld, int nWfr,
                                          double* precPar, int nPrecPar);
 
 /* Partially coherent intensity: nMacroElec electrons drawn from the Gaussian
  * beam pWfr0->partBeam (moments as in srwlib.py:9000-9029, mapping
- * srwlib.py:9793-9803),
+ * srwlib.py:9793-9803; pWfr0's mesh sizes powers of two, 16..2048),
  * each emitting on pWfr0's mesh, propagated through pOpt (drifts) and
  * bilinearly re-gridded onto pStokes->mesh (srwlib.py avg_update_interp),
  * averaged into pStokes->arS0 (float32, nx*ny).
  * arPar: [0] seed, [1] electrons per device batch (0: auto), [2] proposal
  *   mode (0 plain Gaussian MC, 1 learned mixture proposal), [3] target
  *   rel-L2 for early stop (0: none), [4] training steps per batch, [5] (plain
- *   mode) > 0: estimate the variance gain of the optimal proposal (arOut[8]).
- * arOut (optional, >= 8 doubles, >= 9 with arPar[5] > 0): [0] electrons
- *   used, [1] effective sample size, [2] sum of weights, [3] device seconds,
- *   [4] rel-L2 error estimate, [5] batch size used, [6] last training loss,
- *   [7] training steps taken, [8] variance ratio plain / optimal proposal. */
+ *   mode) > 0: estimate the variance gain of the optimal proposal (arOut[8]),
+ *   [6] > 0: randomized quasi-Monte Carlo draws (digitally shifted Sobol'
+ *   points; not with the proposal), [7] its number of independent replicates
+ *   (default 16; electrons are used in whole groups of one per replicate).
+ *   arPar must hold 8 values.
+ * arOut (optional, >= 8 doubles; >= 9 with arPar[5] > 0, >= 10 with arPar[6]
+ *   > 0): [0] electrons used, [1] effective sample size, [2] sum of weights,
+ *   [3] device seconds, [4] rel-L2 error estimate (per-pixel sample variance;
+ *   with replicates, the spread of the replicate averages), [5] batch size
+ *   used, [6] last training loss, [7] training steps taken, [8] variance
+ *   ratio plain / optimal proposal, [9] replicates. */
 SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagFldC* pMagFld,
                                    SRWLOptC* pOpt, double* precPar, int nPrecPar,
                                    long long nMacroElec, double* arPar, double* arOut);

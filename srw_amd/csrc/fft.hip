@@ -20,6 +20,7 @@
 //   B  columns: forward DIF along y, Fresnel kernel, inverse DIT along y
 //   C  rows   : inverse DIT along x, post-multiply (or intensity output)
 #include "srw_fft.h"
+#include "srw_fft_ops.h"
 
 #include <math.h>
 
@@ -27,17 +28,6 @@ namespace {
 
 // Global loads issued back to back per thread before their values are used.
 #define SRWF_BATCH 8
-
-__device__ __forceinline__ float2 cmulf(float2 a, float2 b)
-{
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-__device__ __forceinline__ float2 cis_rev(double rev)
-{   // exp(i 2pi rev) with the argument reduced in FP64
-    float r = (float)(rev - rint(rev));
-    return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
-}
 
 // LDS position of element i of a line: one pad slot after every 8 elements,
 // so the strided accesses of the radix-8 rounds (strides 1, 2, 8 elements)
@@ -216,15 +206,7 @@ __global__ void __launch_bounds__(1024) k_rows_fwd(float2* data, long long slice
             int e = e0 + q * blockDim.x;
             if (e >= total) break;
             int r = e >> logn, ix = e & (N - 1), iy = row0 + r;
-            float2 v = vb[q];
-            if (preOp == 1) {   // quadratic phase removal, sroptdrf.h:812-821
-                double rx = S.xStart + ix * S.xStep - S.xc, rz = S.zStart + iy * S.zStep - S.zc;
-                v = cmulf(v, cis_rev(S.pre_rev * (S.invRx * rx * rx + S.invRz * rz * rz)));
-            } else if (preOp == 2) {   // linear-phase pre-shift of a non-centred input (gmfft.h:250-276)
-                v = cmulf(v, cis_rev(S.sh0x * ((ix - (N >> 1)) * S.xStep) + S.sh0y * ((iy - (ny >> 1)) * S.zStep)));
-            }
-            if ((ix + iy) & 1) { v.x = -v.x; v.y = -v.y; }
-            s[r * P + pad_idx(ix)] = v;
+            s[r * P + pad_idx(ix)] = srwf_pre(S, preOp, ix, iy, N, ny, vb[q]);
         }
     }
     if (inv) lds_dif<true>(s, stw, logn, nrows, P, threadIdx.x, blockDim.x);
@@ -281,17 +263,7 @@ __global__ void __launch_bounds__(1024) k_cols(float2* data, long long sliceStri
         for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
             int c = e >> logny, r = e & (NY - 1);
             int kx = bitrev(c0 + c, lognx), ky = bitrev(r, logny);
-            double qx = (kx - (NX >> 1)) * S.qStepX, qz = (ky - (NY >> 1)) * S.qStepZ;
-            double rev;
-            if (op == 0) {   // sroptdrf.h:615-644
-                double q2 = qx * qx + qz * qz, cq = S.c1 * q2;
-                rev = S.c0_rev * q2 * (1. + cq + 2. * cq * cq) + S.path_rev;
-            } else {         // sroptdrf.h:835-845
-                rev = S.c0x_rev * qx * qx + S.c0z_rev * qz * qz + S.phT_rev;
-            }
-            float2 w = cis_rev(rev);
-            w.x *= S.midScale; w.y *= S.midScale;
-            s[c * P + pad_idx(r)] = cmulf(s[c * P + pad_idx(r)], w);
+            s[c * P + pad_idx(r)] = srwf_mid(S, op, kx, ky, NX, NY, s[c * P + pad_idx(r)]);
         }
         lds_dit(s, stw, logny, colsPerWG, P, threadIdx.x, blockDim.x);
         for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
@@ -303,14 +275,8 @@ __global__ void __launch_bounds__(1024) k_cols(float2* data, long long sliceStri
         for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
             int r = e >> lc, c = e & (colsPerWG - 1);
             int ky = r, kx = bitrev(c0 + c, lognx);
-            float2 v = s[c * P + pad_idx(bitrev(ky, logny))];
-            float sg = ((kx + ky + (NX >> 1) + (NY >> 1)) & 1) ? -S.midScale : S.midScale;
-            v.x *= sg; v.y *= sg;
-            if (S.postShift) {
-                double rev = S.sh1x * ((kx - (NX >> 1)) * S.qStepX) + S.sh1y * ((ky - (NY >> 1)) * S.qStepZ);
-                v = cmulf(v, cis_rev(rev));
-            }
-            out[slice * sliceStride + (long long)ky * NX + kx] = v;
+            out[slice * sliceStride + (long long)ky * NX + kx] =
+                srwf_std_out(S, kx, ky, NX, NY, s[c * P + pad_idx(bitrev(ky, logny))]);
         }
     }
 }
@@ -359,12 +325,7 @@ __global__ void __launch_bounds__(1024) k_rows_inv(float2* data, long long slice
     const srwf_slice& S = sp[slice0];
     for (int e = threadIdx.x; e < nrows * N; e += blockDim.x) {
         int r = e >> logn, ix = e & (N - 1), iy = row0 + r;
-        float2 m = make_float2(((ix + iy) & 1) ? -1.f : 1.f, 0.f);
-        if (postOp >= 1) {
-            double rx = S.nxStart + ix * S.nxStep - S.xc, rz = S.nzStart + iy * S.nzStep - S.zc;
-            float2 w = cis_rev(S.post_rev * (S.invRxL * rx * rx + S.invRzL * rz * rz) + S.path3_rev);
-            m = make_float2(m.x * w.x, m.x * w.y);
-        }
+        float2 m = srwf_post(S, postOp, ix, iy);
         if (postOp == 2) {
             float2 a = s[r * P + pad_idx(ix)], b = s[(nrows + r) * P + pad_idx(ix)];
             // the unit-modulus phase does not change the intensity

@@ -559,7 +559,7 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         arOut[5] = (double)B;
         arOut[6] = lastLoss;
         arOut[7] = useQ ? Q.step : 0;
-        arOut[9] = R;
+        if (rqmc) arOut[9] = R;
         if (diag) {
             // N Var = E||I||^2 - ||mu||^2 for p; (E||I||)^2 - ||mu||^2 for q* ~ p ||I||
             double N = (double)done, m2 = lastA1 / (N * N);

@@ -47,16 +47,69 @@ const float2* twiddles(int logn)
     return (const float2*)p;
 }
 
+// Twiddle tables exp(-i 2pi m / n), m < n, of the mixed-radix passes, cached per size.
+const float2* gen_twiddles(int n)
+{
+    static std::map<int, DBuf> cache;
+    DBuf& b = cache[n];
+    if (b.p) return (const float2*)b.p;
+    std::vector<float2> h(n);
+    for (int m = 0; m < n; m++) {
+        double a = -kTwoPi * m / n;
+        h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    void* p = b.get(sizeof(float2) * h.size());
+    if (!p) return nullptr;
+    if (hipMemcpy(p, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return (const float2*)p;
+}
+
 int fft_ready()
 {
     static bool done = false;
     int rc = ensure_device();
     if (rc) return rc;
     if (!done) {
-        if (srwf_init_attributes()) return SRWL_AMD_ERR_GPU;
+        if (srwf_init_attributes() || srwf_gen_init_attributes()) return SRWL_AMD_ERR_GPU;
         done = true;
     }
     return 0;
+}
+
+// Mixed-radix drift passes over nSlices nx x nz slices in place (sizes that
+// are not both powers of two); tmp holds nSlices transposed slices.
+int drift_passes_gen(float2* d, float2* tmp, long long sliceStride, int nSlices, const srwf_slice* dSp, int nx, int nz,
+                     int mode, int postOp, float* inten, long long intenStride, hipStream_t st)
+{
+    srwf_gplan px, pz;
+    if (srwf_gplan_make(nx, &px) || srwf_gplan_make(nz, &pz)) return SRWL_ERR_IN_FFT;
+    const float2* twx = gen_twiddles(nx);
+    const float2* twz = gen_twiddles(nz);
+    if (!twx || !twz) return SRWL_AMD_ERR_GPU;
+    int pop = postOp;
+    if (pop == 1 && mode != 3) pop = 0;
+    const int pre = mode == 3 ? 1 : 0, col = mode == 3 ? 1 : 0;
+    if (srwf_launch_grows(d, sliceStride, nSlices, dSp, twx, px, nz, 0, pre, 0, 0, nullptr, 0, st)) return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_gtranspose(d, tmp, nz, nx, sliceStride, nSlices, st)) return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_gcols(tmp, sliceStride, nSlices, dSp, twz, pz, nx, col, 0, nullptr, st)) return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_gtranspose(tmp, d, nx, nz, sliceStride, nSlices, st)) return SRWL_AMD_ERR_GPU;
+    if (srwf_launch_grows(d, sliceStride, nSlices, dSp, twx, px, nz, 1, 0, 0, pop, inten, intenStride, st))
+        return SRWL_AMD_ERR_GPU;
+    return 0;
+}
+
+// Sizes of the power-of-2 passes (16..2048 both ways); otherwise the
+// mixed-radix passes take even sizes 8..2048 with factors 2, 3, 5, 7, 11, 13.
+static bool pow2_sizes(long nx, long nz, int& lx, int& lz)
+{
+    lx = log2_exact(nx); lz = log2_exact(nz);
+    return lx >= 4 && lz >= 4 && lx <= 11 && lz <= 11;
+}
+
+static bool gen_sizes(long nx, long nz)
+{
+    srwf_gplan a;
+    return srwf_gplan_make((int)nx, &a) == 0 && srwf_gplan_make((int)nz, &a) == 0;
 }
 
 // Drift element parameters (sroptcnt.cpp:183-211: arProp layout)
@@ -345,8 +398,14 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
         rc = resize_device(pp, pp ? propN(i) : 0);
         if (rc) return rc;
-        int lx = log2_exact(nx), lz = log2_exact(nz);
-        if (lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_ERR_IN_FFT;   // power-of-2 FFT sizes only
+        int lx, lz;
+        const bool p2 = pow2_sizes(nx, nz, lx, lz);
+        if (!p2 && !gen_sizes(nx, nz)) return SRWL_ERR_IN_FFT;
+        float2* dT = nullptr;
+        if (!p2) {
+            dT = (float2*)g_ctx.buf.GenT.get(sizeof(float2) * 2 * (size_t)n);
+            if (!dT) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        }
         mctx.d = dS; mctx.nxy = nxy; mctx.nx = nx;
         int mode, sub;
         drift_mode_from_prop(pp, mode, sub);
@@ -367,9 +426,11 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         }
         HIPCHK(hipMemcpyAsync(dSp, sp.data(), sizeof(srwf_slice) * 2 * ne, hipMemcpyHostToDevice, st));
         // twiddle tables are built and uploaded on first use of a size: outside the timed passes
-        if (!twiddles(lx) || !twiddles(lz)) return SRWL_AMD_ERR_GPU;
+        if (p2 && (!twiddles(lx) || !twiddles(lz))) return SRWL_AMD_ERR_GPU;
+        if (!p2 && (!gen_twiddles((int)nx) || !gen_twiddles((int)nz))) return SRWL_AMD_ERR_GPU;
         if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
-        rc = drift_passes(dS, nxy, (int)(2 * ne), dSp, lx, lz, mode, 1, nullptr, 0, st);
+        rc = p2 ? drift_passes(dS, nxy, (int)(2 * ne), dSp, lx, lz, mode, 1, nullptr, 0, st)
+                : drift_passes_gen(dS, dT, nxy, (int)(2 * ne), dSp, (int)nx, (int)nz, mode, 1, nullptr, 0, st);
         if (rc) return rc;
         if (g_ctx.timing) {
             HIPCHK(hipEventRecord(g_ctx.ev1, st));
@@ -432,9 +493,10 @@ SRWL_EXPORT int srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMes
     if (pcData == nullptr || arMesh == nullptr || nMesh < 6) return SRWL_INCORRECT_PARAM_FOR_FFT;
     if (typeData != 'f') return SRWL_AMD_ERR_UNSUPPORTED;
     long nx = (long)arMesh[2], nz = (long)arMesh[5];
-    int lx = log2_exact(nx), lz = log2_exact(nz);
     if (nx < 2 || nz < 2 || (nx & 1) || (nz & 1)) return SRWL_INCORRECT_PARAM_FOR_FFT;
-    if (lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_AMD_ERR_UNSUPPORTED;
+    int lx, lz;
+    const bool p2 = pow2_sizes(nx, nz, lx, lz);
+    if (!p2 && !gen_sizes(nx, nz)) return SRWL_AMD_ERR_UNSUPPORTED;
     int rc = fft_ready();
     if (rc) return rc;
     hipStream_t st = g_ctx.stream;
@@ -461,13 +523,26 @@ SRWL_EXPORT int srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMes
     S.ny = (int)nz;
     HIPCHK(hipMemcpyAsync(dSp, &S, sizeof(S), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d, pcData, sizeof(float2) * nxy, hipMemcpyHostToDevice, st));
-    const float2* twx = twiddles(lx);
-    const float2* twz = twiddles(lz);
-    if (!twx || !twz) return SRWL_AMD_ERR_GPU;
     int inv = dir > 0 ? 0 : 1;
-    const srwf_tiles* T = fft_tiles(lx, lz, 0, 2, inv, -1, dSp, 1, st);
-    if (srwf_launch_rows_fwd(d, nxy, 1, dSp, twx, lx, (int)nz, 0, inv, T, st)) return SRWL_AMD_ERR_GPU;
-    if (srwf_launch_cols(d, nxy, 1, dSp, twz, lz, lx, 2, inv, o, T, st)) return SRWL_AMD_ERR_GPU;
+    if (p2) {
+        const float2* twx = twiddles(lx);
+        const float2* twz = twiddles(lz);
+        if (!twx || !twz) return SRWL_AMD_ERR_GPU;
+        const srwf_tiles* T = fft_tiles(lx, lz, 0, 2, inv, -1, dSp, 1, st);
+        if (srwf_launch_rows_fwd(d, nxy, 1, dSp, twx, lx, (int)nz, 0, inv, T, st)) return SRWL_AMD_ERR_GPU;
+        if (srwf_launch_cols(d, nxy, 1, dSp, twz, lz, lx, 2, inv, o, T, st)) return SRWL_AMD_ERR_GPU;
+    } else {
+        srwf_gplan px, pz;
+        srwf_gplan_make((int)nx, &px);
+        srwf_gplan_make((int)nz, &pz);
+        const float2* twx = gen_twiddles((int)nx);
+        const float2* twz = gen_twiddles((int)nz);
+        float2* t = (float2*)g_ctx.buf.GenT.get(sizeof(float2) * nxy);
+        if (!twx || !twz || !t) return SRWL_AMD_ERR_GPU;
+        if (srwf_launch_grows(d, nxy, 1, dSp, twx, px, (int)nz, 0, 0, inv, 0, nullptr, 0, st)) return SRWL_AMD_ERR_GPU;
+        if (srwf_launch_gtranspose(d, t, (int)nz, (int)nx, nxy, 1, st)) return SRWL_AMD_ERR_GPU;
+        if (srwf_launch_gcols(t, nxy, 1, dSp, twz, pz, (int)nx, 2, inv, o, st)) return SRWL_AMD_ERR_GPU;
+    }
     HIPCHK(hipMemcpyAsync(pcData, o, sizeof(float2) * nxy, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     arMesh[0] = xStartTr; arMesh[1] = xStepTr; arMesh[3] = zStartTr; arMesh[4] = zStepTr;

@@ -29,6 +29,7 @@ struct Buffers {
     DBuf RsO, RsN;                // resampling
     DBuf AutoTab, AutoFlags, AutoS, AutoF, AutoWork;   // adaptive SR integral (method 1)
     DBuf Tune;                    // scratch slices of the FFT tile autotuner
+    DBuf GenT;                    // transposed slices of the mixed-radix FFT passes
 };
 
 struct Context {

@@ -29,7 +29,32 @@ struct srwf_tiles {
     int rpwF, thrF, cpw, thrC, rpwI, thrI;
 };
 
+// Mixed-radix plan of an even line length n = 2^a 3^b 5^c 7^d 11^e 13^f,
+// 8 <= n <= SRWF_GEN_MAX: the radices of the Stockham stages in order.
+#define SRWF_GEN_MAX 2048
+struct srwf_gplan {
+    int n, nst;
+    int R[16];
+};
+
 extern "C" {
+// Plan of a line length; -1 when n is odd, out of range or has another prime factor.
+int srwf_gplan_make(int n, srwf_gplan* pl);
+int srwf_gen_init_attributes();
+// Mixed-radix passes (fft_gen.hip).  twg: exp(-2 pi i m / n), m < n, of the
+// line length of the pass.  Row pass dir 0: srwf_pre (preOp) then the
+// forward transform (or the inverse direction when inv); dir 1: inverse
+// transform then srwf_post (postOp; 2 writes the intensity of slice pairs).
+int srwf_launch_grows(float2* data, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* twg,
+                      const srwf_gplan& pl, int ny, int dir, int preOp, int inv, int postOp, float* inten,
+                      long long intenStride, hipStream_t st);
+// Column pass on transposed [NX][NY] slices: op 0 / 1 forward, srwf_mid,
+// inverse, in place; op 2 forward (or inverse direction) and srwf_std_out
+// into out ([NY][NX] per slice).
+int srwf_launch_gcols(float2* dataT, long long sliceStride, int nSlices, const srwf_slice* dSp, const float2* twg,
+                      const srwf_gplan& pl, int NX, int op, int inv, float2* out, hipStream_t st);
+int srwf_launch_gtranspose(const float2* src, float2* dst, int rows, int cols, long long sliceStride, int nSlices,
+                           hipStream_t st);
 int srwf_rows_per_wg(int logn);
 int srwf_cols_per_wg(int logn);
 void srwf_default_tiles(int lognx, int logny, int postOp, srwf_tiles* t);

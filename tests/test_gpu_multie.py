@@ -65,3 +65,22 @@ def test_rqmc_estimate_agrees_with_plain_mc():
           (ep, eq, iq["replicates"], diff), flush=True)
     assert iq["electrons"] == 256 and iq["replicates"] == 16
     assert 0 < eq and diff < 4 * np.hypot(ep, eq)
+
+
+def test_error_estimates_match_true_errors():
+    """The run's own error estimates (per-pixel sample variance for plain MC,
+    the spread of 16 replicate averages for RQMC) against the true errors
+    measured from an 8192-electron RQMC reference: each estimate is within a
+    factor 2.5 of the true error (both are noisy at these counts)."""
+    from srw_amd import srwlpy
+    print("case: error estimates vs an 8192-electron reference, 64x64", flush=True)
+    st_r, w0, mag, opt, prec = _setup()
+    srwlpy.EmitPropMultiE(st_r, w0, mag, opt, prec, 8192, _seed=1, _batch=1024, _rqmc=1, _replicates=16)
+    ref = np.array(st_r.arS[:48 * 48])
+    for name, kw, n in (("plain", {}, 512), ("rqmc", {"_rqmc": 1, "_replicates": 16}, 256)):
+        st, _, _, _, _ = _setup()
+        info = srwlpy.EmitPropMultiE(st, w0, mag, opt, prec, n, _seed=7, _batch=256, **kw)
+        true = rel_l2(np.array(st.arS[:48 * 48]), ref)
+        est = info["rel_l2_est"]
+        print("%s: %d electrons, estimated %.4f true %.4f" % (name, n, est, true), flush=True)
+        assert est / 2.5 < true < est * 2.5

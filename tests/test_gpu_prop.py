@@ -144,3 +144,70 @@ def test_drift_tile_shapes_give_same_values(n, mode, tiles):
     print("tuned tiles", srwlpy.fft_tiles(lg, lg, 3 if mode else 0, 1 if mode else 0), flush=True)
     assert np.array_equal(res[0], res[1])
     assert np.array_equal(res[0], res[2])
+
+
+@pytest.mark.parametrize("n,ny,xs,dirn", [(120, 112, None, 1), (66, 52, 3e-4, -1), (390, 30, None, 1),
+                                         (2000, 16, -1e-4, -1), (14, 26, None, 1), (1024, 96, None, 1)])
+def test_uti_fft_mixed_radix(n, ny, xs, dirn):
+    """Even sizes with factors 2, 3, 5, 7, 11, 13 (and one power-of-2 axis
+    beside such a size): the mixed-radix passes."""
+    from srw_amd import srwlpy
+    print("case: UtiFFT mixed radix", n, ny, xs, dirn, flush=True)
+    d1 = _rand_data(n, ny, n * 7 + ny)
+    d2 = array('f', d1)
+    dx, dy = 1e-5, 2e-5
+    x0 = -0.5 * n * dx if xs is None else xs
+    mesh1 = [x0, dx, n, -0.5 * ny * dy, dy, ny]
+    mesh2 = list(mesh1)
+    srwlpy.UtiFFT(d1, mesh1, dirn)
+    assert oracle.uti_fft(d2, mesh2, dirn) == 0
+    a = np.frombuffer(d1, np.float32).view(np.complex64)
+    b = np.frombuffer(d2, np.float32).view(np.complex64)
+    err = rel_l2(a, b)
+    print("rel-L2 %.3e" % err, flush=True)
+    assert err < TOL_FFT
+    assert mesh1 == pytest.approx(mesh2, rel=1e-12)
+
+
+@pytest.mark.parametrize("nx,ny,ne,mode,L", [(120, 112, 1, 2, 20.), (90, 66, 2, 0, 10.), (150, 104, 1, 1, 25.)])
+def test_drift_mixed_radix(nx, ny, ne, mode, L):
+    from srw_amd import srwlpy
+    print("case: drift mixed radix %dx%d ne=%d mode=%d L=%g" % (nx, ny, ne, mode, L), flush=True)
+    w, mag, prec = W.single_electron(n=nx, ne=ne, de_rel=0.03 if ne > 1 else 0.)
+    w.allocate(ne, nx, ny)
+    assert oracle.calc_field(w, mag, prec) == 0
+    wg, wo = clone_wfr(w), clone_wfr(w)
+    opt = W.drift_container(L, mode)
+    srwlpy.PropagElecField(wg, opt)
+    assert oracle.propagate(wo, opt) == 0
+    for attr in ("xStart", "xFin", "yStart", "yFin"):
+        assert getattr(wg.mesh, attr) == pytest.approx(getattr(wo.mesh, attr), rel=1e-6)
+    ig = np.abs(cfield(wg.arEx, wg)) ** 2 + np.abs(cfield(wg.arEy, wg)) ** 2
+    io = np.abs(cfield(wo.arEx, wo)) ** 2 + np.abs(cfield(wo.arEy, wo)) ** 2
+    ex_e = rel_l2(cfield(wg.arEx, wg), cfield(wo.arEx, wo))
+    i_e = rel_l2(ig, io)
+    print("rel-L2 Ex %.3e I %.3e" % (ex_e, i_e), flush=True)
+    assert ex_e < TOL_DRIFT and i_e < 2 * TOL_DRIFT
+
+
+def test_resize_to_good_size_then_drift():
+    """A resize to a non-power-of-2 fast-FFT size (SRW's own size rule),
+    then the drift of that mesh, against the oracle."""
+    from srw_amd import srwlpy
+    from srw_amd.srwlib import SRWLOptC, SRWLOptD
+    print("case: 101^2 resized x1.5 range, x1.3 resolution, 20 m drift", flush=True)
+    w = _sr_wfr(101)
+    wg, wo = clone_wfr(w), clone_wfr(w)
+    opt = SRWLOptC([SRWLOptD(20.)], [[0, 0, 1., 1, 0, 1.5, 1.3, 1.5, 1.3]])
+    srwlpy.PropagElecField(wg, opt)
+    rc, wo = oracle.resize(wo, [0, 1.5, 1.3, 1.5, 1.3, 0.5, 0.5])
+    assert rc == 0
+    assert oracle.propagate(wo, W.drift_container(20., 1)) == 0
+    print("mesh %dx%d" % (wg.mesh.nx, wg.mesh.ny), flush=True)
+    assert (wg.mesh.nx, wg.mesh.ny) == (wo.mesh.nx, wo.mesh.ny)
+    assert wg.mesh.nx & (wg.mesh.nx - 1) != 0
+    ig = np.abs(cfield(wg.arEx, wg)) ** 2 + np.abs(cfield(wg.arEy, wg)) ** 2
+    io = np.abs(cfield(wo.arEx, wo)) ** 2 + np.abs(cfield(wo.arEy, wo)) ** 2
+    i_e = rel_l2(ig, io)
+    print("rel-L2 I %.3e" % i_e, flush=True)
+    assert i_e < 1e-4
