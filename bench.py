@@ -221,6 +221,33 @@ def c2_propagation(srwlpy, w, n_out=1024, L=20.):
     }
 
 
+def c1_methods(srwlpy, W, reps=20):
+    """The CPU-runnable single-electron case (101 x 101 mesh over +/-1 mm at
+    20 m, one photon energy at the first harmonic) with method 0 (manual,
+    trajectory step) and method 1 (adaptive, relative precision 0.01, SRW's
+    usual setting); fields left in HBM, wall time per call."""
+    from array import array
+    out = {}
+    for name, prec in (("method0", None), ("method1_relprec_0.01", array('d', [1, 0.01, 0, 0, 10001, 1, 0]))):
+        w, mag, p0 = W.single_electron(n=101)
+        p = p0 if prec is None else prec
+        srwlpy.set_keep_on_device(True)
+        srwlpy.CalcElecFieldSR(w, 0, mag, p)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            srwlpy.CalcElecFieldSR(w, 0, mag, p)
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"ms_per_call": 1e3 * dt, "points_per_s": 101 * 101 / dt}
+        if prec is not None:
+            import numpy as np
+            lv = srwlpy.last_auto_levels()
+            out[name]["levels"] = [int(lv.min()), int(lv.max())]
+            out[name]["evaluations_per_point"] = float(np.mean(4.0 * 2.0 ** lv + 1))
+    srwlpy.set_keep_on_device(False)
+    out["workload"] = "101x101 mesh over +/-1 mm at 20 m, one photon energy (first harmonic), 10001-point trajectory"
+    return out
+
+
 def cpu_baseline(n, ne_sample, threads):
     """FP64 oracle on a bounded sample of the same workload (fewer energies)."""
     from tests import oracle_lib as oracle
@@ -274,6 +301,8 @@ def main():
     srwlpy.CalcElecFieldSR(w, 0, mag, prec)
     pcie_rate = pts / (time.perf_counter() - t1)
     log("single-electron field: %.4g points/s" % value)
+    c1 = c1_methods(srwlpy, W)
+    log("101x101 methods 0 and 1 done")
     prop = None
     if a.prop:
         prop = c2_propagation(srwlpy, w)
@@ -313,6 +342,7 @@ def main():
                      "flops_per_eval": FLOPS_PER_EVAL, "valu_instr_per_eval": vpe, "valu_trans_per_eval": tpe,
                      "valu_issue_frac": issue},
         "pcie_inclusive_points_per_s": pcie_rate,
+        "single_electron_101x101": c1,
         "propagation": prop,
         "multi_electron": me,
         "uneven_batch": ub,

@@ -4,9 +4,10 @@
 // srradint.h:988-1036 (FillNextLevel), 867-880 (CosAndSin), srradint.cpp:
 // 289-394 (point order, running maximum of |E|^2).
 //
-// Each observation point runs the reference's level doubling on one lane, in
-// FP64 with the reference's Taylor sine and cosine, so that its convergence
-// decisions follow the reference's.  The reference's stopping test becomes
+// Each observation point runs the reference's level doubling on one
+// wavefront (the level's new points split over its 64 lanes), in FP64 with
+// the reference's Taylor sine and cosine, so that its convergence decisions
+// follow the reference's.  The reference's stopping test becomes
 // absolute after the first point of a call: |E|^2 may change by at most
 // relPrec times the largest |E|^2 of the points already done.  That makes a
 // point's final level depend on every earlier point.  Here every point first
@@ -102,6 +103,17 @@ __device__ __forceinline__ void auto_point_coords(const srwd_auto_job& J, long l
     eph = J.eStart + ie * J.eStep;
 }
 
+// Sum over the 64 lanes of a wave, the same value on every lane (lane 0's
+// sum, broadcast), so that decisions taken from it are uniform.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return __shfl(v, 0);
+}
+
+// One wave per observation point: each level's new points are split into
+// contiguous runs over the 64 lanes, the four partial sums are added over the
+// wave and every lane takes the (uniform) convergence decision.
 __global__ void __launch_bounds__(256) k_auto_points(const srwd_auto_job J, const srwd_trj_info* __restrict__ info,
                                                      const srwd_edge* __restrict__ edge,
                                                      const srwd_auto_tab* __restrict__ tab, const double* X,
@@ -110,7 +122,8 @@ __global__ void __launch_bounds__(256) k_auto_points(const srwd_auto_job J, cons
                                                      int nIdx, const double* __restrict__ thr, double* S, float4* F,
                                                      unsigned* guard, int* lvl, int* done, int* err, int* warn)
 {
-    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    long long k = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     long long q;
     if (idx) { if (k >= nIdx) return; q = idx[k]; }
     else { if (k >= n) return; q = k; }
@@ -121,36 +134,46 @@ __global__ void __launch_bounds__(256) k_auto_points(const srwd_auto_job J, cons
     const double actNorm = J.normConst * eph * 0.80654658E-03;
     const double kph = SRWD_KPH(eph);
     const double thrAbs = thr ? thr[q] : -1.;
-    // terminating terms and edge derivatives (srradint.h:345-366)
+    // terminating terms and edge derivatives (srradint.h:345-366), on lane 0
     double o0[4] = {0, 0, 0, 0}, wDD[4] = {0, 0, 0, 0};
     if (J.termin > 0) {
-        cplx RL[2] = {{0, 0}, {0, 0}}, RR[2] = {{0, 0}, {0, 0}}, DL[2] = {{0, 0}, {0, 0}}, DR[2] = {{0, 0}, {0, 0}};
-        bool bad = false;
-        if (J.termin == 1 || J.termin == 2)
-            bad |= !residual(I, edge->v[0], I.sIntegStart, xo, I.yObs, zo, eph, actNorm, RL, DL);
-        if (J.termin == 1 || J.termin == 3)
-            bad |= !residual(I, edge->v[1], I.sIntegFin, xo, I.yObs, zo, eph, actNorm, RR, DR);
-        if (bad) atomicOr(warn, 1);
-        o0[0] = RR[0].re - RL[0].re; o0[1] = RR[0].im - RL[0].im;
-        o0[2] = RR[1].re - RL[1].re; o0[3] = RR[1].im - RL[1].im;
-        const double wd = 1. / 15.;
-        wDD[0] = wd * (DL[0].re - DR[0].re); wDD[1] = wd * (DL[0].im - DR[0].im);
-        wDD[2] = wd * (DL[1].re - DR[1].re); wDD[3] = wd * (DL[1].im - DR[1].im);
+        if (lane == 0) {
+            cplx RL[2] = {{0, 0}, {0, 0}}, RR[2] = {{0, 0}, {0, 0}}, DL[2] = {{0, 0}, {0, 0}}, DR[2] = {{0, 0}, {0, 0}};
+            bool bad = false;
+            if (J.termin == 1 || J.termin == 2)
+                bad |= !residual(I, edge->v[0], I.sIntegStart, xo, I.yObs, zo, eph, actNorm, RL, DL);
+            if (J.termin == 1 || J.termin == 3)
+                bad |= !residual(I, edge->v[1], I.sIntegFin, xo, I.yObs, zo, eph, actNorm, RR, DR);
+            if (bad) atomicOr(warn, 1);
+            o0[0] = RR[0].re - RL[0].re; o0[1] = RR[0].im - RL[0].im;
+            o0[2] = RR[1].re - RL[1].re; o0[3] = RR[1].im - RL[1].im;
+            const double wd = 1. / 15.;
+            wDD[0] = wd * (DL[0].re - DR[0].re); wDD[1] = wd * (DL[0].im - DR[0].im);
+            wDD[2] = wd * (DL[1].re - DR[1].re); wDD[3] = wd * (DL[1].im - DR[1].im);
+        }
+        for (int m = 0; m < 4; m++) { o0[m] = __shfl(o0[m], 0); wDD[m] = __shfl(wDD[m], 0); }
     }
     const double wfe = 7. / 15., wf1 = 16. / 15., wf2 = 14. / 15.;
     const double PI = 3.141592653590;
-    // level 0: ends into wF, odd points into s1, the middle point into s2
-    double wF[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
-    double PhInit = 0.;
-    for (int j = 0; j < 5; j++) {
-        double Ph, Ax, Az, c, sn;
-        auto_term(tab[j], xo, zo, kph, Ph, Ax, Az);
-        cos_sin_taylor(Ph, c, sn);
-        double* a = (j == 0 || j == 4) ? wF : ((j & 1) ? s1 : s2);
-        a[0] += Ax * c; a[1] += Ax * sn; a[2] += Az * c; a[3] += Az * sn;
-        if (j == 0) PhInit = Ph;
+    // level 0 on lanes 0..4: ends into wF, odd points into s1, the middle point into s2
+    double wF[4], s1[4], s2[4], PhInit;
+    {
+        double a[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+        double ph = 0.;
+        if (lane < 5) {
+            double Ax, Az, c, sn;
+            auto_term(tab[lane], xo, zo, kph, ph, Ax, Az);
+            cos_sin_taylor(ph, c, sn);
+            int g = (lane == 0 || lane == 4) ? 0 : ((lane & 1) ? 1 : 2);
+            a[g][0] = Ax * c; a[g][1] = Ax * sn; a[g][2] = Az * c; a[g][3] = Az * sn;
+        }
+        PhInit = __shfl(ph, 0);
+        for (int m = 0; m < 4; m++) {
+            wF[m] = wave_sum(a[0][m]) * wfe;
+            s1[m] = wave_sum(a[1][m]);
+            s2[m] = wave_sum(a[2][m]);
+        }
     }
-    for (int m = 0; m < 4; m++) wF[m] *= wfe;
     double step = (I.sIntegFin - I.sIntegStart) * 0.25;
     double sq = 0.;
     {
@@ -159,8 +182,10 @@ __global__ void __launch_bounds__(256) k_auto_points(const srwd_auto_job J, cons
             v[m] = o0[m] + actNorm * step * (wF[m] + wf1 * s1[m] + wf2 * s2[m] + step * wDD[m]);
             sq += v[m] * v[m];
         }
-        S[q] = sq;
-        F[q] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+        if (lane == 0) {
+            S[q] = sq;
+            F[q] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+        }
     }
     unsigned g = 0;
     int L = 0;
@@ -168,38 +193,40 @@ __global__ void __launch_bounds__(256) k_auto_points(const srwd_auto_job J, cons
     while (L < SRWD_AUTO_MAX_LEVEL) {
         L++;
         for (int m = 0; m < 4; m++) { s2[m] += s1[m]; s1[m] = 0.; }
-        bool mayBeLast = true;
-        double PhPrev = PhInit;
         const double half = 0.5 * step;
         const long long np = srwd_auto_count(L);
-        if (L <= SRWD_AUTO_DEV_TAB_LEVEL) {
-            const srwd_auto_tab* t = tab + srwd_auto_off(L);
-            for (long long i = 0; i < np; i++) {
-                double Ph, Ax, Az, c, sn;
-                auto_term(t[i], xo, zo, kph, Ph, Ax, Az);
-                cos_sin_taylor(Ph, c, sn);
-                s1[0] += Ax * c; s1[1] += Ax * sn; s1[2] += Az * c; s1[3] += Az * sn;
-                if (Ph - PhPrev > PI) mayBeLast = false;
-                PhPrev = Ph;
-            }
-        } else {
-            for (long long i = 0; i < np; i++) {
-                srwd_auto_tab t = auto_point(I, X, Y, IBx, IBz, auto_s(I, L, i), false);
-                double Ph, Ax, Az, c, sn;
-                auto_term(t, xo, zo, kph, Ph, Ax, Az);
-                cos_sin_taylor(Ph, c, sn);
-                s1[0] += Ax * c; s1[1] += Ax * sn; s1[2] += Az * c; s1[3] += Az * sn;
-                if (Ph - PhPrev > PI) mayBeLast = false;
-                PhPrev = Ph;
-            }
+        // this lane's run of the level's points
+        const long long per = (np + 63) >> 6;
+        const long long i0 = lane * per, i1 = (i0 + per < np) ? i0 + per : np;
+        double part[4] = {0, 0, 0, 0};
+        double phFirst = 0., phLast = 0.;
+        bool stepsOk = true;
+        for (long long i = i0; i < i1; i++) {
+            srwd_auto_tab t = (L <= SRWD_AUTO_DEV_TAB_LEVEL) ? tab[srwd_auto_off(L) + i]
+                                                              : auto_point(I, X, Y, IBx, IBz, auto_s(I, L, i), false);
+            double Ph, Ax, Az, c, sn;
+            auto_term(t, xo, zo, kph, Ph, Ax, Az);
+            cos_sin_taylor(Ph, c, sn);
+            part[0] += Ax * c; part[1] += Ax * sn; part[2] += Az * c; part[3] += Az * sn;
+            if (i == i0) phFirst = Ph;
+            else if (Ph - phLast > PI) stepsOk = false;
+            phLast = Ph;
         }
+        // phase step into this run from the previous lane's last point
+        // (from the range start for the first point of the level)
+        double prevLast = __shfl_up(phLast, 1);
+        if (i0 < i1 && phFirst - (lane == 0 ? PhInit : prevLast) > PI) stepsOk = false;
+        const bool mayBeLast = __all(stepsOk ? 1 : 0) != 0;
+        for (int m = 0; m < 4; m++) s1[m] = wave_sum(part[m]);
         double v[4], locSq = 0.;
         for (int m = 0; m < 4; m++) {
             v[m] = o0[m] + actNorm * half * (wF[m] + wf1 * s1[m] + wf2 * s2[m] + half * wDD[m]);
             locSq += v[m] * v[m];
         }
-        S[(long long)L * n + q] = locSq;
-        F[(long long)L * n + q] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+        if (lane == 0) {
+            S[(long long)L * n + q] = locSq;
+            F[(long long)L * n + q] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+        }
         if (mayBeLast) g |= 1u << L;
         bool pass = false;
         if (mayBeLast) {
@@ -210,10 +237,12 @@ __global__ void __launch_bounds__(256) k_auto_points(const srwd_auto_job J, cons
         step = half;
         if (pass) { ok = true; break; }
     }
-    guard[q] = g;
-    lvl[q] = L;
-    done[q] = L;
-    if (!ok) atomicOr(err, 1);
+    if (lane == 0) {
+        guard[q] = g;
+        lvl[q] = L;
+        done[q] = L;
+        if (!ok) atomicOr(err, 1);
+    }
 }
 
 // Exclusive running maximum of the final |E|^2 in point order (one block).
@@ -327,7 +356,8 @@ int srwd_launch_auto_points(const srwd_auto_job& J, const srwd_trj_info* dInfo, 
 {
     long long m = idx ? nIdx : n;
     if (m <= 0) return 0;
-    hipLaunchKernelGGL(k_auto_points, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, J, dInfo, dEdge, dTab, dX,
+    // one wave per point, four per workgroup
+    hipLaunchKernelGGL(k_auto_points, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st, J, dInfo, dEdge, dTab, dX,
                        dY, dIBx, dIBz, p0, n, idx, nIdx, thr, S, F, guard, lvl, done, err, warn);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
