@@ -56,17 +56,18 @@ SRWL_EXPORT int srwlAmdSetResizePerPoint(int on);
 
 /* FFT tile shapes (rows per workgroup and threads of the forward row pass,
  * columns per workgroup and threads of the column pass, rows per workgroup
- * and threads of the inverse row pass).  srwlAmdSetFftTiles(t6) makes every
- * pass use t6 (a shape that does not fit falls back to the default),
+ * and threads of the inverse row pass, 1 when the column pass takes two
+ * adjacent column blocks per workgroup).  srwlAmdSetFftTiles(t7) makes every
+ * pass use t7 (a shape that does not fit falls back to the default),
  * srwlAmdSetFftTiles(NULL) returns to the autotuner; srwlAmdSetTuning(0)
  * selects the fixed default shapes instead of the autotuner.
  * srwlAmdGetFftTiles copies the shapes the autotuner chose for a drift of
  * 2^lognx x 2^lognz slices (mode 0 or 3, postOp 0 field, 1 field with the new
  * quadratic phase, 2 intensity) and returns 1, or returns 0 when that shape
  * has not been tuned yet.  Every shape gives the same values. */
-SRWL_EXPORT int srwlAmdSetFftTiles(const int* t6);
+SRWL_EXPORT int srwlAmdSetFftTiles(const int* t7);
 SRWL_EXPORT int srwlAmdSetTuning(int on);
-SRWL_EXPORT int srwlAmdGetFftTiles(int lognx, int lognz, int mode, int postOp, int* out6);
+SRWL_EXPORT int srwlAmdGetFftTiles(int lognx, int lognz, int mode, int postOp, int* out7);
 
 /* Six standard normals of electron i of the counter-based sampler used by
  * srwlEmitPropMultiE (Philox4x32-10 + Box-Muller, see srw_rng.h). */

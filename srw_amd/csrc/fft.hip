@@ -223,9 +223,44 @@ __global__ void __launch_bounds__(1024) k_rows_fwd(float2* data, long long slice
 // centred-output post-multiply and natural-order output (standalone FFT).
 // Column positions hold bit-reversed x frequencies.
 // ---------------------------------------------------------------------------
+// Transforms the cpw columns c0.. held in LDS (forward, kernel, inverse) and
+// stores them, or for op 2 writes the standalone output.
+__device__ __forceinline__ void cols_block(float2* s, const float2* stw, const srwf_slice& S, float2* base, float2* out,
+                                           long long outOff, int c0, int cpw, int lc, int logny, int lognx, int P,
+                                           int op, int inv)
+{
+    const int NY = 1 << logny, NX = 1 << lognx;
+    if (op == 2 && inv) lds_dif<true>(s, stw, logny, cpw, P, threadIdx.x, blockDim.x);
+    else lds_dif<false>(s, stw, logny, cpw, P, threadIdx.x, blockDim.x);
+    if (op == 0 || op == 1) {
+        for (int e = threadIdx.x; e < NY * cpw; e += blockDim.x) {
+            int c = e >> logny, r = e & (NY - 1);
+            int kx = bitrev(c0 + c, lognx), ky = bitrev(r, logny);
+            s[c * P + pad_idx(r)] = srwf_mid(S, op, kx, ky, NX, NY, s[c * P + pad_idx(r)]);
+        }
+        lds_dit(s, stw, logny, cpw, P, threadIdx.x, blockDim.x);
+        for (int e = threadIdx.x; e < NY * cpw; e += blockDim.x) {
+            int r = e >> lc, c = e & (cpw - 1);
+            base[(long long)r * NX + c0 + c] = s[c * P + pad_idx(r)];
+        }
+    } else {
+        // standalone transform: (-1)^(kx+ky+NX/2+NY/2) * scale * post-shift, natural order
+        for (int e = threadIdx.x; e < NY * cpw; e += blockDim.x) {
+            int r = e >> lc, c = e & (cpw - 1);
+            int ky = r, kx = bitrev(c0 + c, lognx);
+            out[outOff + (long long)ky * NX + kx] = srwf_std_out(S, kx, ky, NX, NY, s[c * P + pad_idx(bitrev(ky, logny))]);
+        }
+    }
+}
+
+// pairs = 0: one block of colsPerWG columns per workgroup.  pairs = 1: two
+// adjacent blocks, read together as full rows of 2 * colsPerWG columns (whole
+// 128-byte lines for 8 columns); the first block goes to LDS, the second to
+// registers (at most SRWF_PAIR_REGS values per thread) until the first is done.
+#define SRWF_PAIR_REGS 16
 __global__ void __launch_bounds__(1024) k_cols(float2* data, long long sliceStride, const srwf_slice* __restrict__ sp,
                                               const float2* __restrict__ twy, int logny, int lognx, int colsPerWG,
-                                              int op, int inv, float2* out)
+                                              int pairs, int op, int inv, float2* out)
 {
     extern __shared__ float2 smem[];
     const int NY = 1 << logny, NX = 1 << lognx;
@@ -237,6 +272,34 @@ __global__ void __launch_bounds__(1024) k_cols(float2* data, long long sliceStri
     const int slice = blockIdx.y;
     const srwf_slice& S = sp[slice];
     float2* base = data + slice * sliceStride;
+    const long long outOff = slice * sliceStride;
+    if (pairs) {
+        const int w = 2 * colsPerWG;            // columns read per row
+        const int c0 = blockIdx.x * w;
+        const int nq = NY * w / blockDim.x;     // loads per thread (host: <= SRWF_PAIR_REGS, exact)
+        const int cHold = threadIdx.x & (w - 1);   // the same column for all of a thread's loads
+        float2 reg[SRWF_PAIR_REGS];
+#pragma unroll
+        for (int q = 0; q < SRWF_PAIR_REGS; q++) {
+            if (q < nq) {
+                int r = (threadIdx.x + q * blockDim.x) >> (lc + 1);
+                float2 v = base[(long long)r * NX + c0 + cHold];
+                if (cHold < colsPerWG) s[cHold * P + pad_idx(r)] = v;
+                else reg[q] = v;
+            }
+        }
+        cols_block(s, stw, S, base, out, outOff, c0, colsPerWG, lc, logny, lognx, P, op, inv);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < SRWF_PAIR_REGS; q++) {
+            if (q < nq && cHold >= colsPerWG) {
+                int r = (threadIdx.x + q * blockDim.x) >> (lc + 1);
+                s[(cHold - colsPerWG) * P + pad_idx(r)] = reg[q];
+            }
+        }
+        cols_block(s, stw, S, base, out, outOff, c0 + colsPerWG, colsPerWG, lc, logny, lognx, P, op, inv);
+        return;
+    }
     const int c0 = blockIdx.x * colsPerWG;
     const int total = NY * colsPerWG;
     for (int e0 = threadIdx.x; e0 < total; e0 += SRWF_BATCH * blockDim.x) {
@@ -257,28 +320,7 @@ __global__ void __launch_bounds__(1024) k_cols(float2* data, long long sliceStri
             s[c * P + pad_idx(r)] = vb[q];
         }
     }
-    if (op == 2 && inv) lds_dif<true>(s, stw, logny, colsPerWG, P, threadIdx.x, blockDim.x);
-    else lds_dif<false>(s, stw, logny, colsPerWG, P, threadIdx.x, blockDim.x);
-    if (op == 0 || op == 1) {
-        for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
-            int c = e >> logny, r = e & (NY - 1);
-            int kx = bitrev(c0 + c, lognx), ky = bitrev(r, logny);
-            s[c * P + pad_idx(r)] = srwf_mid(S, op, kx, ky, NX, NY, s[c * P + pad_idx(r)]);
-        }
-        lds_dit(s, stw, logny, colsPerWG, P, threadIdx.x, blockDim.x);
-        for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
-            int r = e >> lc, c = e & (colsPerWG - 1);
-            base[(long long)r * NX + c0 + c] = s[c * P + pad_idx(r)];
-        }
-    } else {
-        // standalone transform: (-1)^(kx+ky+NX/2+NY/2) * scale * post-shift, natural order
-        for (int e = threadIdx.x; e < NY * colsPerWG; e += blockDim.x) {
-            int r = e >> lc, c = e & (colsPerWG - 1);
-            int ky = r, kx = bitrev(c0 + c, lognx);
-            out[slice * sliceStride + (long long)ky * NX + kx] =
-                srwf_std_out(S, kx, ky, NX, NY, s[c * P + pad_idx(bitrev(ky, logny))]);
-        }
-    }
+    cols_block(s, stw, S, base, out, outOff, c0, colsPerWG, lc, logny, lognx, P, op, inv);
 }
 
 // ---------------------------------------------------------------------------
@@ -444,6 +486,7 @@ void srwf_default_tiles(int lognx, int logny, int postOp, srwf_tiles* t)
     if (postOp == 2) rpw = rpw > 1 ? rpw / 2 : 1;
     t->rpwI = rpw;
     t->thrI = SRWF_ROWS_THREADS;
+    t->pairC = 0;
 }
 
 int srwf_tiles_fit(int lognx, int logny, int postOp, const srwf_tiles* t)
@@ -454,6 +497,13 @@ int srwf_tiles_fit(int lognx, int logny, int postOp, const srwf_tiles* t)
     if (t->rpwF < 1 || t->rpwF > ny || lds_bytes(nx, t->rpwF) > lim) return 0;
     if (t->rpwI < 1 || t->rpwI > ny || lds_bytes(nx, (postOp == 2 ? 2 : 1) * t->rpwI) > lim) return 0;
     if (t->cpw < 1 || t->cpw > nx || (t->cpw & (t->cpw - 1)) != 0 || lds_bytes(ny, t->cpw) > lim) return 0;
+    if (t->pairC != 0 && t->pairC != 1) return 0;
+    if (t->pairC) {
+        // two blocks per workgroup; each thread keeps one column's loads in registers
+        int w = 2 * t->cpw;
+        if (w > nx || t->thrC % w != 0 || (long long)ny * w % t->thrC != 0 || ny * w / t->thrC > SRWF_PAIR_REGS)
+            return 0;
+    }
     return 1;
 }
 
@@ -483,9 +533,9 @@ int srwf_launch_cols(float2* data, long long sliceStride, int nSlices, const srw
     srwf_tiles T = tiles_or_default(lognx, logny, 0, tiles);
     int cpw = T.cpw;
     int nx = 1 << lognx;
-    dim3 grid(nx / cpw, nSlices);
+    dim3 grid(nx / (cpw * (T.pairC ? 2 : 1)), nSlices);
     hipLaunchKernelGGL(k_cols, grid, dim3(T.thrC), lds_bytes(1 << logny, cpw), st, data, sliceStride, dSp, twy, logny,
-                       lognx, cpw, op, inv, out);
+                       lognx, cpw, T.pairC, op, inv, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

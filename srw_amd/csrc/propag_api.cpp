@@ -231,10 +231,10 @@ const srwf_tiles* fft_tiles(int lognx, int lognz, int preOp, int colOp, int inv,
         float bestMs = 1e30f;
         srwf_tiles keep = best;
         for (int a = 1; a <= (pass == 1 ? 64 : 32); a <<= 1)
-            for (int ti = 0; ti < 3; ti++) {
+            for (int ti = 0; ti < 3 * (pass == 1 ? 2 : 1); ti++) {
                 srwf_tiles c = best;
                 if (pass == 0) { c.rpwF = a; c.thrF = thrs[ti]; }
-                else if (pass == 1) { c.cpw = a; c.thrC = thrs[ti]; }
+                else if (pass == 1) { c.cpw = a; c.thrC = thrs[ti % 3]; c.pairC = ti / 3; }
                 else { c.rpwI = a; c.thrI = thrs[ti]; }
                 if (!srwf_tiles_fit(lognx, lognz, postOp, &c)) continue;
                 float ms = 0;
@@ -549,12 +549,13 @@ SRWL_EXPORT int srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMes
     return 0;
 }
 
-SRWL_EXPORT int srwlAmdSetFftTiles(const int* t6)
+SRWL_EXPORT int srwlAmdSetFftTiles(const int* t7)
 {
-    g_ctx.tilesForced = t6 != nullptr;
-    if (t6) {
+    g_ctx.tilesForced = t7 != nullptr;
+    if (t7) {
         srwf_tiles& T = g_ctx.forcedTiles;
-        T.rpwF = t6[0]; T.thrF = t6[1]; T.cpw = t6[2]; T.thrC = t6[3]; T.rpwI = t6[4]; T.thrI = t6[5];
+        T.rpwF = t7[0]; T.thrF = t7[1]; T.cpw = t7[2]; T.thrC = t7[3]; T.rpwI = t7[4]; T.thrI = t7[5];
+        T.pairC = t7[6];
     }
     return 0;
 }
@@ -565,14 +566,17 @@ SRWL_EXPORT int srwlAmdSetTuning(int on)
     return 0;
 }
 
-SRWL_EXPORT int srwlAmdGetFftTiles(int lognx, int lognz, int mode, int postOp, int* out6)
+SRWL_EXPORT int srwlAmdGetFftTiles(int lognx, int lognz, int mode, int postOp, int* out7)
 {
     int pre = mode == 3 ? 1 : 0;
     if (postOp == 1 && mode != 3) postOp = 0;
     auto it = g_tiles.find(TileKey{lognx, lognz, pre, pre, 0, postOp});
     if (it == g_tiles.end()) return 0;
     const srwf_tiles& T = it->second;
-    if (out6) { out6[0] = T.rpwF; out6[1] = T.thrF; out6[2] = T.cpw; out6[3] = T.thrC; out6[4] = T.rpwI; out6[5] = T.thrI; }
+    if (out7) {
+        out7[0] = T.rpwF; out7[1] = T.thrF; out7[2] = T.cpw; out7[3] = T.thrC; out7[4] = T.rpwI; out7[5] = T.thrI;
+        out7[6] = T.pairC;
+    }
     return 1;
 }
 

@@ -48,7 +48,7 @@ struct Context {
     bool resizePerPoint = false;  // test switch: resample per new point instead of per old cell
     bool tuneTiles = true;        // FFT tile shapes from the autotuner (else the fixed defaults)
     bool tilesForced = false;     // test switch: every FFT pass uses forcedTiles
-    srwf_tiles forcedTiles = {0, 0, 0, 0, 0, 0};
+    srwf_tiles forcedTiles = {0, 0, 0, 0, 0, 0, 0};
     double sumSrMs = 0, sumSrEvals = 0;
     int nSrCalls = 0;
     double lastFftMs = 0, lastFftBytes = 0;

@@ -22,11 +22,13 @@ struct srwf_slice {
 
 // Tile shapes of the three passes: rows per workgroup and threads of the
 // forward row pass, columns per workgroup and threads of the column pass,
-// rows per workgroup and threads of the inverse row pass.  Any shape that
+// rows per workgroup and threads of the inverse row pass, and whether the
+// column pass takes two adjacent column blocks per workgroup.  Any shape that
 // fits (srwf_tiles_fit) gives the same values; only the work assignment
 // changes.
 struct srwf_tiles {
     int rpwF, thrF, cpw, thrC, rpwI, thrI;
+    int pairC;   // column pass reads two adjacent blocks per workgroup (full 2*cpw-column rows)
 };
 
 // Mixed-radix plan of an even line length n = 2^a 3^b 5^c 7^d 11^e 13^f,

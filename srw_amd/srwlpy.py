@@ -173,12 +173,12 @@ def set_traj_serial(on=True):
     lib.srwlAmdSetTrajSerial(1 if on else 0)
 
 
-def set_fft_tiles(t6=None):
-    """Forces the FFT tile shapes (6 ints) of every pass; None returns to the autotuner."""
-    if t6 is None:
+def set_fft_tiles(t7=None):
+    """Forces the FFT tile shapes (7 ints) of every pass; None returns to the autotuner."""
+    if t7 is None:
         lib.srwlAmdSetFftTiles(None)
     else:
-        lib.srwlAmdSetFftTiles((C.c_int * 6)(*[int(v) for v in t6]))
+        lib.srwlAmdSetFftTiles((C.c_int * 7)(*[int(v) for v in t7]))
 
 
 def set_tuning(on=True):
@@ -187,7 +187,7 @@ def set_tuning(on=True):
 
 def fft_tiles(lognx, lognz, mode, post_op):
     """Tile shapes the autotuner chose for a drift shape, or None."""
-    out = (C.c_int * 6)()
+    out = (C.c_int * 7)()
     if lib.srwlAmdGetFftTiles(int(lognx), int(lognz), int(mode), int(post_op), out):
         return list(out)
     return None

@@ -116,16 +116,19 @@ def test_intensity(pol):
 
 
 @pytest.mark.parametrize("n,mode,tiles", [
-    (256, 2, (1, 1024, 4, 256, 4, 512)),
-    (256, 0, (8, 256, 16, 1024, 2, 256)),
-    (128, 2, (16, 512, 32, 1024, 16, 1024)),
+    (256, 2, (1, 1024, 4, 256, 4, 512, 0)),
+    (256, 0, (8, 256, 16, 1024, 2, 256, 0)),
+    (128, 2, (16, 512, 32, 1024, 16, 1024, 0)),
+    (1024, 2, (2, 256, 8, 1024, 2, 256, 1)),
+    (256, 0, (4, 256, 4, 512, 4, 256, 1)),
 ])
 def test_drift_tile_shapes_give_same_values(n, mode, tiles):
     """Any tile shape of the FFT passes (the autotuner's search space) gives
     the same bits as the default shape, and the autotuned shape as well."""
     from srw_amd import srwlpy
     print("case: drift tiles n=%d mode=%d tiles=%s" % (n, mode, tiles), flush=True)
-    w = _sr_wfr(n)
+    w, mag, prec = W.single_electron(n=n)
+    srwlpy.CalcElecFieldSR(w, 0, mag, prec)   # the field only feeds the comparison of tile shapes
     opt = W.drift_container(20., mode)
     res = []
     for setting in ("default", "forced", "tuned"):
