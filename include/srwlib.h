@@ -205,8 +205,13 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
                                     double* precPar, int nPrecPar);
 
 /* reference srwlib.h:748 (srwlib.cpp:757-856): intensity from field.
- * Implemented: pol 0..6 and -1..-4, intType 0 (single-electron intensity),
- * depType 3 (vs x and y at photon energy e, linear interpolation in e). */
+ * Implemented: pol 0..6, Stokes -1..-4 and -5 (all four, one after another),
+ * intType 0 (single-electron intensity), 5 (Re E), 6 (Im E); depType 0 (vs e),
+ * 1 (vs x), 2 (vs y), 3 (vs x, y), 4 (vs e, x), 5 (vs e, y), 6 (vs e, x, y).
+ * Coordinates that are not varied are taken at e, x, y, interpolated
+ * linearly between mesh points (clamped at the mesh ends).  Output: float,
+ * photon energy fastest, then x, then y.  Other intType values (multi-electron,
+ * flux, phase) return SRWL_AMD_ERR_UNSUPPORTED. */
 SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, char intType,
                                          char depType, double e, double x, double y,
                                          double* arMeth, void* pFldTrj, double* arParGPU);

@@ -28,7 +28,8 @@
 #define ORC_API __attribute__((visibility("default")))
 
 ORC_API void orc_compute_rad_moments(SRWLWfr* w, double* momX, double* momZ);
-ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e);
+ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e,
+                                         double x, double y);
 
 /* ------------------------------------------------------------------ */
 /* Constants (reference appendix of parity-critical values)            */
@@ -1674,33 +1675,119 @@ static double orc_int_comp(double xr, double xi, double zr, double zi, int pol)
     }
 }
 
-ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e)
+/* srradmnp.h:367-430 IntensityComponent in float: what = 0 intensity, 3 Re,
+ * 6 Im of the polarisation component (Stokes parameters ignore what). */
+static float orc_comp_f(const float* px, const float* pz, int pol, int what)
+{
+    float xr = 0.f, xi = 0.f, zr = 0.f, zi = 0.f;   /* a missing component reads as zero */
+    if (px) { xr = px[0]; xi = px[1]; }
+    if (pz) { zr = pz[0]; zi = pz[1]; }
+    int inten = (what == 0);
+    switch (pol) {
+    case 0: return inten ? xr * xr + xi * xi : (what == 3 ? xr : xi);
+    case 1: return inten ? zr * zr + zi * zi : (what == 3 ? zr : zi);
+    case 2: { float a = xr + zr, b = xi + zi; return (float)(inten ? 0.5 * (a * a + b * b) : 0.70710678 * (what == 3 ? a : b)); }
+    case 3: { float a = xr - zr, b = xi - zi; return (float)(inten ? 0.5 * (a * a + b * b) : 0.70710678 * (what == 3 ? a : b)); }
+    case 5: { float a = xr - zi, b = xi + zr; return (float)(inten ? 0.5 * (a * a + b * b) : 0.70710678 * (what == 3 ? a : b)); }
+    case 4: { float a = xr + zi, b = xi - zr; return (float)(inten ? 0.5 * (a * a + b * b) : 0.70710678 * (what == 3 ? a : b)); }
+    case -1: return xr * xr + xi * xi + zr * zr + zi * zi;
+    case -2: return xr * xr + xi * xi - (zr * zr + zi * zi);
+    case -3: return (float)(2. * (xr * zr + xi * zi));
+    case -4: return (float)(2. * (xr * zi - xi * zr));
+    default: return inten ? xr * xr + xi * xi + zr * zr + zi * zi : (what == 3 ? xr : xi);
+    }
+}
+
+/* srradmnp.cpp:35-79 SetupIntCoord */
+static void orc_int_coord(double arg, double start, double step, long n, long* i0, long* i1, double* t)
+{
+    if (n <= 1) { *i0 = *i1 = 0; *t = 0.; return; }
+    double inv = 1. / step;
+    long a = (long)((arg - start) * inv);
+    if (a < 0) { a = 0; *i1 = 0; }
+    else if (a >= n - 1) { a = n - 1; *i1 = a; }
+    else *i1 = a + 1;
+    *i0 = a;
+    *t = inv * (arg - a * step - start);
+}
+
+/* srradmnp.h:345-363 IntensityComponentSimpleInterpol(2D) */
+static float orc_interp1(float a, float b, double t) { return (float)((b - a) * t + a); }
+static float orc_interp2(float i00, float i10, float i01, float i11, double t1, double t2)
+{
+    double t12 = t1 * t2;
+    return (float)((i00 - i01 - i10 + i11) * t12 + (i10 - i00) * t1 + (i01 - i00) * t2 + i00);
+}
+
+/* srwlib.cpp:757-856 and srradmnp.cpp:224-1179: single-electron intensity
+ * (intType 0), Re E (5) and Im E (6) of polarisation pol vs photon energy
+ * (depType 0), x (1), y (2), x and y (3), energy and x (4), energy and y (5),
+ * energy, x and y (6); coordinates not varied are interpolated (linearly in
+ * one, bilinearly in two).  Outputs are energy fastest, then x, then y. */
+ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e,
+                                         double x, double y)
 {
     if (!pInt || !w) return SRWL_INCORRECT_PARAM_FOR_INT_EXTR;
-    if (intType != 0 || depType != 3) return SRWL_AMD_ERR_UNSUPPORTED;
+    int what = intType == 0 ? 0 : (intType == 5 ? 3 : (intType == 6 ? 6 : -1));
+    if (what < 0 || depType < 0 || depType > 6) return SRWL_AMD_ERR_UNSUPPORTED;
     long nx = w->mesh.nx, nz = w->mesh.ny, ne = w->mesh.ne;
-    double eStep = ne > 1 ? (w->mesh.eFin - w->mesh.eStart) / (ne - 1) : 0;
-    long ie0 = 0, ie1 = 0;
-    double t = 0;
-    if (ne > 1) {
-        double r = (e - w->mesh.eStart) / eStep;
-        ie0 = (long)r;
-        if (ie0 < 0) ie0 = 0;
-        if (ie0 > ne - 2) ie0 = ne - 2;
-        ie1 = ie0 + 1;
-        t = r - ie0;
+    if (pol == -5) {   /* all four Stokes parameters, one block after another (srradmnp.cpp:236-246) */
+        const long long len[7] = {ne, nx, nz, (long long)nx * nz, (long long)ne * nx, (long long)ne * nz,
+                                  (long long)ne * nx * nz};
+        if (what != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+        for (int s = 0; s < 4; s++) {
+            int rc = orc_srwlCalcIntFromElecField(pInt + sizeof(float) * s * len[(int)depType], w, (char)(-1 - s),
+                                                  intType, depType, e, x, y);
+            if (rc) return rc;
+        }
+        return 0;
     }
+    double eStep = ne > 1 ? (w->mesh.eFin - w->mesh.eStart) / (ne - 1) : 0;
+    double xStep = nx > 1 ? (w->mesh.xFin - w->mesh.xStart) / (nx - 1) : 0;
+    double zStep = nz > 1 ? (w->mesh.yFin - w->mesh.yStart) / (nz - 1) : 0;
+    long ie0, ie1, ix0, ix1, iz0, iz1;
+    double te, tx, tz;
+    orc_int_coord(e, w->mesh.eStart, eStep, ne, &ie0, &ie1, &te);
+    orc_int_coord(x, w->mesh.xStart, xStep, nx, &ix0, &ix1, &tx);
+    orc_int_coord(y, w->mesh.yStart, zStep, nz, &iz0, &iz1, &tz);
     const float* fx = (const float*)w->arEx;
     const float* fz = (const float*)w->arEy;
+#define ORC_OFS(ie, ix, iz) ((((long long)(iz) * nx + (ix)) * ne + (ie)) * 2)
+#define ORC_C(ie, ix, iz) orc_comp_f(fx ? fx + ORC_OFS(ie, ix, iz) : 0, fz ? fz + ORC_OFS(ie, ix, iz) : 0, pol, what)
     float* out = (float*)pInt;
-    for (long iz = 0; iz < nz; iz++)
-        for (long ix = 0; ix < nx; ix++) {
-            long long o0 = ((long long)(iz * nx + ix) * ne + ie0) * 2, o1 = ((long long)(iz * nx + ix) * ne + ie1) * 2;
-            double v0 = orc_int_comp(fx[o0], fx[o0 + 1], fz[o0], fz[o0 + 1], pol);
-            double v = v0;
-            if (ne > 1) { double v1 = orc_int_comp(fx[o1], fx[o1 + 1], fz[o1], fz[o1 + 1], pol); v = v0 + (v1 - v0) * t; }
-            out[iz * nx + ix] = (float)v;
-        }
+    long long k = 0;
+    switch (depType) {
+    case 0:
+        for (long ie = 0; ie < ne; ie++)
+            out[k++] = orc_interp2(ORC_C(ie, ix0, iz0), ORC_C(ie, ix1, iz0), ORC_C(ie, ix0, iz1), ORC_C(ie, ix1, iz1), tx, tz);
+        break;
+    case 1:
+        for (long ix = 0; ix < nx; ix++)
+            out[k++] = orc_interp2(ORC_C(ie0, ix, iz0), ORC_C(ie1, ix, iz0), ORC_C(ie0, ix, iz1), ORC_C(ie1, ix, iz1), te, tz);
+        break;
+    case 2:
+        for (long iz = 0; iz < nz; iz++)
+            out[k++] = orc_interp2(ORC_C(ie0, ix0, iz), ORC_C(ie1, ix0, iz), ORC_C(ie0, ix1, iz), ORC_C(ie1, ix1, iz), te, tx);
+        break;
+    case 3:
+        for (long iz = 0; iz < nz; iz++)
+            for (long ix = 0; ix < nx; ix++) out[k++] = orc_interp1(ORC_C(ie0, ix, iz), ORC_C(ie1, ix, iz), te);
+        break;
+    case 4:
+        for (long ix = 0; ix < nx; ix++)
+            for (long ie = 0; ie < ne; ie++) out[k++] = orc_interp1(ORC_C(ie, ix, iz0), ORC_C(ie, ix, iz1), tz);
+        break;
+    case 5:
+        for (long iz = 0; iz < nz; iz++)
+            for (long ie = 0; ie < ne; ie++) out[k++] = orc_interp1(ORC_C(ie, ix0, iz), ORC_C(ie, ix1, iz), tx);
+        break;
+    default:
+        for (long iz = 0; iz < nz; iz++)
+            for (long ix = 0; ix < nx; ix++)
+                for (long ie = 0; ie < ne; ie++) out[k++] = ORC_C(ie, ix, iz);
+    }
+#undef ORC_C
+#undef ORC_OFS
     return 0;
 }
 
@@ -1769,7 +1856,7 @@ ORC_API int orc_emit_prop_multi_e(SRWLStokes* st, SRWLWfr* w0, SRWLMagFldC* mag,
             int rc = orc_srwlCalcElecFieldSR_mt(&w, 0, mag, prec, nprec, 1);
             if (rc > 0) { err = rc; continue; }
             if (opt) { rc = orc_srwlPropagElecField(&w, opt); if (rc) { err = rc; continue; } }
-            orc_srwlCalcIntFromElecField((char*)Iarr, &w, 6, 0, 3, w.mesh.eStart);
+            orc_srwlCalcIntFromElecField((char*)Iarr, &w, 6, 0, 3, w.mesh.eStart, 0., 0.);
             double wxS = w.mesh.xStart, wxD = (w.mesh.xFin - w.mesh.xStart) / (nx - 1);
             double wzS = w.mesh.yStart, wzD = (w.mesh.yFin - w.mesh.yStart) / (nz - 1);
             for (long iz = 0; iz < rnz; iz++) {
@@ -1920,7 +2007,7 @@ static void orc_intensity_limits(const SRWLWfr* w, const orc_mesh3* m, long ie, 
     double* ax = (double*)malloc(sizeof(double) * (nz > nx ? nz : nx));
     if (!Iv || !ax) { free(Iv); free(ax); return; }
     /* single-electron intensity, total polarisation, at this slice's energy (srradmnp.h:367-415) */
-    orc_srwlCalcIntFromElecField((char*)Iv, (SRWLWfr*)w, 6, 0, 3, m->eStart + ie * m->eStep);
+    orc_srwlCalcIntFromElecField((char*)Iv, (SRWLWfr*)w, 6, 0, 3, m->eStart + ie * m->eStep, 0., 0.);
     float tot = (float)(orc_sum_f(Iv, 0, nx * nz - 1, 1) * m->xStep * m->zStep);
     double stopAbs = (double)tot * (1. - 0.9) * 0.25;
     float lim[5];

@@ -5,7 +5,7 @@
 //   srwlPropagElecField      cpp/src/lib/srwlib.cpp:1023-1074, sroptcnt.cpp:87-425
 //   drift buffer variables   cpp/src/core/sroptdrf.cpp:889-1116
 //   srwlUtiFFT               cpp/src/lib/srwlib.cpp:1090-1204, gmfft.cpp:295-893
-//   srwlCalcIntFromElecField cpp/src/lib/srwlib.cpp:757-856, srradmnp.h:367-430
+//   srwlCalcIntFromElecField cpp/src/lib/srwlib.cpp:757-856, srradmnp.cpp:35-1180
 #include "srw_api_internal.h"
 #include "srw_fft.h"
 #include "srw_prop.h"
@@ -583,34 +583,48 @@ SRWL_EXPORT int srwlAmdGetFftTiles(int lognx, int lognz, int mode, int postOp, i
 SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, char intType, char depType, double e,
                                          double x, double y, double* arMeth, void* pFldTrj, double* arParGPU)
 {
-    (void)x; (void)y; (void)arMeth; (void)arParGPU;
+    (void)arMeth; (void)arParGPU;
     if (pWfr == nullptr || pInt == nullptr) return SRWL_INCORRECT_PARAM_FOR_INT_EXTR;
-    if (intType != 0 || depType != 3 || pFldTrj != nullptr) return SRWL_AMD_ERR_UNSUPPORTED;
-    if (pWfr->arEx == nullptr || pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
+    // single-electron intensity (0), Re E (5), Im E (6); srwlib.cpp:781-792
+    const int what = intType == 0 ? 0 : (intType == 5 ? 3 : (intType == 6 ? 6 : -1));
+    if (what < 0 || depType < 0 || depType > 6 || pFldTrj != nullptr) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (pol < -5 || pol > 6 || (pol == -5 && what != 0)) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (pWfr->arEx == nullptr && pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
+    const SRWLRadMesh& m = pWfr->mesh;
+    if (m.nx <= 0 || m.ny <= 0 || m.ne <= 0) return SRWL_INCORRECT_WFR_STRUCT;
     int rc = ensure_device();
     if (rc) return rc;
     hipStream_t st = g_ctx.stream;
-    long nx = pWfr->mesh.nx, nz = pWfr->mesh.ny, ne = pWfr->mesh.ne;
-    long long n = (long long)nx * nz * ne;
+    srwx_extract P{};
+    P.pol = pol; P.what = what; P.dep = depType;
+    P.ne = m.ne; P.nx = m.nx; P.nz = m.ny;
+    // srradmnp.cpp:35-79: cell index and fractional offset of an off-mesh coordinate
+    auto coord = [](double arg, double start, double fin, long long n, long long& i0, long long& i1, double& t) {
+        if (n <= 1) { i0 = i1 = 0; t = 0.; return; }
+        const double step = (fin - start) / (n - 1), inv = 1. / step;
+        long long a = (long long)((arg - start) * inv);
+        if (a < 0) { a = 0; i1 = 0; }
+        else if (a >= n - 1) { a = n - 1; i1 = a; }
+        else i1 = a + 1;
+        i0 = a;
+        t = inv * (arg - a * step - start);
+    };
+    coord(e, m.eStart, m.eFin, P.ne, P.ie0, P.ie1, P.te);
+    coord(x, m.xStart, m.xFin, P.nx, P.ix0, P.ix1, P.tx);
+    coord(y, m.yStart, m.yFin, P.nz, P.iz0, P.iz1, P.tz);
+    const long long lens[7] = {P.ne, P.nx, P.nz, P.nx * P.nz, P.ne * P.nx, P.ne * P.nz, P.ne * P.nx * P.nz};
+    P.nout = lens[(int)depType];
+    const int nOut = pol == -5 ? 4 : 1;
+    const long long n = P.ne * P.nx * P.nz;
     float2* d = (float2*)g_ctx.buf.W0.get(sizeof(float2) * 2 * n);
-    float* o = (float*)g_ctx.buf.W1.get(sizeof(float) * nx * nz);
+    float* o = (float*)g_ctx.buf.W1.get(sizeof(float) * nOut * P.nout);
     if (!d || !o) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    HIPCHK(hipMemcpyAsync(d, pWfr->arEx, sizeof(float2) * n, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d + n, pWfr->arEy, sizeof(float2) * n, hipMemcpyHostToDevice, st));
-    int ie0 = 0, ie1 = 0;
-    double t = 0;
-    if (ne > 1) {
-        double eStep = (pWfr->mesh.eFin - pWfr->mesh.eStart) / (ne - 1);
-        double r = (e - pWfr->mesh.eStart) / eStep;
-        ie0 = (int)r;
-        if (ie0 < 0) ie0 = 0;
-        if (ie0 > ne - 2) ie0 = (int)ne - 2;
-        ie1 = ie0 + 1;
-        t = r - ie0;
-    }
-    if (srwp_launch_intensity(d, d + n, (int)ne, ie0, ie1, (float)t, (int)pol, (long long)nx * nz, o, st))
-        return SRWL_AMD_ERR_GPU;
-    HIPCHK(hipMemcpyAsync(pInt, o, sizeof(float) * nx * nz, hipMemcpyDeviceToHost, st));
+    float2* dx = pWfr->arEx ? d : nullptr;
+    float2* dz = pWfr->arEy ? d + n : nullptr;
+    if (dx) HIPCHK(hipMemcpyAsync(dx, pWfr->arEx, sizeof(float2) * n, hipMemcpyHostToDevice, st));
+    if (dz) HIPCHK(hipMemcpyAsync(dz, pWfr->arEy, sizeof(float2) * n, hipMemcpyHostToDevice, st));
+    if (srwx_launch_extract(dx, dz, P, o, st)) return SRWL_AMD_ERR_GPU;
+    HIPCHK(hipMemcpyAsync(pInt, o, sizeof(float) * nOut * P.nout, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }

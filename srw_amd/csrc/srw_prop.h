@@ -21,7 +21,19 @@ int drift_passes(float2* d, long long sliceStride, int nSlices, const srwf_slice
 
 }  // namespace srwamd
 
+// One extraction request of srwlCalcIntFromElecField: component what
+// (0 intensity, 3 Re E, 6 Im E) of polarisation pol (-5: the four Stokes
+// parameters one after another) vs dependence dep, with the (i0, i1, t)
+// interpolation of every coordinate that is not varied.
+struct srwx_extract {
+    int pol, what, dep, pad;
+    long long ne, nx, nz, nout;
+    long long ie0, ie1, ix0, ix1, iz0, iz1;
+    double te, tx, tz;
+};
+
 extern "C" {
+int srwx_launch_extract(const float2* ex, const float2* ez, const srwx_extract& P, float* out, hipStream_t st);
 int srwp_launch_intensity(const float2* ex, const float2* ez, int ne, int ie0, int ie1, float t, int pol,
                           long long nxy, float* out, hipStream_t st);
 }

@@ -26,7 +26,8 @@ def load():
                                                C.c_int, C.c_int, P(C.c_int), P(C.c_long)]
     lib.orc_srwlCalcPartTraj.argtypes = [P(S.SRWLPrtTrj), P(S.SRWLMagFldC), P(C.c_double)]
     lib.orc_srwlPropagElecField.argtypes = [P(S.SRWLWfr), P(S.SRWLOptC)]
-    lib.orc_srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_byte, C.c_byte, C.c_byte, C.c_double]
+    lib.orc_srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_byte, C.c_byte, C.c_byte, C.c_double,
+                                                 C.c_double, C.c_double]
     lib.orc_srwlUtiFFT.argtypes = [C.c_void_p, C.c_char, P(C.c_double), C.c_int, C.c_int]
     lib.orc_field_B.argtypes = [P(S.SRWLMagFldC), C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.orc_trj_interp.argtypes = [P(S.SRWLPrtTrj), P(C.c_double), C.c_longlong, P(C.c_double)]
@@ -132,11 +133,11 @@ def propagate(w, optc):
     return rc
 
 
-def intensity(arI, w, pol=6, int_type=0, dep_type=3, e=None):
+def intensity(arI, w, pol=6, int_type=0, dep_type=3, e=None, x=0., y=0.):
     keep = M.Keep()
     cw = M.wfr(w, keep)
     e = w.mesh.eStart if e is None else e
-    return lib.orc_srwlCalcIntFromElecField(M._addr(arI), C.byref(cw), pol, int_type, dep_type, e)
+    return lib.orc_srwlCalcIntFromElecField(M._addr(arI), C.byref(cw), pol, int_type, dep_type, e, x, y)
 
 
 def uti_fft(data, mesh, direction):
