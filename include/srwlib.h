@@ -130,6 +130,20 @@ typedef struct SRWLStructOpticsDrift {
     char treat;   /* 1: keep absolute optical path in the phase */
 } SRWLOptD;
 
+/* reference srwlib.h:312-318: aperture or obstacle ("aperture" / "obstacle") */
+typedef struct SRWLStructOpticsAperture {
+    char shape;      /* 'r' rectangular, 'c' circular */
+    char ap_or_ob;   /* 'a' aperture, 'o' obstacle */
+    double Dx, Dy;   /* transverse sizes [m]; the diameter is Dx for 'c' */
+    double x, y;     /* centre [m] */
+} SRWLOptA;
+
+/* reference srwlib.h:324-328: thin lens ("lens") */
+typedef struct SRWLStructOpticsLens {
+    double Fx, Fy;   /* focal lengths [m] */
+    double x, y;     /* centre [m] */
+} SRWLOptL;
+
 /* reference srwlib.h:523-532 */
 typedef struct SRWLStructOpticsContainer {
     void **arOpt;

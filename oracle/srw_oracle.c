@@ -1577,13 +1577,175 @@ static void orc_drift_4x4(double* P, double L)
     for (int i = 0; i < 4; i++) P[16 + i] = V[i];
 }
 
-/* srwlib.cpp:1023-1074 + sroptcnt.cpp:87-425 for drift-only containers */
+static void orc_cos_sin_f(double x, float* c, float* s);   /* sroptelm.h:856-872, defined below */
+
+/* Pointwise transmission of a thin lens (sroptfoc.h:227-242) or an
+ * aperture / obstacle (sroptapt.h:150-166, 244-258, 330-343, 425-440) over
+ * every (x, y, e) point of the wavefront. kind: 0 lens, 1 rectangular
+ * aperture, 2 rectangular obstacle, 3 circular aperture, 4 circular obstacle. */
+static void orc_point_modifier(SRWLWfr* w, int kind, double p0, double p1, double cx, double cy)
+{
+    long nx = w->mesh.nx, nz = w->mesh.ny, ne = w->mesh.ne;
+    double xStep = nx > 1 ? (w->mesh.xFin - w->mesh.xStart) / (nx - 1) : 0;
+    double zStep = nz > 1 ? (w->mesh.yFin - w->mesh.yStart) / (nz - 1) : 0;
+    double eStep = ne > 1 ? (w->mesh.eFin - w->mesh.eStart) / (ne - 1) : 0;
+    float* fx = (float*)w->arEx;
+    float* fz = (float*)w->arEy;
+    const double hx = 0.5 * p0 + 1e-10, hz = 0.5 * p1 + 1e-10, r2 = 0.25 * p0 * p0;
+    for (long iz = 0; iz < nz; iz++)
+        for (long ix = 0; ix < nx; ix++)
+            for (long ie = 0; ie < ne; ie++) {
+                double xr = w->mesh.xStart + ix * xStep - cx, zr = w->mesh.yStart + iz * zStep - cy;
+                long long o = ((long long)(iz * nx + ix) * ne + ie) * 2;
+                if (kind == 0) {
+                    double e = w->mesh.eStart + ie * eStep;
+                    double ph = -(e * 2.533865612E+06) * (xr * xr / p0 + zr * zr / p1);
+                    float c, sn;
+                    orc_cos_sin_f(ph, &c, &sn);
+                    float a = fx[o], b = fx[o + 1];
+                    fx[o] = a * c - b * sn; fx[o + 1] = a * sn + b * c;
+                    a = fz[o]; b = fz[o + 1];
+                    fz[o] = a * c - b * sn; fz[o + 1] = a * sn + b * c;
+                    continue;
+                }
+                int inside = (kind <= 2) ? (xr >= -hx && xr <= hx && zr >= -hz && zr <= hz) : (xr * xr + zr * zr <= r2);
+                int zero = (kind == 1 || kind == 3) ? !inside : inside;
+                if (zero) { fx[o] = fx[o + 1] = fz[o] = fz[o + 1] = 0.f; }
+            }
+}
+
+/* sroptelm.cpp:2094-2180 GenAuxPropagateRadMoments with the lens matrices
+ * ax = [[1, 0], [-1/Fx, 1]], az = [[1, 0], [-1/Fy, 1]] (sroptfoc.h:30-42). */
+static void orc_lens_moments(long ne, double Fx, double Fz, double* mom)
+{
+    double cx = -1. / Fx, cz = -1. / Fz;
+    for (long ie = 0; ie < ne; ie++) {
+        double* M = mom + 11 * ie;
+        double X = M[1], XP = M[2], Z = M[3], ZP = M[4];
+        double XX = M[5], XXP = M[6], XPXP = M[7], ZZ = M[8], ZZP = M[9], ZPZP = M[10];
+        M[2] = cx * X + XP;
+        M[4] = cz * Z + ZP;
+        M[6] = cx * XX + XXP;
+        M[7] = cx * cx * XX + 2. * cx * XXP + XPXP;
+        M[9] = cz * ZZ + ZZP;
+        M[10] = cz * cz * ZZ + 2. * cz * ZZP + ZPZP;
+    }
+}
+
+/* srstraux.h:331-384 (centre and rms size with the 0.1 safety floor) and the
+ * aperture's recompute test (sroptapt.cpp:248-300, sroptapt.h:204-212, 383-394):
+ * returns 1 when the 1.7-sigma extent of either component's moments leaves the
+ * opening. */
+static int orc_aperture_needs_moments(const double* mx, const double* mz, int circ, double Dx, double Dz, double cx,
+                                      double cy)
+{
+    double x0 = circ ? cx - 0.5 * Dx : cx - 0.5 * Dx, x1 = circ ? cx + 0.5 * Dx : cx + 0.5 * Dx;
+    double z0 = circ ? cy - 0.5 * Dx : cy - 0.5 * Dz, z1 = circ ? cy + 0.5 * Dx : cy + 0.5 * Dz;
+    const double* Ms[2] = {mx, mz};
+    for (int k = 0; k < 2; k++) {
+        const double* M = Ms[k];
+        double mxx = M[5] - M[1] * M[1], mzz = M[8] - M[3] * M[3];
+        if (mxx < 0.1 * M[5]) mxx = 0.1 * M[5];
+        if (mzz < 0.1 * M[8]) mzz = 0.1 * M[8];
+        double sx = 1.7 * sqrt(fabs(mxx)), sz = 1.7 * sqrt(fabs(mzz));
+        if (M[1] - sx < x0 || M[1] + sx > x1 || M[3] - sz < z0 || M[3] + sz > z1) return 1;
+    }
+    return 0;
+}
+
+/* srradstr.cpp:2610-2674: would the linear phase term be subtracted for a
+ * lens centred at (newXc, newZc)?  The library reports that case as
+ * unsupported. */
+static int orc_lin_phase_would_subtract(const SRWLWfr* w, double newXc, double newZc)
+{
+    double lambda = 3.1415926535898 / (w->mesh.eStart * 2.53384080189E+06);
+    long nx = w->mesh.nx, nz = w->mesh.ny;
+    double xr = (w->mesh.xFin - w->mesh.xStart), zr = (w->mesh.yFin - w->mesh.yStart);
+    (void)nx; (void)nz;
+    int sx = 0, sz = 0;
+    if (w->Rx != 0)
+        sx = fabs(w->Rx) > fabs(w->dRx) && 0.25 * xr * xr / (lambda * w->Rx) > 10 && fabs(newXc - w->xc) / xr > 0.2;
+    if (w->Ry != 0)
+        sz = fabs(w->Ry) > fabs(w->dRy) && 0.25 * zr * zr / (lambda * w->Ry) > 10 && fabs(newZc - w->yc) / zr > 0.2;
+    return sx || sz;
+}
+
+/* Thin lens or aperture element of a container (sroptfoc.h:111-117, 182-195;
+ * sroptapt.h:53-62; sroptel2.cpp:41-185): field, moments, wavefront radius,
+ * 4x4 matrix. */
+static int orc_point_element(SRWLWfr* w, const char* type, const void* el)
+{
+    long ne = w->mesh.ne;
+    if (strcmp(type, "lens") == 0) {
+        const SRWLOptL* L = (const SRWLOptL*)el;
+        if (orc_lin_phase_would_subtract(w, L->x, L->y)) return SRWL_AMD_ERR_UNSUPPORTED;
+        orc_point_modifier(w, 0, L->Fx, L->Fy, L->x, L->y);
+        if (w->arMomX) orc_lens_moments(ne, L->Fx, L->Fy, w->arMomX);
+        if (w->arMomY) orc_lens_moments(ne, L->Fx, L->Fy, w->arMomY);
+        double difX = (L->Fx == w->Rx) ? 1e-23 : L->Fx - w->Rx;
+        double difZ = (L->Fy == w->Ry) ? 1e-23 : L->Fy - w->Ry;
+        double mX = L->Fx / difX, mZ = L->Fy / difZ;
+        w->Rx *= mX; w->Ry *= mZ;
+        w->dRx *= mX * mX; w->dRy *= mZ * mZ;
+        w->xc = L->x - (L->x - w->xc) * mX;
+        w->yc = L->y - (L->y - w->yc) * mZ;
+        if (w->arElecPropMatr) {   /* sroptfoc.h:100-108, sroptelm.cpp:5594-5640 */
+            double* P = w->arElecPropMatr;
+            double F[16] = {1, 0, 0, 0, -1. / L->Fx, 1, 0, 0, 0, 0, 1, 0, 0, 0, -1. / L->Fy, 1};
+            double Fv[4] = {0, L->x / L->Fx, 0, L->y / L->Fy};
+            double R[16], V[4];
+            for (int i = 0; i < 4; i++) {
+                V[i] = 0;
+                for (int j = 0; j < 4; j++) {
+                    double r = 0;
+                    for (int k = 0; k < 4; k++) r += F[4 * i + k] * P[4 * k + j];
+                    R[4 * i + j] = r;
+                    V[i] += F[4 * i + j] * P[16 + j];
+                }
+            }
+            for (int i = 0; i < 16; i++) P[i] = R[i];
+            for (int i = 0; i < 4; i++) P[16 + i] = V[i] + Fv[i];
+        }
+    } else {
+        const SRWLOptA* A = (const SRWLOptA*)el;
+        int circ = (A->shape == 'c');
+        int ob = (A->ap_or_ob == 'o' || A->ap_or_ob == 'O');
+        if ((A->shape != 'r' && !circ) || (!ob && A->ap_or_ob != 'a' && A->ap_or_ob != 'A')) return SRWL_AMD_ERR_UNSUPPORTED;
+        orc_point_modifier(w, circ ? (ob ? 4 : 3) : (ob ? 2 : 1), A->Dx, A->Dy, A->x, A->y);
+        if (w->arMomX && w->arMomY) {
+            double* nmx = (double*)malloc(sizeof(double) * 22 * ne);
+            if (!nmx) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+            int any = 0;
+            for (long ie = 0; ie < ne; ie++)
+                any |= ob || orc_aperture_needs_moments(w->arMomX + 11 * ie, w->arMomY + 11 * ie, circ, A->Dx, A->Dy, A->x, A->y);
+            if (any) {
+                orc_compute_rad_moments(w, nmx, nmx + 11 * ne);
+                for (long ie = 0; ie < ne; ie++)
+                    if (ob || orc_aperture_needs_moments(w->arMomX + 11 * ie, w->arMomY + 11 * ie, circ, A->Dx, A->Dy, A->x, A->y)) {
+                        memcpy(w->arMomX + 11 * ie, nmx + 11 * ie, 11 * sizeof(double));
+                        memcpy(w->arMomY + 11 * ie, nmx + 11 * (ne + ie), 11 * sizeof(double));
+                    }
+            }
+            free(nmx);
+        }
+    }
+    /* sroptel2.cpp:147-148 */
+    if (w->Rx != 0 && w->dRx == 0) w->dRx = fabs(0.1 * w->Rx);
+    if (w->Ry != 0 && w->dRy == 0) w->dRy = fabs(0.1 * w->Ry);
+    return 0;
+}
+
+/* srwlib.cpp:1023-1074 + sroptcnt.cpp:87-425 for containers of drifts,
+ * thin lenses, apertures and obstacles */
 ORC_API int orc_srwlPropagElecField(SRWLWfr* w, SRWLOptC* opt)
 {
     if (!w || !opt) return SRWL_INCORRECT_PARAM_FOR_WFR_PROP;
     if (w->presCA != 0 || w->presFT != 0) return SRWL_AMD_ERR_UNSUPPORTED;
     for (int iel = 0; iel < opt->nElem; iel++) {
-        if (strcmp(opt->arOptTypes[iel], "drift") != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+        const char* ty = opt->arOptTypes[iel];
+        int isDrift = strcmp(ty, "drift") == 0;
+        if (!isDrift && strcmp(ty, "lens") != 0 && strcmp(ty, "aperture") != 0 && strcmp(ty, "obstacle") != 0)
+            return SRWL_AMD_ERR_UNSUPPORTED;
         const SRWLOptD* d = (const SRWLOptD*)opt->arOpt[iel];
         double* pp = (opt->arProp && iel < opt->nProp) ? opt->arProp[iel] : 0;
         int mode = 0, sub = 0;
@@ -1591,6 +1753,13 @@ ORC_API int orc_srwlPropagElecField(SRWLWfr* w, SRWLOptC* opt)
             if (pp[0] != 0 || pp[1] != 0) return SRWL_AMD_ERR_UNSUPPORTED;
             if (fabs(pp[5] - 1) > 1e-4 || fabs(pp[6] - 1) > 1e-4 || fabs(pp[7] - 1) > 1e-4 || fabs(pp[8] - 1) > 1e-4)
                 return SRWL_AMD_ERR_UNSUPPORTED;
+        }
+        if (!isDrift) {
+            int rc = orc_point_element(w, ty, opt->arOpt[iel]);
+            if (rc) return rc;
+            continue;
+        }
+        if (pp) {
             int at = (int)(char)pp[3];
             if (at == 0) mode = 0;
             else if (at == 1 || at == 2) { mode = 3; sub = at; }

@@ -158,12 +158,20 @@ def opt_cnt(oc, keep):
     ptrs = (C.c_void_p * n)()
     types = (C.c_char_p * n)()
     for i, el in enumerate(oc.arOpt):
-        if not hasattr(el, "L"):
-            raise TypeError("only drift spaces (SRWLOptD) are supported by this build")
-        d = S.SRWLOptD(float(el.L), int(getattr(el, "treat", 0)))
+        if hasattr(el, "L"):
+            d, t = S.SRWLOptD(float(el.L), int(getattr(el, "treat", 0))), b"drift"
+        elif hasattr(el, "Fx"):
+            d, t = S.SRWLOptL(float(el.Fx), float(el.Fy), float(el.x), float(el.y)), b"lens"
+        elif hasattr(el, "ap_or_ob"):
+            ob = str(el.ap_or_ob)
+            d = S.SRWLOptA(str(el.shape).encode()[:1], ob.encode()[:1], float(el.Dx), float(el.Dy),
+                           float(el.x), float(el.y))
+            t = b"obstacle" if ob in ("o", "O") else b"aperture"
+        else:
+            raise TypeError("optical element %r is not supported (drift, lens, aperture, obstacle)" % (el,))
         keep.append(d)
         ptrs[i] = C.cast(C.pointer(d), C.c_void_p)
-        types[i] = b"drift"
+        types[i] = t
     nprop = len(oc.arProp)
     props = (S.c_double_p * max(nprop, 1))()
     propn = C.create_string_buffer(max(nprop, 1))

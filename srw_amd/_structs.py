@@ -78,6 +78,15 @@ class SRWLOptD(C.Structure):
     _fields_ = [("L", C.c_double), ("treat", C.c_char)]
 
 
+class SRWLOptA(C.Structure):
+    _fields_ = [("shape", C.c_char), ("ap_or_ob", C.c_char), ("Dx", C.c_double), ("Dy", C.c_double),
+                ("x", C.c_double), ("y", C.c_double)]
+
+
+class SRWLOptL(C.Structure):
+    _fields_ = [("Fx", C.c_double), ("Fy", C.c_double), ("x", C.c_double), ("y", C.c_double)]
+
+
 class SRWLOptC(C.Structure):
     _fields_ = [("arOpt", C.POINTER(C.c_void_p)), ("arOptTypes", C.POINTER(C.c_char_p)),
                 ("nElem", C.c_int), ("arProp", C.POINTER(c_double_p)),

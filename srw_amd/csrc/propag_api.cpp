@@ -289,6 +289,76 @@ using namespace srwamd;
 
 extern "C" {
 
+// sroptelm.cpp:2094-2180 with the lens matrices ax = [[1, 0], [-1/Fx, 1]],
+// az = [[1, 0], [-1/Fy, 1]] (sroptfoc.h:30-42), for every photon energy.
+static void lens_moments(long ne, double Fx, double Fz, double* mom)
+{
+    const double cx = -1. / Fx, cz = -1. / Fz;
+    for (long ie = 0; ie < ne; ie++) {
+        double* M = mom + 11 * ie;
+        const double X = M[1], XP = M[2], Z = M[3], ZP = M[4];
+        const double XX = M[5], XXP = M[6], XPXP = M[7], ZZ = M[8], ZZP = M[9], ZPZP = M[10];
+        M[2] = cx * X + XP;
+        M[4] = cz * Z + ZP;
+        M[6] = cx * XX + XXP;
+        M[7] = cx * cx * XX + 2. * cx * XXP + XPXP;
+        M[9] = cz * ZZ + ZZP;
+        M[10] = cz * cz * ZZ + 2. * cz * ZZP + ZPZP;
+    }
+}
+
+// sroptfoc.h:100-108 and sroptelm.cpp:5594-5640: M <- F M, V <- F V + f.
+static void lens_4x4(double* P, const SRWLOptL& L)
+{
+    const double F[16] = {1., 0., 0., 0., -1. / L.Fx, 1., 0., 0., 0., 0., 1., 0., 0., 0., -1. / L.Fy, 1.};
+    const double f[4] = {0., L.x / L.Fx, 0., L.y / L.Fy};
+    double R[16], V[4];
+    for (int i = 0; i < 4; i++) {
+        V[i] = 0.;
+        for (int j = 0; j < 4; j++) {
+            double r = 0.;
+            for (int k = 0; k < 4; k++) r += F[4 * i + k] * P[4 * k + j];
+            R[4 * i + j] = r;
+            V[i] += F[4 * i + j] * P[16 + j];
+        }
+    }
+    std::memcpy(P, R, sizeof(R));
+    for (int i = 0; i < 4; i++) P[16 + i] = V[i] + f[i];
+}
+
+// srstraux.h:331-384 (centre and rms size with the 0.1 floor) and the
+// aperture's test (sroptapt.h:204-212, 383-394): true when the 1.7-sigma
+// extent of either component's moments leaves the opening.
+static bool aperture_needs_moments(const double* mx, const double* mz, bool circ, double Dx, double Dz, double cx,
+                                   double cy)
+{
+    const double hx = 0.5 * Dx, hz = circ ? 0.5 * Dx : 0.5 * Dz;
+    const double* Ms[2] = {mx, mz};
+    for (int k = 0; k < 2; k++) {
+        const double* M = Ms[k];
+        double mxx = M[5] - M[1] * M[1], mzz = M[8] - M[3] * M[3];
+        if (mxx < 0.1 * M[5]) mxx = 0.1 * M[5];
+        if (mzz < 0.1 * M[8]) mzz = 0.1 * M[8];
+        const double sx = 1.7 * std::sqrt(std::fabs(mxx)), sz = 1.7 * std::sqrt(std::fabs(mzz));
+        if (M[1] - sx < cx - hx || M[1] + sx > cx + hx || M[3] - sz < cy - hz || M[3] + sz > cy + hz) return true;
+    }
+    return false;
+}
+
+// srradstr.cpp:2610-2674: whether the reference would subtract a linear phase
+// term before a lens centred at (newXc, newZc); that case is not supported.
+static bool lin_phase_would_subtract(const SRWLWfr& w, double newXc, double newZc)
+{
+    const double lambda = 3.1415926535898 / (w.mesh.eStart * 2.53384080189E+06);
+    const double xr = w.mesh.xFin - w.mesh.xStart, zr = w.mesh.yFin - w.mesh.yStart;
+    bool sx = false, sz = false;
+    if (w.Rx != 0)
+        sx = std::fabs(w.Rx) > std::fabs(w.dRx) && 0.25 * xr * xr / (lambda * w.Rx) > 10 && std::fabs(newXc - w.xc) / xr > 0.2;
+    if (w.Ry != 0)
+        sz = std::fabs(w.Ry) > std::fabs(w.dRy) && 0.25 * zr * zr / (lambda * w.Ry) > 10 && std::fabs(newZc - w.yc) / zr > 0.2;
+    return sx || sz;
+}
+
 // Resize instructions of one arProp entry (sroptcnt.cpp:183-211: [5..8] =
 // pxm, pxd, pzm, pzd, [9..11] centring); par for resize_plan().
 static bool prop_resize_par(const double* pp, int npp, double* par)
@@ -314,7 +384,15 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
     if (nx < 2 || nz < 2 || ne < 1) return SRWL_INCORRECT_WFR_STRUCT;
     auto propN = [&](int i) -> int { return (pOpt->arPropN != nullptr) ? (int)(unsigned char)pOpt->arPropN[i] : 9; };
     for (int i = 0; i < pOpt->nElem; i++) {
-        if (std::strcmp(pOpt->arOptTypes[i], "drift") != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+        const char* ty = pOpt->arOptTypes[i];
+        if (std::strcmp(ty, "aperture") == 0 || std::strcmp(ty, "obstacle") == 0) {
+            const SRWLOptA* A = (const SRWLOptA*)pOpt->arOpt[i];
+            const char ao = A->ap_or_ob;
+            if ((A->shape != 'r' && A->shape != 'c') || (ao != 'a' && ao != 'A' && ao != 'o' && ao != 'O'))
+                return SRWL_AMD_ERR_UNSUPPORTED;
+        } else if (std::strcmp(ty, "lens") != 0 && std::strcmp(ty, "drift") != 0) {
+            return SRWL_AMD_ERR_UNSUPPORTED;
+        }
         int mode, sub;
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
         int rc = drift_mode_from_prop(pp, mode, sub);
@@ -393,11 +471,72 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
         xStep = P.x.step; zStep = P.z.step;
         return 0;
     };
+    // Thin lens, aperture or obstacle (sroptfoc.h:111-117, 182-195; sroptapt.h:53-62;
+    // sroptel2.cpp:41-185): field, moments, wavefront radius, 4x4 matrix.
+    auto point_element = [&](const char* ty, const void* el) -> int {
+        srwo_elem E{};
+        E.nx = nx; E.xStart = pWfr->mesh.xStart; E.xStep = xStep; E.zStart = pWfr->mesh.yStart; E.zStep = zStep;
+        E.eStart = pWfr->mesh.eStart; E.eStep = eStep;
+        const bool lens = std::strcmp(ty, "lens") == 0;
+        const SRWLOptL* L = (const SRWLOptL*)el;
+        const SRWLOptA* A = (const SRWLOptA*)el;
+        bool circ = false, ob = false;
+        if (lens) {
+            if (lin_phase_would_subtract(*pWfr, L->x, L->y)) return SRWL_AMD_ERR_UNSUPPORTED;
+            E.kind = 0; E.p0 = L->Fx; E.p1 = L->Fy; E.cx = L->x; E.cy = L->y;
+        } else {
+            circ = A->shape == 'c';
+            ob = A->ap_or_ob == 'o' || A->ap_or_ob == 'O';
+            E.kind = circ ? (ob ? 4 : 3) : (ob ? 2 : 1);
+            E.p0 = A->Dx; E.p1 = A->Dy; E.cx = A->x; E.cy = A->y;
+        }
+        if (srwo_launch_point_elem(dS, nxy, (int)ne, E, st)) return SRWL_AMD_ERR_GPU;
+        if (lens) {
+            lens_moments(ne, L->Fx, L->Fy, momX);
+            lens_moments(ne, L->Fx, L->Fy, momZ);
+            const double difX = (L->Fx == pWfr->Rx) ? 1E-23 : L->Fx - pWfr->Rx;
+            const double difZ = (L->Fy == pWfr->Ry) ? 1E-23 : L->Fy - pWfr->Ry;
+            const double mX = L->Fx / difX, mZ = L->Fy / difZ;
+            pWfr->Rx *= mX; pWfr->Ry *= mZ;
+            pWfr->dRx *= mX * mX; pWfr->dRy *= mZ * mZ;
+            pWfr->xc = L->x - (L->x - pWfr->xc) * mX;
+            pWfr->yc = L->y - (L->y - pWfr->yc) * mZ;
+            if (pWfr->arElecPropMatr) lens_4x4(pWfr->arElecPropMatr, *L);
+        } else if (pWfr->arMomX && pWfr->arMomY) {
+            // sroptapt.cpp:248-300: moments recomputed from the field for the
+            // energies whose 1.7-sigma extent leaves the opening (always for obstacles)
+            std::vector<char> need(ne);
+            bool any = false;
+            for (long ie = 0; ie < ne; ie++) {
+                need[ie] = ob || aperture_needs_moments(momX + 11 * ie, momZ + 11 * ie, circ, A->Dx, A->Dy, A->x, A->y);
+                any = any || need[ie];
+            }
+            if (any) {
+                std::vector<double> nm(22 * (size_t)ne);
+                if (wfr_moments(*pWfr, dS, dS + nxy, 2 * nxy, 1, nx, nm.data(), nm.data() + 11 * ne))
+                    return SRWL_AMD_ERR_GPU;
+                for (long ie = 0; ie < ne; ie++)
+                    if (need[ie]) {
+                        std::memcpy(momX + 11 * ie, nm.data() + 11 * ie, 11 * sizeof(double));
+                        std::memcpy(momZ + 11 * ie, nm.data() + 11 * (ne + ie), 11 * sizeof(double));
+                    }
+            }
+        }
+        // sroptel2.cpp:147-148
+        if (pWfr->Rx != 0 && pWfr->dRx == 0) pWfr->dRx = std::fabs(0.1 * pWfr->Rx);
+        if (pWfr->Ry != 0 && pWfr->dRy == 0) pWfr->dRy = std::fabs(0.1 * pWfr->Ry);
+        return 0;
+    };
     for (int i = 0; i < pOpt->nElem; i++) {
         const SRWLOptD* d = (const SRWLOptD*)pOpt->arOpt[i];
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
         rc = resize_device(pp, pp ? propN(i) : 0);
         if (rc) return rc;
+        if (std::strcmp(pOpt->arOptTypes[i], "drift") != 0) {
+            rc = point_element(pOpt->arOptTypes[i], pOpt->arOpt[i]);
+            if (rc) return rc;
+            continue;
+        }
         int lx, lz;
         const bool p2 = pow2_sizes(nx, nz, lx, lz);
         if (!p2 && !gen_sizes(nx, nz)) return SRWL_ERR_IN_FFT;

@@ -32,7 +32,19 @@ struct srwx_extract {
     double te, tx, tz;
 };
 
+// Pointwise element of a container on the slice-major field (slice 2 ie +
+// component holds [iz][ix]): kind 0 thin lens (p0, p1 = Fx, Fy), 1 / 2
+// rectangular aperture / obstacle (p0, p1 = Dx, Dy), 3 / 4 circular aperture /
+// obstacle (p0 = diameter); (cx, cy) the element's centre.
+struct srwo_elem {
+    int kind, pad;
+    long long nx;
+    double p0, p1, cx, cy;
+    double xStart, xStep, zStart, zStep, eStart, eStep;
+};
+
 extern "C" {
+int srwo_launch_point_elem(float2* d, long long nxy, int ne, const srwo_elem& E, hipStream_t st);
 int srwx_launch_extract(const float2* ex, const float2* ez, const srwx_extract& P, float* out, hipStream_t st);
 int srwp_launch_intensity(const float2* ex, const float2* ez, int ne, int ie0, int ie1, float t, int pol,
                           long long nxy, float* out, hipStream_t st);

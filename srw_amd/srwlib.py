@@ -168,6 +168,26 @@ class SRWLOptD(object):
         self.treat = _treat
 
 
+class SRWLOptA(object):
+    """Aperture or obstacle (reference srwlib.py:2922-2936)."""
+    def __init__(self, _shape='r', _ap_or_ob='a', _Dx=0, _Dy=0, _x=0, _y=0):
+        self.shape = _shape
+        self.ap_or_ob = _ap_or_ob
+        self.Dx = _Dx
+        self.Dy = _Dy
+        self.x = _x
+        self.y = _y
+
+
+class SRWLOptL(object):
+    """Thin lens (reference srwlib.py:2941-2951)."""
+    def __init__(self, _Fx=1e+23, _Fy=1e+23, _x=0, _y=0):
+        self.Fx = _Fx
+        self.Fy = _Fy
+        self.x = _x
+        self.y = _y
+
+
 class SRWLOptC(object):
     def __init__(self, _arOpt=None, _arProp=None):
         self.arOpt = [] if _arOpt is None else list(_arOpt)
