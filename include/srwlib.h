@@ -231,7 +231,9 @@ SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, ch
                                          double* arMeth, void* pFldTrj, double* arParGPU);
 
 /* reference srwlib.h:821 (srwlib.cpp:1023-1074): propagation through a
- * container of drift spaces ("drift"); arProp[3] = 0 (angular representation)
+ * container of drift spaces ("drift"), thin lenses ("lens", SRWLOptL) and
+ * rectangular / circular apertures and obstacles ("aperture", "obstacle",
+ * SRWLOptA); arProp[3] = 0 (angular representation)
  * or 1, 2 (quadratic phase treated analytically with the wavefront radius
  * from the moments or from Rx, Ry); arProp[5..8] resize the mesh first.
  * Mesh sizes: even, 8..2048, with factors 2, 3, 5, 7, 11, 13 (SRW's own
