@@ -216,8 +216,18 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     if ((1L << lx) != nx || (1L << lz) != nz || lx < 4 || lz < 4 || lx > 11 || lz > 11) return SRWL_ERR_IN_FFT;
     int nDrift = pOpt ? pOpt->nElem : 0;
     std::vector<int> modes(nDrift), subs(nDrift);
+    std::vector<char> isDrift(nDrift);
     for (int i = 0; i < nDrift; i++) {
-        if (std::strcmp(pOpt->arOptTypes[i], "drift") != 0) return SRWL_AMD_ERR_UNSUPPORTED;
+        const char* ty = pOpt->arOptTypes[i];
+        isDrift[i] = std::strcmp(ty, "drift") == 0;
+        if (std::strcmp(ty, "aperture") == 0 || std::strcmp(ty, "obstacle") == 0) {
+            const SRWLOptA* A = (const SRWLOptA*)pOpt->arOpt[i];
+            const char ao = A->ap_or_ob;
+            if ((A->shape != 'r' && A->shape != 'c') || (ao != 'a' && ao != 'A' && ao != 'o' && ao != 'O'))
+                return SRWL_AMD_ERR_UNSUPPORTED;
+        } else if (!isDrift[i] && std::strcmp(ty, "lens") != 0) {
+            return SRWL_AMD_ERR_UNSUPPORTED;
+        }
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
         int rc = drift_mode_from_prop(pp, modes[i], subs[i]);
         if (rc) return rc;
@@ -296,7 +306,8 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         // beam / beamline descriptor: log-scaled rms sizes, divergences, energy spread, total drift
         const double* m2 = pWfr0->partBeam.arStatMom2;
         double Ltot = 0;
-        for (int d = 0; d < nDrift; d++) Ltot += ((const SRWLOptD*)pOpt->arOpt[d])->L;
+        for (int d = 0; d < nDrift; d++)
+            if (isDrift[d]) Ltot += ((const SRWLOptD*)pOpt->arOpt[d])->L;
         float in[8] = {(float)std::log(std::sqrt(m2[0]) / 1e-5 + 1e-30), (float)std::log(std::sqrt(m2[2]) / 1e-6 + 1e-30),
                        (float)std::log(std::sqrt(m2[3]) / 1e-5 + 1e-30), (float)std::log(std::sqrt(m2[5]) / 1e-6 + 1e-30),
                        (float)std::log(std::sqrt(m2[10]) / 1e-3 + 1e-30), (float)(Ltot / 10.), 0.f, 0.f};
@@ -404,6 +415,48 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
             return wfr_moments(*m->w, m->d, m->d + m->nxy, 0, 1, m->nx, m->mx, m->mz);
         };
         for (int d = 0; d < nDrift; d++) {
+            if (!isDrift[d]) {
+                // thin lens / aperture / obstacle on each electron's own mesh
+                // (sroptfoc.h:44-70, 227-242; sroptapt.h:150-166, 244-258, 330-343,
+                // 425-440).  The oracle's per-electron wavefronts carry no
+                // moments, so a later drift that needs them computes them from
+                // the field again.
+                const char* ty = pOpt->arOptTypes[d];
+                const bool lens = std::strcmp(ty, "lens") == 0;
+                const SRWLOptL* L = (const SRWLOptL*)pOpt->arOpt[d];
+                const SRWLOptA* A = (const SRWLOptA*)pOpt->arOpt[d];
+                for (long long j = 0; j < nb; j++) {
+                    srwo_elem E{};
+                    E.nx = nx; E.xStart = em[j].xStart; E.xStep = em[j].xStep; E.zStart = em[j].zStart;
+                    E.zStep = em[j].zStep; E.eStart = pWfr0->mesh.eStart; E.eStep = 0.;
+                    if (lens) {
+                        emesh_to_wfr(j);
+                        ws[j].xc = res[j].xc; ws[j].yc = res[j].zc;
+                        if (lin_phase_would_subtract(ws[j], L->x, L->y)) return SRWL_AMD_ERR_UNSUPPORTED;
+                        E.kind = 0; E.p0 = L->Fx; E.p1 = L->Fy; E.cx = L->x; E.cy = L->y;
+                    } else {
+                        const bool circ = A->shape == 'c', ob = A->ap_or_ob == 'o' || A->ap_or_ob == 'O';
+                        E.kind = circ ? (ob ? 4 : 3) : (ob ? 2 : 1);
+                        E.p0 = A->Dx; E.p1 = A->Dy; E.cx = A->x; E.cy = A->y;
+                    }
+                    if (srwo_launch_point_elem((float2*)dF + 2 * j * nxy, nxy, 1, E, st)) return SRWL_AMD_ERR_GPU;
+                    if (lens) {
+                        const double difX = (L->Fx == ws[j].Rx) ? 1E-23 : L->Fx - ws[j].Rx;
+                        const double difZ = (L->Fy == ws[j].Ry) ? 1E-23 : L->Fy - ws[j].Ry;
+                        const double mX = L->Fx / difX, mZ = L->Fy / difZ;
+                        ws[j].Rx *= mX; ws[j].Ry *= mZ;
+                        ws[j].dRx *= mX * mX; ws[j].dRy *= mZ * mZ;
+                        res[j].xc = L->x - (L->x - res[j].xc) * mX;
+                        res[j].zc = L->y - (L->y - res[j].zc) * mZ;
+                        ws[j].xc = res[j].xc; ws[j].yc = res[j].zc;
+                        Rx[j] = ws[j].Rx; Rz[j] = ws[j].Ry;
+                    }
+                    if (ws[j].Rx != 0 && ws[j].dRx == 0) ws[j].dRx = std::fabs(0.1 * ws[j].Rx);
+                    if (ws[j].Ry != 0 && ws[j].dRy == 0) ws[j].dRy = std::fabs(0.1 * ws[j].Ry);
+                }
+                momValid = false;
+                continue;
+            }
             const SRWLOptD* D = (const SRWLOptD*)pOpt->arOpt[d];
             if (modes[d] == 3 && subs[d] == 1 && !momValid) {
                 mom.assign(22 * (size_t)nb, 0.);
@@ -459,12 +512,13 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
             rc = drift_passes((float2*)dF, nxy, (int)(2 * nb), dSp, lx, lz, modes[d], last ? 2 : 1, dI, nxy, st);
             if (rc) return rc;
         }
-        if (nDrift == 0) {
-            // intensity of the emitted field itself
-            for (long long j = 0; j < nb; j++)
-                if (srwp_launch_intensity((const float2*)(dF + 2 * off[j]), (const float2*)(dF + 2 * off[j] + 2 * nxy),
-                                          1, 0, 0, 0.f, 6, nxy, dI + j * nxy, st))
+        if (nDrift == 0 || !isDrift[nDrift - 1]) {
+            // intensity of the emitted field itself, or after a final lens / aperture
+            for (long long j = 0; j < nb; j++) {
+                const float2* ex = nDrift == 0 ? (const float2*)(dF + 2 * off[j]) : (const float2*)dF + 2 * j * nxy;
+                if (srwp_launch_intensity(ex, ex + nxy, 1, 0, 0, 0.f, 6, nxy, dI + j * nxy, st))
                     return SRWL_AMD_ERR_GPU;
+            }
         }
         HIPCHK(hipMemcpyAsync(dEm, em.data(), sizeof(srwm_emesh) * nb, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dW, w.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st));

@@ -281,6 +281,20 @@ int drift_passes(float2* d, long long sliceStride, int nSlices, const srwf_slice
     return 0;
 }
 
+// srradstr.cpp:2610-2674: whether the reference would subtract a linear phase
+// term before a lens centred at (newXc, newZc); that case is not supported.
+bool lin_phase_would_subtract(const SRWLWfr& w, double newXc, double newZc)
+{
+    const double lambda = 3.1415926535898 / (w.mesh.eStart * 2.53384080189E+06);
+    const double xr = w.mesh.xFin - w.mesh.xStart, zr = w.mesh.yFin - w.mesh.yStart;
+    bool sx = false, sz = false;
+    if (w.Rx != 0)
+        sx = std::fabs(w.Rx) > std::fabs(w.dRx) && 0.25 * xr * xr / (lambda * w.Rx) > 10 && std::fabs(newXc - w.xc) / xr > 0.2;
+    if (w.Ry != 0)
+        sz = std::fabs(w.Ry) > std::fabs(w.dRy) && 0.25 * zr * zr / (lambda * w.Ry) > 10 && std::fabs(newZc - w.yc) / zr > 0.2;
+    return sx || sz;
+}
+
 }  // namespace srwamd
 
 using namespace srwamd;
@@ -343,20 +357,6 @@ static bool aperture_needs_moments(const double* mx, const double* mz, bool circ
         if (M[1] - sx < cx - hx || M[1] + sx > cx + hx || M[3] - sz < cy - hz || M[3] + sz > cy + hz) return true;
     }
     return false;
-}
-
-// srradstr.cpp:2610-2674: whether the reference would subtract a linear phase
-// term before a lens centred at (newXc, newZc); that case is not supported.
-static bool lin_phase_would_subtract(const SRWLWfr& w, double newXc, double newZc)
-{
-    const double lambda = 3.1415926535898 / (w.mesh.eStart * 2.53384080189E+06);
-    const double xr = w.mesh.xFin - w.mesh.xStart, zr = w.mesh.yFin - w.mesh.yStart;
-    bool sx = false, sz = false;
-    if (w.Rx != 0)
-        sx = std::fabs(w.Rx) > std::fabs(w.dRx) && 0.25 * xr * xr / (lambda * w.Rx) > 10 && std::fabs(newXc - w.xc) / xr > 0.2;
-    if (w.Ry != 0)
-        sz = std::fabs(w.Ry) > std::fabs(w.dRy) && 0.25 * zr * zr / (lambda * w.Ry) > 10 && std::fabs(newZc - w.yc) / zr > 0.2;
-    return sx || sz;
 }
 
 // Resize instructions of one arProp entry (sroptcnt.cpp:183-211: [5..8] =

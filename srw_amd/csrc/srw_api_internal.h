@@ -115,6 +115,10 @@ int sr_auto_run(const std::vector<SrJob>& jobs, const std::vector<srwd_trj_info>
 
 // Wavefront moments (moments_api.cpp).  Component element (ix, iz, ie) of a
 // device wavefront is at base[ie*perE + iz*perZ + ix*perX] (float2 units).
+// srradstr.cpp:2610-2674: whether the reference would subtract a linear phase
+// term before a lens centred at (newXc, newZc) (reported as unsupported).
+bool lin_phase_would_subtract(const SRWLWfr& w, double newXc, double newZc);
+
 int wfr_moments(const SRWLWfr& w, const float2* dEx, const float2* dEz, long long perE, long long perX,
                 long long perZ, double* momX, double* momZ);
 int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums, std::vector<int>& lims,

@@ -84,3 +84,27 @@ def test_error_estimates_match_true_errors():
         est = info["rel_l2_est"]
         print("%s: %d electrons, estimated %.4f true %.4f" % (name, n, est, true), flush=True)
         assert est / 2.5 < true < est * 2.5
+
+
+@pytest.mark.parametrize("beamline", ["aperture-lens-drift", "drift-aperture"])
+def test_multi_e_with_lens_and_aperture_matches_oracle(beamline):
+    """Multi-electron sum through a beamline with a slit, a thin lens and a
+    drift (and one that ends on an aperture, so the intensity is taken after
+    it), electron by electron against the oracle on the same draws."""
+    from srw_amd import srwlpy
+    from srw_amd.srwlib import SRWLOptA, SRWLOptC, SRWLOptD, SRWLOptL
+    p0, p2 = [0, 0, 1.0, 0, 0, 1.0, 1.0, 1.0, 1.0], [0, 0, 1.0, 2, 0, 1.0, 1.0, 1.0, 1.0]
+    if beamline == "aperture-lens-drift":
+        mk = lambda: SRWLOptC([SRWLOptA('r', 'a', 1.4e-3, 1.0e-3), SRWLOptL(15., 15.), SRWLOptD(10.)], [p0, p0, p2])
+    else:
+        mk = lambda: SRWLOptC([SRWLOptD(30.), SRWLOptA('c', 'a', 3.0e-3, 0., 2e-4, 0.)], [p2, p0])
+    print("case: multi-e 64x64, 16 electrons, beamline", beamline, flush=True)
+    st_g, w0, mag, _, prec = _setup()
+    st_o, _, _, _, _ = _setup()
+    info = srwlpy.EmitPropMultiE(st_g, w0, mag, mk(), prec, 16, _seed=7, _batch=8)
+    assert oracle.multi_e(st_o, w0, mag, mk(), prec, 16, seed=7) == 0
+    a, b = np.array(st_g.arS[:48 * 48]), np.array(st_o.arS[:48 * 48])
+    err = rel_l2(a, b)
+    print("rel-L2 %.3e electrons %d" % (err, info["electrons"]), flush=True)
+    assert info["electrons"] == 16 and np.max(b) > 0
+    assert err < 2e-4
