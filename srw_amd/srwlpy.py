@@ -248,7 +248,7 @@ def ResizeElecFieldMesh(_wfr, _mesh, _par):
 
 
 def PropagElecField(_wfr, _opt_cnt, _int_list=None, _gpu_idx=None):
-    """Propagation through a container of drift spaces (reference srwlpy.cpp:5214-5320)."""
+    """Propagation through a container of drifts, thin lenses, apertures and obstacles (reference srwlpy.cpp:5214-5320)."""
     if _int_list is not None:
         raise RuntimeError(err_text(23301))
     keep = M.Keep()
