@@ -99,8 +99,11 @@ static int orc_fld_from_c(const SRWLMagFldC* c, orc_fld* f)
             u->K[k] = 93.37290417576577 * U->per * H->B;
             u->hn[k] = H->n; u->ph[k] = H->ph; u->s[k] = H->s;
         }
+        /* limits as GetMagnFieldLongLim reports them: the stored cz -/+ half
+         * (srmagfld.cpp:148-149) translated by the centre again
+         * (srmagelem.h:79-90), i.e. 2 cz -/+ half */
         double half = 0.5 * u->totLen + 4 * u->per;
-        double s0 = u->cz - half, s1 = u->cz + half;
+        double s0 = 2. * u->cz - half, s1 = 2. * u->cz + half;
         if (s0 < f->sStart) f->sStart = s0;
         if (s1 > f->sEnd) f->sEnd = s1;
     }
@@ -590,6 +593,9 @@ typedef struct {
      * relative precision and the running maximum of |E|^2 that turns the
      * convergence test absolute after the first point of a call */
     double relPrec, maxFlux, absPrec;
+    /* relative precision of the near-field edge integral (srradint.cpp:1054):
+     * 1e-4 for the manual method, relPrec/100 for the adaptive one */
+    double nearPrec;
     int sameLoop, nLvl;
     double* lvl[ORC_AUTO_MAX_LEVEL + 1];   /* 6 arrays of n: Btx, X, IBx, Btz, Z, IBz */
     long long lvlN[ORC_AUTO_MAX_LEVEL + 1];
@@ -630,10 +636,103 @@ static void orc_fun_rad_int(const orc_sr* S, double s, double xo, double yo, dou
     F[1] = az * c + I * az * sn;
 }
 
-/* srradint.cpp:398-871 (coordinate representation, far-field expansion with
- * up to 3 terms; the near-field fallback is not restated: when the expansion
- * is rejected the residual is zero and a warning is raised, and the edge
- * derivative keeps its previous value exactly like the reference). */
+/* Integrand on the field-free straight line through the edge point sc
+ * (position x0/z0, angles btx/btz, integral of the squared angles ib0 at sc):
+ * phase and amplitudes at s (srradint.cpp:1083-1091). */
+static void orc_free_term(double s, double sc, double btx, double x0, double btz, double z0, double ib0,
+                          double xo, double yo, double zo, double kph, double gEm2, double* Ph, double* Ax,
+                          double* Az)
+{
+    double inv = 1. / (yo - s);
+    double dx = xo - (x0 + btx * (s - sc)), dz = zo - (z0 + btz * (s - sc));
+    double Nx = dx * inv, Nz = dz * inv;
+    *Ph = kph * (s * gEm2 + dx * Nx + dz * Nz + (btx * btx + btz * btz) * (s - sc) + ib0);
+    *Ax = (btx - Nx) * inv;
+    *Az = (btz - Nz) * inv;
+}
+
+/* srradint.cpp:1052-1201 ComputePreResid: the radiation integral along the
+ * field-free straight line from sSt to sFi (the line passes through the edge
+ * point sc = sFi for the left edge), added with sign -1 (left) or +1 (right)
+ * to R.  Composite rule of 5 points, then level doubling until a level whose
+ * midpoint phases advance by at most 5*pi/4 changes |E|^2 by at most
+ * prec*|E|^2; the end-derivative correction uses dd, the difference of the
+ * integrand's derivative between the two ends.  The reference has no level
+ * cap; this one stops after 22 levels. */
+static void orc_pre_resid(double sSt, double sFi, double btx, double x0, double ibx0, double btz, double z0,
+                          double ibz0, const double complex* dd, double complex* R, int right,
+                          double xo, double yo, double zo, double kph, double gEm2, double actNorm, double prec)
+{
+    const double wfe = 0.46666666666667, wf1 = 1.0666666666667, wf2 = 0.93333333333333, wd = 0.066666666666667;
+    const double fivePIdFour = 1.25 * kPI_SR;
+    double sign = right ? 1. : -1., sc = right ? sSt : sFi, ib0 = ibx0 + ibz0;
+    long long npl = 5;
+    double step = (sFi - sSt) / (npl - 1);
+    double Ph, Ax, Az, c, sn;
+    double complex s1[2] = {0, 0}, s2[2] = {0, 0}, fe[2];
+    orc_free_term(sSt, sc, btx, x0, btz, z0, ib0, xo, yo, zo, kph, gEm2, &Ph, &Ax, &Az);
+    orc_cos_sin(Ph, &c, &sn);
+    fe[0] = Ax * c + I * (Ax * sn); fe[1] = Az * c + I * (Az * sn);
+    double phInit = Ph;
+    double s = sSt + step;
+    for (long long i = 0; i < ((npl - 3) >> 1); i++) {
+        orc_free_term(s, sc, btx, x0, btz, z0, ib0, xo, yo, zo, kph, gEm2, &Ph, &Ax, &Az);
+        orc_cos_sin(Ph, &c, &sn);
+        s1[0] += Ax * c + I * (Ax * sn); s1[1] += Az * c + I * (Az * sn); s += step;
+        orc_free_term(s, sc, btx, x0, btz, z0, ib0, xo, yo, zo, kph, gEm2, &Ph, &Ax, &Az);
+        orc_cos_sin(Ph, &c, &sn);
+        s2[0] += Ax * c + I * (Ax * sn); s2[1] += Az * c + I * (Az * sn); s += step;
+    }
+    orc_free_term(s, sc, btx, x0, btz, z0, ib0, xo, yo, zo, kph, gEm2, &Ph, &Ax, &Az);
+    orc_cos_sin(Ph, &c, &sn);
+    s1[0] += Ax * c + I * (Ax * sn); s1[1] += Az * c + I * (Az * sn); s += step;
+    orc_free_term(s, sc, btx, x0, btz, z0, ib0, xo, yo, zo, kph, gEm2, &Ph, &Ax, &Az);
+    orc_cos_sin(Ph, &c, &sn);
+    fe[0] += Ax * c + I * (Ax * sn); fe[1] += Az * c + I * (Az * sn);
+    fe[0] *= wfe; fe[1] *= wfe;
+    double complex wdd[2] = {wd * dd[0], wd * dd[1]}, out[2];
+    double an = sign * actNorm * step;
+    for (int k = 0; k < 2; k++) out[k] = R[k] + an * (fe[k] + wf1 * s1[k] + wf2 * s2[k] + step * wdd[k]);
+    double sq = 0.;
+    for (int k = 0; k < 2; k++) sq += creal(out[k]) * creal(out[k]) + cimag(out[k]) * cimag(out[k]);
+    npl--;
+    for (int lev = 0; lev < 22; lev++) {
+        s2[0] += s1[0]; s2[1] += s1[1]; s1[0] = s1[1] = 0;
+        int last = 1;
+        double phPrev = phInit, half = 0.5 * step;
+        s = sSt + half;
+        for (long long i = 0; i < npl; i++) {
+            orc_free_term(s, sc, btx, x0, btz, z0, ib0, xo, yo, zo, kph, gEm2, &Ph, &Ax, &Az);
+            orc_cos_sin(Ph, &c, &sn);
+            s1[0] += Ax * c + I * (Ax * sn); s1[1] += Az * c + I * (Az * sn);
+            s += step;
+            if (Ph - phPrev > fivePIdFour) last = 0;
+            phPrev = Ph;
+        }
+        double ah = sign * actNorm * half, lsq = 0.;
+        double complex loc[2];
+        for (int k = 0; k < 2; k++) {
+            loc[k] = R[k] + ah * (fe[k] + wf1 * s1[k] + wf2 * s2[k] + half * wdd[k]);
+            lsq += creal(loc[k]) * creal(loc[k]) + cimag(loc[k]) * cimag(loc[k]);
+        }
+        int notYet = 1;
+        if (last) notYet = fabs(lsq - sq) > prec * lsq;
+        out[0] = loc[0]; out[1] = loc[1]; sq = lsq;
+        step = half; npl *= 2;
+        if (!notYet) break;
+    }
+    R[0] = out[0]; R[1] = out[1];
+}
+
+/* srradint.cpp:398-871, coordinate representation: far-field expansion with
+ * up to 3 terms; when the 2-term expansion is rejected at the left edge and
+ * the field there is zero (|B| < 1e-10 T, srtrjdat.h:119) the near-field
+ * residual of :589-652 (SRWLIB enables it for the left edge only,
+ * srradint.cpp:3610-3611): the far-field term at a point sNew further
+ * upstream on the straight line plus the integral from sNew to the edge.
+ * Otherwise a rejected expansion gives zero with a warning.  On every
+ * rejection (near-field branch included) the edge derivative keeps its
+ * previous value, as in the reference (the branches return before :841). */
 static void orc_residual(orc_sr* S, double s, int computeDer, double xo, double yo, double zo,
                          double eph, double complex* R)
 {
@@ -684,6 +783,44 @@ static void orc_residual(orc_sr* S, double s, int computeDer, double xo, double 
     if (!ok3) {
         if (!ok2) fail = 1;
         else { pX = pX2; pZ = pZ2; }
+    }
+    if (fail && computeDer == 1 && fabs(Bx) < 1.E-10 && fabs(Bz) < 1.E-10) {
+        /* srradint.cpp:591-652 */
+        const double PI = kPI_SR;
+        double gEm2 = T->gammaEm2;
+        double sNew1 = s - 2. * (PI / (kph * gEm2));
+        double sNew2 = 0.5 * (yo - sqrt(yo * yo + 4. * (dx * dx + dz * dz) / (gEm2 + Btx * Btx + Btz * Btz)));
+        double sNew = (sNew1 < sNew2) ? sNew1 : sNew2;
+        double invN = 1. / (yo - sNew);
+        double dxN = xo - (X + Btx * (sNew - s)), dzN = zo - (Z + Btz * (sNew - s));
+        double bxN = Btx - dxN * invN, bzN = Btz - dzN * invN;
+        double dPhN = kph * (gEm2 + bxN * bxN + bzN * bzN);
+        double two2 = 2. * invN * invN;
+        double AxN = bxN * invN, AzN = bzN * invN;
+        double dAxN = bxN * two2, dAzN = bzN * two2;
+        double d2N = tk * (bxN * AxN + bzN * AzN);
+        double odN = 1. / dPhN, qN = d2N * odN;
+        double PhN = kph * (sNew * gEm2 + (IBx + IBz) + (Btx * Btx + Btz * Btz) * (sNew - s) +
+                            (dxN * dxN + dzN * dzN) * invN);
+        double cN, sN;
+        orc_cos_sin(PhN, &cN, &sN);
+        double complex dd[2];
+        dd[0] = ((dAxN * cN - AxN * dPhN * sN) - (dAxds * cph - Ax * dPhds * sph)) +
+                I * ((AxN * dPhN * cN + dAxN * sN) - (Ax * dPhds * cph + dAxds * sph));
+        dd[1] = ((dAzN * cN - AzN * dPhN * sN) - (dAzds * cph - Az * dPhds * sph)) +
+                I * ((AzN * dPhN * cN + dAzN * sN) - (Az * dPhds * cph + dAzds * sph));
+        double pXr = (AxN * qN - dAxN) * odN * odN, pXi = AxN * odN;
+        double pZr = (AzN * qN - dAzN) * odN * odN, pZi = AzN * odN;
+        if (fabs(pXi) < fabs(pXr) || fabs(pZi) < fabs(pZr)) {
+            R[0] = R[1] = 0;
+            S->warn = SRWL_WARN_BAD_RAD_INT_TERMINATIONS;
+            return;
+        }
+        double ac = actNorm * cN, as = actNorm * sN;
+        R[0] = (pXr * ac - pXi * as) + I * (pXi * ac + pXr * as);
+        R[1] = (pZr * ac - pZi * as) + I * (pZi * ac + pZr * as);
+        orc_pre_resid(sNew, s, Btx, X, IBx, Btz, Z, IBz, dd, R, 0, xo, yo, zo, kph, gEm2, actNorm, S->nearPrec);
+        return;
     }
     if (fail) {
         R[0] = R[1] = 0;
@@ -1062,6 +1199,7 @@ ORC_API int orc_srwlCalcElecFieldSR_ex(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
         }
         S.termin = (nPrecPar <= 0 || nPrecPar > 5) ? (int)(char)precPar[5] : 1;
         S.normConst = sqrt((1. / 137.0360411) * pWfr->partBeam.Iavg * 1.E+09 / 1.602189246E-19);
+        S.nearPrec = 0.0001;
         /* srradint.cpp:92-123 */
         if (T.gamma <= 1.) { res = SRWL_ERR_ELEC_BEAM_NOT_ULTRARELATIVISTIC; goto done1; }
         if (T.s0 < T.sStart || T.s0 > T.sStart + T.sStep * (T.np - 1)) { res = SRWL_ERR_S0_OUT_OF_FIELD_LIMITS; goto done1; }
@@ -1069,6 +1207,7 @@ ORC_API int orc_srwlCalcElecFieldSR_ex(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
         if ((yObs - S.sIntegFin) < 3. * 1.239842E-06 / m->eStart) { res = SRWL_ERR_WAVELENGTH_COMPARABLE_WITH_DISTANCE; goto done1; }
         if (meth == 1) {   /* srradint.cpp:3590-3594, 293; srradint.h:1027-1030 */
             S.relPrec = precPar[1];
+            S.nearPrec = 0.01 * precPar[1];
             S.sameLoop = 1;
             S.maxFlux = S.absPrec = 0.;
             S.sIntegStep = (S.sIntegFin - S.sIntegStart) / 4.;
@@ -1515,6 +1654,16 @@ static void orc_rad_sub1(SRWLWfr* w, double L, double* momX, double* momZ, doubl
     double ax = coefAngRange * sigXp, az = coefAngRange * sigZp;
     *pRx = orc_rad_sub1_axis(w->Rx, L, lam / (fourPi * ax * ax));
     *pRz = orc_rad_sub1_axis(w->Ry, L, lam / (fourPi * az * az));
+}
+
+/* Test hook: the drift's radius estimate of arProp[3] = 1 (sub = 1,
+ * EstimateWfrRadToSub) or 2 (sub = 2, EstimateWfrRadToSub2) for a drift of
+ * length L; out[0..1] = effective Rx, Rz; the moment arrays are updated as
+ * the estimate leaves them. */
+ORC_API void orc_rad_to_sub(SRWLWfr* w, double L, int sub, double* momX, double* momZ, double* out)
+{
+    if (sub == 2) orc_rad_sub2(w, L, out, out + 1);
+    else orc_rad_sub1(w, L, momX, momZ, out, out + 1);
 }
 
 /* sroptdrf.cpp:889-1081 with UseExactRxRz off: buffer variables of the

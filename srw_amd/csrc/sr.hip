@@ -868,9 +868,10 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
         const srwd_edge& E = edges[IT.traj];
         bool bad = false;
         if (termin == 1 || termin == 2)
-            bad |= !residual(I, E.v[0], I.sIntegStart, xo, I.yObs, zo, IT.eph, IT.actNorm, RL, DL);
+            bad |= residual(I, E.v[0], I.sIntegStart, xo, I.yObs, zo, IT.eph, IT.actNorm, RL, DL, true, 1e-4) ==
+                   SRWD_RESID_BAD;
         if (termin == 1 || termin == 3)
-            bad |= !residual(I, E.v[1], I.sIntegFin, xo, I.yObs, zo, IT.eph, IT.actNorm, RR, DR);
+            bad |= residual(I, E.v[1], I.sIntegFin, xo, I.yObs, zo, IT.eph, IT.actNorm, RR, DR) == SRWD_RESID_BAD;
         if (bad) atomicOr(warn, 1);
         double wd = IT.actNorm * h * h / 15.;
         ex.re += RR[0].re - RL[0].re + wd * (DL[0].re - DR[0].re);

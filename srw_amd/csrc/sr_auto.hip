@@ -18,24 +18,6 @@
 #include "srw_sr_dev.h"
 #include "srw_auto.h"
 
-// srradint.h:867-880 with the constants of srradint.cpp:48-60.
-__device__ __forceinline__ void cos_sin_taylor(double x, double& c, double& s)
-{
-    const double PI = 3.141592653590, TwoPI = 2. * PI, ThreePIdTwo = 1.5 * PI, HalfPI = 0.5 * PI;
-    const double One_dTwoPI = 1. / TwoPI;
-    const double a2c = -0.5, a4c = 1. / 24., a6c = -1. / 720., a8c = 1. / 40320., a10c = -1. / 3628800.;
-    const double a3s = -1. / 6., a5s = 1. / 120., a7s = -1. / 5040., a9s = 1. / 362880., a11s = -1. / 39916800.;
-    x -= TwoPI * (double)(int)(x * One_dTwoPI);
-    if (x < 0.) x += TwoPI;
-    bool chs = false;
-    if (x > ThreePIdTwo) x -= TwoPI;
-    else if (x > HalfPI) { x -= PI; chs = true; }
-    double x2 = x * x;
-    c = 1. + x2 * (a2c + x2 * (a4c + x2 * (a6c + x2 * (a8c + x2 * a10c))));
-    s = x * (1. + x2 * (a3s + x2 * (a5s + x2 * (a7s + x2 * (a9s + x2 * a11s)))));
-    if (chs) { c = -c; s = -s; }
-}
-
 // Trajectory data at s (srtrjdat.cpp:1185-1248 when byDiv, srtrjdat.h:431-472
 // otherwise), with the straight line of a plane without motion.
 __device__ __forceinline__ srwd_auto_tab auto_point(const srwd_trj_info& I, const double* X, const double* Y,
@@ -141,9 +123,10 @@ __global__ void __launch_bounds__(256) k_auto_points(const srwd_auto_job J, cons
             cplx RL[2] = {{0, 0}, {0, 0}}, RR[2] = {{0, 0}, {0, 0}}, DL[2] = {{0, 0}, {0, 0}}, DR[2] = {{0, 0}, {0, 0}};
             bool bad = false;
             if (J.termin == 1 || J.termin == 2)
-                bad |= !residual(I, edge->v[0], I.sIntegStart, xo, I.yObs, zo, eph, actNorm, RL, DL);
+                bad |= residual(I, edge->v[0], I.sIntegStart, xo, I.yObs, zo, eph, actNorm, RL, DL, true,
+                                0.01 * J.relPrec) == SRWD_RESID_BAD;
             if (J.termin == 1 || J.termin == 3)
-                bad |= !residual(I, edge->v[1], I.sIntegFin, xo, I.yObs, zo, eph, actNorm, RR, DR);
+                bad |= residual(I, edge->v[1], I.sIntegFin, xo, I.yObs, zo, eph, actNorm, RR, DR) == SRWD_RESID_BAD;
             if (bad) atomicOr(warn, 1);
             o0[0] = RR[0].re - RL[0].re; o0[1] = RR[0].im - RL[0].im;
             o0[2] = RR[1].re - RL[1].re; o0[3] = RR[1].im - RL[1].im;

@@ -73,13 +73,117 @@ __device__ __forceinline__ void cos_sin_ref(double x, double& c, double& s)
     s = sin(x); c = cos(x);
 }
 
+// srradint.h:867-880 with the constants of srradint.cpp:48-60.
+__device__ __forceinline__ void cos_sin_taylor(double x, double& c, double& s)
+{
+    const double PI = 3.141592653590, TwoPI = 2. * PI, ThreePIdTwo = 1.5 * PI, HalfPI = 0.5 * PI;
+    const double One_dTwoPI = 1. / TwoPI;
+    const double a2c = -0.5, a4c = 1. / 24., a6c = -1. / 720., a8c = 1. / 40320., a10c = -1. / 3628800.;
+    const double a3s = -1. / 6., a5s = 1. / 120., a7s = -1. / 5040., a9s = 1. / 362880., a11s = -1. / 39916800.;
+    x -= TwoPI * (double)(int)(x * One_dTwoPI);
+    if (x < 0.) x += TwoPI;
+    bool chs = false;
+    if (x > ThreePIdTwo) x -= TwoPI;
+    else if (x > HalfPI) { x -= PI; chs = true; }
+    double x2 = x * x;
+    c = 1. + x2 * (a2c + x2 * (a4c + x2 * (a6c + x2 * (a8c + x2 * a10c))));
+    s = x * (1. + x2 * (a3s + x2 * (a5s + x2 * (a7s + x2 * (a9s + x2 * a11s)))));
+    if (chs) { c = -c; s = -s; }
+}
+
 struct cplx { double re, im; };
 __device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 
-// srradint.cpp:398-871, coordinate representation, 3-term far-field expansion.
-// Returns false when the expansion is rejected (residual zero, no derivative).
-static __device__ bool residual(const srwd_trj_info& I, const double* ev, double s, double xo, double yo, double zo,
-                         double eph, double actNorm, cplx* R, cplx* D)
+// Radiation integral along the field-free straight line through the edge
+// point sc (srradint.cpp:1052-1201), from sSt to sFi, added with sign -1 to R
+// (the left edge, sc = sFi): composite rule of 5 points, then level doubling
+// until a level whose midpoint phases advance by at most 5*pi/4 changes |E|^2
+// by at most prec*|E|^2; dd is the difference of the integrand's derivative
+// between the ends.  Capped at 22 levels (the reference has no cap).
+struct free_line { double sc, btx, x0, btz, z0, ib0, xo, yo, zo, kph, gEm2; };
+
+__device__ __forceinline__ void free_term(const free_line& L, double s, double& c, double& sn, double& Ax,
+                                          double& Az, double& Ph)
+{
+    double inv = 1. / (L.yo - s);
+    double dx = L.xo - (L.x0 + L.btx * (s - L.sc)), dz = L.zo - (L.z0 + L.btz * (s - L.sc));
+    double Nx = dx * inv, Nz = dz * inv;
+    Ph = L.kph * (s * L.gEm2 + dx * Nx + dz * Nz + (L.btx * L.btx + L.btz * L.btz) * (s - L.sc) + L.ib0);
+    Ax = (L.btx - Nx) * inv;
+    Az = (L.btz - Nz) * inv;
+    cos_sin_taylor(Ph, c, sn);
+}
+
+static __device__ void left_pre_resid(const free_line& L, double sSt, double sFi, const cplx* dd, double actNorm,
+                                      double prec, cplx* R)
+{
+    const double wfe = 0.46666666666667, wf1 = 1.0666666666667, wf2 = 0.93333333333333, wd = 0.066666666666667;
+    const double fivePIdFour = 1.25 * 3.141592653590;
+    long long npl = 5;
+    double step = (sFi - sSt) / (npl - 1);
+    double c, sn, Ax, Az, Ph;
+    double fe[4], s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+    free_term(L, sSt, c, sn, Ax, Az, Ph);
+    fe[0] = Ax * c; fe[1] = Ax * sn; fe[2] = Az * c; fe[3] = Az * sn;
+    double phInit = Ph, s = sSt + step;
+    free_term(L, s, c, sn, Ax, Az, Ph);
+    s1[0] += Ax * c; s1[1] += Ax * sn; s1[2] += Az * c; s1[3] += Az * sn; s += step;
+    free_term(L, s, c, sn, Ax, Az, Ph);
+    s2[0] += Ax * c; s2[1] += Ax * sn; s2[2] += Az * c; s2[3] += Az * sn; s += step;
+    free_term(L, s, c, sn, Ax, Az, Ph);
+    s1[0] += Ax * c; s1[1] += Ax * sn; s1[2] += Az * c; s1[3] += Az * sn; s += step;
+    free_term(L, s, c, sn, Ax, Az, Ph);
+    fe[0] += Ax * c; fe[1] += Ax * sn; fe[2] += Az * c; fe[3] += Az * sn;
+    for (int m = 0; m < 4; m++) fe[m] *= wfe;
+    double wdd[4] = {wd * dd[0].re, wd * dd[0].im, wd * dd[1].re, wd * dd[1].im};
+    double r0[4] = {R[0].re, R[0].im, R[1].re, R[1].im}, out[4];
+    double an = -actNorm * step, sq = 0.;
+    for (int m = 0; m < 4; m++) {
+        out[m] = r0[m] + an * (fe[m] + wf1 * s1[m] + wf2 * s2[m] + step * wdd[m]);
+        sq += out[m] * out[m];
+    }
+    npl--;
+    for (int lev = 0; lev < 22; lev++) {
+        for (int m = 0; m < 4; m++) { s2[m] += s1[m]; s1[m] = 0.; }
+        bool last = true;
+        double phPrev = phInit, half = 0.5 * step;
+        s = sSt + half;
+        for (long long i = 0; i < npl; i++) {
+            free_term(L, s, c, sn, Ax, Az, Ph);
+            s1[0] += Ax * c; s1[1] += Ax * sn; s1[2] += Az * c; s1[3] += Az * sn;
+            s += step;
+            if (Ph - phPrev > fivePIdFour) last = false;
+            phPrev = Ph;
+        }
+        double ah = -actNorm * half, lsq = 0.;
+        for (int m = 0; m < 4; m++) {
+            out[m] = r0[m] + ah * (fe[m] + wf1 * s1[m] + wf2 * s2[m] + half * wdd[m]);
+            lsq += out[m] * out[m];
+        }
+        bool notYet = last ? (fabs(lsq - sq) > prec * lsq) : true;
+        sq = lsq;
+        step = half; npl *= 2;
+        if (!notYet) break;
+    }
+    R[0] = {out[0], out[1]};
+    R[1] = {out[2], out[3]};
+}
+
+// Result of residual(): the far-field expansion with the edge derivative set,
+// the near-field residual (no derivative, no warning), or a rejection (zero
+// residual, no derivative; the caller raises the warning).
+enum { SRWD_RESID_OK = 1, SRWD_RESID_NEAR = 0, SRWD_RESID_BAD = -1 };
+
+// srradint.cpp:398-871, coordinate representation: 3-term far-field
+// expansion.  When the 2-term expansion is rejected at the left edge
+// (left = true) and the field there is zero (|B| < 1e-10 T, srtrjdat.h:119),
+// the near-field residual of srradint.cpp:589-652, which SRWLIB enables for
+// the left edge only (srradint.cpp:3610-3611): the far-field term at a point
+// sNew further upstream on the straight line plus the integral from sNew to
+// the edge (relative precision nearPrec).
+static __device__ int residual(const srwd_trj_info& I, const double* ev, double s, double xo, double yo,
+                               double zo, double eph, double actNorm, cplx* R, cplx* D, bool left = false,
+                               double nearPrec = 1e-4)
 {
     double dBzds = ev[0], Bz = ev[1], Btx = ev[2], X = ev[3], IBx = ev[4];
     double dBxds = ev[5], Bx = ev[6], Btz = ev[7], Z = ev[8], IBz = ev[9];
@@ -109,7 +213,42 @@ static __device__ bool residual(const srwd_trj_info& I, const double* ev, double
     double t1x = Ax;
     bool ok2 = ((t2x == 0.) && (t1x == 0.)) || ((t2x != 0.) && (fabs(t2x) < 0.7 * fabs(t1x)));
     bool ok3 = ok2 && (((t3x == 0.) && (t2x == 0.)) || ((t3x != 0.) && (fabs(t3x) < 0.7 * fabs(t2x))));
-    if (!ok2) return false;
+    if (!ok2) {
+        if (!left || !(fabs(Bx) < 1.E-10) || !(fabs(Bz) < 1.E-10)) return SRWD_RESID_BAD;
+        // srradint.cpp:591-652
+        double gEm2 = I.gammaEm2;
+        double sNew1 = s - 2. * (3.141592653590 / (kph * gEm2));
+        double sNew2 = 0.5 * (yo - sqrt(yo * yo + 4. * (dx * dx + dz * dz) / (gEm2 + Btx * Btx + Btz * Btz)));
+        double sNew = (sNew1 < sNew2) ? sNew1 : sNew2;
+        double invN = 1. / (yo - sNew);
+        double dxN = xo - (X + Btx * (sNew - s)), dzN = zo - (Z + Btz * (sNew - s));
+        double bxN = Btx - dxN * invN, bzN = Btz - dzN * invN;
+        double dPhN = kph * (gEm2 + bxN * bxN + bzN * bzN);
+        double two2 = 2. * invN * invN;
+        double AxN = bxN * invN, AzN = bzN * invN;
+        double dAxN = bxN * two2, dAzN = bzN * two2;
+        double d2N = tk * (bxN * AxN + bzN * AzN);
+        double odN = 1. / dPhN, qN = d2N * odN;
+        double PhN = kph * (sNew * gEm2 + (IBx + IBz) + (Btx * Btx + Btz * Btz) * (sNew - s) +
+                            (dxN * dxN + dzN * dzN) * invN);
+        double cN, sN, c0, s0;
+        cos_sin_taylor(PhN, cN, sN);
+        cos_sin_taylor(Ph, c0, s0);
+        cplx dd[2];
+        dd[0] = {(dAxN * cN - AxN * dPhN * sN) - (dAxds * c0 - Ax * dPhds * s0),
+                 (AxN * dPhN * cN + dAxN * sN) - (Ax * dPhds * c0 + dAxds * s0)};
+        dd[1] = {(dAzN * cN - AzN * dPhN * sN) - (dAzds * c0 - Az * dPhds * s0),
+                 (AzN * dPhN * cN + dAzN * sN) - (Az * dPhds * c0 + dAzds * s0)};
+        double pXr = (AxN * qN - dAxN) * odN * odN, pXi = AxN * odN;
+        double pZr = (AzN * qN - dAzN) * odN * odN, pZi = AzN * odN;
+        if (fabs(pXi) < fabs(pXr) || fabs(pZi) < fabs(pZr)) return SRWD_RESID_BAD;
+        double ac = actNorm * cN, as = actNorm * sN;
+        R[0] = {pXr * ac - pXi * as, pXi * ac + pXr * as};
+        R[1] = {pZr * ac - pZi * as, pZi * ac + pZr * as};
+        free_line L = {s, Btx, X, Btz, Z, IBx + IBz, xo, yo, zo, kph, gEm2};
+        left_pre_resid(L, sNew, s, dd, actNorm, nearPrec, R);
+        return SRWD_RESID_NEAR;
+    }
     cplx pX = {Ax, t2x}, pZ = {Az, t2z};
     if (ok3) { pX.re += t3x; pZ.re += t3z; }
     // multiply by i/dPhds then by exp(i Ph)
@@ -119,6 +258,5 @@ static __device__ bool residual(const srwd_trj_info& I, const double* ev, double
     R[1] = {actNorm * ez.re, actNorm * ez.im};
     D[0] = cmul({dAxds, Ax * dPhds}, eip);
     D[1] = cmul({dAzds, Az * dPhds}, eip);
-    return true;
+    return SRWD_RESID_OK;
 }
-

@@ -117,8 +117,14 @@ int field_from_c(const SRWLMagFldC* c, srwd_field& f, double& gsStart, double& g
             h.sL = (ttL < 0) ? -1. : 1.;
             h.sR = (ttR > 0) ? -1. : 1.;
         }
-        gsStart = std::min(gsStart, u.cz - u.halfTotTerm);
-        gsEnd = std::max(gsEnd, u.cz + u.halfTotTerm);
+        // Longitudinal limits of the element as the container reports them:
+        // srTMagFieldPeriodic stores cz -/+ halfTotTerm (srmagfld.cpp:148-149)
+        // and GetMagnFieldLongLim translates them by the element's centre
+        // once more (srmagelem.h:79-90), so the default trajectory range of
+        // an element centred at cz is 2 cz -/+ halfTotTerm.  The field itself
+        // is placed at cz.
+        gsStart = std::min(gsStart, 2. * u.cz - u.halfTotTerm);
+        gsEnd = std::max(gsEnd, 2. * u.cz + u.halfTotTerm);
     }
     return 0;
 }

@@ -99,7 +99,14 @@ def err_text(code):
     return buf.value.decode()
 
 
+# Return code of the last library call: 0, or the first warning (< 0) of a
+# call that succeeded with warnings (the reference's srwlpy drops warnings).
+last_code = 0
+
+
 def _check(code):
+    global last_code
+    last_code = int(code)
     if code > 0:
         raise RuntimeError(err_text(code))
     return code

@@ -39,6 +39,7 @@ def load():
     lib.orc_resize_fill.argtypes = [P(S.SRWLWfr), P(S.SRWLWfr), P(C.c_double), P(C.c_double)]
     lib.orc_resize_fill_mesh.argtypes = [P(S.SRWLWfr), P(S.SRWLWfr), P(C.c_double)]
     lib.orc_compute_rad_moments.argtypes = [P(S.SRWLWfr), C.c_void_p, C.c_void_p]
+    lib.orc_rad_to_sub.argtypes = [P(S.SRWLWfr), C.c_double, C.c_int, C.c_void_p, C.c_void_p, P(C.c_double)]
     lib.orc_emit_prop_multi_e.argtypes = [P(S.SRWLStokes), P(S.SRWLWfr), P(S.SRWLMagFldC), P(S.SRWLOptC),
                                           P(C.c_double), C.c_int, C.c_longlong, C.c_ulonglong, C.c_longlong,
                                           C.c_int, P(C.c_double)]
@@ -211,3 +212,16 @@ def resize_to_mesh(w, mesh, par=(0, 1, 1)):
     cp = M.dvec(list(par), keep)
     rc = lib.orc_resize_fill_mesh(C.byref(cw), C.byref(cn), cp)
     return rc, wn
+
+
+def rad_to_sub(w, L, subtype=1):
+    """The drift's radius estimate of arProp[3] = 1 (subtype 1) or 2
+    (subtype 2); returns (effRx, effRz, arMomX, arMomY as left by it)."""
+    import numpy as np
+    keep = M.Keep()
+    cw = M.wfr(w, keep)
+    mx = np.array(w.arMomX, dtype=np.float64)
+    my = np.array(w.arMomY, dtype=np.float64)
+    out = (C.c_double * 2)()
+    lib.orc_rad_to_sub(C.byref(cw), float(L), int(subtype), mx.ctypes.data, my.ctypes.data, out)
+    return out[0], out[1], mx, my

@@ -1,0 +1,156 @@
+"""Runner of the reference SRW sources for the tests (oracle/_ref/ref_run).
+
+oracle/ref.mk compiles the reference's own C++ sources (under
+/root/reference/cpp/src, only where that tree exists) together with the
+command-line runner oracle/ref_run.cpp.  The functions here take the same
+srw_amd.srwlib objects as the oracle wrappers in tests/oracle_lib.py, run the
+reference on them in a child process (the reference keeps global state and
+is not re-entrant) and copy the results back.
+
+`available()` is False where the runner is not built; tests that need it skip.
+"""
+import os
+import subprocess
+import tempfile
+from array import array
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUNNER = os.path.join(ROOT, "oracle", "_ref", "ref_run")
+REF_SRC = "/root/reference/cpp/src"
+
+
+def build():
+    """Builds the runner when the reference sources are present."""
+    if os.path.isdir(REF_SRC):
+        subprocess.run(["make", "-j8", "-f", os.path.join("oracle", "ref.mk")], cwd=ROOT, check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+def available():
+    return os.path.exists(RUNNER)
+
+
+def _mesh_tokens(m):
+    return [m.ne, repr(float(m.eStart)), repr(float(m.eFin)), m.nx, repr(float(m.xStart)), repr(float(m.xFin)),
+            m.ny, repr(float(m.yStart)), repr(float(m.yFin)), repr(float(m.zStart))]
+
+
+def _run(tokens, timeout=600):
+    text = " ".join(str(t) for t in tokens) + "\n"
+    r = subprocess.run([RUNNER], input=text.encode(), stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError("ref_run failed (%d): %s" % (r.returncode, r.stderr.decode()[-2000:]))
+
+
+def _mag_tokens(mag):
+    tok = [len(mag.arMagFld)]
+    for i, u in enumerate(mag.arMagFld):
+        tok += [repr(float(mag.arXc[i])), repr(float(mag.arYc[i])), repr(float(mag.arZc[i])),
+                repr(float(u.per)), int(u.nPer), len(u.arHarm)]
+        for h in u.arHarm:
+            tok += [int(h.n), ord(h.h_or_v), repr(float(h.B)), repr(float(h.ph)), int(h.s), repr(float(h.a))]
+    return tok
+
+
+def calc_traj(trj, mag):
+    """srwlCalcPartTraj of the reference (srwlib.cpp:328-373) for trajectory
+    trj (np, ctStart, ctEnd, partInitCond); fills trj.arX .. arZp and the ct
+    range; returns rc."""
+    p = trj.partInitCond
+    tok = ["TRJ", None] + [repr(float(v)) for v in (p.gamma, p.x, p.y, p.z, p.xp, p.yp)]
+    tok += _mag_tokens(mag) + [int(trj.np), repr(float(trj.ctStart)), repr(float(trj.ctEnd))]
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "out.bin")
+        tok[1] = out
+        _run(tok)
+        raw = open(out, "rb").read()
+    rc = int(np.frombuffer(raw, dtype=np.int32, count=1)[0])
+    g = np.frombuffer(raw, dtype=np.float64, count=2, offset=8)
+    a = np.frombuffer(raw, dtype=np.float64, count=6 * trj.np, offset=24).reshape(6, trj.np)
+    trj.ctStart, trj.ctEnd = float(g[0]), float(g[1])
+    for k, name in enumerate(("arX", "arXp", "arY", "arYp", "arZ", "arZp")):
+        setattr(trj, name, array('d', a[k].tobytes()))
+    return rc
+
+
+def calc_field(w, mag, prec, repeat=1, timeout=600):
+    """srwlCalcElecFieldSR of the reference on wavefront w (its mesh and
+    partBeam.partStatMom1/Iavg), undulator container mag and precPar prec.
+    Fills w.arEx, w.arEy, w.Rx/Ry/dRx/dRy/xc/yc, w.arMomX/arMomY; returns
+    (rc, best wall seconds over `repeat` calls)."""
+    p = w.partBeam.partStatMom1
+    tok = ["SR", None, int(repeat)]
+    tok += [repr(float(v)) for v in (p.gamma, p.x, p.y, p.z, p.xp, p.yp, w.partBeam.Iavg)]
+    tok += _mag_tokens(mag)
+    tok += _mesh_tokens(w.mesh)
+    tok += [len(prec)] + [repr(float(v)) for v in prec]
+    m = w.mesh
+    nf = 2 * m.ne * m.nx * m.ny
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "out.bin")
+        tok[1] = out
+        _run(tok, timeout)
+        raw = open(out, "rb").read()
+    hdr = np.frombuffer(raw, dtype=np.int32, count=2)
+    g = np.frombuffer(raw, dtype=np.float64, count=7, offset=8)
+    off = 8 + 56
+    ex = np.frombuffer(raw, dtype=np.float32, count=nf, offset=off); off += 4 * nf
+    ey = np.frombuffer(raw, dtype=np.float32, count=nf, offset=off); off += 4 * nf
+    mx = np.frombuffer(raw, dtype=np.float64, count=11 * m.ne, offset=off); off += 88 * m.ne
+    my = np.frombuffer(raw, dtype=np.float64, count=11 * m.ne, offset=off)
+    w.arEx = array('f', ex.tobytes())
+    w.arEy = array('f', ey.tobytes())
+    w.Rx, w.Ry, w.dRx, w.dRy, w.xc, w.yc = (float(v) for v in g[:6])
+    w.arMomX = array('d', mx.tobytes())
+    w.arMomY = array('d', my.tobytes())
+    return int(hdr[0]), float(g[6])
+
+
+def rad_to_sub(w, L, subtype=1):
+    """srTDriftSpace::EstimateWfrRadToSub{,2}_AnalytTreatQuadPhaseTerm of the
+    reference (sroptdrf.cpp:1085-1381) on wavefront w for a drift of length L.
+    Returns (effRx, effRz, arMomX, arMomY as left by the call)."""
+    m = w.mesh
+    nf = 2 * m.ne * m.nx * m.ny
+    with tempfile.TemporaryDirectory() as td:
+        ff, fm, out = (os.path.join(td, n) for n in ("f.bin", "m.bin", "out.bin"))
+        with open(ff, "wb") as f:
+            f.write(np.asarray(w.arEx, dtype=np.float32)[:nf].tobytes())
+            f.write(np.asarray(w.arEy, dtype=np.float32)[:nf].tobytes())
+        with open(fm, "wb") as f:
+            f.write(np.asarray(w.arMomX, dtype=np.float64)[:11 * m.ne].tobytes())
+            f.write(np.asarray(w.arMomY, dtype=np.float64)[:11 * m.ne].tobytes())
+        tok = ["RADSUB", out, ff, fm, repr(float(L)), int(subtype)] + _mesh_tokens(m)
+        tok += [repr(float(v)) for v in (w.Rx, w.Ry, w.dRx, w.dRy, w.xc, w.yc, w.avgPhotEn)]
+        _run(tok)
+        raw = open(out, "rb").read()
+    eff = np.frombuffer(raw, dtype=np.float64, count=2)
+    mx = np.frombuffer(raw, dtype=np.float64, count=11 * m.ne, offset=16)
+    my = np.frombuffer(raw, dtype=np.float64, count=11 * m.ne, offset=16 + 88 * m.ne)
+    return float(eff[0]), float(eff[1]), mx.copy(), my.copy()
+
+
+def calc_int(w, pol, int_type, dep_type, e, x, y, nout, meth=None, prefill=None):
+    """srwlCalcIntFromElecField of the reference (srwlib.cpp:757-856) on
+    wavefront w; returns (rc, float32 array of nout values).  `prefill` sets
+    the output buffer's values before the call (for the averaging modes)."""
+    m = w.mesh
+    nf = 2 * m.ne * m.nx * m.ny
+    meth = [] if meth is None else list(meth)
+    pre = np.zeros(nout, dtype=np.float32) if prefill is None else np.asarray(prefill, dtype=np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        ff, out = os.path.join(td, "f.bin"), os.path.join(td, "out.bin")
+        with open(ff, "wb") as f:
+            f.write(np.asarray(w.arEx, dtype=np.float32)[:nf].tobytes())
+            f.write(np.asarray(w.arEy, dtype=np.float32)[:nf].tobytes())
+        tok = ["INT", out, ff, int(pol), int(int_type), int(dep_type), repr(float(e)), repr(float(x)),
+               repr(float(y)), len(meth)] + [repr(float(v)) for v in meth]
+        tok += _mesh_tokens(m) + [repr(float(v)) for v in (w.Rx, w.Ry, w.xc, w.yc)]
+        tok += [int(nout)] + [repr(float(v)) for v in pre]
+        _run(tok)
+        raw = open(out, "rb").read()
+    rc = int(np.frombuffer(raw, dtype=np.int32, count=1)[0])
+    return rc, np.frombuffer(raw, dtype=np.float32, count=nout, offset=8).copy()

@@ -1,0 +1,91 @@
+"""The device path against the reference SRW sources' own results on
+identical inputs: every golden case of tests/golden/ref_sr.npz (Rx/xc
+search, terminating terms with the near-field left residual, offsets, limits,
+two undulators, method 1), and the full 201 x 201 mesh of the 201-energy
+workload at 9 energies (both +/-5 % ends) against the FP64 oracle, and,
+where the reference runner is built, against the reference run live.
+
+Tolerance: single-electron fields within 1e-4 relative L2 of the
+reference computation, per energy (Ex, Ey and intensity)."""
+import numpy as np
+import pytest
+
+from srw_amd import workloads as W
+from tests import oracle_lib as oracle
+from tests import ref_lib
+from tests.golden import cases
+from tests.golden.compare import check_against_golden
+from tests.util import cfield, clone_wfr, rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+@pytest.mark.parametrize("case", cases.SR_CASES, ids=[c["name"] for c in cases.SR_CASES])
+def test_gpu_sr_vs_reference_golden(case):
+    from srw_amd import srwlpy
+    w, mag, prec = cases.sr_case(case)
+    print("case", case["name"], flush=True)
+    srwlpy.CalcElecFieldSR(w, 0, mag, prec)
+    check_against_golden(w, srwlpy.last_code, case["name"], tol=TOL, geo_rel=1e-9)
+
+
+def _per_energy(wg, wr):
+    out = []
+    exg, exr = cfield(wg.arEx, wg), cfield(wr.arEx, wr)
+    eyg, eyr = cfield(wg.arEy, wg), cfield(wr.arEy, wr)
+    for ie in range(wg.mesh.ne):
+        ig = np.abs(exg[:, :, ie]) ** 2 + np.abs(eyg[:, :, ie]) ** 2
+        ir = np.abs(exr[:, :, ie]) ** 2 + np.abs(eyr[:, :, ie]) ** 2
+        ey = rel_l2(eyg[:, :, ie], eyr[:, :, ie]) if np.linalg.norm(eyr[:, :, ie]) > 1e-6 * np.linalg.norm(
+            exr[:, :, ie]) else 0.
+        out.append((rel_l2(exg[:, :, ie], exr[:, :, ie]), ey, rel_l2(ig, ir)))
+    return out
+
+
+def test_gpu_full_mesh_201_energies_vs_oracle_and_reference():
+    """201 x 201 mesh over +/-1 mm at 20 m, energies 0.95 E1 ... 1.05 E1 on
+    the 201-point grid, of which 9 are computed here: both ends, the
+    resonance and six between (the grid's own values, so each matches the
+    201-energy workload's slice)."""
+    from srw_amd import srwlpy
+    w_full, mag, prec = W.single_electron(n=201, ne=201, de_rel=0.05)
+    e0, e1 = w_full.mesh.eStart, w_full.mesh.eFin
+    grid = np.linspace(e0, e1, 201)
+    picks = [0, 25, 50, 75, 100, 125, 150, 175, 200]
+    worst = 0.
+    for k in picks:
+        w, _, _ = W.single_electron(n=201, ne=1)
+        w.mesh.eStart = w.mesh.eFin = float(grid[k])
+        wg, wo = clone_wfr(w), clone_wfr(w)
+        srwlpy.CalcElecFieldSR(wg, 0, mag, prec)
+        assert oracle.calc_field(wo, mag, prec, nthreads=16) == 0
+        ex, ey, ei = _per_energy(wg, wo)[0]
+        print("energy %3d (%.2f eV) vs oracle: rel-L2 Ex %.2e Ey %.2e I %.2e" % (k, grid[k], ex, ey, ei),
+              flush=True)
+        assert ex < TOL and ey < TOL and ei < 2 * TOL
+        worst = max(worst, ex, ei)
+        if ref_lib.available() and k in (0, 200):
+            wr = clone_wfr(w)
+            rc, _ = ref_lib.calc_field(wr, mag, prec)
+            assert rc == 0
+            ex, ey, ei = _per_energy(wg, wr)[0]
+            print("energy %3d vs reference: rel-L2 Ex %.2e Ey %.2e I %.2e" % (k, ex, ey, ei), flush=True)
+            assert ex < TOL and ey < TOL and ei < 2 * TOL
+            assert wg.Rx == pytest.approx(wr.Rx, rel=1e-9) and wg.xc == pytest.approx(wr.xc, rel=1e-9, abs=1e-15)
+    print("worst rel-L2 over the 9 energies: %.2e" % worst)
+
+
+def test_gpu_multi_energy_call_equals_single_energy_slices():
+    """One call over 9 grid energies (a 9-energy mesh spanning the same
+    +/-5 %) equals the single-energy calls slice by slice."""
+    from srw_amd import srwlpy
+    w, mag, prec = W.single_electron(n=41, ne=9, de_rel=0.05)
+    wg = clone_wfr(w)
+    srwlpy.CalcElecFieldSR(wg, 0, mag, prec)
+    ex = cfield(wg.arEx, wg)
+    for ie in range(9):
+        w1, _, _ = W.single_electron(n=41, ne=1)
+        w1.mesh.eStart = w1.mesh.eFin = w.mesh.eStart + ie * (w.mesh.eFin - w.mesh.eStart) / 8
+        srwlpy.CalcElecFieldSR(w1, 0, mag, prec)
+        assert rel_l2(ex[:, :, ie], cfield(w1.arEx, w1)[:, :, 0]) < 1e-6
