@@ -3,11 +3,16 @@
 Workload (one step): the SR electric field of a 3 GeV electron in a planar
 undulator (lambda_u = 20 mm, 100 periods, K = 1.5), 10001-point FP64
 trajectory, observed at 20 m on a 201 x 201 mesh over +/-1 mm at 201 photon
-energies spanning +/-5 % around the first harmonic; results stay in HBM.
+energies spanning +/-5 % around the first harmonic.  `value` is the
+throughput of srwlCalcElecFieldSR as the API delivers it: the field in the
+caller's host arrays on return (the rate with the field left in HBM is
+reported beside it).
 
 Prints one JSON line with the single-electron field throughput (points/s),
-the roofline of the dominant kernel (the SR integral) and a CPU baseline
-(the FP64 oracle on the host cores, bounded sample).
+the roofline of the dominant kernel (the SR integral), the CPU baseline (the
+reference SRW sources when their runner is built, else the FP64 oracle, on
+the host cores, bounded sample) and the per-energy relative L2 error of the
+device field against the FP64 oracle and the reference.
 """
 import argparse
 import json
@@ -18,9 +23,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_VALU_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak
+FP32_VALU_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed FMA)
 HBM_PEAK_GBS = 8000.0
-FLOPS_PER_EVAL = 64                 # algorithmic FP32-equivalent flops per integrand evaluation
+# FP32 flops the k_sr_integral loop executes per integrand evaluation: two
+# phase FMAs, two amplitude FMAs and four accumulating FMAs (8 FMA = 16 flops);
+# the hardware sine and cosine are not counted.
+FLOPS_PER_EVAL = 16
 VALU_LANE_ISSUE_PER_S = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes per cycle x clock
 
 
@@ -248,19 +256,109 @@ def c1_methods(srwlpy, W, reps=20):
     return out
 
 
-def cpu_baseline(n, ne_sample, threads):
-    """FP64 oracle on a bounded sample of the same workload (fewer energies)."""
-    from tests import oracle_lib as oracle
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baselines(w_gpu, mag, prec, workers):
+    """CPU legs on bounded samples of the same workload, beside the device.
+
+    reference: the reference SRW sources' srwlCalcElecFieldSR (its runner
+    built by oracle/ref.mk), one process per worker core, each computing one
+    photon energy of the 201-energy grid on the 101 x 101 subgrid (every
+    second point) of the 201 x 201 mesh; the reference's SR integral is
+    serial, so cores scale by processes as its MPI ranks would.
+    port: the FP64 oracle (oracle/, -O3 -march=x86-64-v3), one OpenMP thread per
+    worker core, 4 energies on the 201 x 201 mesh; 1 warm-up, median of 5.
+    Both also check the device field of the timed run: relative L2 of Ex, Ey
+    and the intensity per sampled energy."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
     from srw_amd import workloads as W
-    w, mag, prec = W.single_electron(n=n, ne=ne_sample, de_rel=0.05)
-    t0 = time.perf_counter()
-    rc = oracle.calc_field(w, mag, prec, nthreads=threads)
-    dt = time.perf_counter() - t0
-    if rc > 0:
-        raise RuntimeError("oracle failed: %d" % rc)
-    pts = n * n * ne_sample
-    return {"value": pts / dt, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": "%dx%d mesh, %d photon energies of the same undulator (%.1f s)" % (n, n, ne_sample, dt)}
+    from tests import oracle_lib as oracle
+    from tests import ref_lib
+    from tests.util import cfield, clone_wfr, rel_l2
+    n, ne = w_gpu.mesh.nx, w_gpu.mesh.ne
+    eg = np.linspace(w_gpu.mesh.eStart, w_gpu.mesh.eFin, ne)
+    exg, eyg = cfield(w_gpu.arEx, w_gpu), cfield(w_gpu.arEy, w_gpu)
+
+    def errs(ie, wr, step):
+        a, b = exg[::step, ::step, ie], cfield(wr.arEx, wr)[:, :, 0]
+        c, d = eyg[::step, ::step, ie], cfield(wr.arEy, wr)[:, :, 0]
+        return {"energy_index": int(ie), "eV": float(eg[ie]), "Ex": rel_l2(a, b), "Ey": rel_l2(c, d),
+                "I": rel_l2(np.abs(a) ** 2 + np.abs(c) ** 2, np.abs(b) ** 2 + np.abs(d) ** 2)}
+
+    out = {}
+    # the reference
+    if ref_lib.available():
+        picks = sorted(set(int(round(k * (ne - 1) / max(workers - 1, 1))) for k in range(workers)))
+        sub = (n + 1) // 2
+
+        def one(ie):
+            w, _, _ = W.single_electron(n=sub, ne=1)
+            w.mesh.eStart = w.mesh.eFin = float(eg[ie])
+            rc, secs = ref_lib.calc_field(w, mag, prec)
+            if rc > 0:
+                raise RuntimeError("reference runner failed: %d" % rc)
+            return ie, w, secs
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(len(picks)) as ex:
+            runs = list(ex.map(one, picks))
+        wall = time.perf_counter() - t0
+        secs = sorted(r[2] for r in runs)
+        med = secs[len(secs) // 2]
+        out["reference"] = {"value": len(picks) * sub * sub / med, "unit": "points/s", "cores": len(picks),
+                            "kind": "reference",
+                            "sample": "%d processes, each one photon energy of the 201-energy grid on the %dx%d "
+                                      "subgrid (every second point) of the %dx%d mesh; median per-process call "
+                                      "%.2f s, wall %.2f s" % (len(picks), sub, sub, n, n, med, wall),
+                            "cpu_model": cpu_model(),
+                            "rel_l2_device_vs_reference": [errs(ie, w, 2) for ie, w, _ in runs]}
+        log("cpu baseline (reference) done")
+    # the FP64 oracle
+    picks = [0, ne // 3, (2 * ne) // 3, ne - 1]
+    w, _, _ = W.single_electron(n=n, ne=len(picks))
+    times = []
+    for rep in range(6):
+        wo = clone_wfr(w)
+        wo.mesh.eStart, wo.mesh.eFin = float(eg[picks[0]]), float(eg[picks[-1]])
+        wo.mesh.ne = 2
+        # four separate energies: one call per pair keeps the grid's own values
+        t0 = time.perf_counter()
+        res = []
+        for a, b in ((picks[0], picks[1]), (picks[2], picks[3])):
+            wp, _, _ = W.single_electron(n=n, ne=2)
+            wp.mesh.eStart, wp.mesh.eFin = float(eg[a]), float(eg[b])
+            rc = oracle.calc_field(wp, mag, prec, nthreads=workers)
+            if rc > 0:
+                raise RuntimeError("oracle failed: %d" % rc)
+            res.append(wp)
+        if rep > 0:
+            times.append(time.perf_counter() - t0)
+    med = sorted(times)[len(times) // 2]
+    port_errs = []
+    for wp, (a, b) in zip(res, ((picks[0], picks[1]), (picks[2], picks[3]))):
+        for k, ie in enumerate((a, b)):
+            w1 = clone_wfr(wp)
+            w1.mesh.ne = 1
+            w1.arEx = np.asarray(cfield(wp.arEx, wp)[:, :, k]).astype(np.complex64).view(np.float32).ravel()
+            w1.arEy = np.asarray(cfield(wp.arEy, wp)[:, :, k]).astype(np.complex64).view(np.float32).ravel()
+            port_errs.append(errs(ie, w1, 1))
+    out["port"] = {"value": len(picks) * n * n / med, "unit": "points/s", "cores": workers, "kind": "port",
+                   "sample": "4 photon energies of the 201-energy grid on the %dx%d mesh; 1 warm-up, median of 5 "
+                             "(%.2f s)" % (n, n, med),
+                   "cpu_model": cpu_model(),
+                   "build": "oracle/Makefile: gcc -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (built off the box; AVX2 is the ISA level both hosts have; no FMA contraction, as the reference)",
+                   "rel_l2_device_vs_oracle": port_errs}
+    log("cpu baseline (port) done")
+    return out
 
 
 def main():
@@ -276,7 +374,9 @@ def main():
     from srw_amd import workloads as W
 
     w, mag, prec = W.single_electron(n=a.n, ne=a.ne, de_rel=0.05)
-    srwlpy.set_keep_on_device(True)
+    pts = a.n * a.n * a.ne
+    # the API's own path: the field in the caller's host arrays on return
+    srwlpy.set_keep_on_device(False)
     srwlpy.set_timing(True)
     for _ in range(a.warmup):
         srwlpy.CalcElecFieldSR(w, 0, mag, prec)
@@ -286,26 +386,34 @@ def main():
         srwlpy.CalcElecFieldSR(w, 0, mag, prec)
     dt = time.perf_counter() - t0
     st = srwlpy.stats()
-    pts = a.n * a.n * a.ne
     value = pts * a.steps / dt
     k_ms = st["sum_sr_ms"] / max(st["n_sr"], 1)
     evals = st["sum_sr_evals"] / max(st["n_sr"], 1)
-    achieved = FLOPS_PER_EVAL * evals / (k_ms * 1e-3) / 1e12
+    fp64_points = int(st["sr_fp64_points"])
+    log("single-electron field into host arrays: %.4g points/s" % value)
+    # the same with the field left in HBM (device throughput)
+    srwlpy.set_keep_on_device(True)
+    wd = W.single_electron(n=a.n, ne=a.ne, de_rel=0.05)[0]
+    srwlpy.CalcElecFieldSR(wd, 0, mag, prec)
+    srwlpy.reset_stats()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        srwlpy.CalcElecFieldSR(wd, 0, mag, prec)
+    dt_dev = time.perf_counter() - t1
+    st_dev = srwlpy.stats()
+    srwlpy.set_keep_on_device(False)
+    k_ms_dev = st_dev["sum_sr_ms"] / max(st_dev["n_sr"], 1)
+    achieved = FLOPS_PER_EVAL * evals / (k_ms_dev * 1e-3) / 1e12
     vpe, tpe = sr_valu_per_eval(a.n, a.ne)
     # issue rate of the non-transcendental vector instructions (the transcendental
     # ones overlap them) over one wave64 instruction per 4 cycles per SIMD
-    issue = (vpe - tpe) * evals / (k_ms * 1e-3) / VALU_LANE_ISSUE_PER_S if vpe and tpe is not None else None
-    # PCIe-inclusive rate (results copied to host buffers), reported aside
-    srwlpy.set_keep_on_device(False)
-    t1 = time.perf_counter()
-    srwlpy.CalcElecFieldSR(w, 0, mag, prec)
-    pcie_rate = pts / (time.perf_counter() - t1)
-    log("single-electron field: %.4g points/s" % value)
+    issue = (vpe - tpe) * evals / (k_ms_dev * 1e-3) / VALU_LANE_ISSUE_PER_S if vpe and tpe is not None else None
     c1 = c1_methods(srwlpy, W)
     log("101x101 methods 0 and 1 done")
     prop = None
     if a.prop:
-        prop = c2_propagation(srwlpy, w)
+        import copy
+        prop = c2_propagation(srwlpy, copy.deepcopy(w))
         log("propagation leg done")
     me = multi_e_runs(srwlpy, W, a.me_n, a.me_max) if a.multi_e else None
     if me:
@@ -314,10 +422,11 @@ def main():
     if ub:
         log("uneven batch done")
     cpu = None
+    cpu_all = None
     if not a.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(a.n, a.cpu_energies, threads)
-        log("cpu baseline done")
+        workers = min(os.cpu_count() or 1, 16)
+        cpu_all = cpu_baselines(w, mag, prec, workers)
+        cpu = dict(cpu_all.get("reference") or cpu_all["port"])
     out = {
         "metric": "single-e field pts/s",
         "value": value,
@@ -330,23 +439,28 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "precision": "FP32 integrand and sums (compensated); FP64 trajectory, tables and per-chunk phase reference",
+        "precision": ("FP32 integrand, 32-point FP32 block sums added with Kahan compensation per 256-point chunk, "
+                      "FP64 chunk sums; FP64 trajectory, tables and per-chunk phase reference; points failing the "
+                      "FP32 condition check recomputed in FP64 (%d in the timed calls' last pass)" % fp64_points),
         "data": "synthetic",
         "config": {"workload": "planar undulator 3 GeV, lambda_u 20 mm x 100, K 1.5; %dx%d mesh at 20 m, %d photon "
-                               "energies +/-5%% around the 1st harmonic; 10001-point trajectory, method 0"
-                               % (a.n, a.n, a.ne),
+                               "energies +/-5%% around the 1st harmonic; 10001-point trajectory, method 0; field "
+                               "delivered into the caller's host arrays" % (a.n, a.n, a.ne),
                    "mesh": [a.n, a.n], "energies": a.ne, "trajectory_points": 10001, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": sr_traffic(a.n, a.ne),
-                     "kernel": "k_sr_integral", "kernel_ms": k_ms, "evals_per_launch": evals,
-                     "flops_per_eval": FLOPS_PER_EVAL, "valu_instr_per_eval": vpe, "valu_trans_per_eval": tpe,
-                     "valu_issue_frac": issue},
-        "pcie_inclusive_points_per_s": pcie_rate,
+                     "kernel": "k_sr_integral", "kernel_ms": k_ms_dev, "evals_per_launch": evals,
+                     "flops_per_eval": FLOPS_PER_EVAL, "flops_counted": "executed FP32 FMAs x 2 (sin/cos excluded)",
+                     "valu_instr_per_eval": vpe, "valu_trans_per_eval": tpe, "valu_issue_frac": issue,
+                     "nonpacked_fma_ceiling_frac": 0.5},
+        "device_resident_points_per_s": pts * a.steps / dt_dev,
+        "sr_kernel_ms_in_api_run": k_ms,
         "single_electron_101x101": c1,
         "propagation": prop,
         "multi_electron": me,
         "uneven_batch": ub,
         "cpu_baseline": cpu,
+        "cpu_legs": cpu_all,
     }
     print(json.dumps(out))
 

@@ -19,13 +19,14 @@ extern "C" {
  * the library's stream.  Returns 0 or an SRWL error code. */
 SRWL_EXPORT int srwlAmdSetTiming(int on);
 
-/* Copies up to n (<= 15) statistics into out: [0] last SR-kernel ms,
+/* Copies up to n (<= 16) statistics into out: [0] last SR-kernel ms,
  * [1] last integrand evaluations, [2] last SR work blocks, [3] last s-range
  * split, [4] last FFT-pass ms, [5] last FFT-pass bytes, [6] sum SR ms,
  * [7] sum SR evaluations, [8] SR calls, [9] sum FFT ms, [10] sum FFT bytes,
  * [11] FFT calls (sums since srwlAmdResetStats), [12] last resampling ms,
  * [13] last resampling bytes, [14] device ms of the last propagation
- * (after the upload, before the download). */
+ * (after the upload, before the download), [15] points of the last SR pass
+ * whose FP32 sums failed the condition check and were recomputed in FP64. */
 SRWL_EXPORT int srwlAmdGetStats(double* out, int n);
 SRWL_EXPORT int srwlAmdResetStats(void);
 

@@ -187,6 +187,7 @@ typedef struct SRWLStructOpticsContainer {
 #define SRWL_AMD_ERR_BATCH                            (302 + SRWL_FIRST_ERR)
 /* Warnings */
 #define SRWL_WARN_BAD_RAD_INT_TERMINATIONS            (12 + SRWL_WARN_OFFSET)
+#define SRWL_WARN_FLUX_ONLY_VS_PHOTON_ENERGY          (21 + SRWL_WARN_OFFSET)
 #define SRWL_WARN_ELEC_BEAM_NOT_ULTRARELATIVISTIC     (23 + SRWL_WARN_OFFSET)
 
 /* ---------------------------------------------------------------------- */

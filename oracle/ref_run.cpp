@@ -34,8 +34,8 @@
 //
 //   INT <out> <fieldfile> pol intType depType e x y nMeth meth[0..nMeth-1]
 //      ne eStart eFin nx xStart xFin ny yStart yFin zStart
-//      Rx Ry xc yc nout prefill[0..nout-1]
-//   -> int32 rc, int32 pad, float out[nout]
+//      Rx Ry xc yc nout dbl prefill[0..nout-1]
+//   -> int32 rc, int32 pad, out[nout] (double when dbl, else float)
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -202,19 +202,25 @@ int cmd_int(Reader& r) {
     read_wfr(r, w);
     w.Rx = r.d(); w.Ry = r.d(); w.xc = r.d(); w.yc = r.d();
     long nout = r.l();
-    std::vector<float> res(nout > 0 ? nout : 1);
-    for (long i = 0; i < nout; ++i) res[i] = (float)r.d();
+    int dbl = (int)r.l();
+    size_t esz = dbl ? 8 : 4;
+    std::vector<char> res(esz * (nout > 0 ? nout : 1) + 64, 0);
+    for (long i = 0; i < nout; ++i) {
+        double v = r.d();
+        if (dbl) ((double*)res.data())[i] = v;
+        else ((float*)res.data())[i] = (float)v;
+    }
     size_t nf = 2 * (size_t)w.mesh.ne * w.mesh.nx * w.mesh.ny;
     std::vector<char> fld = slurp(ff, 2 * nf * 4);
     w.arEx = fld.data(); w.arEy = fld.data() + nf * 4;
     w.numTypeElFld = 'f'; w.unitElFld = 1;
-    int rc = srwlCalcIntFromElecField((char*)res.data(), &w, pol, intType, depType, e, x, y,
+    int rc = srwlCalcIntFromElecField(res.data(), &w, pol, intType, depType, e, x, y,
                                       nMeth > 0 ? meth.data() : nullptr, nullptr);
     FILE* f = std::fopen(out.c_str(), "wb");
     if (!f) return 3;
     int hdr[2] = {rc, 0};
     put(f, hdr, sizeof hdr);
-    put(f, res.data(), (size_t)(nout > 0 ? nout : 0) * 4);
+    put(f, res.data(), esz * (size_t)(nout > 0 ? nout : 0));
     std::fclose(f);
     return 0;
 }

@@ -30,6 +30,8 @@
 ORC_API void orc_compute_rad_moments(SRWLWfr* w, double* momX, double* momZ);
 ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e,
                                          double x, double y);
+ORC_API int orc_srwlCalcIntFromElecField_m(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e,
+                                           double x, double y, const double* arMeth);
 
 /* ------------------------------------------------------------------ */
 /* Constants (reference appendix of parity-critical values)            */
@@ -1993,26 +1995,45 @@ static double orc_int_comp(double xr, double xi, double zr, double zi, int pol)
     }
 }
 
-/* srradmnp.h:367-430 IntensityComponent in float: what = 0 intensity, 3 Re,
- * 6 Im of the polarisation component (Stokes parameters ignore what). */
+/* srradmnp.h:434-454 FormalPhase: the argument of Re + i Im in (-pi, pi], the
+ * ratio formed in float as there. */
+static double orc_formal_phase(float re, float im)
+{
+    const double HalfPi = 1.5707963267949, Pi = 3.1415926535898;
+    if (re != 0.f) {
+        if (im <= 0.f) return (re < 0.f) ? atan((double)(im / re)) - Pi : atan((double)(im / re));
+        return (re < 0.f) ? atan((double)(im / re)) + Pi : atan((double)(im / re));
+    }
+    if (im == 0.f) return 0.;
+    return (im > 0.f) ? HalfPi : -HalfPi;
+}
+
+/* srradmnp.h:367-430 IntensityComponent in float.  what (the reference's
+ * internal Int_or_ReE): 0 and 1 intensity, 2 phase, 3 Re E, anything else
+ * Im E, of the polarisation component; the Stokes parameters ignore it. */
 static float orc_comp_f(const float* px, const float* pz, int pol, int what)
 {
     float xr = 0.f, xi = 0.f, zr = 0.f, zi = 0.f;   /* a missing component reads as zero */
     if (px) { xr = px[0]; xi = px[1]; }
     if (pz) { zr = pz[0]; zi = pz[1]; }
-    int inten = (what == 0);
+    int inten = (what == 0 || what == 1);
     switch (pol) {
-    case 0: return inten ? xr * xr + xi * xi : (what == 3 ? xr : xi);
-    case 1: return inten ? zr * zr + zi * zi : (what == 3 ? zr : zi);
-    case 2: { float a = xr + zr, b = xi + zi; return (float)(inten ? 0.5 * (a * a + b * b) : 0.70710678 * (what == 3 ? a : b)); }
-    case 3: { float a = xr - zr, b = xi - zi; return (float)(inten ? 0.5 * (a * a + b * b) : 0.70710678 * (what == 3 ? a : b)); }
-    case 5: { float a = xr - zi, b = xi + zr; return (float)(inten ? 0.5 * (a * a + b * b) : 0.70710678 * (what == 3 ? a : b)); }
-    case 4: { float a = xr + zi, b = xi - zr; return (float)(inten ? 0.5 * (a * a + b * b) : 0.70710678 * (what == 3 ? a : b)); }
+    case 0: return inten ? xr * xr + xi * xi : (what == 2 ? (float)orc_formal_phase(xr, xi) : (what == 3 ? xr : xi));
+    case 1: return inten ? zr * zr + zi * zi : (what == 2 ? (float)orc_formal_phase(zr, zi) : (what == 3 ? zr : zi));
+    case 2: { float a = xr + zr, b = xi + zi;
+        return (float)(inten ? 0.5 * (a * a + b * b) : (what == 2 ? orc_formal_phase(a, b) : 0.70710678 * (what == 3 ? a : b))); }
+    case 3: { float a = xr - zr, b = xi - zi;
+        return (float)(inten ? 0.5 * (a * a + b * b) : (what == 2 ? orc_formal_phase(a, b) : 0.70710678 * (what == 3 ? a : b))); }
+    case 5: { float a = xr - zi, b = xi + zr;
+        return (float)(inten ? 0.5 * (a * a + b * b) : (what == 2 ? orc_formal_phase(a, b) : 0.70710678 * (what == 3 ? a : b))); }
+    case 4: { float a = xr + zi, b = xi - zr;
+        return (float)(inten ? 0.5 * (a * a + b * b) : (what == 2 ? orc_formal_phase(a, b) : 0.70710678 * (what == 3 ? a : b))); }
     case -1: return xr * xr + xi * xi + zr * zr + zi * zi;
     case -2: return xr * xr + xi * xi - (zr * zr + zi * zi);
     case -3: return (float)(2. * (xr * zr + xi * zi));
     case -4: return (float)(2. * (xr * zi - xi * zr));
-    default: return inten ? xr * xr + xi * xi + zr * zr + zi * zi : (what == 3 ? xr : xi);
+    default: return inten ? xr * xr + xi * xi + zr * zr + zi * zi
+                          : (what == 2 ? (float)orc_formal_phase(xr, xi) : (what == 3 ? xr : xi));
     }
 }
 
@@ -2029,84 +2050,147 @@ static void orc_int_coord(double arg, double start, double step, long n, long* i
     *t = inv * (arg - a * step - start);
 }
 
-/* srradmnp.h:345-363 IntensityComponentSimpleInterpol(2D) */
-static float orc_interp1(float a, float b, double t) { return (float)((b - a) * t + a); }
-static float orc_interp2(float i00, float i10, float i01, float i11, double t1, double t2)
+/* srradmnp.h:345-363 IntensityComponentSimpleInterpol(2D); the phase is
+ * taken at the first point, without interpolation */
+static float orc_interp1w(float a, float b, double t, int what) { return what == 2 ? a : (float)((b - a) * t + a); }
+static float orc_interp2w(float i00, float i10, float i01, float i11, double t1, double t2, int what)
 {
+    if (what == 2) return i00;
     double t12 = t1 * t2;
     return (float)((i00 - i01 - i10 + i11) * t12 + (i10 - i00) * t1 + (i01 - i00) * t2 + i00);
 }
 
-/* srwlib.cpp:757-856 and srradmnp.cpp:224-1179: single-electron intensity
- * (intType 0), Re E (5) and Im E (6) of polarisation pol vs photon energy
- * (depType 0), x (1), y (2), x and y (3), energy and x (4), energy and y (5),
- * energy, x and y (6); coordinates not varied are interpolated (linearly in
- * one, bilinearly in two).  Outputs are energy fastest, then x, then y. */
-ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e,
-                                         double x, double y)
+/* gmmeth.h:171-203 Integ1D_FuncDefByArray, with its loop bounds as written:
+ * the Simpson sums start at the second interior pair, so for n >= 5 points
+ * two of them are left out; an even count ends with one trapezoid. */
+static double orc_integ1d(const double* f, long long np, double step)
+{
+    if (!f || np < 2 || step == 0) return 0;
+    if (np == 2) return 0.5 * step * (f[0] + f[1]);
+    const double* t = f + 1;
+    int even = (np == ((np >> 1) << 1));
+    long long nsim = even ? np - 1 : np;
+    double s1 = 0, s2 = 0;
+    for (long long i = 1; i < ((nsim - 3) >> 1); i++) { s1 += *(t++); s2 += *(t++); }
+    s1 += *(t++);
+    double res = (step / 3.) * (f[0] + 4. * s1 + 2. * s2 + (*t));
+    if (even) res += 0.5 * step * (f[nsim - 1] + f[nsim]);
+    return res;
+}
+
+/* srwlib.cpp:757-856 with srradmnp.h:87-130 and srradmnp.cpp:83-143, 224-1179.
+ * intType: 0 single-electron intensity, 2 single-electron flux (vs photon
+ * energy only; otherwise a warning and a zero first value), 4 phase (double
+ * output), 5 Re E, 6 Im E, 7 intensity integrated over photon energy (vs x,
+ * vs y, vs x and y when ne > 1; the plain intensity when ne = 1; for the
+ * dependence types with energy the reference's component evaluation returns
+ * Im E).  Dependence types 0 (e), 1 (x), 2 (y), 3 (x, y), 4 (e, x), 5 (e, y),
+ * 6 (e, x, y); coordinates not varied are interpolated (linearly in one,
+ * bilinearly in two).  arMeth[0] = 1 (running average over arMeth[1]
+ * previous values) or 2 (sum) applies to dependence type 3 only, as in the
+ * reference; other dependence types overwrite.  Outputs are energy fastest,
+ * then x, then y; pol = -5 gives the four Stokes parameters one block after
+ * another. */
+ORC_API int orc_srwlCalcIntFromElecField_m(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e,
+                                           double x, double y, const double* arMeth)
 {
     if (!pInt || !w) return SRWL_INCORRECT_PARAM_FOR_INT_EXTR;
-    int what = intType == 0 ? 0 : (intType == 5 ? 3 : (intType == 6 ? 6 : -1));
-    if (what < 0 || depType < 0 || depType > 6) return SRWL_AMD_ERR_UNSUPPORTED;
+    static const int conv[] = {0, 1, 4, 5, 2, 3, 6, 7, 8};   /* srwlib.cpp:782 */
+    if (intType < 0 || intType > 7 || depType < 0 || depType > 6) return SRWL_AMD_ERR_UNSUPPORTED;
+    int what = conv[(int)intType];
+    if (what == 1 || what == 5) return SRWL_AMD_ERR_UNSUPPORTED;   /* multi-electron intensity / flux */
     long nx = w->mesh.nx, nz = w->mesh.ny, ne = w->mesh.ne;
-    if (pol == -5) {   /* all four Stokes parameters, one block after another (srradmnp.cpp:236-246) */
-        const long long len[7] = {ne, nx, nz, (long long)nx * nz, (long long)ne * nx, (long long)ne * nz,
-                                  (long long)ne * nx * nz};
-        if (what != 0) return SRWL_AMD_ERR_UNSUPPORTED;
-        for (int s = 0; s < 4; s++) {
-            int rc = orc_srwlCalcIntFromElecField(pInt + sizeof(float) * s * len[(int)depType], w, (char)(-1 - s),
-                                                  intType, depType, e, x, y);
-            if (rc) return rc;
-        }
-        return 0;
-    }
     double eStep = ne > 1 ? (w->mesh.eFin - w->mesh.eStart) / (ne - 1) : 0;
     double xStep = nx > 1 ? (w->mesh.xFin - w->mesh.xStart) / (nx - 1) : 0;
     double zStep = nz > 1 ? (w->mesh.yFin - w->mesh.yStart) / (nz - 1) : 0;
+    const float* fx = (const float*)w->arEx;
+    const float* fz = (const float*)w->arEy;
+#define ORC_OFS(ie, ix, iz) ((((long long)(iz) * nx + (ix)) * ne + (ie)) * 2)
+    if (what == 4) {   /* single-electron flux vs photon energy (srradmnp.cpp:83-143, srradmnp.h:282-295) */
+        float* out = (float*)pInt;
+        if (depType != 0) { out[0] = 0.f; return SRWL_WARN_FLUX_ONLY_VS_PHOTON_ENERGY; }
+        for (long ie = 0; ie < ne; ie++) {
+            double sum = 0;
+            for (long iz = 0; iz < nz; iz++) {
+                double sx = 0.;
+                for (long ix = 0; ix < nx; ix++) {
+                    double c = orc_comp_f(fx ? fx + ORC_OFS(ie, ix, iz) : 0, fz ? fz + ORC_OFS(ie, ix, iz) : 0, pol, 0);
+                    if (ix == 0 || ix == nx - 1) c *= 0.5;
+                    sx += c;
+                }
+                if (iz == 0 || iz == nz - 1) sx *= 0.5;
+                sum += sx;
+            }
+            out[ie] = (float)(sum * xStep * zStep * 1E+06);
+        }
+        return 0;
+    }
+    const long long len[7] = {ne, nx, nz, (long long)nx * nz, (long long)ne * nx, (long long)ne * nz,
+                              (long long)ne * nx * nz};
+    const long long n = len[(int)depType];
+    const int dbl = (what == 2);
+    /* arMeth (srradmnp.cpp:742-752), dependence on x and y only */
+    double iter = 0., invIt = 1.;
+    if (arMeth && depType == 3) {
+        if (arMeth[0] == 1) { iter = arMeth[1]; invIt = 1. / (iter + 1.); }
+        else if (arMeth[0] == 2) iter = -1;
+    }
     long ie0, ie1, ix0, ix1, iz0, iz1;
     double te, tx, tz;
     orc_int_coord(e, w->mesh.eStart, eStep, ne, &ie0, &ie1, &te);
     orc_int_coord(x, w->mesh.xStart, xStep, nx, &ix0, &ix1, &tx);
     orc_int_coord(y, w->mesh.yStart, zStep, nz, &iz0, &iz1, &tz);
-    const float* fx = (const float*)w->arEx;
-    const float* fz = (const float*)w->arEy;
-#define ORC_OFS(ie, ix, iz) ((((long long)(iz) * nx + (ix)) * ne + (ie)) * 2)
-#define ORC_C(ie, ix, iz) orc_comp_f(fx ? fx + ORC_OFS(ie, ix, iz) : 0, fz ? fz + ORC_OFS(ie, ix, iz) : 0, pol, what)
-    float* out = (float*)pInt;
-    long long k = 0;
-    switch (depType) {
-    case 0:
-        for (long ie = 0; ie < ne; ie++)
-            out[k++] = orc_interp2(ORC_C(ie, ix0, iz0), ORC_C(ie, ix1, iz0), ORC_C(ie, ix0, iz1), ORC_C(ie, ix1, iz1), tx, tz);
-        break;
-    case 1:
-        for (long ix = 0; ix < nx; ix++)
-            out[k++] = orc_interp2(ORC_C(ie0, ix, iz0), ORC_C(ie1, ix, iz0), ORC_C(ie0, ix, iz1), ORC_C(ie1, ix, iz1), te, tz);
-        break;
-    case 2:
-        for (long iz = 0; iz < nz; iz++)
-            out[k++] = orc_interp2(ORC_C(ie0, ix0, iz), ORC_C(ie1, ix0, iz), ORC_C(ie0, ix1, iz), ORC_C(ie1, ix1, iz), te, tx);
-        break;
-    case 3:
-        for (long iz = 0; iz < nz; iz++)
-            for (long ix = 0; ix < nx; ix++) out[k++] = orc_interp1(ORC_C(ie0, ix, iz), ORC_C(ie1, ix, iz), te);
-        break;
-    case 4:
-        for (long ix = 0; ix < nx; ix++)
-            for (long ie = 0; ie < ne; ie++) out[k++] = orc_interp1(ORC_C(ie, ix, iz0), ORC_C(ie, ix, iz1), tz);
-        break;
-    case 5:
-        for (long iz = 0; iz < nz; iz++)
-            for (long ie = 0; ie < ne; ie++) out[k++] = orc_interp1(ORC_C(ie, ix0, iz), ORC_C(ie, ix1, iz), tx);
-        break;
-    default:
-        for (long iz = 0; iz < nz; iz++)
-            for (long ix = 0; ix < nx; ix++)
-                for (long ie = 0; ie < ne; ie++) out[k++] = ORC_C(ie, ix, iz);
-    }
+    /* fluence over photon energy (srradmnp.cpp:329, 372-420, 505, 550-600, 686, 733-860) */
+    int intOverE = (what == 7) && (ne > 1) && (depType >= 1 && depType <= 3);
+    int whatC = (what == 7 && depType >= 1 && depType <= 3) ? 0 : what;
+    double* aux = intOverE ? (double*)malloc(sizeof(double) * ne) : 0;
+    const int nStk = pol == -5 ? 4 : 1;
+    for (int sk = 0; sk < nStk; sk++) {
+        const int pl = pol == -5 ? -1 - sk : pol;
+#define ORC_C(ie, ix, iz) orc_comp_f(fx ? fx + ORC_OFS(ie, ix, iz) : 0, fz ? fz + ORC_OFS(ie, ix, iz) : 0, pl, whatC)
+        for (long long k = 0; k < n; k++) {
+            double v;
+            if (intOverE) {
+                for (long ie = 0; ie < ne; ie++) {
+                    if (depType == 1) aux[ie] = orc_interp1w(ORC_C(ie, k, iz0), ORC_C(ie, k, iz1), tz, 0);
+                    else if (depType == 2) aux[ie] = orc_interp1w(ORC_C(ie, ix0, k), ORC_C(ie, ix1, k), tx, 0);
+                    else aux[ie] = ORC_C(ie, k % nx, k / nx);
+                }
+                v = orc_integ1d(aux, ne, eStep);
+            } else {
+                switch (depType) {
+                case 0: v = orc_interp2w(ORC_C(k, ix0, iz0), ORC_C(k, ix1, iz0), ORC_C(k, ix0, iz1), ORC_C(k, ix1, iz1), tx, tz, whatC); break;
+                case 1: v = orc_interp2w(ORC_C(ie0, k, iz0), ORC_C(ie1, k, iz0), ORC_C(ie0, k, iz1), ORC_C(ie1, k, iz1), te, tz, whatC); break;
+                case 2: v = orc_interp2w(ORC_C(ie0, ix0, k), ORC_C(ie1, ix0, k), ORC_C(ie0, ix1, k), ORC_C(ie1, ix1, k), te, tx, whatC); break;
+                case 3: v = orc_interp1w(ORC_C(ie0, k % nx, k / nx), ORC_C(ie1, k % nx, k / nx), te, whatC); break;
+                case 4: v = orc_interp1w(ORC_C(k % ne, k / ne, iz0), ORC_C(k % ne, k / ne, iz1), tz, whatC); break;
+                case 5: v = orc_interp1w(ORC_C(k % ne, ix0, k / ne), ORC_C(k % ne, ix1, k / ne), tx, whatC); break;
+                default: { long long ie = k % ne, r = k / ne; v = ORC_C(ie, r % nx, r / nx); }
+                }
+            }
+            if (dbl) {
+                double* o = (double*)pInt + sk * n + k;
+                if (iter == 0) *o = v;
+                else if (iter > 0) *o = ((*o) * iter + v) * invIt;
+                else *o += v;
+            } else {
+                float* o = (float*)pInt + sk * n + k;
+                if (iter == 0) *o = (float)v;
+                else if (iter > 0) *o = (float)(((*o) * iter + v) * invIt);
+                else *o += (float)v;
+            }
+        }
 #undef ORC_C
+    }
 #undef ORC_OFS
+    free(aux);
     return 0;
+}
+
+ORC_API int orc_srwlCalcIntFromElecField(char* pInt, SRWLWfr* w, char pol, char intType, char depType, double e,
+                                         double x, double y)
+{
+    return orc_srwlCalcIntFromElecField_m(pInt, w, pol, intType, depType, e, x, y, 0);
 }
 
 /* Layout probe: sizes of the C structures as compiled from include/srwlib.h,

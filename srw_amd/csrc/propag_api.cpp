@@ -722,21 +722,46 @@ SRWL_EXPORT int srwlAmdGetFftTiles(int lognx, int lognz, int mode, int postOp, i
 SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, char intType, char depType, double e,
                                          double x, double y, double* arMeth, void* pFldTrj, double* arParGPU)
 {
-    (void)arMeth; (void)arParGPU;
+    (void)arParGPU;
     if (pWfr == nullptr || pInt == nullptr) return SRWL_INCORRECT_PARAM_FOR_INT_EXTR;
-    // single-electron intensity (0), Re E (5), Im E (6); srwlib.cpp:781-792
-    const int what = intType == 0 ? 0 : (intType == 5 ? 3 : (intType == 6 ? 6 : -1));
-    if (what < 0 || depType < 0 || depType > 6 || pFldTrj != nullptr) return SRWL_AMD_ERR_UNSUPPORTED;
-    if (pol < -5 || pol > 6 || (pol == -5 && what != 0)) return SRWL_AMD_ERR_UNSUPPORTED;
+    // srwlib.cpp:782: intType -> the reference's internal Int_or_ReE
+    static const int conv[] = {0, 1, 4, 5, 2, 3, 6, 7, 8};
+    if (intType < 0 || intType > 7 || depType < 0 || depType > 6 || pFldTrj != nullptr) return SRWL_AMD_ERR_UNSUPPORTED;
+    const int what = conv[(int)intType];
+    // multi-electron intensity and flux (convolution with the electron beam) are not in this build
+    if (what == 1 || what == 5) return SRWL_AMD_ERR_UNSUPPORTED;
+    if (pol < -5 || pol > 6) return SRWL_AMD_ERR_UNSUPPORTED;
     if (pWfr->arEx == nullptr && pWfr->arEy == nullptr) return SRWL_INCORRECT_WFR_STRUCT;
     const SRWLRadMesh& m = pWfr->mesh;
     if (m.nx <= 0 || m.ny <= 0 || m.ne <= 0) return SRWL_INCORRECT_WFR_STRUCT;
+    if (what == 4 && depType != 0) {
+        // srradmnp.h:284-288: flux only vs photon energy
+        *(float*)pInt = 0.f;
+        return SRWL_WARN_FLUX_ONLY_VS_PHOTON_ENERGY;
+    }
     int rc = ensure_device();
     if (rc) return rc;
     hipStream_t st = g_ctx.stream;
     srwx_extract P{};
     P.pol = pol; P.what = what; P.dep = depType;
+    P.dbl = (what == 2);
     P.ne = m.ne; P.nx = m.nx; P.nz = m.ny;
+    P.eStep = m.ne > 1 ? (m.eFin - m.eStart) / (m.ne - 1) : 0.;
+    P.xStep = m.nx > 1 ? (m.xFin - m.xStart) / (m.nx - 1) : 0.;
+    P.zStep = m.ny > 1 ? (m.yFin - m.yStart) / (m.ny - 1) : 0.;
+    // intensity integrated over photon energy: vs x, vs y, vs x and y with
+    // ne > 1 (srradmnp.cpp:372, 550, 733); there the components are
+    // intensities (srradmnp.cpp:329, 505, 686)
+    if (what == 7 && depType >= 1 && depType <= 3) {
+        P.intOverE = m.ne > 1;
+        P.what = 0;
+    }
+    // srradmnp.cpp:742-752: arMeth applies to the dependence on x and y
+    P.iter = 0.; P.invIter = 1.;
+    if (arMeth && depType == 3) {
+        if (arMeth[0] == 1) { P.iter = arMeth[1]; P.invIter = 1. / (P.iter + 1.); }
+        else if (arMeth[0] == 2) P.iter = -1.;
+    }
     // srradmnp.cpp:35-79: cell index and fractional offset of an off-mesh coordinate
     auto coord = [](double arg, double start, double fin, long long n, long long& i0, long long& i1, double& t) {
         if (n <= 1) { i0 = i1 = 0; t = 0.; return; }
@@ -753,17 +778,22 @@ SRWL_EXPORT int srwlCalcIntFromElecField(char* pInt, SRWLWfr* pWfr, char pol, ch
     coord(y, m.yStart, m.yFin, P.nz, P.iz0, P.iz1, P.tz);
     const long long lens[7] = {P.ne, P.nx, P.nz, P.nx * P.nz, P.ne * P.nx, P.ne * P.nz, P.ne * P.nx * P.nz};
     P.nout = lens[(int)depType];
-    const int nOut = pol == -5 ? 4 : 1;
+    const int nOut = (pol == -5 && what != 4) ? 4 : 1;
+    const size_t esz = P.dbl ? sizeof(double) : sizeof(float);
+    const size_t outBytes = esz * nOut * P.nout;
     const long long n = P.ne * P.nx * P.nz;
     float2* d = (float2*)g_ctx.buf.W0.get(sizeof(float2) * 2 * n);
-    float* o = (float*)g_ctx.buf.W1.get(sizeof(float) * nOut * P.nout);
-    if (!d || !o) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    void* o = g_ctx.buf.W1.get(outBytes);
+    double* rows = (double*)g_ctx.buf.W2.get(sizeof(double) * (size_t)(P.ne * P.nz));
+    if (!d || !o || !rows) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     float2* dx = pWfr->arEx ? d : nullptr;
     float2* dz = pWfr->arEy ? d + n : nullptr;
     if (dx) HIPCHK(hipMemcpyAsync(dx, pWfr->arEx, sizeof(float2) * n, hipMemcpyHostToDevice, st));
     if (dz) HIPCHK(hipMemcpyAsync(dz, pWfr->arEy, sizeof(float2) * n, hipMemcpyHostToDevice, st));
-    if (srwx_launch_extract(dx, dz, P, o, st)) return SRWL_AMD_ERR_GPU;
-    HIPCHK(hipMemcpyAsync(pInt, o, sizeof(float) * nOut * P.nout, hipMemcpyDeviceToHost, st));
+    // the averaging and sum modes update the caller's previous values
+    if (P.iter != 0.) HIPCHK(hipMemcpyAsync(o, pInt, outBytes, hipMemcpyHostToDevice, st));
+    if (srwx_launch_extract(dx, dz, P, o, rows, st)) return SRWL_AMD_ERR_GPU;
+    HIPCHK(hipMemcpyAsync(pInt, o, what == 4 ? sizeof(float) * P.ne : outBytes, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }

@@ -10,7 +10,9 @@
 //
 // Precision: trajectory, tables and the per-chunk phase reference are FP64;
 // the per-point phase increment (|t| < ~2 revolutions) and the integrand sum
-// are FP32, with Kahan-compensated accumulation of 16-point block sums.
+// are FP32: 64-point FMA block sums, added with Kahan compensation within a
+// 256-point chunk, chunk sums in FP64.  A condition check of every point's
+// FP32 sum sends ill-conditioned points to an FP64 recomputation.
 #include "srw_sr_dev.h"
 
 // ------------------------------------------------------------------------
@@ -724,8 +726,18 @@ __global__ void __launch_bounds__(256) k_phase_table(const double* __restrict__ 
 // gam dV (revolutions), with Tc the point's FP64 phase at the chunk reference
 // reduced to [-1/2, 1/2]; the loop leaves Tc out and each chunk sum is turned
 // by exp(2 pi i Tc) once.
-#define SRWD_BLOCK 32
+#define SRWD_BLOCK 64
 #define SRWD_ROWS 4
+
+// Kahan summation step: sum += v, with c holding the negated low-order part
+// lost so far (the compensated total is sum - c).
+__device__ __forceinline__ void kahan_add(float& sum, float& c, float v)
+{
+    float y = v - c;
+    float t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+}
 static_assert(SRWD_CHUNK == 256, "k_sr_integral stages one chunk per 256-thread block");
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_sr_integral(const srwd_item* __restrict__ items,
@@ -743,7 +755,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     const long long flat = IT.pointBase + local;
     const bool valid = local < IT.npts;
     const srwd_trj_info& I = info[IT.traj];
-    int ix = valid ? local % IT.nxc : 0, iz = valid ? local / IT.nxc : 0;
+    int ix = valid ? local % IT.nxc : 0, iz = valid ? IT.iz0 + local / IT.nxc : 0;
     double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
     double kph = SRWD_KPH(IT.eph) * (0.5 / M_PI);
     double r2 = xo * xo + zo * zo;
@@ -765,7 +777,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     __shared__ float4 sA[SRWD_ROWS][SRWD_CHUNK];
     const int first = (blockIdx.x - IT.blockBase) * 256;
     const int last = min(first + 255, IT.npts - 1);
-    const int rLo = first / IT.nxc, nRows = last / IT.nxc - rLo + 1;
+    const int rLo = IT.iz0 + first / IT.nxc, nRows = IT.iz0 + last / IT.nxc - rLo + 1;
     const bool byRow = nRows <= SRWD_ROWS;
     const int rs = byRow && valid ? iz - rLo : 0;
     const float nfxo = -fxo, nfzo = -fzo;
@@ -792,9 +804,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
             }
         }
         __syncthreads();
-        // chunk sums of b (P - r_o) exp(2 pi i (t - Tc)): 32-point FP32 block
-        // sums added in FP32, then turned by exp(2 pi i Tc) into FP64
+        // chunk sums of b (P - r_o) exp(2 pi i (t - Tc)): 64-point FP32 block
+        // sums added with Kahan compensation (c* carry the lost low parts),
+        // then turned by exp(2 pi i Tc) into FP64
         float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
         const float4* S = sA[rs];
         for (int b0 = 0; b0 < n; b0 += SRWD_BLOCK) {
             float bx = 0.f, bxi = 0.f, bz = 0.f, bzi = 0.f;
@@ -825,9 +839,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
                     bz = fmaf(waz, cs, bz); bzi = fmaf(waz, sn, bzi);
                 }
             }
-            q0 += bx; q1 += bxi; q2 += bz; q3 += bzi;
+            kahan_add(q0, c0, bx); kahan_add(q1, c1, bxi); kahan_add(q2, c2, bz); kahan_add(q3, c3, bzi);
         }
-        double rxr = q0, rxi = q1, rzr = q2, rzi = q3;
+        double rxr = (double)q0 - (double)c0, rxi = (double)q1 - (double)c1;
+        double rzr = (double)q2 - (double)c2, rzi = (double)q3 - (double)c3;
         double cT = __builtin_amdgcn_cosf(Tc), sT = __builtin_amdgcn_sinf(Tc);
         ox += rxr * cT - rxi * sT; oxi += rxr * sT + rxi * cT;
         oz += rzr * cT - rzi * sT; ozi += rzr * sT + rzi * cT;
@@ -840,32 +855,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 // ------------------------------------------------------------------------
 // Finalisation: partial sums, terminating terms (srw_sr_dev.h), symmetry
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict__ items,
-                                                     const int* __restrict__ blockItem,
-                                                     const srwd_trj_info* __restrict__ info,
-                                                     const srwd_edge* __restrict__ edges, int termin,
-                                                     const double* __restrict__ part, int nSplit,
-                                                     long long nPointsPad, float2* Ex, float2* Ez, int* warn)
+// Error model of the FP32 sums.  Each term's phase is formed in FP32 within
+// two revolutions of its chunk reference and its sine and cosine come from
+// the hardware FP32 instructions, so a term carries an error of about
+// SRWD_EPS_TERM times its magnitude; over many s points these add like a
+// random walk, to SRWD_EPS_TERM * sqrt(sum of squared terms).  A point whose
+// field (terminating terms included) is not at least SRWD_COND_MAX times
+// that size is recomputed in FP64 (k_sr_fp64): the estimate then stays below
+// 1e-5 of the point's field.
+#define SRWD_EPS_TERM 4e-6
+#define SRWD_COND_MAX 1e-5
+
+// Terminating terms, the derivative correction, the float store and the
+// mirrored parts of one point; S = sum over the integration points of
+// w_i * (A_x, A_z) * exp(i phi) with w_i in units of h/15.
+__device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, const srwd_edge& E, int termin,
+                                int ix, int iz, const double* S, float2* Ex, float2* Ez, int* warn,
+                                double sumSq, int* fpList, int* fpCount, int listEntry)
 {
-    const int it = blockItem[blockIdx.x];
-    const srwd_item& IT = items[it];
-    const int local = (blockIdx.x - IT.blockBase) * 256 + threadIdx.x;
-    if (local >= IT.npts) return;
-    const long long flat = IT.pointBase + local;
-    const srwd_trj_info& I = info[IT.traj];
-    int ix = local % IT.nxc, iz = local / IT.nxc;
     double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
-    double S[4] = {0, 0, 0, 0};
-    for (int k = 0; k < nSplit; k++) {
-        const double* p = part + ((long long)k * nPointsPad + flat) * 4;
-        S[0] += p[0]; S[1] += p[1]; S[2] += p[2]; S[3] += p[3];
-    }
     double h = I.h;
     double w = IT.actNorm * h / 15.;
     cplx ex = {w * S[0], w * S[1]}, ez = {w * S[2], w * S[3]};
     if (termin > 0) {
         cplx RL[2] = {{0, 0}, {0, 0}}, RR[2] = {{0, 0}, {0, 0}}, DL[2] = {{0, 0}, {0, 0}}, DR[2] = {{0, 0}, {0, 0}};
-        const srwd_edge& E = edges[IT.traj];
         bool bad = false;
         if (termin == 1 || termin == 2)
             bad |= residual(I, E.v[0], I.sIntegStart, xo, I.yObs, zo, IT.eph, IT.actNorm, RL, DL, true, 1e-4) ==
@@ -878,6 +891,16 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
         ex.im += RR[0].im - RL[0].im + wd * (DL[0].im - DR[0].im);
         ez.re += RR[1].re - RL[1].re + wd * (DL[1].re - DR[1].re);
         ez.im += RR[1].im - RL[1].im + wd * (DL[1].im - DR[1].im);
+    }
+    if (fpList) {
+        // condition check of the FP32 sum: hand the point to the FP64 pass
+        double mag2 = ex.re * ex.re + ex.im * ex.im + ez.re * ez.re + ez.im * ez.im;
+        double err = SRWD_EPS_TERM * w * sqrt(fmax(sumSq, 0.));
+        if (err * err > SRWD_COND_MAX * SRWD_COND_MAX * mag2) {
+            int k = atomicAdd(fpCount, 1);
+            fpList[k] = listEntry;
+            return;
+        }
     }
     float2 vx = make_float2((float)ex.re, (float)ex.im), vz = make_float2((float)ez.re, (float)ez.im);
     long long ne = IT.ne;
@@ -893,6 +916,116 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
     if (mz) put(ix, IT.nz - 1 - iz, 1.f, -1.f);
     if (mx) put(IT.nx - 1 - ix, iz, -1.f, 1.f);
     if (mz && mx) put(IT.nx - 1 - ix, IT.nz - 1 - iz, -1.f, -1.f);
+}
+
+// Sums of squared terms per trajectory, from the integration records:
+// cond[0..3] = sum b^2, sum b bPx, sum b bPz, sum (bPx^2 + bPz^2), so that the
+// squared terms of point (xo, zo) add up to
+// cond[0] (xo^2 + zo^2) - 2 xo cond[1] - 2 zo cond[2] + cond[3].
+__global__ void __launch_bounds__(256) k_cond_sums(const srwd_trj_info* __restrict__ info,
+                                                   const srwd_rec* __restrict__ rec, long long ldI, double* cond)
+{
+    int tr = blockIdx.x;
+    long long n = info[tr].nInt;
+    const srwd_rec* R = rec + (long long)tr * ldI;
+    double a = 0, b = 0, c = 0, d = 0;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+        double bb = R[i].b, px = R[i].bPx, pz = R[i].bPz;
+        a += bb * bb; b += bb * px; c += bb * pz; d += px * px + pz * pz;
+    }
+    __shared__ double sh[4][256];
+    sh[0][threadIdx.x] = a; sh[1][threadIdx.x] = b; sh[2][threadIdx.x] = c; sh[3][threadIdx.x] = d;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) cond[4 * tr + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict__ items,
+                                                     const int* __restrict__ blockItem,
+                                                     const srwd_trj_info* __restrict__ info,
+                                                     const srwd_edge* __restrict__ edges, int termin,
+                                                     const double* __restrict__ part, int nSplit,
+                                                     long long nPointsPad, float2* Ex, float2* Ez, int* warn,
+                                                     const double* __restrict__ cond, int* fpList, int* fpCount)
+{
+    const int it = blockItem[blockIdx.x];
+    const srwd_item& IT = items[it];
+    const int local = (blockIdx.x - IT.blockBase) * 256 + threadIdx.x;
+    if (local >= IT.npts) return;
+    const long long flat = IT.pointBase + local;
+    const srwd_trj_info& I = info[IT.traj];
+    int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
+    double S[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nSplit; k++) {
+        const double* p = part + ((long long)k * nPointsPad + flat) * 4;
+        S[0] += p[0]; S[1] += p[1]; S[2] += p[2]; S[3] += p[3];
+    }
+    double sumSq = 0.;
+    if (cond) {
+        const double* c = cond + 4 * IT.traj;
+        double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
+        sumSq = c[0] * (xo * xo + zo * zo) - 2. * (xo * c[1] + zo * c[2]) + c[3];
+    }
+    sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, S, Ex, Ez, warn, sumSq, cond ? fpList : nullptr,
+                    fpCount, (int)(blockIdx.x * 256 + threadIdx.x));
+}
+
+// FP64 recomputation of the points the condition check handed over: one wave
+// per point, the integration points split over the lanes, the trajectory
+// interpolated at each point as the reference's manual methods do
+// (srradint.cpp:1214-1258, 1379-1604), the phase reduced and its sine and
+// cosine taken with the reference's Taylor series.  A fixed grid of waves
+// walks the list (its length is read on the device).
+__global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ items,
+                                                 const int* __restrict__ blockItem,
+                                                 const srwd_trj_info* __restrict__ info,
+                                                 const srwd_edge* __restrict__ edges, int termin,
+                                                 const double* X, const double* Y, const double* IBx,
+                                                 const double* IBz, long long ldT, float2* Ex, float2* Ez,
+                                                 int* warn, const int* __restrict__ fpList,
+                                                 const int* __restrict__ fpCount)
+{
+    const int lane = threadIdx.x & 63;
+    const int nWaves = gridDim.x * (blockDim.x >> 6);
+    const int count = *fpCount;
+    for (int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < count; q += nWaves) {
+        const int entry = fpList[q];
+        const int blk = entry >> 8, thr = entry & 255;
+        const srwd_item& IT = items[blockItem[blk]];
+        const int local = (blk - IT.blockBase) * 256 + thr;
+        const srwd_trj_info I = info[IT.traj];
+        int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
+        double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
+        double kph = SRWD_KPH(IT.eph);
+        bool byDiv = I.nInt < 10000;
+        const double* cx = X + IT.traj * ldT; const double* cz = Y + IT.traj * ldT;
+        const double* bx = IBx + IT.traj * ldT; const double* bz = IBz + IT.traj * ldT;
+        double bmx = -1. / I.betaNorm, bmz = 1. / I.betaNorm;
+        double acc[4] = {0, 0, 0, 0};
+        for (long long i = lane; i < I.nInt; i += 64) {
+            double sv = (i == I.nInt - 1) ? I.sIntegFin : I.sIntegStart + (double)i * I.h;
+            double Xv, Zv, Btx, Btz, IBxv, IBzv;
+            if (I.verNZ) { trj_pt p = eval_plane(I, cx, bx, bmx, sv, byDiv); Xv = p.crd; Btx = p.bt; IBxv = p.ib; }
+            else { double d = sv - I.s0; Btx = I.dxds0; Xv = I.x0 + I.dxds0 * d; IBxv = I.dxds0 * I.dxds0 * d; }
+            if (I.horNZ) { trj_pt p = eval_plane(I, cz, bz, bmz, sv, byDiv); Zv = p.crd; Btz = p.bt; IBzv = p.ib; }
+            else { double d = sv - I.s0; Btz = I.dzds0; Zv = I.z0 + I.dzds0 * d; IBzv = I.dzds0 * I.dzds0 * d; }
+            double inv = 1. / (I.yObs - sv), dx = xo - Xv, dz = zo - Zv;
+            double ph = kph * (sv * I.gammaEm2 + (IBxv + IBzv) + (dx * dx + dz * dz) * inv);
+            double c, sn;
+            cos_sin_taylor(ph, c, sn);
+            double ax = (Btx - dx * inv) * inv, az = (Btz - dz * inv) * inv;
+            double wt = (i == 0 || i == I.nInt - 1) ? 7. : ((i & 1) ? 16. : 14.);
+            acc[0] += wt * ax * c; acc[1] += wt * ax * sn; acc[2] += wt * az * c; acc[3] += wt * az * sn;
+        }
+        for (int m = 0; m < 4; m++)
+            for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off);
+        if (lane == 0)
+            sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., nullptr, nullptr, 0);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -952,10 +1085,28 @@ int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, 
 
 int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                          const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
-                         long long nPointsPad, float* dEx, float* dEz, int* dWarn, hipStream_t st)
+                         long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
+                         int* dFpList, int* dFpCount, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sr_finalize, dim3(nBlocks), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin,
-                       dPart, nSplit, nPointsPad, (float2*)dEx, (float2*)dEz, dWarn);
+                       dPart, nSplit, nPointsPad, (float2*)dEx, (float2*)dEz, dWarn, dCond, dFpList, dFpCount);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec* dRec, long long ldI, double* dCond,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cond_sums, dim3(nTraj), dim3(256), 0, st, dInfo, dRec, ldI, dCond);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
+                        const srwd_edge* dEdge, int termin, const double* dX, const double* dY, const double* dIBx,
+                        const double* dIBz, long long ldT, float* dEx, float* dEz, int* dWarn, const int* dFpList,
+                        const int* dFpCount, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sr_fp64, dim3(256), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin, dX, dY,
+                       dIBx, dIBz, ldT, (float2*)dEx, (float2*)dEz, dWarn, dFpList, dFpCount);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

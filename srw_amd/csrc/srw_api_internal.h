@@ -21,6 +21,7 @@ struct DBuf {
 struct Buffers {
     DBuf X, Xp, Y, Yp, IBx, IBz, Samp, Jobs, Fields, Info;
     DBuf Rec, Chunk, Q, Edge, Mesh, Rx, Items, BlockItem, RowTraj, RowE, RowN, A, Part, Warn;
+    DBuf Cond, FpList, FpCount;   // condition check of the FP32 sums and its FP64 recomputation list
     DBuf Ex, Ez;                  // field outputs
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
     DBuf Stokes, Draws, Wt, Net, Grad, Rep;
@@ -57,6 +58,17 @@ struct Context {
     double lastRsMs = 0, lastRsBytes = 0;   // last resampling of a propagation
     double lastPropDevMs = 0;               // device part of the last propagation (after upload, before download)
     int* warnHost = nullptr;                // pinned: termination warning of the last SR pass
+    // Streamed download of field results: a second stream copies finished
+    // row bands into pinned staging while the next band is computed; the
+    // calling thread moves them on into the caller's arrays.
+    hipStream_t copyStream = nullptr;
+    char* pin = nullptr;                    // pinned staging (grow-only)
+    size_t pinBytes = 0;
+    std::vector<hipEvent_t> bandEv;         // per band: computed, downloaded, SR start, SR end
+    int nTimePairs = 0;                     // timed SR launches of the last pass (bandEv[4b+2], [4b+3])
+    int* fpCountHost = nullptr;             // pinned: points recomputed in FP64, per band (at most 64 bands)
+    int fpBands = 0;
+    long long lastFp64Points = 0;           // points of the last SR pass recomputed in FP64
     long long pendingEvals = 0;             // integrand evaluations of the last SR pass
     std::vector<int> autoLevels;            // final levels of the last method-1 pass, in point order
     int autoSweeps = 0;                     // fixed-point sweeps of the last method-1 pass (largest chunk)
@@ -99,9 +111,22 @@ struct SrResult {
 int field_from_c(const SRWLMagFldC* c, srwd_field& f, double& gsStart, double& gsEnd);
 int setup_job_from_field(SrJob& J, const SRWLParticle& p, const SRWLMagFldC* mag, const double* precPar,
                          std::vector<srwd_field>& fields);
+// Host destination of a single request's field, filled band by band while
+// later bands are computed (see sr_run).
+struct HostOut {
+    float* ex = nullptr;
+    float* ez = nullptr;
+};
 int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
            const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin,
-           bool defer = false, bool geomOnly = false);
+           bool defer = false, bool geomOnly = false, const HostOut* hostOut = nullptr);
+// Pinned staging of at least `bytes` (grow-only); nullptr on failure.
+char* pinned_staging(size_t bytes);
+// Copies device memory to host memory through the pinned staging on the copy
+// stream, in pieces, after the work already queued on the compute stream.
+int download(void* dst, const void* src, size_t bytes);
+// Host memcpy split over several threads (large copies out of pinned memory).
+void par_memcpy(void* dst, const void* src, size_t bytes);
 // Mesh size of one axis for the automatic mesh of precPar[6] > 0.
 long sr_auto_mesh_size(double start, double fin, long n, double c, double R, double lambda, double f);
 // Completes a deferred sr_run once the stream has been synchronised.

@@ -85,6 +85,7 @@ struct srwd_item {
     double xStart, xStep, zStart, zStep;
     int nx, nz;               // full mesh
     int nxc, nzc;             // computed region (symmetry)
+    int iz0;                  // first mesh row of this item (items may cover a band of the computed rows)
     int symX, symZ;
     int ne, ie;               // output layout: energies per point and this energy's index
     long long outOffset;      // float2 offset of (iz=0, ix=0, ie) in the output arrays
@@ -122,7 +123,20 @@ int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj
 int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                    const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA, long long ldI,
                    long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st);
+// dCond (4 doubles per trajectory, from srwd_launch_cond_sums) enables the
+// condition check of the FP32 sums: points that fail it are appended to dFpList
+// (block * 256 + thread; *dFpCount entries, zeroed by the caller) instead of
+// being stored, for srwd_launch_sr_fp64.  dCond == nullptr stores every point.
 int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                          const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
-                         long long nPointsPad, float* dEx, float* dEz, int* dWarn, hipStream_t st);
+                         long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
+                         int* dFpList, int* dFpCount, hipStream_t st);
+int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec* dRec, long long ldI, double* dCond,
+                          hipStream_t st);
+// FP64 recomputation and store of the listed points (block indices relative
+// to dBlockItem, as passed to srwd_launch_finalize).
+int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
+                        const srwd_edge* dEdge, int termin, const double* dX, const double* dY, const double* dIBx,
+                        const double* dIBz, long long ldT, float* dEx, float* dEz, int* dWarn, const int* dFpList,
+                        const int* dFpCount, hipStream_t st);
 }

@@ -26,10 +26,13 @@ int drift_passes(float2* d, long long sliceStride, int nSlices, const srwf_slice
 // parameters one after another) vs dependence dep, with the (i0, i1, t)
 // interpolation of every coordinate that is not varied.
 struct srwx_extract {
-    int pol, what, dep, pad;
+    int pol, what, dep, dbl;      // what: the reference's internal Int_or_ReE; dbl: double output (phase)
+    int intOverE, pad;            // integrate the intensity over photon energy at each output point
     long long ne, nx, nz, nout;
     long long ie0, ie1, ix0, ix1, iz0, iz1;
     double te, tx, tz;
+    double eStep, xStep, zStep;
+    double iter, invIter;         // arMeth: 0 overwrite, > 0 running average over iter values, < 0 sum
 };
 
 // Pointwise element of a container on the slice-major field (slice 2 ie +
@@ -45,7 +48,9 @@ struct srwo_elem {
 
 extern "C" {
 int srwo_launch_point_elem(float2* d, long long nxy, int ne, const srwo_elem& E, hipStream_t st);
-int srwx_launch_extract(const float2* ex, const float2* ez, const srwx_extract& P, float* out, hipStream_t st);
+// rows: ne * nz doubles of scratch for the flux (what == 4)
+int srwx_launch_extract(const float2* ex, const float2* ez, const srwx_extract& P, void* out, double* rows,
+                        hipStream_t st);
 int srwp_launch_intensity(const float2* ex, const float2* ez, int ne, int ie0, int ie1, float t, int pol,
                           long long nxy, float* out, hipStream_t st);
 }

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace srwamd {
@@ -35,7 +36,10 @@ int ensure_device()
     if (dev >= n) return SRWL_AMD_ERR_GPU;
     if (hipSetDevice(dev) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    if (hipStreamCreateWithFlags(&g_ctx.copyStream, hipStreamNonBlocking) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipHostMalloc((void**)&g_ctx.warnHost, sizeof(int), hipHostMallocDefault) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    if (hipHostMalloc((void**)&g_ctx.fpCountHost, sizeof(int) * 64, hipHostMallocDefault) != hipSuccess)
+        return SRWL_AMD_ERR_GPU;
     *g_ctx.warnHost = 0;
     g_ctx.device = dev;
     g_ctx.ready = true;
@@ -148,7 +152,8 @@ static int field_moments(SRWLWfr& w, const float2* dEx, const float2* dEz)
 // stay on the device: request j writes ne*nx*ny complex values of Ex and Ez
 // starting at float2 offset outOff[j] of dEx / dEz.
 int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
-           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin, bool defer, bool geomOnly)
+           const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin, bool defer, bool geomOnly,
+           const HostOut* hostOut)
 {
     int rc = ensure_device();
     if (rc) return rc;
@@ -295,6 +300,13 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         rc = sr_auto_run(jobs, info, dInfo, dX, dY, dIBx, dIBz, ldT, dEdge, termin, dEx, dEz, outOff, res);
         if (rc) return rc;
         if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev1, st));
+        if (hostOut) {
+            const SRWLRadMesh& m = jobs[0].mesh;
+            size_t n = sizeof(float) * 2 * (size_t)m.ne * m.nx * m.ny;
+            rc = download(hostOut->ex, dEx, n);
+            if (!rc) rc = download(hostOut->ez, dEz, n);
+            if (rc) return rc;
+        }
         long long ev = 0;
         for (int L : g_ctx.autoLevels) ev += 4LL * (1LL << L) + 1;
         g_ctx.pendingEvals = ev;
@@ -305,87 +317,266 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         return 0;
     }
     // --- items -------------------------------------------------------------
-    std::vector<srwd_item> items;
-    std::vector<int> blockItem;
+    // One item per (trajectory, photon energy) and band of computed mesh rows.
+    // A single request whose field goes to host arrays is cut into row bands:
+    // band b's download (rows b and their mirror images) overlaps the
+    // computation of band b + 1.
     std::vector<int> rowTraj;
     std::vector<double> rowE;
     std::vector<long long> rowN;
     const double normBase = 1. / 137.0360411;
+    std::vector<int> rowBase(nT);
     for (int j = 0; j < nT; j++) {
-        SrJob& J = jobs[j];
-        const srwd_trj_info& I = info[j];
-        const SRWLRadMesh& m = J.mesh;
-        int nx = (int)m.nx, nz = (int)m.ny, ne = (int)m.ne;
-        SrRegion G = sr_region(m, I);
-        double xStep = G.xStep, zStep = G.zStep;
-        double eStep = ne > 1 ? (m.eFin - m.eStart) / (ne - 1) : 0;
-        double xS = m.xStart, zS = m.yStart;
-        int symX = G.symX, symZ = G.symZ, nzc = G.nzc, nxc = G.nxc;
-        double normConst = std::sqrt(normBase * J.Iavg * 1.E+09 / 1.602189246E-19);
-        for (int ie = 0; ie < ne; ie++) {
-            srwd_item it;
-            std::memset(&it, 0, sizeof(it));
-            it.traj = j;
-            it.arow = (int)rowTraj.size();
-            it.eph = m.eStart + ie * eStep;
-            it.xStart = xS; it.xStep = xStep; it.zStart = zS; it.zStep = zStep;
-            it.nx = nx; it.nz = nz; it.nxc = nxc; it.nzc = nzc;
-            it.symX = symX; it.symZ = symZ;
-            it.ne = ne; it.ie = ie;
-            it.outOffset = outOff[j] + ie;
-            it.npts = nxc * nzc;
-            it.blockBase = (int)blockItem.size();
-            it.pointBase = (long long)it.blockBase * 256;
-            it.actNorm = normConst * it.eph * 0.80654658E-03;
-            int nb = (it.npts + 255) / 256;
-            for (int b = 0; b < nb; b++) blockItem.push_back((int)items.size());
-            items.push_back(it);
-            rowTraj.push_back(j); rowE.push_back(it.eph); rowN.push_back(I.nInt);
+        rowBase[j] = (int)rowTraj.size();
+        const SRWLRadMesh& m = jobs[j].mesh;
+        double eStep = m.ne > 1 ? (m.eFin - m.eStart) / (m.ne - 1) : 0;
+        for (int ie = 0; ie < (int)m.ne; ie++) {
+            rowTraj.push_back(j); rowE.push_back(m.eStart + ie * eStep); rowN.push_back(info[j].nInt);
         }
     }
+    int nBands = 1;
+    if (hostOut && nT == 1) {
+        SrRegion G = sr_region(jobs[0].mesh, info[0]);
+        const SRWLRadMesh& m = jobs[0].mesh;
+        double outBytes = 16. * m.ne * m.nx * m.ny;
+        if (outBytes >= 16. * (1 << 20)) nBands = std::min(8, G.nzc);
+    }
+    std::vector<srwd_item> items;
+    std::vector<int> blockItem;
+    std::vector<int> bandBlock0(nBands + 1, 0), bandRow0(nBands + 1, 0);
+    for (int b = 0; b < nBands; b++) {
+        bandBlock0[b] = (int)blockItem.size();
+        for (int j = 0; j < nT; j++) {
+            SrJob& J = jobs[j];
+            const srwd_trj_info& I = info[j];
+            const SRWLRadMesh& m = J.mesh;
+            int nx = (int)m.nx, nz = (int)m.ny, ne = (int)m.ne;
+            SrRegion G = sr_region(m, I);
+            int r0 = (int)(((long long)G.nzc * b) / nBands), r1 = (int)(((long long)G.nzc * (b + 1)) / nBands);
+            if (nBands > 1) { bandRow0[b] = r0; bandRow0[b + 1] = r1; }
+            if (r1 <= r0) continue;
+            double eStep = ne > 1 ? (m.eFin - m.eStart) / (ne - 1) : 0;
+            double normConst = std::sqrt(normBase * J.Iavg * 1.E+09 / 1.602189246E-19);
+            for (int ie = 0; ie < ne; ie++) {
+                srwd_item it;
+                std::memset(&it, 0, sizeof(it));
+                it.traj = j;
+                it.eph = m.eStart + ie * eStep;
+                it.arow = rowBase[j] + ie;
+                it.xStart = m.xStart; it.xStep = G.xStep; it.zStart = m.yStart; it.zStep = G.zStep;
+                it.nx = nx; it.nz = nz; it.nxc = G.nxc; it.nzc = G.nzc;
+                it.iz0 = r0;
+                it.symX = G.symX; it.symZ = G.symZ;
+                it.ne = ne; it.ie = ie;
+                it.outOffset = outOff[j] + ie;
+                it.npts = G.nxc * (r1 - r0);
+                it.blockBase = (int)blockItem.size() - bandBlock0[b];   // relative to the band's first block
+                it.pointBase = (long long)blockItem.size() * 256;
+                it.actNorm = normConst * it.eph * 0.80654658E-03;
+                int nb = (it.npts + 255) / 256;
+                for (int q = 0; q < nb; q++) blockItem.push_back((int)items.size());
+                items.push_back(it);
+            }
+        }
+    }
+    bandBlock0[nBands] = (int)blockItem.size();
     int nItems = (int)items.size();
+    int nRows = (int)rowTraj.size();
     int nBlocks = (int)blockItem.size();
     if (nBlocks == 0) return 0;
     long long nPointsPad = (long long)nBlocks * 256;
-    // split the integration range when the mesh alone does not fill the GPU
+    // split the integration range when a band's mesh alone does not fill the GPU
     long long minChunks = ldC;
     for (auto& I : info) minChunks = std::min(minChunks, (I.nInt + SRWD_CHUNK - 1) / SRWD_CHUNK);
-    int nSplit = (int)std::min<long long>(std::max<long long>(1, (4096 + nBlocks - 1) / nBlocks), minChunks);
+    int bandBlocks = (nBlocks + nBands - 1) / nBands;
+    int nSplit = (int)std::min<long long>(std::max<long long>(1, (4096 + bandBlocks - 1) / bandBlocks), minChunks);
     if (g_ctx.forceSplit > 0) nSplit = (int)std::min<long long>(g_ctx.forceSplit, minChunks);
     srwd_item* dItems = (srwd_item*)B.Items.get(sizeof(srwd_item) * nItems);
     int* dBlockItem = (int*)B.BlockItem.get(sizeof(int) * nBlocks);
-    int* dRowTraj = (int*)B.RowTraj.get(sizeof(int) * nItems);
-    double* dRowE = (double*)B.RowE.get(sizeof(double) * nItems);
-    long long* dRowN = (long long*)B.RowN.get(sizeof(long long) * nItems);
-    float* dA = (float*)B.A.get(sizeof(float) * (size_t)ldI * nItems);
+    int* dRowTraj = (int*)B.RowTraj.get(sizeof(int) * nRows);
+    double* dRowE = (double*)B.RowE.get(sizeof(double) * nRows);
+    long long* dRowN = (long long*)B.RowN.get(sizeof(long long) * nRows);
+    float* dA = (float*)B.A.get(sizeof(float) * (size_t)ldI * nRows);
     double* dPart = (double*)B.Part.get(sizeof(double) * 4 * (size_t)nSplit * nPointsPad);
     int* dWarn = (int*)B.Warn.get(sizeof(int));
-    if (!dItems || !dBlockItem || !dRowTraj || !dRowE || !dRowN || !dA || !dPart || !dWarn)
+    // condition check of the FP32 sums and the FP64 recomputation list (one band at a time)
+    int maxBandBlocks = 0;
+    for (int b = 0; b < nBands; b++) maxBandBlocks = std::max(maxBandBlocks, bandBlock0[b + 1] - bandBlock0[b]);
+    double* dCond = (double*)B.Cond.get(sizeof(double) * 4 * nT);
+    int* dFpList = (int*)B.FpList.get(sizeof(int) * 256 * (size_t)maxBandBlocks);
+    int* dFpCount = (int*)B.FpCount.get(sizeof(int) * (size_t)nBands);
+    if (!dItems || !dBlockItem || !dRowTraj || !dRowE || !dRowN || !dA || !dPart || !dWarn || !dCond || !dFpList ||
+        !dFpCount)
         return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     HIPCHK(hipMemcpyAsync(dItems, items.data(), sizeof(srwd_item) * nItems, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dBlockItem, blockItem.data(), sizeof(int) * nBlocks, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dRowTraj, rowTraj.data(), sizeof(int) * nItems, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dRowE, rowE.data(), sizeof(double) * nItems, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dRowN, rowN.data(), sizeof(long long) * nItems, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dRowTraj, rowTraj.data(), sizeof(int) * nRows, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dRowE, rowE.data(), sizeof(double) * nRows, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dRowN, rowN.data(), sizeof(long long) * nRows, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(dWarn, 0, sizeof(int), st));
-    if (srwd_launch_phase_table(dQ, ldI, dRowTraj, dRowE, dRowN, nItems, maxInt, dA, st)) return SRWL_AMD_ERR_GPU;
+    if (srwd_launch_phase_table(dQ, ldI, dRowTraj, dRowE, dRowN, nRows, maxInt, dA, st)) return SRWL_AMD_ERR_GPU;
+    if (srwd_launch_cond_sums(dInfo, nT, dRec, ldI, dCond, st)) return SRWL_AMD_ERR_GPU;
+    HIPCHK(hipMemsetAsync(dFpCount, 0, sizeof(int) * (size_t)nBands, st));
     g_ctx.lastSrBlocks = nBlocks;
     g_ctx.lastSrSplit = nSplit;
-    if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
-    if (srwd_launch_sr(dItems, dBlockItem, nBlocks, dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart, st))
-        return SRWL_AMD_ERR_GPU;
-    if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev1, st));
-    if (srwd_launch_finalize(dItems, dBlockItem, nBlocks, dInfo, dEdge, termin, dPart, nSplit, nPointsPad, dEx, dEz,
-                             dWarn, st))
-        return SRWL_AMD_ERR_GPU;
-    // the warning flag is read back with the caller's final synchronisation
+    // events: per band a timing pair around the SR launch, "computed" and "downloaded"
+    size_t needEv = 4 * (size_t)nBands;
+    while (g_ctx.bandEv.size() < needEv) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        g_ctx.bandEv.push_back(e);
+    }
+    g_ctx.nTimePairs = g_ctx.timing ? nBands : 0;
+    // host side of the streamed download
+    const SRWLRadMesh& m0 = jobs[0].mesh;
+    size_t rowBytes = sizeof(float) * 2 * (size_t)m0.nx * m0.ne;
+    size_t compBytes = rowBytes * m0.ny;
+    char* pin = nullptr;
+    if (nBands > 1) {
+        pin = pinned_staging(2 * compBytes);
+        if (!pin) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    }
+    auto rows_of_band = [&](int b, int* ranges) {   // [a, b) of computed rows and of their mirror images
+        int nz = (int)m0.ny, hz = nz >> 1;
+        ranges[0] = bandRow0[b]; ranges[1] = bandRow0[b + 1];
+        ranges[2] = ranges[3] = 0;
+        if (items[0].symZ) {
+            int hi = std::min(bandRow0[b + 1], hz);
+            if (bandRow0[b] < hi) { ranges[2] = nz - hi; ranges[3] = nz - bandRow0[b]; }
+        }
+    };
+    auto host_copy_band = [&](int b) {
+        int r[4];
+        rows_of_band(b, r);
+        for (int k = 0; k < 4; k += 2) {
+            if (r[k + 1] <= r[k]) continue;
+            size_t o = rowBytes * r[k], n = rowBytes * (r[k + 1] - r[k]);
+            par_memcpy((char*)hostOut->ex + o, pin + o, n);
+            par_memcpy((char*)hostOut->ez + o, pin + compBytes + o, n);
+        }
+    };
+    for (int b = 0; b < nBands; b++) {
+        int b0 = bandBlock0[b], nb = bandBlock0[b + 1] - b0;
+        if (nb > 0) {
+            if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 2], st));
+            if (srwd_launch_sr(dItems, dBlockItem + b0, nb, dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad,
+                               dPart, st))
+                return SRWL_AMD_ERR_GPU;
+            if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 3], st));
+            if (srwd_launch_finalize(dItems, dBlockItem + b0, nb, dInfo, dEdge, termin, dPart, nSplit, nPointsPad,
+                                     dEx, dEz, dWarn, dCond, dFpList, dFpCount + b, st))
+                return SRWL_AMD_ERR_GPU;
+            if (srwd_launch_sr_fp64(dItems, dBlockItem + b0, dInfo, dEdge, termin, dX, dY, dIBx, dIBz, ldT, dEx, dEz,
+                                    dWarn, dFpList, dFpCount + b, st))
+                return SRWL_AMD_ERR_GPU;
+        }
+        if (nBands > 1) {
+            // download band b on the copy stream once computed
+            HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b], st));
+            HIPCHK(hipStreamWaitEvent(g_ctx.copyStream, g_ctx.bandEv[4 * b], 0));
+            int r[4];
+            rows_of_band(b, r);
+            for (int k = 0; k < 4; k += 2) {
+                if (r[k + 1] <= r[k]) continue;
+                size_t o = rowBytes * r[k], n = rowBytes * (r[k + 1] - r[k]);
+                HIPCHK(hipMemcpyAsync(pin + o, (char*)dEx + o, n, hipMemcpyDeviceToHost, g_ctx.copyStream));
+                HIPCHK(hipMemcpyAsync(pin + compBytes + o, (char*)dEz + o, n, hipMemcpyDeviceToHost,
+                                      g_ctx.copyStream));
+            }
+            HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 1], g_ctx.copyStream));
+        }
+    }
+    // the warning flag and the FP64 point counts are read back with the caller's final synchronisation
     HIPCHK(hipMemcpyAsync(g_ctx.warnHost, dWarn, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(g_ctx.fpCountHost, dFpCount, sizeof(int) * (size_t)nBands, hipMemcpyDeviceToHost, st));
+    g_ctx.fpBands = nBands;
     long long ev = 0;
     for (auto& it : items) ev += (long long)it.npts * info[it.traj].nInt;
     g_ctx.pendingEvals = ev;
+    if (nBands > 1) {
+        // every band is queued; the host moves each downloaded band on into the
+        // caller's arrays while the later ones are still computed
+        for (int b = 0; b < nBands; b++) {
+            HIPCHK(hipEventSynchronize(g_ctx.bandEv[4 * b + 1]));
+            host_copy_band(b);
+        }
+    } else if (hostOut) {
+        size_t n = compBytes;
+        rc = download(hostOut->ex, dEx, n);
+        if (!rc) rc = download(hostOut->ez, dEz, n);
+        if (rc) return rc;
+    }
     if (defer) return 0;
     HIPCHK(hipStreamSynchronize(st));
     sr_complete(res);
+    return 0;
+}
+
+char* pinned_staging(size_t bytes)
+{
+    if (bytes > g_ctx.pinBytes) {
+        if (g_ctx.pin) (void)hipHostFree(g_ctx.pin);
+        g_ctx.pin = nullptr;
+        g_ctx.pinBytes = 0;
+        if (hipHostMalloc((void**)&g_ctx.pin, bytes, hipHostMallocDefault) != hipSuccess) {
+            g_ctx.pin = nullptr;
+            return nullptr;
+        }
+        g_ctx.pinBytes = bytes;
+    }
+    return g_ctx.pin;
+}
+
+void par_memcpy(void* dst, const void* src, size_t bytes)
+{
+    const size_t minPiece = 4 << 20;
+    unsigned nt = std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency()));
+    size_t pieces = std::min<size_t>(nt, (bytes + minPiece - 1) / minPiece);
+    if (pieces <= 1) { std::memcpy(dst, src, bytes); return; }
+    std::vector<std::thread> th;
+    size_t per = (bytes + pieces - 1) / pieces;
+    for (size_t k = 1; k < pieces; k++) {
+        size_t o = k * per;
+        if (o >= bytes) break;
+        size_t n = std::min(per, bytes - o);
+        th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, n); });
+    }
+    std::memcpy(dst, src, std::min(per, bytes));
+    for (auto& t : th) t.join();
+}
+
+int download(void* dst, const void* src, size_t bytes)
+{
+    // pieces of up to 32 MiB through the pinned staging: the DMA of piece k + 1
+    // overlaps the host copy of piece k
+    const size_t piece = 32 << 20;
+    size_t np = (bytes + piece - 1) / piece;
+    char* pin = pinned_staging(std::min(bytes, 2 * piece));
+    if (!pin) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    while (g_ctx.bandEv.size() < 4) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        g_ctx.bandEv.push_back(e);
+    }
+    HIPCHK(hipEventRecord(g_ctx.bandEv[0], g_ctx.stream));
+    HIPCHK(hipStreamWaitEvent(g_ctx.copyStream, g_ctx.bandEv[0], 0));
+    for (size_t k = 0; k < np; k++) {
+        size_t o = k * piece, n = std::min(piece, bytes - o);
+        char* stage = pin + (k & 1) * piece;
+        if (k >= 2) HIPCHK(hipEventSynchronize(g_ctx.bandEv[2 + (k & 1)]));   // stage free again
+        HIPCHK(hipMemcpyAsync(stage, (const char*)src + o, n, hipMemcpyDeviceToHost, g_ctx.copyStream));
+        HIPCHK(hipEventRecord(g_ctx.bandEv[2 + (k & 1)], g_ctx.copyStream));
+        if (k >= 1) {
+            size_t op = (k - 1) * piece, npv = std::min(piece, bytes - op);
+            HIPCHK(hipEventSynchronize(g_ctx.bandEv[2 + ((k - 1) & 1)]));
+            par_memcpy((char*)dst + op, pin + ((k - 1) & 1) * piece, npv);
+        }
+    }
+    if (np > 0) {
+        size_t k = np - 1, o = k * piece, n = std::min(piece, bytes - o);
+        HIPCHK(hipEventSynchronize(g_ctx.bandEv[2 + (k & 1)]));
+        par_memcpy((char*)dst + o, pin + (k & 1) * piece, n);
+    }
     return 0;
 }
 
@@ -438,7 +629,16 @@ void sr_complete(std::vector<SrResult>& res)
 {
     if (g_ctx.timing) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
+        if (g_ctx.nTimePairs > 0) {
+            for (int b = 0; b < g_ctx.nTimePairs; b++) {
+                float t = 0;
+                (void)hipEventElapsedTime(&t, g_ctx.bandEv[4 * b + 2], g_ctx.bandEv[4 * b + 3]);
+                ms += t;
+            }
+            g_ctx.nTimePairs = 0;
+        } else {
+            (void)hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
+        }
         g_ctx.lastSrMs = ms;
         g_ctx.lastSrEvals = g_ctx.pendingEvals;
         g_ctx.sumSrMs += ms;
@@ -447,6 +647,10 @@ void sr_complete(std::vector<SrResult>& res)
     }
     if (*g_ctx.warnHost)
         for (auto& r : res) if (!r.warn) r.warn = SRWL_WARN_BAD_RAD_INT_TERMINATIONS;
+    long long fp = 0;
+    for (int b = 0; b < g_ctx.fpBands; b++) fp += g_ctx.fpCountHost[b];
+    g_ctx.lastFp64Points = fp;
+    g_ctx.fpBands = 0;
 }
 
 // srwlib.cpp:403-452 + 328-373: trajectory range for a request without a
@@ -673,12 +877,11 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     float* dEx = (float*)g_ctx.buf.Ex.get(sizeof(float) * 2 * n);
     float* dEz = (float*)g_ctx.buf.Ez.get(sizeof(float) * 2 * n);
     if (!dEx || !dEz) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    rc = sr_run(jobs, fields, dEx, dEz, off, res, termin, true);
+    // the field reaches the caller's arrays band by band while the rest is computed
+    HostOut ho;
+    ho.ex = (float*)pWfr->arEx; ho.ez = (float*)pWfr->arEy;
+    rc = sr_run(jobs, fields, dEx, dEz, off, res, termin, true, false, g_ctx.keepOnDevice ? nullptr : &ho);
     if (rc) return rc;
-    if (!g_ctx.keepOnDevice) {
-        HIPCHK(hipMemcpyAsync(pWfr->arEx, dEx, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
-        HIPCHK(hipMemcpyAsync(pWfr->arEy, dEz, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, g_ctx.stream));
-    }
     pWfr->Rx = pWfr->Ry = res[0].Robs;
     pWfr->dRx = pWfr->dRy = res[0].RobsErr;
     pWfr->xc = res[0].xc; pWfr->yc = res[0].zc;
@@ -792,11 +995,11 @@ SRWL_EXPORT int srwlAmdSetTiming(int on)
 SRWL_EXPORT int srwlAmdGetStats(double* out, int n)
 {
     if (out == nullptr) return 0;
-    double v[15] = {g_ctx.lastSrMs, (double)g_ctx.lastSrEvals, (double)g_ctx.lastSrBlocks, (double)g_ctx.lastSrSplit,
+    double v[16] = {g_ctx.lastSrMs, (double)g_ctx.lastSrEvals, (double)g_ctx.lastSrBlocks, (double)g_ctx.lastSrSplit,
                     g_ctx.lastFftMs, g_ctx.lastFftBytes, g_ctx.sumSrMs, g_ctx.sumSrEvals, (double)g_ctx.nSrCalls,
                     g_ctx.sumFftMs, g_ctx.sumFftBytes, (double)g_ctx.nFftCalls,
-                    g_ctx.lastRsMs, g_ctx.lastRsBytes, g_ctx.lastPropDevMs};
-    for (int i = 0; i < n && i < 15; i++) out[i] = v[i];
+                    g_ctx.lastRsMs, g_ctx.lastRsBytes, g_ctx.lastPropDevMs, (double)g_ctx.lastFp64Points};
+    for (int i = 0; i < n && i < 16; i++) out[i] = v[i];
     return 0;
 }
 

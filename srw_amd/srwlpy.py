@@ -157,10 +157,11 @@ def CalcElecFieldSRBatch(_arWfr, _arMag, _prec_par):
 
 
 def stats():
-    out = (C.c_double * 15)()
-    lib.srwlAmdGetStats(out, 15)
+    out = (C.c_double * 16)()
+    lib.srwlAmdGetStats(out, 16)
     keys = ("sr_ms", "sr_evals", "sr_blocks", "sr_split", "fft_ms", "fft_bytes", "sum_sr_ms", "sum_sr_evals",
-            "n_sr", "sum_fft_ms", "sum_fft_bytes", "n_fft", "resize_ms", "resize_bytes", "prop_device_ms")
+            "n_sr", "sum_fft_ms", "sum_fft_bytes", "n_fft", "resize_ms", "resize_bytes", "prop_device_ms",
+            "sr_fp64_points")
     return dict(zip(keys, list(out)))
 
 
@@ -273,11 +274,14 @@ def PropagElecField(_wfr, _opt_cnt, _int_list=None, _gpu_idx=None):
 
 def CalcIntFromElecField(_arI, _wfr, _pol, _int_type, _dep_type, _e, _x, _y, _meth=None, _fldtrj=None,
                          _gpu_idx=None):
-    """Intensity from the electric field (reference srwlpy.cpp:4873)."""
+    """Intensity from the electric field (reference srwlpy.cpp:4873).  _arI is
+    float ('f'), or double ('d') for the phase (_int_type 4); _meth is arMeth
+    (srwlib.h:727: [0] = 1 running average over [1] previous values, 2 sum)."""
     keep = M.Keep()
     cw = M.wfr(_wfr, keep)
+    pm = M.dvec(_meth, keep) if _meth is not None else None
     _check(lib.srwlCalcIntFromElecField(M._addr(_arI), C.byref(cw), int(_pol), int(_int_type), int(_dep_type),
-                                        float(_e), float(_x), float(_y), None, None, None))
+                                        float(_e), float(_x), float(_y), pm, None, None))
     return _arI
 
 

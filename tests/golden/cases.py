@@ -150,3 +150,60 @@ def radsub_wavefront(d, e_ev=1000.):
     w.dRx = w.dRy = d["dRf"] * abs(R)
     w.avgPhotEn = e_ev
     return w
+
+
+# Extraction (srwlCalcIntFromElecField) golden cases: a seeded random field on
+# a 5-energy, 7 x 6 mesh, every polarisation, intType 0, 2, 4, 5, 6, 7, every
+# dependence type, the averaging / sum modes of arMeth for the x-y dependence;
+# at one off-mesh (e, x, y).  The reference's own bugs are left out: the
+# phase vs (e, x), (e, y), (e, x, y) (written through a null float pointer)
+# and the fourth Stokes block vs (e, x, y) (a copy of the third).
+INT_POLS = [0, 1, 2, 3, 4, 5, 6, -1, -2, -3, -4, -5]
+INT_TYPES = [0, 2, 4, 5, 6, 7]
+INT_METHS = [None, [1, 3], [2]]
+INT_POINT = (None, 0.13e-3, -0.21e-3)   # e: mid-band + 3.3 eV (set in int_wavefront)
+
+
+def int_wavefront(seed=3):
+    import numpy as np
+    from srw_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    w, _, _ = W.single_electron(n=7, ne=5, de_rel=0.05)
+    w.allocate(5, 7, 6)
+    w.mesh.ny = 6
+    w.mesh.yFin = 0.8e-3
+    n = 2 * 5 * 7 * 6
+    w.arEx = array('f', rng.normal(size=n).astype(np.float32).tobytes())
+    w.arEy = array('f', (0.3 * rng.normal(size=n)).astype(np.float32).tobytes())
+    e = 0.5 * (w.mesh.eStart + w.mesh.eFin) + 3.3
+    return w, (e, INT_POINT[1], INT_POINT[2])
+
+
+def int_cases():
+    """(intType, depType, pol, meth) of the extraction golden set."""
+    out = []
+    for it in INT_TYPES:
+        for dep in range(7):
+            if it == 4 and dep >= 4:
+                continue
+            for pol in INT_POLS:
+                if dep == 6 and pol == -5:
+                    continue
+                for meth in INT_METHS:
+                    if meth is not None and dep != 3:
+                        continue
+                    out.append((it, dep, pol, meth))
+    return out
+
+
+def int_len(w, dep, pol, it):
+    m = w.mesh
+    lens = [m.ne, m.nx, m.ny, m.nx * m.ny, m.ne * m.nx, m.ne * m.ny, m.ne * m.nx * m.ny]
+    return lens[dep] * (4 if (pol == -5 and it != 2) else 1)
+
+
+def int_prefill(k, n, it):
+    """Prefilled output buffer of extraction case k (seeded by k)."""
+    import numpy as np
+    rng = np.random.default_rng(1000 + k)
+    return rng.normal(size=n).astype(np.float64 if it == 4 else np.float32)

@@ -48,6 +48,22 @@ def main():
     rs["_cases"] = np.frombuffer(json.dumps(cases.RADSUB_CASES).encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(HERE, "ref_radsub.npz"), **rs)
 
+    # extraction: SHA-256 of each output's bytes (the checks are bit-exact),
+    # from a prefill seeded by the case index (the averaging and sum modes read it)
+    import hashlib
+    w, (e, x, y) = cases.int_wavefront()
+    digests, rcs = [], []
+    for k, (it, dep, pol, meth) in enumerate(cases.int_cases()):
+        n = cases.int_len(w, dep, pol, it)
+        pre = cases.int_prefill(k, n, it)
+        rc, out = ref_lib.calc_int(w, pol, it, dep, e, x, y, n, meth=meth, prefill=pre)
+        digests.append(hashlib.sha256(out.tobytes()).hexdigest())
+        rcs.append(rc)
+    with open(os.path.join(HERE, "ref_int.json"), "w") as f:
+        json.dump({"cases": [[it, dep, pol, meth] for (it, dep, pol, meth) in cases.int_cases()],
+                   "rc": rcs, "sha256": digests}, f, indent=0)
+    print("extraction cases:", len(cases.int_cases()))
+
 
 if __name__ == "__main__":
     main()

@@ -28,6 +28,8 @@ def load():
     lib.orc_srwlPropagElecField.argtypes = [P(S.SRWLWfr), P(S.SRWLOptC)]
     lib.orc_srwlCalcIntFromElecField.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_byte, C.c_byte, C.c_byte, C.c_double,
                                                  C.c_double, C.c_double]
+    lib.orc_srwlCalcIntFromElecField_m.argtypes = [C.c_void_p, P(S.SRWLWfr), C.c_byte, C.c_byte, C.c_byte,
+                                                   C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.orc_srwlUtiFFT.argtypes = [C.c_void_p, C.c_char, P(C.c_double), C.c_int, C.c_int]
     lib.orc_field_B.argtypes = [P(S.SRWLMagFldC), C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.orc_trj_interp.argtypes = [P(S.SRWLPrtTrj), P(C.c_double), C.c_longlong, P(C.c_double)]
@@ -134,11 +136,12 @@ def propagate(w, optc):
     return rc
 
 
-def intensity(arI, w, pol=6, int_type=0, dep_type=3, e=None, x=0., y=0.):
+def intensity(arI, w, pol=6, int_type=0, dep_type=3, e=None, x=0., y=0., meth=None):
     keep = M.Keep()
     cw = M.wfr(w, keep)
     e = w.mesh.eStart if e is None else e
-    return lib.orc_srwlCalcIntFromElecField(M._addr(arI), C.byref(cw), pol, int_type, dep_type, e, x, y)
+    pm = M.dvec(meth, keep) if meth is not None else None
+    return lib.orc_srwlCalcIntFromElecField_m(M._addr(arI), C.byref(cw), pol, int_type, dep_type, e, x, y, pm)
 
 
 def uti_fft(data, mesh, direction):

@@ -135,12 +135,15 @@ def rad_to_sub(w, L, subtype=1):
 
 def calc_int(w, pol, int_type, dep_type, e, x, y, nout, meth=None, prefill=None):
     """srwlCalcIntFromElecField of the reference (srwlib.cpp:757-856) on
-    wavefront w; returns (rc, float32 array of nout values).  `prefill` sets
-    the output buffer's values before the call (for the averaging modes)."""
+    wavefront w; returns (rc, array of nout values: float64 for the phase,
+    intType 4, float32 otherwise).  `prefill` sets the output buffer's values
+    before the call (for the averaging modes)."""
     m = w.mesh
     nf = 2 * m.ne * m.nx * m.ny
     meth = [] if meth is None else list(meth)
-    pre = np.zeros(nout, dtype=np.float32) if prefill is None else np.asarray(prefill, dtype=np.float32)
+    dbl = int(int_type) == 4
+    dt = np.float64 if dbl else np.float32
+    pre = np.zeros(nout, dtype=dt) if prefill is None else np.asarray(prefill, dtype=dt)
     with tempfile.TemporaryDirectory() as td:
         ff, out = os.path.join(td, "f.bin"), os.path.join(td, "out.bin")
         with open(ff, "wb") as f:
@@ -149,8 +152,8 @@ def calc_int(w, pol, int_type, dep_type, e, x, y, nout, meth=None, prefill=None)
         tok = ["INT", out, ff, int(pol), int(int_type), int(dep_type), repr(float(e)), repr(float(x)),
                repr(float(y)), len(meth)] + [repr(float(v)) for v in meth]
         tok += _mesh_tokens(m) + [repr(float(v)) for v in (w.Rx, w.Ry, w.xc, w.yc)]
-        tok += [int(nout)] + [repr(float(v)) for v in pre]
+        tok += [int(nout), int(dbl)] + [repr(float(v)) for v in pre]
         _run(tok)
         raw = open(out, "rb").read()
     rc = int(np.frombuffer(raw, dtype=np.int32, count=1)[0])
-    return rc, np.frombuffer(raw, dtype=np.float32, count=nout, offset=8).copy()
+    return rc, np.frombuffer(raw, dtype=dt, count=nout, offset=8).copy()

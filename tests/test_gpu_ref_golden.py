@@ -89,3 +89,33 @@ def test_gpu_multi_energy_call_equals_single_energy_slices():
         w1.mesh.eStart = w1.mesh.eFin = w.mesh.eStart + ie * (w.mesh.eFin - w.mesh.eStart) / 8
         srwlpy.CalcElecFieldSR(w1, 0, mag, prec)
         assert rel_l2(ex[:, :, ie], cfield(w1.arEx, w1)[:, :, 0]) < 1e-6
+
+
+def test_gpu_per_point_error_at_mesh_corners_and_band_edges():
+    """Per-point relative error of the field (|Ex|^2 + |Ey|^2 weighted) at the
+    dimmest places of the 201-energy workload: the 20 x 20 corner blocks of
+    the 201 x 201 mesh at the -5 % and +5 % energies.  Points whose FP32 sums
+    are ill-conditioned are recomputed in FP64 (counted in the statistics)."""
+    from srw_amd import srwlpy
+    w_full, mag, prec = W.single_electron(n=201, ne=201, de_rel=0.05)
+    for e in (w_full.mesh.eStart, w_full.mesh.eFin):
+        w, _, _ = W.single_electron(n=201, ne=1)
+        w.mesh.eStart = w.mesh.eFin = e
+        wg, wo = clone_wfr(w), clone_wfr(w)
+        srwlpy.CalcElecFieldSR(wg, 0, mag, prec)
+        fp64 = int(srwlpy.stats()["sr_fp64_points"])
+        assert oracle.calc_field(wo, mag, prec, nthreads=16) == 0
+        exg, exo = cfield(wg.arEx, wg)[:, :, 0], cfield(wo.arEx, wo)[:, :, 0]
+        eyg, eyo = cfield(wg.arEy, wg)[:, :, 0], cfield(wo.arEy, wo)[:, :, 0]
+        num = np.sqrt(np.abs(exg - exo) ** 2 + np.abs(eyg - eyo) ** 2)
+        den = np.sqrt(np.abs(exo) ** 2 + np.abs(eyo) ** 2)
+        rel = num / np.maximum(den, 1e-300)
+        worst = 0.
+        for sl in ((slice(0, 20), slice(0, 20)), (slice(0, 20), slice(-20, None)),
+                   (slice(-20, None), slice(0, 20)), (slice(-20, None), slice(-20, None))):
+            worst = max(worst, float(rel[sl].max()))
+        ratio = float(den[:20, :20].max() / den.max())
+        print("%.2f eV: corner |E| / peak |E| <= %.2e, worst per-point rel error %.2e, whole mesh %.2e, "
+              "FP64 points %d" % (e, ratio, worst, float(rel.max()), fp64), flush=True)
+        assert worst < 1e-4
+        assert float(rel.max()) < 1e-4
