@@ -231,6 +231,18 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         const double* pp = (pOpt->arProp && i < pOpt->nProp) ? pOpt->arProp[i] : nullptr;
         int rc = drift_mode_from_prop(pp, modes[i], subs[i]);
         if (rc) return rc;
+        // resampling instructions (arProp[5..8] != 1, sroptcnt.cpp:314-327) are
+        // not applied per electron here: refuse them instead of ignoring them
+        const int npp = (pOpt->arPropN != nullptr) ? (int)(unsigned char)pOpt->arPropN[i] : 9;
+        if (pp && npp > 8 && (pp[5] != 1. || pp[6] != 1. || pp[7] != 1. || pp[8] != 1.))
+            return SRWL_AMD_ERR_UNSUPPORTED;
+    }
+    // a trailing post-resize instruction (sroptcnt.cpp:408-423) likewise
+    if (pOpt && pOpt->arProp && pOpt->nProp > pOpt->nElem) {
+        const double* pp = pOpt->arProp[pOpt->nElem];
+        const int npp = (pOpt->arPropN != nullptr) ? (int)(unsigned char)pOpt->arPropN[pOpt->nElem] : 9;
+        if (pp && npp > 8 && (pp[5] != 1. || pp[6] != 1. || pp[7] != 1. || pp[8] != 1.))
+            return SRWL_AMD_ERR_UNSUPPORTED;
     }
     BeamSampler bs;
     if (!bs.init(pWfr0->partBeam)) return SRWL_AMD_ERR_BATCH;

@@ -10,6 +10,7 @@
 #include "srw_fft.h"
 #include "srw_prop.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -546,6 +547,10 @@ SRWL_EXPORT int srwlPropagElecField(SRWLWfr* pWfr, SRWLOptC* pOpt, int nInt, cha
             if (!dT) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
         }
         mctx.d = dS; mctx.nxy = nxy; mctx.nx = nx;
+        // a wavefront without moment arrays starts every drift from empty
+        // scratch moments, so that the radius estimate recomputes them from
+        // the current field (e.g. after an aperture clipped it)
+        if (!tmpMom.empty()) std::fill(tmpMom.begin(), tmpMom.end(), 0.);
         int mode, sub;
         drift_mode_from_prop(pp, mode, sub);
         DriftGeom g;

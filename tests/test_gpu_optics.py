@@ -73,3 +73,37 @@ def test_lens_with_linear_phase_term_is_rejected():
     assert oracle.propagate(clone_wfr(w), opt()) != 0
     with pytest.raises(RuntimeError):
         srwlpy.PropagElecField(clone_wfr(w), opt())
+
+
+def test_wavefront_without_moment_arrays_aperture_then_drift():
+    """A wavefront with no moment arrays: the arProp[3] = 1 drift after an
+    aperture takes its radius from moments of the clipped field, as the
+    oracle (fresh scratch moments for every drift)."""
+    from srw_amd import srwlpy
+    w = _sr_wfr(128)
+    w.arMomX = w.arMomY = None
+    mk = lambda: SRWLOptC([SRWLOptA('r', 'a', 0.6e-3, 0.5e-3, 1e-4, 0.), SRWLOptD(10.)], [_prop(), _prop(1)])
+    wg, wo = clone_wfr(w), clone_wfr(w)
+    srwlpy.PropagElecField(wg, mk())
+    assert oracle.propagate(wo, mk()) == 0
+    for attr in ("xStart", "xFin", "yStart", "yFin"):
+        assert getattr(wg.mesh, attr) == pytest.approx(getattr(wo.mesh, attr), rel=1e-9)
+    assert rel_l2(cfield(wg.arEx, wg), cfield(wo.arEx, wo)) < TOL
+
+
+def test_multi_electron_beamline_with_resize_is_refused():
+    """srwlEmitPropMultiE does not resample per electron: a beamline whose
+    arProp asks for resampling is refused instead of being run unresampled."""
+    from srw_amd import srwlpy
+    from srw_amd.srwlib import SRWLStokes
+    beam = W.beam_for_multi_e()
+    e1 = W.first_harmonic_eV(beam.partStatMom1.gamma, 1.5, 0.02)
+    w0 = W.wavefront(beam, 1, e1, e1, 64, 64, 1e-3, 1e-3, 20.)
+    st = SRWLStokes()
+    st.allocate(1, 64, 64, _n_comp=1)
+    st.mesh.eStart = st.mesh.eFin = e1
+    st.mesh.xStart, st.mesh.xFin, st.mesh.yStart, st.mesh.yFin = -2e-3, 2e-3, -2e-3, 2e-3
+    for opt in (SRWLOptC([SRWLOptL(10., 10.), SRWLOptD(5.)], [[0, 0, 1.0, 0, 0, 2.0, 1.0, 2.0, 1.0], _prop(2)]),
+                SRWLOptC([SRWLOptD(5.)], [_prop(2), [0, 0, 1.0, 0, 0, 1.0, 2.0, 1.0, 2.0]])):
+        with pytest.raises(RuntimeError):
+            srwlpy.EmitPropMultiE(st, w0, W.planar_undulator(), opt, W.traj_prec(10001), 16, _seed=3, _batch=16)
