@@ -349,9 +349,13 @@ __device__ __forceinline__ double* picard_chain(const rk_ctx& c, const double* y
     const long long nSteps = n - 1;
     const long long per = nSteps > 0 ? (nSteps + SRWD_PIC_THREADS - 1) / SRWD_PIC_THREADS : 1;
     perOut = per;
-    {   // initial iterate: straight line along the initial derivative
+    {   // initial iterate: free flight along the initial velocity.  The
+        // accelerations are left out: extrapolated over a whole chain they
+        // carry the orbit far off axis, where the field's cosh growth makes
+        // the next iterate overflow.
         double d0[5];
         rk_deriv_t<ONE>(c, y0, d0);
+        d0[1] = 0.; d0[3] = 0.;
         for (long long j = t; j < n; j += SRWD_PIC_THREADS) {
             long long a = pic_at(j, per);
             for (int k = 0; k < 5; k++) Ya[k * cap + a] = (j == 0) ? y0[k] : y0[k] + (double)j * h * d0[k];
@@ -379,7 +383,9 @@ __device__ __forceinline__ double* picard_chain(const rk_ctx& c, const double* y
             long long b = pic_at(j + 1, per);
             for (int k = 0; k < 5; k++) {
                 acc[k] += Yb[k * cap + b];
-                m[k] = fmax(m[k], fabs(acc[k] - Ya[k * cap + b]));
+                double dv = fabs(acc[k] - Ya[k * cap + b]);
+                // fmax drops a NaN: a non-finite iterate is recorded as infinite instead
+                m[k] = isfinite(dv) ? fmax(m[k], dv) : INFINITY;
                 m[5 + k] = fmax(m[5 + k], fabs(acc[k]));
                 Yb[k * cap + b] = acc[k];
             }
@@ -387,8 +393,13 @@ __device__ __forceinline__ double* picard_chain(const rk_ctx& c, const double* y
         if (t == 0) for (int k = 0; k < 5; k++) Yb[k * cap + per * SRWD_PIC_THREADS] = y0[k];
         block_max10(m, lds + 5 * (SRWD_PIC_THREADS / 64));
         double* sw = Ya; Ya = Yb; Yb = sw;
+        bool diverged = false;
         done = true;
-        for (int k = 0; k < 5; k++) done = done && (m[k] <= 1e-15 * m[5 + k]);
+        for (int k = 0; k < 5; k++) {
+            done = done && (m[k] <= 1e-15 * m[5 + k]);
+            diverged = diverged || !isfinite(m[k]);
+        }
+        if (diverged) break;   // every thread holds the same m: the serial recurrence takes over
     }
     if (!done) {
         // serial recurrence, one thread
@@ -629,7 +640,7 @@ __global__ void __launch_bounds__(256) k_sr_tables(const srwd_trj_info* __restri
                                                    const double* X, const double* Y, const double* IBx,
                                                    const double* IBz, long long ldT,
                                                    srwd_rec* rec, srwd_chunk* chunk, double* Qout,
-                                                   long long ldI, long long ldC, long long maxInt)
+                                                   long long ldI, long long ldC, long long maxInt, double* T64)
 {
     long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int tr = blockIdx.y;
@@ -678,6 +689,12 @@ __global__ void __launch_bounds__(256) k_sr_tables(const srwd_trj_info* __restri
     R.bPz = (float)(bd * Pzd);
     rec[tr * ldI + gi] = R;
     Qout[tr * ldI + gi] = Q;
+    // FP64 trajectory values at the point, for the FP64 recomputation of
+    // ill-conditioned points (k_sr_fp64)
+    if (T64) {
+        double* t = T64 + (tr * ldI + gi) * 5;
+        t[0] = Xv; t[1] = Zv; t[2] = Btx; t[3] = Btz; t[4] = IB;
+    }
     if (c0 == gi) {
         srwd_chunk C; C.inv = inv; C.U = U; C.V = V; C.pad = 0;
         chunk[tr * ldC + gi / SRWD_CHUNK] = C;
@@ -859,19 +876,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 // two revolutions of its chunk reference and its sine and cosine come from
 // the hardware FP32 instructions, so a term carries an error of about
 // SRWD_EPS_TERM times its magnitude; over many s points these add like a
-// random walk, to SRWD_EPS_TERM * sqrt(sum of squared terms).  A point whose
-// field (terminating terms included) is not at least SRWD_COND_MAX times
-// that size is recomputed in FP64 (k_sr_fp64): the estimate then stays below
-// 1e-5 of the point's field.
+// random walk, to SRWD_EPS_TERM * sqrt(sum of squared terms) per field
+// component.  A point where that estimate exceeds condTol times the
+// component's value (terminating terms included) is recomputed in FP64
+// (k_sr_fp64).
 #define SRWD_EPS_TERM 4e-6
-#define SRWD_COND_MAX 1e-5
 
 // Terminating terms, the derivative correction, the float store and the
 // mirrored parts of one point; S = sum over the integration points of
 // w_i * (A_x, A_z) * exp(i phi) with w_i in units of h/15.
 __device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, const srwd_edge& E, int termin,
                                 int ix, int iz, const double* S, float2* Ex, float2* Ez, int* warn,
-                                double sumSq, int* fpList, int* fpCount, int listEntry)
+                                double sumSqX, double sumSqZ, double condTol, int* fpList, int* fpCount,
+                                int listEntry)
 {
     double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
     double h = I.h;
@@ -893,10 +910,11 @@ __device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, con
         ez.im += RR[1].im - RL[1].im + wd * (DL[1].im - DR[1].im);
     }
     if (fpList) {
-        // condition check of the FP32 sum: hand the point to the FP64 pass
-        double mag2 = ex.re * ex.re + ex.im * ex.im + ez.re * ez.re + ez.im * ez.im;
-        double err = SRWD_EPS_TERM * w * sqrt(fmax(sumSq, 0.));
-        if (err * err > SRWD_COND_MAX * SRWD_COND_MAX * mag2) {
+        // condition check of the FP32 sums, per component: hand the point to the FP64 pass
+        const double e2 = SRWD_EPS_TERM * SRWD_EPS_TERM * w * w, t2 = condTol * condTol;
+        const bool badX = e2 * fmax(sumSqX, 0.) > t2 * (ex.re * ex.re + ex.im * ex.im);
+        const bool badZ = e2 * fmax(sumSqZ, 0.) > t2 * (ez.re * ez.re + ez.im * ez.im);
+        if (badX || badZ) {
             int k = atomicAdd(fpCount, 1);
             fpList[k] = listEntry;
             return;
@@ -919,29 +937,29 @@ __device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, con
 }
 
 // Sums of squared terms per trajectory, from the integration records:
-// cond[0..3] = sum b^2, sum b bPx, sum b bPz, sum (bPx^2 + bPz^2), so that the
-// squared terms of point (xo, zo) add up to
-// cond[0] (xo^2 + zo^2) - 2 xo cond[1] - 2 zo cond[2] + cond[3].
+// cond[0..4] = sum b^2, sum b bPx, sum b bPz, sum bPx^2, sum bPz^2, so that
+// the squared x terms of point (xo, zo) add up to
+// cond[0] xo^2 - 2 xo cond[1] + cond[3] (and the z terms likewise).
 __global__ void __launch_bounds__(256) k_cond_sums(const srwd_trj_info* __restrict__ info,
                                                    const srwd_rec* __restrict__ rec, long long ldI, double* cond)
 {
     int tr = blockIdx.x;
     long long n = info[tr].nInt;
     const srwd_rec* R = rec + (long long)tr * ldI;
-    double a = 0, b = 0, c = 0, d = 0;
+    double v[5] = {0, 0, 0, 0, 0};
     for (long long i = threadIdx.x; i < n; i += blockDim.x) {
         double bb = R[i].b, px = R[i].bPx, pz = R[i].bPz;
-        a += bb * bb; b += bb * px; c += bb * pz; d += px * px + pz * pz;
+        v[0] += bb * bb; v[1] += bb * px; v[2] += bb * pz; v[3] += px * px; v[4] += pz * pz;
     }
-    __shared__ double sh[4][256];
-    sh[0][threadIdx.x] = a; sh[1][threadIdx.x] = b; sh[2][threadIdx.x] = c; sh[3][threadIdx.x] = d;
+    __shared__ double sh[5][256];
+    for (int k = 0; k < 5; k++) sh[k][threadIdx.x] = v[k];
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off)
-            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+            for (int k = 0; k < 5; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x < 4) cond[4 * tr + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 5) cond[5 * tr + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict__ items,
@@ -950,7 +968,8 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
                                                      const srwd_edge* __restrict__ edges, int termin,
                                                      const double* __restrict__ part, int nSplit,
                                                      long long nPointsPad, float2* Ex, float2* Ez, int* warn,
-                                                     const double* __restrict__ cond, int* fpList, int* fpCount)
+                                                     const double* __restrict__ cond, double condTol, int* fpList,
+                                                     int* fpCount)
 {
     const int it = blockItem[blockIdx.x];
     const srwd_item& IT = items[it];
@@ -964,14 +983,15 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
         const double* p = part + ((long long)k * nPointsPad + flat) * 4;
         S[0] += p[0]; S[1] += p[1]; S[2] += p[2]; S[3] += p[3];
     }
-    double sumSq = 0.;
+    double sumSqX = 0., sumSqZ = 0.;
     if (cond) {
-        const double* c = cond + 4 * IT.traj;
+        const double* c = cond + 5 * IT.traj;
         double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
-        sumSq = c[0] * (xo * xo + zo * zo) - 2. * (xo * c[1] + zo * c[2]) + c[3];
+        sumSqX = c[0] * xo * xo - 2. * xo * c[1] + c[3];
+        sumSqZ = c[0] * zo * zo - 2. * zo * c[2] + c[4];
     }
-    sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, S, Ex, Ez, warn, sumSq, cond ? fpList : nullptr,
-                    fpCount, (int)(blockIdx.x * 256 + threadIdx.x));
+    sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, S, Ex, Ez, warn, sumSqX, sumSqZ, condTol,
+                    cond ? fpList : nullptr, fpCount, (int)(blockIdx.x * 256 + threadIdx.x));
 }
 
 // FP64 recomputation of the points the condition check handed over: one wave
@@ -984,9 +1004,8 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
                                                  const int* __restrict__ blockItem,
                                                  const srwd_trj_info* __restrict__ info,
                                                  const srwd_edge* __restrict__ edges, int termin,
-                                                 const double* X, const double* Y, const double* IBx,
-                                                 const double* IBz, long long ldT, float2* Ex, float2* Ez,
-                                                 int* warn, const int* __restrict__ fpList,
+                                                 const double* __restrict__ T64, long long ldI, float2* Ex,
+                                                 float2* Ez, int* warn, const int* __restrict__ fpList,
                                                  const int* __restrict__ fpCount)
 {
     const int lane = threadIdx.x & 63;
@@ -1001,20 +1020,14 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
         int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
         double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
         double kph = SRWD_KPH(IT.eph);
-        bool byDiv = I.nInt < 10000;
-        const double* cx = X + IT.traj * ldT; const double* cz = Y + IT.traj * ldT;
-        const double* bx = IBx + IT.traj * ldT; const double* bz = IBz + IT.traj * ldT;
-        double bmx = -1. / I.betaNorm, bmz = 1. / I.betaNorm;
+        const double* T = T64 + (long long)IT.traj * ldI * 5;
         double acc[4] = {0, 0, 0, 0};
         for (long long i = lane; i < I.nInt; i += 64) {
             double sv = (i == I.nInt - 1) ? I.sIntegFin : I.sIntegStart + (double)i * I.h;
-            double Xv, Zv, Btx, Btz, IBxv, IBzv;
-            if (I.verNZ) { trj_pt p = eval_plane(I, cx, bx, bmx, sv, byDiv); Xv = p.crd; Btx = p.bt; IBxv = p.ib; }
-            else { double d = sv - I.s0; Btx = I.dxds0; Xv = I.x0 + I.dxds0 * d; IBxv = I.dxds0 * I.dxds0 * d; }
-            if (I.horNZ) { trj_pt p = eval_plane(I, cz, bz, bmz, sv, byDiv); Zv = p.crd; Btz = p.bt; IBzv = p.ib; }
-            else { double d = sv - I.s0; Btz = I.dzds0; Zv = I.z0 + I.dzds0 * d; IBzv = I.dzds0 * I.dzds0 * d; }
+            const double* t = T + i * 5;
+            double Xv = t[0], Zv = t[1], Btx = t[2], Btz = t[3], IB = t[4];
             double inv = 1. / (I.yObs - sv), dx = xo - Xv, dz = zo - Zv;
-            double ph = kph * (sv * I.gammaEm2 + (IBxv + IBzv) + (dx * dx + dz * dz) * inv);
+            double ph = kph * (sv * I.gammaEm2 + IB + (dx * dx + dz * dz) * inv);
             double c, sn;
             cos_sin_taylor(ph, c, sn);
             double ax = (Btx - dx * inv) * inv, az = (Btz - dz * inv) * inv;
@@ -1024,7 +1037,8 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
         for (int m = 0; m < 4; m++)
             for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off);
         if (lane == 0)
-            sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., nullptr, nullptr, 0);
+            sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr, nullptr,
+                            0);
     }
 }
 
@@ -1055,12 +1069,12 @@ int srwd_launch_traj(const srwd_traj_job* dJobs, const srwd_field* dFields, int 
 int srwd_launch_tables(const srwd_trj_info* dInfo, int nTraj, long long maxInt, const double* dX,
                        const double* dXp, const double* dY, const double* dYp, long long ldT,
                        double* dIBx, double* dIBz, srwd_rec* dRec, srwd_chunk* dChunk, double* dQ,
-                       long long ldI, long long ldC, srwd_edge* dEdge, hipStream_t st)
+                       long long ldI, long long ldC, srwd_edge* dEdge, double* dT64, hipStream_t st)
 {
     hipLaunchKernelGGL(k_quad_terms, dim3(2 * nTraj), dim3(SRWD_QT_THREADS), 0, st, dInfo, dXp, dYp, ldT, dIBx,
                        dIBz);
     hipLaunchKernelGGL(k_sr_tables, dim3((unsigned)((maxInt + 255) / 256), nTraj), dim3(256), 0, st, dInfo,
-                       dX, dY, dIBx, dIBz, ldT, dRec, dChunk, dQ, ldI, ldC, maxInt);
+                       dX, dY, dIBx, dIBz, ldT, dRec, dChunk, dQ, ldI, ldC, maxInt, dT64);
     hipLaunchKernelGGL(k_sr_edges, dim3((2 * nTraj + 63) / 64), dim3(64), 0, st, dInfo, nTraj, dX, dY,
                        dIBx, dIBz, ldT, dEdge);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1086,10 +1100,11 @@ int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, 
 int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                          const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
                          long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
-                         int* dFpList, int* dFpCount, hipStream_t st)
+                         double condTol, int* dFpList, int* dFpCount, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sr_finalize, dim3(nBlocks), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin,
-                       dPart, nSplit, nPointsPad, (float2*)dEx, (float2*)dEz, dWarn, dCond, dFpList, dFpCount);
+                       dPart, nSplit, nPointsPad, (float2*)dEx, (float2*)dEz, dWarn, dCond, condTol, dFpList,
+                       dFpCount);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1101,12 +1116,11 @@ int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec*
 }
 
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
-                        const srwd_edge* dEdge, int termin, const double* dX, const double* dY, const double* dIBx,
-                        const double* dIBz, long long ldT, float* dEx, float* dEz, int* dWarn, const int* dFpList,
-                        const int* dFpCount, hipStream_t st)
+                        const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
+                        float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sr_fp64, dim3(256), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin, dX, dY,
-                       dIBx, dIBz, ldT, (float2*)dEx, (float2*)dEz, dWarn, dFpList, dFpCount);
+    hipLaunchKernelGGL(k_sr_fp64, dim3(256), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin, dT64, ldI,
+                       (float2*)dEx, (float2*)dEz, dWarn, dFpList, dFpCount);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -10,7 +10,8 @@
 
 namespace srwamd {
 
-// Grow-only device buffer.
+// Grow-only device buffer.  While an Arena is active (the no-allocation
+// entry point), get() hands out pieces of the arena instead of allocating.
 struct DBuf {
     void* p = nullptr;
     size_t n = 0;
@@ -18,10 +19,25 @@ struct DBuf {
     void release();
 };
 
+// Bump allocator over caller-provided device memory: each DBuf asking within
+// one arena session gets its own 256-byte aligned piece, kept for the
+// session (asked again, a piece at least as large is returned as is).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    bool overflow = false;
+    struct Piece { const DBuf* buf; void* p; size_t bytes; };
+    Piece pieces[64];             // one per buffer of the call (fixed: the arena allocates nothing)
+    int nPieces = 0;
+    void* get(const DBuf* b, size_t bytes);
+};
+extern Arena* g_arena;   // non-null while a no-allocation call runs
+
 struct Buffers {
     DBuf X, Xp, Y, Yp, IBx, IBz, Samp, Jobs, Fields, Info;
     DBuf Rec, Chunk, Q, Edge, Mesh, Rx, Items, BlockItem, RowTraj, RowE, RowN, A, Part, Warn;
     DBuf Cond, FpList, FpCount;   // condition check of the FP32 sums and its FP64 recomputation list
+    DBuf T64;                     // FP64 trajectory values per integration point (FP64 recomputation)
     DBuf Ex, Ez;                  // field outputs
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
     DBuf Stokes, Draws, Wt, Net, Grad, Rep;
@@ -69,6 +85,7 @@ struct Context {
     int* fpCountHost = nullptr;             // pinned: points recomputed in FP64, per band (at most 64 bands)
     int fpBands = 0;
     long long lastFp64Points = 0;           // points of the last SR pass recomputed in FP64
+    double condTol = 1e-4;                  // condition check of the FP32 sums (0: off)
     long long pendingEvals = 0;             // integrand evaluations of the last SR pass
     std::vector<int> autoLevels;            // final levels of the last method-1 pass, in point order
     int autoSweeps = 0;                     // fixed-point sweeps of the last method-1 pass (largest chunk)

@@ -114,7 +114,7 @@ int srwd_launch_nz(srwd_trj_info* dInfo, int nTraj, const double* dX, const doub
 int srwd_launch_tables(const srwd_trj_info* dInfo, int nTraj, long long maxInt, const double* dX,
                        const double* dXp, const double* dY, const double* dYp, long long ldT,
                        double* dIBx, double* dIBz, srwd_rec* dRec, srwd_chunk* dChunk, double* dQ,
-                       long long ldI, long long ldC, srwd_edge* dEdge, hipStream_t st);
+                       long long ldI, long long ldC, srwd_edge* dEdge, double* dT64, hipStream_t st);
 int srwd_launch_rx(const srwd_trj_info* dInfo, int nTraj, long long maxNp, const double* dX, const double* dY,
                    const double* dIBx, const double* dIBz, long long ldT, double* dSamp, const double* dMesh,
                    double* dRx, hipStream_t st);
@@ -123,20 +123,22 @@ int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj
 int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                    const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA, long long ldI,
                    long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st);
-// dCond (4 doubles per trajectory, from srwd_launch_cond_sums) enables the
-// condition check of the FP32 sums: points that fail it are appended to dFpList
-// (block * 256 + thread; *dFpCount entries, zeroed by the caller) instead of
-// being stored, for srwd_launch_sr_fp64.  dCond == nullptr stores every point.
+// dCond (5 doubles per trajectory, from srwd_launch_cond_sums) enables the
+// condition check of the FP32 sums (estimated error <= condTol of each field
+// component): points that fail it are appended to dFpList (block * 256 +
+// thread; *dFpCount entries, zeroed by the caller) instead of being stored,
+// for srwd_launch_sr_fp64.  dCond == nullptr stores every point.
 int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                          const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
                          long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
-                         int* dFpList, int* dFpCount, hipStream_t st);
+                         double condTol, int* dFpList, int* dFpCount, hipStream_t st);
 int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec* dRec, long long ldI, double* dCond,
                           hipStream_t st);
 // FP64 recomputation and store of the listed points (block indices relative
 // to dBlockItem, as passed to srwd_launch_finalize).
+// dT64: FP64 trajectory values (X, Z, X', Z', integral of the squared angles)
+// per integration point, 5 doubles, as written by srwd_launch_tables.
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
-                        const srwd_edge* dEdge, int termin, const double* dX, const double* dY, const double* dIBx,
-                        const double* dIBz, long long ldT, float* dEx, float* dEz, int* dWarn, const int* dFpList,
-                        const int* dFpCount, hipStream_t st);
+                        const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
+                        float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, hipStream_t st);
 }

@@ -68,11 +68,6 @@ __device__ __forceinline__ trj_pt eval_plane(const srwd_trj_info& I, const doubl
 // ------------------------------------------------------------------------
 // End terms (FP64)
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void cos_sin_ref(double x, double& c, double& s)
-{
-    s = sin(x); c = cos(x);
-}
-
 // srradint.h:867-880 with the constants of srradint.cpp:48-60.
 __device__ __forceinline__ void cos_sin_taylor(double x, double& c, double& s)
 {
@@ -190,8 +185,10 @@ static __device__ int residual(const srwd_trj_info& I, const double* ev, double 
     double kph = SRWD_KPH(eph);
     double inv = 1. / (yo - s), dx = xo - X, dz = zo - Z;
     double Ph = kph * (s * I.gammaEm2 + IBx + IBz + (dx * dx + dz * dz) * inv);
+    // the reference's own sine and cosine (srradint.cpp:449): a point whose
+    // integral nearly cancels against its end terms depends on them
     double cph, sph;
-    cos_sin_ref(Ph, cph, sph);
+    cos_sin_taylor(Ph, cph, sph);
     cplx eip = {cph, sph};
     double bxn = Btx - dx * inv, bzn = Btz - dz * inv;
     double dPhds = kph * (I.gammaEm2 + bxn * bxn + bzn * bzn);
@@ -231,9 +228,8 @@ static __device__ int residual(const srwd_trj_info& I, const double* ev, double 
         double odN = 1. / dPhN, qN = d2N * odN;
         double PhN = kph * (sNew * gEm2 + (IBx + IBz) + (Btx * Btx + Btz * Btz) * (sNew - s) +
                             (dxN * dxN + dzN * dzN) * invN);
-        double cN, sN, c0, s0;
+        double cN, sN, c0 = cph, s0 = sph;
         cos_sin_taylor(PhN, cN, sN);
-        cos_sin_taylor(Ph, c0, s0);
         cplx dd[2];
         dd[0] = {(dAxN * cN - AxN * dPhN * sN) - (dAxds * c0 - Ax * dPhds * s0),
                  (AxN * dPhN * cN + dAxN * sN) - (Ax * dPhds * c0 + dAxds * s0)};

@@ -46,9 +46,27 @@ int ensure_device()
     return 0;
 }
 
+Arena* g_arena = nullptr;
+
+void* Arena::get(const DBuf* b, size_t bytes)
+{
+    int k = 0;
+    while (k < nPieces && pieces[k].buf != b) k++;
+    if (k < nPieces && pieces[k].bytes >= bytes) return pieces[k].p;
+    // new buffer, or larger now: a new piece (the old one is left unused)
+    size_t need = (bytes + 255) & ~(size_t)255;
+    if (used + need > cap || (k == nPieces && nPieces == 64)) { overflow = true; return nullptr; }
+    void* p = base + used;
+    used += need;
+    if (k == nPieces) nPieces++;
+    pieces[k] = {b, p, bytes};
+    return p;
+}
+
 void* DBuf::get(size_t bytes)
 {
     if (bytes == 0) bytes = 256;
+    if (g_arena) return g_arena->get(this, bytes);
     if (bytes > n) {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -271,9 +289,11 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     srwd_edge* dEdge = (srwd_edge*)B.Edge.get(sizeof(srwd_edge) * nT);
     double* dMesh = (double*)B.Mesh.get(sizeof(double) * 4 * nT);
     double* dRx = (double*)B.Rx.get(sizeof(double) * 4 * nT);
-    if (!dRec || !dChunk || !dQ || !dEdge || !dMesh || !dRx) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    double* dT64 = g_ctx.condTol > 0 ? (double*)B.T64.get(sizeof(double) * 5 * (size_t)ldI * nT) : nullptr;
+    if (!dRec || !dChunk || !dQ || !dEdge || !dMesh || !dRx || (g_ctx.condTol > 0 && !dT64))
+        return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     if (srwd_launch_tables(dInfo, nT, maxInt, dX, dXp, dY, dYp, ldT, dIBx, dIBz, dRec, dChunk, dQ, ldI, ldC,
-                           dEdge, st))
+                           dEdge, dT64, st))
         return SRWL_AMD_ERR_GPU;
     std::vector<double> meshLim(4 * nT);
     for (int j = 0; j < nT; j++) {
@@ -335,7 +355,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         }
     }
     int nBands = 1;
-    if (hostOut && nT == 1) {
+    if (hostOut && nT == 1 && !g_arena) {
         SrRegion G = sr_region(jobs[0].mesh, info[0]);
         const SRWLRadMesh& m = jobs[0].mesh;
         double outBytes = 16. * m.ne * m.nx * m.ny;
@@ -402,7 +422,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     // condition check of the FP32 sums and the FP64 recomputation list (one band at a time)
     int maxBandBlocks = 0;
     for (int b = 0; b < nBands; b++) maxBandBlocks = std::max(maxBandBlocks, bandBlock0[b + 1] - bandBlock0[b]);
-    double* dCond = (double*)B.Cond.get(sizeof(double) * 4 * nT);
+    double* dCond = (double*)B.Cond.get(sizeof(double) * 5 * nT);
     int* dFpList = (int*)B.FpList.get(sizeof(int) * 256 * (size_t)maxBandBlocks);
     int* dFpCount = (int*)B.FpCount.get(sizeof(int) * (size_t)nBands);
     if (!dItems || !dBlockItem || !dRowTraj || !dRowE || !dRowN || !dA || !dPart || !dWarn || !dCond || !dFpList ||
@@ -419,14 +439,17 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     HIPCHK(hipMemsetAsync(dFpCount, 0, sizeof(int) * (size_t)nBands, st));
     g_ctx.lastSrBlocks = nBlocks;
     g_ctx.lastSrSplit = nSplit;
-    // events: per band a timing pair around the SR launch, "computed" and "downloaded"
-    size_t needEv = 4 * (size_t)nBands;
-    while (g_ctx.bandEv.size() < needEv) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        g_ctx.bandEv.push_back(e);
+    // events of a banded pass: per band a timing pair around the SR launch,
+    // "computed" and "downloaded"; a single band is timed with ev0 / ev1
+    if (nBands > 1) {
+        size_t needEv = 4 * (size_t)nBands;
+        while (g_ctx.bandEv.size() < needEv) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            g_ctx.bandEv.push_back(e);
+        }
     }
-    g_ctx.nTimePairs = g_ctx.timing ? nBands : 0;
+    g_ctx.nTimePairs = (g_ctx.timing && nBands > 1) ? nBands : 0;
     // host side of the streamed download
     const SRWLRadMesh& m0 = jobs[0].mesh;
     size_t rowBytes = sizeof(float) * 2 * (size_t)m0.nx * m0.ne;
@@ -458,16 +481,17 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     for (int b = 0; b < nBands; b++) {
         int b0 = bandBlock0[b], nb = bandBlock0[b + 1] - b0;
         if (nb > 0) {
-            if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 2], st));
+            if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 2] : g_ctx.ev0, st));
             if (srwd_launch_sr(dItems, dBlockItem + b0, nb, dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad,
                                dPart, st))
                 return SRWL_AMD_ERR_GPU;
-            if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 3], st));
+            if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 3] : g_ctx.ev1, st));
             if (srwd_launch_finalize(dItems, dBlockItem + b0, nb, dInfo, dEdge, termin, dPart, nSplit, nPointsPad,
-                                     dEx, dEz, dWarn, dCond, dFpList, dFpCount + b, st))
+                                     dEx, dEz, dWarn, g_ctx.condTol > 0 ? dCond : nullptr, g_ctx.condTol, dFpList,
+                                     dFpCount + b, st))
                 return SRWL_AMD_ERR_GPU;
-            if (srwd_launch_sr_fp64(dItems, dBlockItem + b0, dInfo, dEdge, termin, dX, dY, dIBx, dIBz, ldT, dEx, dEz,
-                                    dWarn, dFpList, dFpCount + b, st))
+            if (g_ctx.condTol > 0 && srwd_launch_sr_fp64(dItems, dBlockItem + b0, dInfo, dEdge, termin, dT64, ldI, dEx,
+                                                         dEz, dWarn, dFpList, dFpCount + b, st))
                 return SRWL_AMD_ERR_GPU;
         }
         if (nBands > 1) {
@@ -500,6 +524,11 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             HIPCHK(hipEventSynchronize(g_ctx.bandEv[4 * b + 1]));
             host_copy_band(b);
         }
+    } else if (hostOut && g_arena) {
+        // no pinned staging in a no-allocation call: plain copies, finished
+        // by the caller's synchronisation
+        HIPCHK(hipMemcpyAsync(hostOut->ex, dEx, compBytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hostOut->ez, dEz, compBytes, hipMemcpyDeviceToHost, st));
     } else if (hostOut) {
         size_t n = compBytes;
         rc = download(hostOut->ex, dEx, n);
@@ -738,6 +767,7 @@ SRWL_EXPORT int srwlUtiGetErrText(char* t, int erNo)
     case SRWL_AMD_ERR_GPU: s = "GPU (HIP) error or no usable GPU"; break;
     case SRWL_AMD_ERR_UNSUPPORTED: s = "Parameter combination not supported by this build"; break;
     case SRWL_AMD_ERR_BATCH: s = "Inconsistent batch request"; break;
+    case SRWL_WARN_FLUX_ONLY_VS_PHOTON_ENERGY: s = "Flux can only be extracted vs photon energy"; break;
     case SRWL_WARN_BAD_RAD_INT_TERMINATIONS: s = "Terminating terms of the radiation integral were not computed"; break;
     case SRWL_WARN_ELEC_BEAM_NOT_ULTRARELATIVISTIC: s = "Electron beam may be not ultra-relativistic"; break;
     default: break;
@@ -1025,6 +1055,13 @@ SRWL_EXPORT int srwlAmdSetSplit(int nSplit)
 SRWL_EXPORT int srwlAmdSetTrajSerial(int on)
 {
     g_ctx.trajSerial = on != 0;
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetCondTol(double tol)
+{
+    if (!(tol >= 0)) return SRWL_AMD_ERR_UNSUPPORTED;
+    g_ctx.condTol = tol;
     return 0;
 }
 

@@ -181,6 +181,11 @@ def set_traj_serial(on=True):
     lib.srwlAmdSetTrajSerial(1 if on else 0)
 
 
+def set_cond_tol(tol=1e-4):
+    """Tolerance of the FP32 condition check of the SR sums (0: off)."""
+    _check(lib.srwlAmdSetCondTol(C.c_double(tol)))
+
+
 def set_fft_tiles(t7=None):
     """Forces the FFT tile shapes (7 ints) of every pass; None returns to the autotuner."""
     if t7 is None:
