@@ -183,7 +183,7 @@ def uneven_batch(srwlpy, W, n):
     dt = time.perf_counter() - t0
     st = srwlpy.stats()
     return {"requests": n, "points": pts, "seconds": dt, "points_per_s": pts / dt,
-            "sr_kernel_ms": st["sr_ms"], "sr_evals": st["sr_evals"],
+            "sr_kernel_ms": st["sr_ms"], "sr_evals": st["sr_evals"], "sr_fp64_points": int(st["sr_fp64_points"]),
             "sr_kernel_tflops": FLOPS_PER_EVAL * st["sr_evals"] / (st["sr_ms"] * 1e-3) / 1e12 if st["sr_ms"] else None,
             "workload": "%d requests, meshes 128^2-1024^2 over +/-1 mm at 20 m, harmonic 1-5 of K in [0.5, 2.5], "
                         "one device pass, fields left in HBM" % n}

@@ -43,9 +43,10 @@ SRWL_EXPORT int srwlAmdSetSplit(int nSplit);
  * recurrence, the switch exists to compare them. */
 SRWL_EXPORT int srwlAmdSetTrajSerial(int on);
 
-/* Tolerance of the condition check of the FP32 SR sums (method 0/2): a
- * point whose estimated rounding error exceeds tol times either field
- * component is recomputed in FP64.  Default 1e-4; 0 turns the check off. */
+/* Tolerance of the condition check of the FP32 SR sums (method 0): a point
+ * whose estimated rounding error exceeds tol times the magnitude of its field
+ * (both components) is recomputed in FP64.  Default 1e-4; 0 turns the check
+ * off. */
 SRWL_EXPORT int srwlAmdSetCondTol(double tol);
 
 /* Final level (1..20) of every computed point of the last method-1 field

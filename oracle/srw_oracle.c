@@ -603,6 +603,13 @@ typedef struct {
     long long lvlN[ORC_AUTO_MAX_LEVEL + 1];
 } orc_sr;
 
+/* Test knob (not part of the reference's algorithm): scale applied to the
+ * magnetic field and its derivative at the integration edges as the
+ * terminating terms see them.  Outside the magnet these values are the
+ * rounding noise of the trajectory interpolation's higher derivatives; 0
+ * shows which points of a case depend on that noise. */
+double orc_edge_field_scale = 1.;
+
 /* srradint.h:867-880 with srradint.cpp:48-60 constants */
 static void orc_cos_sin(double x, double* c, double* s)
 {
@@ -743,6 +750,8 @@ static void orc_residual(orc_sr* S, double s, int computeDer, double xo, double 
     double dBzds, Bz, Btx, X, IBx, dBxds, Bx, Btz, Z, IBz;
     orc_trj_eval_fld(T, 0, s, &dBzds, &Bz, &Btx, &X, &IBx);
     orc_trj_eval_fld(T, 1, s, &dBxds, &Bx, &Btz, &Z, &IBz);
+    dBzds *= orc_edge_field_scale; Bz *= orc_edge_field_scale;
+    dBxds *= orc_edge_field_scale; Bx *= orc_edge_field_scale;
     double kph = kPI_SR * 1.E+06 * 0.80654658 * eph;
     double inv = 1. / (yo - s);
     double dx = xo - X, dz = zo - Z;
@@ -836,6 +845,61 @@ static void orc_residual(orc_sr* S, double s, int computeDer, double xo, double 
     double complex* D = computeDer == 1 ? S->initDer : S->finDer;
     D[0] = (dAxds + I * (Ax * dPhds)) * eip;
     D[1] = (dAzds + I * (Az * dPhds)) * eip;
+}
+
+/* The acceptance test of orc_residual's far-field expansion at edge s
+ * (srradint.cpp:432-510, the same arithmetic) and, when accepted, the
+ * integrand's derivative there (:841-846) into D.  Returns 1 when accepted. */
+static int orc_edge_der(const orc_sr* S, int end, double xo, double yo, double zo, double eph, double complex* D)
+{
+    const orc_trj* T = S->T;
+    double s = end ? S->sIntegFin : S->sIntegStart;
+    double dBzds, Bz, Btx, X, IBx, dBxds, Bx, Btz, Z, IBz;
+    orc_trj_eval_fld(T, 0, s, &dBzds, &Bz, &Btx, &X, &IBx);
+    orc_trj_eval_fld(T, 1, s, &dBxds, &Bx, &Btz, &Z, &IBz);
+    Bz *= orc_edge_field_scale; Bx *= orc_edge_field_scale;
+    (void)dBzds; (void)dBxds;
+    double kph = kPI_SR * 1.E+06 * 0.80654658 * eph;
+    double inv = 1. / (yo - s);
+    double dx = xo - X, dz = zo - Z;
+    double Nx = dx * inv, bxn = Btx - Nx;
+    double Nz = dz * inv, bzn = Btz - Nz;
+    double dPhds = kph * (T->gammaEm2 + bxn * bxn + bzn * bzn);
+    double odp = 1. / dPhds;
+    double Ax = bxn * inv, Az = bzn * inv;
+    double cBxBz = T->betaNorm * Bz, cBzBx = -T->betaNorm * Bx;
+    double cxA = cBxBz + Ax, czA = cBzBx + Az, tk = 2. * kph;
+    double d2Phds2 = tk * (bxn * cxA + bzn * czA);
+    double dAxds = (2. * Ax + cBxBz) * inv, dAzds = (2. * Az + cBzBx) * inv;
+    double q = d2Phds2 * odp;
+    double t1x = Ax, t2x = (dAxds - Ax * q) * odp;
+    int ok2 = ((t2x == 0.) && (t1x == 0.)) || ((t2x != 0.) && (fabs(t2x) < 0.7 * fabs(t1x)));
+    if (!ok2) return 0;
+    double a0 = IBx + IBz + (dx * dx + dz * dz) * inv;
+    double Ph = kph * (s * T->gammaEm2 + a0);
+    double cph, sph;
+    orc_cos_sin(Ph, &cph, &sph);
+    double complex eip = cph + I * sph;
+    D[0] = (dAxds + I * (Ax * dPhds)) * eip;
+    D[1] = (dAzds + I * (Az * dPhds)) * eip;
+    return 1;
+}
+
+/* The edge derivatives the reference's sequential loop (rows, columns,
+ * photon energies, srradint.cpp:342-388) holds when it reaches point q0 of
+ * the computed region: each that of the last earlier point whose expansion
+ * at that edge was accepted, zero if none.  xs, zs, es: the loop's
+ * coordinates and energies. */
+static void orc_carried_state(orc_sr* S, long long q0, long nxc, long ne, const double* xs, const double* zs,
+                              const double* es, double yObs)
+{
+    int needL = S->termin == 1 || S->termin == 2, needR = S->termin == 1 || S->termin == 3;
+    S->initDer[0] = S->initDer[1] = S->finDer[0] = S->finDer[1] = 0;
+    for (long long q = q0 - 1; q >= 0 && (needL || needR); q--) {
+        long ie = (long)(q % ne), ix = (long)((q / ne) % nxc), iz = (long)(q / ne / nxc);
+        if (needL && orc_edge_der(S, 0, xs[ix], yObs, zs[iz], es[ie], S->initDer)) needL = 0;
+        if (needR && orc_edge_der(S, 1, xs[ix], yObs, zs[iz], es[ie], S->finDer)) needR = 0;
+    }
 }
 
 /* srradint.cpp:1214-1258 (slower) and 1379-1604 (faster1): the composite
@@ -1295,13 +1359,29 @@ ORC_API int orc_srwlCalcElecFieldSR_ex(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
             orc_lvl_free(&S);
             if (errAll) { res = errAll; goto done1; }
         } else
+        {
+        /* the loop's coordinates and energies, for the edge-derivative state
+         * each row starts from */
+        double* xs = (double*)malloc(sizeof(double) * (size_t)(nxc + nzc + ne + 1));
+        double* zs = xs + nxc;
+        double* es = zs + nzc;
+        {
+            double v = xS;
+            for (long ix = 0; ix < nxc; ix++) { xs[ix] = v; v += xStep; }
+            v = zS;
+            for (long iz = 0; iz < nzc; iz++) { zs[iz] = v; v += zStep; }
+            v = m->eStart;
+            for (long ie = 0; ie < ne; ie++) { es[ie] = v; v += eStep; }
+        }
 #pragma omp parallel num_threads(nthreads) reduction(|:warnAll)
         {
             orc_sr L = S;
-            L.initDer[0] = L.initDer[1] = L.finDer[0] = L.finDer[1] = 0;
             L.warn = 0;
 #pragma omp for schedule(dynamic, 1)
             for (long iz = 0; iz < nzc; iz++) {
+                /* rows run in parallel: each starts from the state the
+                 * sequential loop has at its first point */
+                if (L.termin > 0) orc_carried_state(&L, (long long)iz * nxc * ne, nxc, ne, xs, zs, es, yObs);
                 double z = zS;
                 for (long k = 0; k < iz; k++) z += zStep;
                 double x = xS;
@@ -1325,6 +1405,8 @@ ORC_API int orc_srwlCalcElecFieldSR_ex(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagF
                 }
             }
             if (L.warn) warnAll = 1;
+        }
+        free(xs);
         }
         if (warnAll) S.warn = SRWL_WARN_BAD_RAD_INT_TERMINATIONS;
         /* srradint.cpp:3264-3350 */

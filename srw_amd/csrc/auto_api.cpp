@@ -26,7 +26,7 @@ static const int kMaxSweeps = 4096;
 int sr_auto_run(const std::vector<SrJob>& jobs, const std::vector<srwd_trj_info>& info, const srwd_trj_info* dInfo,
                 const double* dX, const double* dY, const double* dIBx, const double* dIBz, long long ldT,
                 const srwd_edge* dEdge, int termin, float* dEx, float* dEz, const std::vector<long long>& outOff,
-                std::vector<SrResult>& res)
+                std::vector<SrResult>& res, const srwd_req* dReq, const double* dCoord, const long long* dLast)
 {
     hipStream_t st = g_ctx.stream;
     Buffers& B = g_ctx.buf;
@@ -61,6 +61,9 @@ int sr_auto_run(const std::vector<SrJob>& jobs, const std::vector<srwd_trj_info>
         J.nx = (int)m.nx; J.nz = (int)m.ny; J.nxc = G.nxc; J.nzc = G.nzc; J.ne = (int)m.ne;
         J.symX = G.symX; J.symZ = G.symZ; J.termin = termin;
         J.outOffset = outOff[j];
+        J.req = dReq + j;
+        J.coord = dCoord;
+        J.last = dLast;
         const srwd_trj_info* dI = dInfo + j;
         const double *x = dX + j * ldT, *y = dY + j * ldT, *ibx = dIBx + j * ldT, *ibz = dIBz + j * ldT;
         if (srwd_launch_auto_tables(dI, x, y, ibx, ibz, dTab, st)) return SRWL_AMD_ERR_GPU;

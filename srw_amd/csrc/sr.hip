@@ -876,32 +876,41 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 // two revolutions of its chunk reference and its sine and cosine come from
 // the hardware FP32 instructions, so a term carries an error of about
 // SRWD_EPS_TERM times its magnitude; over many s points these add like a
-// random walk, to SRWD_EPS_TERM * sqrt(sum of squared terms) per field
-// component.  A point where that estimate exceeds condTol times the
-// component's value (terminating terms included) is recomputed in FP64
-// (k_sr_fp64).
+// random walk, to SRWD_EPS_TERM * sqrt(sum of squared terms).  A point where
+// that estimate exceeds condTol times the magnitude of its field (both
+// components, terminating terms included) is recomputed in FP64 (k_sr_fp64):
+// the points near the nodes of the pattern, where the sums cancel.
 #define SRWD_EPS_TERM 4e-6
 
 // Terminating terms, the derivative correction, the float store and the
 // mirrored parts of one point; S = sum over the integration points of
 // w_i * (A_x, A_z) * exp(i phi) with w_i in units of h/15.
+// Rq, coord: the point's request (observation coordinates as the reference's
+// loop forms them); last: the edge-derivative state table (nullptr without
+// terminating terms).
 __device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, const srwd_edge& E, int termin,
                                 int ix, int iz, const double* S, float2* Ex, float2* Ez, int* warn,
                                 double sumSqX, double sumSqZ, double condTol, int* fpList, int* fpCount,
-                                int listEntry)
+                                int listEntry, const srwd_req& Rq, const double* coord, const long long* last)
 {
-    double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
+    double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
     double h = I.h;
     double w = IT.actNorm * h / 15.;
     cplx ex = {w * S[0], w * S[1]}, ez = {w * S[2], w * S[3]};
     if (termin > 0) {
         cplx RL[2] = {{0, 0}, {0, 0}}, RR[2] = {{0, 0}, {0, 0}}, DL[2] = {{0, 0}, {0, 0}}, DR[2] = {{0, 0}, {0, 0}};
         bool bad = false;
-        if (termin == 1 || termin == 2)
-            bad |= residual(I, E.v[0], I.sIntegStart, xo, I.yObs, zo, IT.eph, IT.actNorm, RL, DL, true, 1e-4) ==
-                   SRWD_RESID_BAD;
-        if (termin == 1 || termin == 3)
-            bad |= residual(I, E.v[1], I.sIntegFin, xo, I.yObs, zo, IT.eph, IT.actNorm, RR, DR) == SRWD_RESID_BAD;
+        const long long p = ((long long)iz * IT.nxc + ix) * IT.ne + IT.ie;
+        if (termin == 1 || termin == 2) {
+            int r = residual(I, E.v[0], I.sIntegStart, xo, I.yObs, zo, IT.eph, IT.actNorm, RL, DL, true, 1e-4);
+            bad |= r == SRWD_RESID_BAD;
+            if (r != SRWD_RESID_OK) carried_derivative(I, E.v[0], I.sIntegStart, I.yObs, Rq, coord, last, 0, p, DL);
+        }
+        if (termin == 1 || termin == 3) {
+            int r = residual(I, E.v[1], I.sIntegFin, xo, I.yObs, zo, IT.eph, IT.actNorm, RR, DR);
+            bad |= r == SRWD_RESID_BAD;
+            if (r != SRWD_RESID_OK) carried_derivative(I, E.v[1], I.sIntegFin, I.yObs, Rq, coord, last, 1, p, DR);
+        }
         if (bad) atomicOr(warn, 1);
         double wd = IT.actNorm * h * h / 15.;
         ex.re += RR[0].re - RL[0].re + wd * (DL[0].re - DR[0].re);
@@ -910,11 +919,11 @@ __device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, con
         ez.im += RR[1].im - RL[1].im + wd * (DL[1].im - DR[1].im);
     }
     if (fpList) {
-        // condition check of the FP32 sums, per component: hand the point to the FP64 pass
+        // condition check of the FP32 sums against the point's field
+        // magnitude: hand the point to the FP64 pass
         const double e2 = SRWD_EPS_TERM * SRWD_EPS_TERM * w * w, t2 = condTol * condTol;
-        const bool badX = e2 * fmax(sumSqX, 0.) > t2 * (ex.re * ex.re + ex.im * ex.im);
-        const bool badZ = e2 * fmax(sumSqZ, 0.) > t2 * (ez.re * ez.re + ez.im * ez.im);
-        if (badX || badZ) {
+        const double mag2 = ex.re * ex.re + ex.im * ex.im + ez.re * ez.re + ez.im * ez.im;
+        if (e2 * (fmax(sumSqX, 0.) + fmax(sumSqZ, 0.)) > t2 * mag2) {
             int k = atomicAdd(fpCount, 1);
             fpList[k] = listEntry;
             return;
@@ -969,7 +978,9 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
                                                      const double* __restrict__ part, int nSplit,
                                                      long long nPointsPad, float2* Ex, float2* Ez, int* warn,
                                                      const double* __restrict__ cond, double condTol, int* fpList,
-                                                     int* fpCount)
+                                                     int* fpCount, const srwd_req* __restrict__ reqs,
+                                                     const double* __restrict__ coord,
+                                                     const long long* __restrict__ last)
 {
     const int it = blockItem[blockIdx.x];
     const srwd_item& IT = items[it];
@@ -983,15 +994,16 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
         const double* p = part + ((long long)k * nPointsPad + flat) * 4;
         S[0] += p[0]; S[1] += p[1]; S[2] += p[2]; S[3] += p[3];
     }
+    const srwd_req Rq = reqs[IT.traj];
     double sumSqX = 0., sumSqZ = 0.;
     if (cond) {
         const double* c = cond + 5 * IT.traj;
-        double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
+        double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
         sumSqX = c[0] * xo * xo - 2. * xo * c[1] + c[3];
         sumSqZ = c[0] * zo * zo - 2. * zo * c[2] + c[4];
     }
     sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, S, Ex, Ez, warn, sumSqX, sumSqZ, condTol,
-                    cond ? fpList : nullptr, fpCount, (int)(blockIdx.x * 256 + threadIdx.x));
+                    cond ? fpList : nullptr, fpCount, (int)(blockIdx.x * 256 + threadIdx.x), Rq, coord, last);
 }
 
 // FP64 recomputation of the points the condition check handed over: one wave
@@ -1006,7 +1018,9 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
                                                  const srwd_edge* __restrict__ edges, int termin,
                                                  const double* __restrict__ T64, long long ldI, float2* Ex,
                                                  float2* Ez, int* warn, const int* __restrict__ fpList,
-                                                 const int* __restrict__ fpCount)
+                                                 const int* __restrict__ fpCount, const srwd_req* __restrict__ reqs,
+                                                 const double* __restrict__ coord,
+                                                 const long long* __restrict__ last)
 {
     const int lane = threadIdx.x & 63;
     const int nWaves = gridDim.x * (blockDim.x >> 6);
@@ -1018,7 +1032,8 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
         const int local = (blk - IT.blockBase) * 256 + thr;
         const srwd_trj_info I = info[IT.traj];
         int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
-        double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
+        const srwd_req& Rq = reqs[IT.traj];
+        double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
         double kph = SRWD_KPH(IT.eph);
         const double* T = T64 + (long long)IT.traj * ldI * 5;
         double acc[4] = {0, 0, 0, 0};
@@ -1038,7 +1053,86 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
             for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off);
         if (lane == 0)
             sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr, nullptr,
-                            0);
+                            0, Rq, coord, last);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Edge-derivative state of the reference's point loop
+// ------------------------------------------------------------------------
+// The derivative of the integrand at an integration edge enters each point's
+// integral through the end correction (srradint.cpp:1583-1586).  The
+// reference's loop (rows, columns, photon energies of the computed region)
+// sets it only at points where the far-field expansion at that edge is
+// accepted (:841-846) and keeps the previous point's value elsewhere.
+// k_edge_blocks: per block of SRWD_EDGE_BLOCK points and edge, the last
+// accepted point of the block (-1: none).
+__global__ void __launch_bounds__(SRWD_EDGE_BLOCK) k_edge_blocks(const srwd_req* __restrict__ reqs,
+                                                                 const int* __restrict__ blkReq,
+                                                                 const srwd_trj_info* __restrict__ info,
+                                                                 const srwd_edge* __restrict__ edges,
+                                                                 const double* __restrict__ coord, int termin,
+                                                                 long long* last)
+{
+    const long long b = blockIdx.x;
+    const int r = blkReq[b];
+    const srwd_req Rq = reqs[r];
+    const srwd_trj_info& I = info[r];
+    const long long q = (b - Rq.blk0) * SRWD_EDGE_BLOCK + threadIdx.x;
+    long long v[2] = {-1, -1};
+    if (q < Rq.nPts) {
+        double xo, zo, eph;
+        req_point(Rq, coord, q, xo, zo, eph);
+        edge_geo g;
+        if (termin == 1 || termin == 2) {
+            edge_geometry(I, edges[r].v[0], I.sIntegStart, xo, I.yObs, zo, eph, g);
+            if (g.ok2) v[0] = q;
+        }
+        if (termin == 1 || termin == 3) {
+            edge_geometry(I, edges[r].v[1], I.sIntegFin, xo, I.yObs, zo, eph, g);
+            if (g.ok2) v[1] = q;
+        }
+    }
+    __shared__ long long sh[2][SRWD_EDGE_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int e = 0; e < 2; e++) {
+        long long x = v[e];
+        for (int off = 32; off > 0; off >>= 1) x = max(x, (long long)__shfl_xor(x, off, 64));
+        if (lane == 0) sh[e][wv] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        long long x = -1;
+        for (int k = 0; k < SRWD_EDGE_BLOCK / 64; k++) x = max(x, sh[threadIdx.x][k]);
+        last[2 * b + threadIdx.x] = x;
+    }
+}
+
+// Running maximum over the blocks of each request (one workgroup per request):
+// afterwards last[2b + e] is the last accepted point at or before the end of
+// block b.
+__global__ void __launch_bounds__(256) k_edge_scan(const srwd_req* __restrict__ reqs, long long* last)
+{
+    const srwd_req Rq = reqs[blockIdx.x];
+    const long long nb = (Rq.nPts + SRWD_EDGE_BLOCK - 1) / SRWD_EDGE_BLOCK;
+    __shared__ long long sh[2][256];
+    long long carry[2] = {-1, -1};
+    for (long long c0 = 0; c0 < nb; c0 += 256) {
+        const long long b = c0 + threadIdx.x;
+        for (int e = 0; e < 2; e++) sh[e][threadIdx.x] = b < nb ? last[2 * (Rq.blk0 + b) + e] : -1;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            long long t0 = threadIdx.x >= (unsigned)off ? sh[0][threadIdx.x - off] : -1;
+            long long t1 = threadIdx.x >= (unsigned)off ? sh[1][threadIdx.x - off] : -1;
+            __syncthreads();
+            sh[0][threadIdx.x] = max(sh[0][threadIdx.x], t0);
+            sh[1][threadIdx.x] = max(sh[1][threadIdx.x], t1);
+            __syncthreads();
+        }
+        if (b < nb)
+            for (int e = 0; e < 2; e++) last[2 * (Rq.blk0 + b) + e] = max(carry[e], sh[e][threadIdx.x]);
+        for (int e = 0; e < 2; e++) carry[e] = max(carry[e], sh[e][255]);
+        __syncthreads();
     }
 }
 
@@ -1100,11 +1194,12 @@ int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, 
 int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                          const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
                          long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
-                         double condTol, int* dFpList, int* dFpCount, hipStream_t st)
+                         double condTol, int* dFpList, int* dFpCount, const srwd_req* dReq, const double* dCoord,
+                         const long long* dLast, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sr_finalize, dim3(nBlocks), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin,
                        dPart, nSplit, nPointsPad, (float2*)dEx, (float2*)dEz, dWarn, dCond, condTol, dFpList,
-                       dFpCount);
+                       dFpCount, dReq, dCoord, dLast);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1117,10 +1212,22 @@ int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec*
 
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
                         const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
-                        float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, hipStream_t st)
+                        float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, const srwd_req* dReq,
+                        const double* dCoord, const long long* dLast, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sr_fp64, dim3(256), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin, dT64, ldI,
-                       (float2*)dEx, (float2*)dEz, dWarn, dFpList, dFpCount);
+    hipLaunchKernelGGL(k_sr_fp64, dim3(2048), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin, dT64, ldI,
+                       (float2*)dEx, (float2*)dEz, dWarn, dFpList, dFpCount, dReq, dCoord, dLast);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_edge_state(const srwd_req* dReq, int nReq, const int* dBlkReq, long long nBlk,
+                           const srwd_trj_info* dInfo, const srwd_edge* dEdge, const double* dCoord, int termin,
+                           long long* dLast, hipStream_t st)
+{
+    if (nBlk <= 0 || nReq <= 0) return 0;
+    hipLaunchKernelGGL(k_edge_blocks, dim3((unsigned)nBlk), dim3(SRWD_EDGE_BLOCK), 0, st, dReq, dBlkReq, dInfo, dEdge,
+                       dCoord, termin, dLast);
+    hipLaunchKernelGGL(k_edge_scan, dim3(nReq), dim3(256), 0, st, dReq, dLast);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -80,9 +80,7 @@ __device__ __forceinline__ void auto_point_coords(const srwd_auto_job& J, long l
     long long r = p / J.ne;
     ix = (int)(r % J.nxc);
     iz = (int)(r / J.nxc);
-    xo = J.xStart + ix * J.xStep;
-    zo = J.zStart + iz * J.zStep;
-    eph = J.eStart + ie * J.eStep;
+    req_point(*J.req, J.coord, p, xo, zo, eph);
 }
 
 // Sum over the 64 lanes of a wave, the same value on every lane (lane 0's
@@ -122,11 +120,20 @@ __global__ void __launch_bounds__(256) k_auto_points(const srwd_auto_job J, cons
         if (lane == 0) {
             cplx RL[2] = {{0, 0}, {0, 0}}, RR[2] = {{0, 0}, {0, 0}}, DL[2] = {{0, 0}, {0, 0}}, DR[2] = {{0, 0}, {0, 0}};
             bool bad = false;
-            if (J.termin == 1 || J.termin == 2)
-                bad |= residual(I, edge->v[0], I.sIntegStart, xo, I.yObs, zo, eph, actNorm, RL, DL, true,
-                                0.01 * J.relPrec) == SRWD_RESID_BAD;
-            if (J.termin == 1 || J.termin == 3)
-                bad |= residual(I, edge->v[1], I.sIntegFin, xo, I.yObs, zo, eph, actNorm, RR, DR) == SRWD_RESID_BAD;
+            const long long p = p0 + q;
+            if (J.termin == 1 || J.termin == 2) {
+                int r = residual(I, edge->v[0], I.sIntegStart, xo, I.yObs, zo, eph, actNorm, RL, DL, true,
+                                 0.01 * J.relPrec);
+                bad |= r == SRWD_RESID_BAD;
+                if (r != SRWD_RESID_OK)
+                    carried_derivative(I, edge->v[0], I.sIntegStart, I.yObs, *J.req, J.coord, J.last, 0, p, DL);
+            }
+            if (J.termin == 1 || J.termin == 3) {
+                int r = residual(I, edge->v[1], I.sIntegFin, xo, I.yObs, zo, eph, actNorm, RR, DR);
+                bad |= r == SRWD_RESID_BAD;
+                if (r != SRWD_RESID_OK)
+                    carried_derivative(I, edge->v[1], I.sIntegFin, I.yObs, *J.req, J.coord, J.last, 1, p, DR);
+            }
             if (bad) atomicOr(warn, 1);
             o0[0] = RR[0].re - RL[0].re; o0[1] = RR[0].im - RL[0].im;
             o0[2] = RR[1].re - RL[1].re; o0[3] = RR[1].im - RL[1].im;

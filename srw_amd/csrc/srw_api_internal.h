@@ -38,6 +38,7 @@ struct Buffers {
     DBuf Rec, Chunk, Q, Edge, Mesh, Rx, Items, BlockItem, RowTraj, RowE, RowN, A, Part, Warn;
     DBuf Cond, FpList, FpCount;   // condition check of the FP32 sums and its FP64 recomputation list
     DBuf T64;                     // FP64 trajectory values per integration point (FP64 recomputation)
+    DBuf Req, Coord, BlkReq, Last;   // requests, observation coordinates, edge-derivative state
     DBuf Ex, Ez;                  // field outputs
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
     DBuf Stokes, Draws, Wt, Net, Grad, Rep;
@@ -153,7 +154,7 @@ void sr_complete(std::vector<SrResult>& res);
 int sr_auto_run(const std::vector<SrJob>& jobs, const std::vector<srwd_trj_info>& info, const srwd_trj_info* dInfo,
                 const double* dX, const double* dY, const double* dIBx, const double* dIBz, long long ldT,
                 const srwd_edge* dEdge, int termin, float* dEx, float* dEz, const std::vector<long long>& outOff,
-                std::vector<SrResult>& res);
+                std::vector<SrResult>& res, const srwd_req* dReq, const double* dCoord, const long long* dLast);
 
 // Wavefront moments (moments_api.cpp).  Component element (ix, iz, ie) of a
 // device wavefront is at base[ie*perE + iz*perZ + ix*perX] (float2 units).

@@ -34,6 +34,9 @@ struct srwd_auto_job {
     double relPrec;
     int nx, nz, nxc, nzc, ne, symX, symZ, termin;
     long long outOffset;  // float2 offset of (0, 0, 0) in the output arrays
+    const srwd_req* req;  // the request (device): observation coordinates of the reference's loop
+    const double* coord;  // coordinate table (device)
+    const long long* last;   // edge-derivative state table (device; nullptr without terminating terms)
 };
 
 extern "C" {

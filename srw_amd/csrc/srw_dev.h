@@ -97,6 +97,20 @@ struct srwd_item {
 
 #define SRWD_CHUNK 256
 
+// One request of a field computation, for the parts that follow the
+// reference's point loop (rows, then columns, then photon energies of the
+// computed region, srradint.cpp:342-388): its observation coordinates as that
+// loop forms them, running sums of the steps (x[nxc], z[nzc] and e[ne] at
+// xOff, zOff, eOff of the coordinate table), and its blocks of 256 points in
+// the edge-derivative state table.
+struct srwd_req {
+    int nxc, nzc, ne, pad;
+    int xOff, zOff, eOff, pad2;
+    long long blk0;           // first block of the request in the state table
+    long long nPts;           // nzc * nxc * ne
+};
+#define SRWD_EDGE_BLOCK 256
+
 extern "C" {
 // Launch wrappers implemented in sr.hip; all asynchronous on stream st.
 // Iterate-buffer length per component of the parallel-in-s RK4 for
@@ -123,15 +137,24 @@ int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj
 int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                    const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA, long long ldI,
                    long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st);
+// Edge-derivative state of the reference's point loop: per block of
+// SRWD_EDGE_BLOCK points of each request (dBlkReq: its request) and edge, the
+// last point at or before the block's end whose far-field expansion at that
+// edge is accepted (request-local index; -1: none).  dLast holds 2 values per
+// block.
+int srwd_launch_edge_state(const srwd_req* dReq, int nReq, const int* dBlkReq, long long nBlk,
+                           const srwd_trj_info* dInfo, const srwd_edge* dEdge, const double* dCoord, int termin,
+                           long long* dLast, hipStream_t st);
 // dCond (5 doubles per trajectory, from srwd_launch_cond_sums) enables the
-// condition check of the FP32 sums (estimated error <= condTol of each field
-// component): points that fail it are appended to dFpList (block * 256 +
+// condition check of the FP32 sums (estimated error <= condTol of the point's
+// field magnitude): points that fail it are appended to dFpList (block * 256 +
 // thread; *dFpCount entries, zeroed by the caller) instead of being stored,
 // for srwd_launch_sr_fp64.  dCond == nullptr stores every point.
 int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
                          const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
                          long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
-                         double condTol, int* dFpList, int* dFpCount, hipStream_t st);
+                         double condTol, int* dFpList, int* dFpCount, const srwd_req* dReq, const double* dCoord,
+                         const long long* dLast, hipStream_t st);
 int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec* dRec, long long ldI, double* dCond,
                           hipStream_t st);
 // FP64 recomputation and store of the listed points (block indices relative
@@ -140,5 +163,6 @@ int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec*
 // per integration point, 5 doubles, as written by srwd_launch_tables.
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
                         const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
-                        float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, hipStream_t st);
+                        float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, const srwd_req* dReq,
+                        const double* dCoord, const long long* dLast, hipStream_t st);
 }

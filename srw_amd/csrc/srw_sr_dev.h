@@ -164,6 +164,109 @@ static __device__ void left_pre_resid(const free_line& L, double sSt, double sFi
     R[1] = {out[2], out[3]};
 }
 
+// Far-field quantities at an integration edge (srradint.cpp:432-503): the
+// phase derivative, the amplitudes and their derivatives, the second term of
+// the expansion and its acceptance test |t2x| < 0.7 |t1x| (:507-510).
+struct edge_geo {
+    double kph, inv, dx, dz, bxn, bzn, dPhds, odp, Ax, Az, cBxBz, cBzBx, cxA, czA, tk, d2, dAxds, dAzds, q, t2x, t2z;
+    bool ok2;
+};
+
+__device__ __forceinline__ void edge_geometry(const srwd_trj_info& I, const double* ev, double s, double xo,
+                                              double yo, double zo, double eph, edge_geo& g)
+{
+    double Bz = ev[1], Btx = ev[2], X = ev[3];
+    double Bx = ev[6], Btz = ev[7], Z = ev[8];
+    g.kph = SRWD_KPH(eph);
+    g.inv = 1. / (yo - s);
+    g.dx = xo - X;
+    g.dz = zo - Z;
+    g.bxn = Btx - g.dx * g.inv;
+    g.bzn = Btz - g.dz * g.inv;
+    g.dPhds = g.kph * (I.gammaEm2 + g.bxn * g.bxn + g.bzn * g.bzn);
+    g.odp = 1. / g.dPhds;
+    g.Ax = g.bxn * g.inv;
+    g.Az = g.bzn * g.inv;
+    g.cBxBz = I.betaNorm * Bz;
+    g.cBzBx = -I.betaNorm * Bx;
+    g.cxA = g.cBxBz + g.Ax;
+    g.czA = g.cBzBx + g.Az;
+    g.tk = 2. * g.kph;
+    g.d2 = g.tk * (g.bxn * g.cxA + g.bzn * g.czA);
+    g.dAxds = (2. * g.Ax + g.cBxBz) * g.inv;
+    g.dAzds = (2. * g.Az + g.cBzBx) * g.inv;
+    g.q = g.d2 * g.odp;
+    g.t2x = (g.dAxds - g.Ax * g.q) * g.odp;
+    g.t2z = (g.dAzds - g.Az * g.q) * g.odp;
+    const double t1x = g.Ax;
+    g.ok2 = ((g.t2x == 0.) && (t1x == 0.)) || ((g.t2x != 0.) && (fabs(g.t2x) < 0.7 * fabs(t1x)));
+}
+
+// Phase at the edge with the reference's own sine and cosine (srradint.cpp:449):
+// a point whose integral nearly cancels against its end terms depends on them.
+__device__ __forceinline__ void edge_phase(const srwd_trj_info& I, const double* ev, double s, const edge_geo& g,
+                                           double& c, double& sn)
+{
+    double Ph = g.kph * (s * I.gammaEm2 + ev[4] + ev[9] + (g.dx * g.dx + g.dz * g.dz) * g.inv);
+    cos_sin_taylor(Ph, c, sn);
+}
+
+// Derivative of the integrand at the edge (srradint.cpp:841-846).
+__device__ __forceinline__ void edge_derivative(const edge_geo& g, double c, double sn, cplx* D)
+{
+    cplx eip = {c, sn};
+    D[0] = cmul({g.dAxds, g.Ax * g.dPhds}, eip);
+    D[1] = cmul({g.dAzds, g.Az * g.dPhds}, eip);
+}
+
+// Observation coordinates and photon energy of point q (request-local, the
+// reference's order: rows, columns, photon energies of the computed region).
+__device__ __forceinline__ void req_point(const srwd_req& Rq, const double* coord, long long q, double& xo,
+                                          double& zo, double& eph)
+{
+    int ie = (int)(q % Rq.ne);
+    long long r = q / Rq.ne;
+    int ix = (int)(r % Rq.nxc), iz = (int)(r / Rq.nxc);
+    xo = coord[Rq.xOff + ix];
+    zo = coord[Rq.zOff + iz];
+    eph = coord[Rq.eOff + ie];
+}
+
+// The edge derivative that the reference carries into point p when p's own
+// expansion at that edge is rejected: it sets the derivative only where the
+// expansion is accepted and keeps the previous point's value otherwise (zero
+// before the first accepted point of a call).  So: the derivative of the last
+// earlier point with an accepted expansion, found by walking back through p's
+// block of SRWD_EDGE_BLOCK points and then from the block table `last`
+// (2 values per block: left, right edge).
+__device__ __noinline__ void carried_derivative(const srwd_trj_info& I, const double* ev, double s, double yo,
+                                                const srwd_req& Rq, const double* coord, const long long* last,
+                                                int end, long long p, cplx* D)
+{
+    const long long b = p / SRWD_EDGE_BLOCK, lo = b * SRWD_EDGE_BLOCK;
+    double xo, zo, eph;
+    edge_geo g;
+    long long q = p - 1;
+    for (; q >= lo; q--) {
+        req_point(Rq, coord, q, xo, zo, eph);
+        edge_geometry(I, ev, s, xo, yo, zo, eph, g);
+        if (g.ok2) break;
+    }
+    if (q < lo) {
+        q = b > 0 ? last[2 * (Rq.blk0 + b - 1) + end] : -1;
+        if (q < 0) {
+            D[0] = {0., 0.};
+            D[1] = {0., 0.};
+            return;
+        }
+        req_point(Rq, coord, q, xo, zo, eph);
+        edge_geometry(I, ev, s, xo, yo, zo, eph, g);
+    }
+    double c, sn;
+    edge_phase(I, ev, s, g, c, sn);
+    edge_derivative(g, c, sn, D);
+}
+
 // Result of residual(): the far-field expansion with the edge derivative set,
 // the near-field residual (no derivative, no warning), or a rejection (zero
 // residual, no derivative; the caller raises the warning).
@@ -182,24 +285,15 @@ static __device__ int residual(const srwd_trj_info& I, const double* ev, double 
 {
     double dBzds = ev[0], Bz = ev[1], Btx = ev[2], X = ev[3], IBx = ev[4];
     double dBxds = ev[5], Bx = ev[6], Btz = ev[7], Z = ev[8], IBz = ev[9];
-    double kph = SRWD_KPH(eph);
-    double inv = 1. / (yo - s), dx = xo - X, dz = zo - Z;
-    double Ph = kph * (s * I.gammaEm2 + IBx + IBz + (dx * dx + dz * dz) * inv);
-    // the reference's own sine and cosine (srradint.cpp:449): a point whose
-    // integral nearly cancels against its end terms depends on them
+    edge_geo g;
+    edge_geometry(I, ev, s, xo, yo, zo, eph, g);
     double cph, sph;
-    cos_sin_taylor(Ph, cph, sph);
+    edge_phase(I, ev, s, g, cph, sph);
     cplx eip = {cph, sph};
-    double bxn = Btx - dx * inv, bzn = Btz - dz * inv;
-    double dPhds = kph * (I.gammaEm2 + bxn * bxn + bzn * bzn);
-    double odp = 1. / dPhds;
-    double Ax = bxn * inv, Az = bzn * inv;
-    double cBxBz = I.betaNorm * Bz, cBzBx = -I.betaNorm * Bx;
-    double cxA = cBxBz + Ax, czA = cBzBx + Az, tk = 2. * kph;
-    double d2 = tk * (bxn * cxA + bzn * czA);
-    double dAxds = (2. * Ax + cBxBz) * inv, dAzds = (2. * Az + cBzBx) * inv;
-    double q = d2 * odp;
-    double t2x = (dAxds - Ax * q) * odp, t2z = (dAzds - Az * q) * odp;
+    const double kph = g.kph, dx = g.dx, dz = g.dz, bxn = g.bxn, bzn = g.bzn, dPhds = g.dPhds;
+    const double inv = g.inv, odp = g.odp, Ax = g.Ax, Az = g.Az, cBxBz = g.cBxBz, cBzBx = g.cBzBx;
+    const double cxA = g.cxA, czA = g.czA, tk = g.tk, d2 = g.d2, dAxds = g.dAxds, dAzds = g.dAzds;
+    const double t2x = g.t2x, t2z = g.t2z;
     double cdBz = I.betaNorm * dBzds, cdBx = -I.betaNorm * dBxds;
     double d3 = tk * (cxA * cxA + czA * czA + bxn * (inv * (2. * bxn * inv + cBxBz) + cdBz) +
                       bzn * (inv * (2. * bzn * inv + cBzBx) + cdBx));
@@ -207,8 +301,7 @@ static __device__ int residual(const srwd_trj_info& I, const double* ev, double 
     double odp2 = odp * odp, qq = d2 * d2 * odp2;
     double t3x = (-d2Ax + (3. * dAxds * d2 + Ax * d3) * odp - 3. * Ax * qq) * odp2;
     double t3z = (-d2Az + (3. * dAzds * d2 + Az * d3) * odp - 3. * Az * qq) * odp2;
-    double t1x = Ax;
-    bool ok2 = ((t2x == 0.) && (t1x == 0.)) || ((t2x != 0.) && (fabs(t2x) < 0.7 * fabs(t1x)));
+    const bool ok2 = g.ok2;
     bool ok3 = ok2 && (((t3x == 0.) && (t2x == 0.)) || ((t3x != 0.) && (fabs(t3x) < 0.7 * fabs(t2x))));
     if (!ok2) {
         if (!left || !(fabs(Bx) < 1.E-10) || !(fabs(Bz) < 1.E-10)) return SRWD_RESID_BAD;
@@ -252,7 +345,6 @@ static __device__ int residual(const srwd_trj_info& I, const double* ev, double 
     cplx ex = cmul(mX, eip), ez = cmul(mZ, eip);
     R[0] = {actNorm * ex.re, actNorm * ex.im};
     R[1] = {actNorm * ez.re, actNorm * ez.im};
-    D[0] = cmul({dAxds, Ax * dPhds}, eip);
-    D[1] = cmul({dAzds, Az * dPhds}, eip);
+    edge_derivative(g, cph, sph, D);
     return SRWD_RESID_OK;
 }

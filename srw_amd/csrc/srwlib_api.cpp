@@ -314,10 +314,54 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         res[j].xc = rx[4 * j + 2]; res[j].zc = rx[4 * j + 3];
     }
     if (geomOnly) return 0;
+    // --- requests in the reference's point order ----------------------------
+    // Observation coordinates as the reference's loop forms them (running
+    // sums of the steps, srradint.cpp:342-388) and the edge-derivative state.
+    std::vector<srwd_req> reqs(nT);
+    std::vector<double> coord;
+    std::vector<int> blkReq;
+    for (int j = 0; j < nT; j++) {
+        const SRWLRadMesh& m = jobs[j].mesh;
+        SrRegion G = sr_region(m, info[j]);
+        srwd_req& R = reqs[j];
+        std::memset(&R, 0, sizeof(R));
+        R.nxc = G.nxc; R.nzc = G.nzc; R.ne = (int)m.ne;
+        R.nPts = (long long)G.nzc * G.nxc * m.ne;
+        R.blk0 = (long long)blkReq.size();
+        R.xOff = (int)coord.size();
+        double v = m.xStart;
+        for (int ix = 0; ix < G.nxc; ix++) { coord.push_back(v); v += G.xStep; }
+        R.zOff = (int)coord.size();
+        v = m.yStart;
+        for (int iz = 0; iz < G.nzc; iz++) { coord.push_back(v); v += G.zStep; }
+        R.eOff = (int)coord.size();
+        double eStep = m.ne > 1 ? (m.eFin - m.eStart) / (m.ne - 1) : 0;
+        v = m.eStart;
+        for (int ie = 0; ie < (int)m.ne; ie++) { coord.push_back(v); v += eStep; }
+        if (termin > 0) {
+            long long nb = (R.nPts + SRWD_EDGE_BLOCK - 1) / SRWD_EDGE_BLOCK;
+            blkReq.insert(blkReq.end(), (size_t)nb, j);
+        }
+    }
+    srwd_req* dReq = (srwd_req*)B.Req.get(sizeof(srwd_req) * nT);
+    double* dCoord = (double*)B.Coord.get(sizeof(double) * std::max<size_t>(1, coord.size()));
+    long long nEdgeBlk = (long long)blkReq.size();
+    int* dBlkReq = nEdgeBlk ? (int*)B.BlkReq.get(sizeof(int) * (size_t)nEdgeBlk) : nullptr;
+    long long* dLast = nEdgeBlk ? (long long*)B.Last.get(sizeof(long long) * 2 * (size_t)nEdgeBlk) : nullptr;
+    if (!dReq || !dCoord || (nEdgeBlk && (!dBlkReq || !dLast))) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    HIPCHK(hipMemcpyAsync(dReq, reqs.data(), sizeof(srwd_req) * nT, hipMemcpyHostToDevice, st));
+    if (!coord.empty())
+        HIPCHK(hipMemcpyAsync(dCoord, coord.data(), sizeof(double) * coord.size(), hipMemcpyHostToDevice, st));
+    if (nEdgeBlk) {
+        HIPCHK(hipMemcpyAsync(dBlkReq, blkReq.data(), sizeof(int) * (size_t)nEdgeBlk, hipMemcpyHostToDevice, st));
+        if (srwd_launch_edge_state(dReq, nT, dBlkReq, nEdgeBlk, dInfo, dEdge, dCoord, termin, dLast, st))
+            return SRWL_AMD_ERR_GPU;
+    }
     if (jobs[0].meth == 1) {
         for (auto& J : jobs) if (J.meth != 1) return SRWL_AMD_ERR_BATCH;
         if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
-        rc = sr_auto_run(jobs, info, dInfo, dX, dY, dIBx, dIBz, ldT, dEdge, termin, dEx, dEz, outOff, res);
+        rc = sr_auto_run(jobs, info, dInfo, dX, dY, dIBx, dIBz, ldT, dEdge, termin, dEx, dEz, outOff, res, dReq, dCoord,
+                         dLast);
         if (rc) return rc;
         if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev1, st));
         if (hostOut) {
@@ -349,9 +393,8 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     for (int j = 0; j < nT; j++) {
         rowBase[j] = (int)rowTraj.size();
         const SRWLRadMesh& m = jobs[j].mesh;
-        double eStep = m.ne > 1 ? (m.eFin - m.eStart) / (m.ne - 1) : 0;
         for (int ie = 0; ie < (int)m.ne; ie++) {
-            rowTraj.push_back(j); rowE.push_back(m.eStart + ie * eStep); rowN.push_back(info[j].nInt);
+            rowTraj.push_back(j); rowE.push_back(coord[reqs[j].eOff + ie]); rowN.push_back(info[j].nInt);
         }
     }
     int nBands = 1;
@@ -375,13 +418,12 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             int r0 = (int)(((long long)G.nzc * b) / nBands), r1 = (int)(((long long)G.nzc * (b + 1)) / nBands);
             if (nBands > 1) { bandRow0[b] = r0; bandRow0[b + 1] = r1; }
             if (r1 <= r0) continue;
-            double eStep = ne > 1 ? (m.eFin - m.eStart) / (ne - 1) : 0;
             double normConst = std::sqrt(normBase * J.Iavg * 1.E+09 / 1.602189246E-19);
             for (int ie = 0; ie < ne; ie++) {
                 srwd_item it;
                 std::memset(&it, 0, sizeof(it));
                 it.traj = j;
-                it.eph = m.eStart + ie * eStep;
+                it.eph = coord[reqs[j].eOff + ie];
                 it.arow = rowBase[j] + ie;
                 it.xStart = m.xStart; it.xStep = G.xStep; it.zStart = m.yStart; it.zStep = G.zStep;
                 it.nx = nx; it.nz = nz; it.nxc = G.nxc; it.nzc = G.nzc;
@@ -488,10 +530,10 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 3] : g_ctx.ev1, st));
             if (srwd_launch_finalize(dItems, dBlockItem + b0, nb, dInfo, dEdge, termin, dPart, nSplit, nPointsPad,
                                      dEx, dEz, dWarn, g_ctx.condTol > 0 ? dCond : nullptr, g_ctx.condTol, dFpList,
-                                     dFpCount + b, st))
+                                     dFpCount + b, dReq, dCoord, dLast, st))
                 return SRWL_AMD_ERR_GPU;
             if (g_ctx.condTol > 0 && srwd_launch_sr_fp64(dItems, dBlockItem + b0, dInfo, dEdge, termin, dT64, ldI, dEx,
-                                                         dEz, dWarn, dFpList, dFpCount + b, st))
+                                                         dEz, dWarn, dFpList, dFpCount + b, dReq, dCoord, dLast, st))
                 return SRWL_AMD_ERR_GPU;
         }
         if (nBands > 1) {

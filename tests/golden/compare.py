@@ -36,15 +36,20 @@ def both_symmetries_middle_row(w, name):
     loop stops before the middle row (srradint.cpp:3276-3290 runs iz <
     ny/2), so the right half of that row is never filled.  This build
     mirrors the whole computed region."""
+    from tests.golden import cases
+    d = next((c for c in cases.SR_CASES if c["name"] == name), {})
     m = np.zeros((w.mesh.ny, w.mesh.nx, w.mesh.ne), dtype=bool)
-    if name == "zero_field" and w.mesh.ny % 2 == 1:
+    if d.get("K", 1.) == 0. and w.mesh.ny % 2 == 1:
         m[w.mesh.ny // 2, w.mesh.nx - w.mesh.nx // 2:, :] = True
     return m
 
 
-def check_against_golden(w, rc, name, tol=FIELD_TOL, geo_rel=1e-12, mom_tol=1e-4):
+def check_against_golden(w, rc, name, tol=FIELD_TOL, geo_rel=1e-12, mom_tol=1e-4, exclude=None):
     """Field, geometry, moments and return code of wavefront w (computed by
-    the oracle or the device path) against the reference's stored result."""
+    the oracle or the device path) against the reference's stored result.
+    exclude: mask (ny, nx, ne) of points left out of the field comparison
+    (points whose reference value depends on rounding noise); the moments,
+    which include those points, are then not compared."""
     gex, gey = SR[name + "/ex"], SR[name + "/ey"]
     ex, ey = cfield(w.arEx, w), cfield(w.arEy, w)
     rex, rey = _cfield(gex, w), _cfield(gey, w)
@@ -56,6 +61,8 @@ def check_against_golden(w, rc, name, tol=FIELD_TOL, geo_rel=1e-12, mom_tol=1e-4
         for ix in range(nx // 2):
             np.testing.assert_allclose(ex[mid, nx - 1 - ix], -ex[mid, ix], rtol=1e-6, atol=0)
         ex, ey, rex, rey = (np.where(quirk, 0, a) for a in (ex, ey, rex, rey))
+    if exclude is not None:
+        ex, ey, rex, rey = (np.where(exclude, 0, a) for a in (ex, ey, rex, rey))
     e_x = rel_l2(ex, rex)
     e_y = rel_l2(ey, rey) if np.linalg.norm(rey) > 1e-6 * np.linalg.norm(rex) else 0.
     i_g = np.abs(ex) ** 2 + np.abs(ey) ** 2
@@ -73,6 +80,8 @@ def check_against_golden(w, rc, name, tol=FIELD_TOL, geo_rel=1e-12, mom_tol=1e-4
     assert rc == int(SR[name + "/rc"][0])
     if quirk.any():
         return   # the reference's moments come from its partly unfilled field
+    if exclude is not None and exclude.any():
+        return   # the moments weigh the excluded points too
     mom = SR[name + "/mom"]
     ne = w.mesh.ne
     for comp, arr in enumerate((w.arMomX, w.arMomY)):

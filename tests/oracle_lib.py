@@ -228,3 +228,31 @@ def rad_to_sub(w, L, subtype=1):
     out = (C.c_double * 2)()
     lib.orc_rad_to_sub(C.byref(cw), float(L), int(subtype), mx.ctypes.data, my.ctypes.data, out)
     return out[0], out[1], mx, my
+
+
+def edge_noise_mask(w, mag, prec, tol, nthreads=8):
+    """Points of the field computation on wavefront w (not modified) whose
+    result depends on the rounding noise of the field at the integration
+    edges: the oracle is run with the edge field and its derivative as
+    computed, zeroed and doubled (orc_edge_field_scale), and a point is
+    marked where either variant moves Ex or Ey by more than tol times that
+    component's largest magnitude.  Returns a (ny, nx, ne) bool array."""
+    import numpy as np
+    from tests.util import cfield, clone_wfr
+    knob = C.c_double.in_dll(lib, "orc_edge_field_scale")
+    out = []
+    try:
+        for sc in (1., 0., 2.):
+            knob.value = sc
+            wv = clone_wfr(w)
+            calc_field(wv, mag, prec, nthreads=nthreads)
+            out.append((cfield(wv.arEx, wv), cfield(wv.arEy, wv)))
+    finally:
+        knob.value = 1.
+    mask = np.zeros(out[0][0].shape, dtype=bool)
+    for k in (1, 2):
+        for c in (0, 1):
+            ref = out[0][c]
+            scale = max(np.abs(ref).max(), 1e-300)
+            mask |= np.abs(out[k][c] - ref) > tol * scale
+    return mask

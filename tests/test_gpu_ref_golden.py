@@ -26,8 +26,16 @@ def test_gpu_sr_vs_reference_golden(case):
     from srw_amd import srwlpy
     w, mag, prec = cases.sr_case(case)
     print("case", case["name"], flush=True)
+    tol = case.get("tol", TOL)
+    exclude = None
+    if case.get("edge_noise"):
+        # points whose reference value is decided by rounding noise (see the
+        # case's note) are left out; they must stay few
+        exclude = oracle.edge_noise_mask(w, mag, prec, tol)
+        print("  %d of %d points depend on the edge-field noise" % (exclude.sum(), exclude.size), flush=True)
+        assert exclude.mean() < 0.05
     srwlpy.CalcElecFieldSR(w, 0, mag, prec)
-    check_against_golden(w, srwlpy.last_code, case["name"], tol=TOL, geo_rel=1e-9)
+    check_against_golden(w, srwlpy.last_code, case["name"], tol=tol, geo_rel=1e-9, exclude=exclude)
 
 
 def _per_energy(wg, wr):
