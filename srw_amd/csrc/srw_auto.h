@@ -49,8 +49,8 @@ int srwd_launch_auto_tables(const srwd_trj_info* dInfo, const double* dX, const 
 // Per level L it stores |E|^2 in S[L*n + q] and the field (Ex re, im, Ez re,
 // im) in F[L*n + q]; bit L of guard[q] is set when the level's phase steps
 // stay within pi.  lvl[q] and done[q] receive the final level.  err is set to 1 when a
-// point needs more than SRWD_AUTO_MAX_LEVEL levels, warn when a terminating
-// term was rejected.
+// point given its threshold (thr) needs more than SRWD_AUTO_MAX_LEVEL levels,
+// warn when a terminating term was rejected.
 int srwd_launch_auto_points(const srwd_auto_job& J, const srwd_trj_info* dInfo, const srwd_edge* dEdge,
                             const srwd_auto_tab* dTab, const double* dX, const double* dY, const double* dIBx,
                             const double* dIBz, long long p0, long long n, const int* idx, int nIdx,
