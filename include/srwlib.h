@@ -185,6 +185,8 @@ typedef struct SRWLStructOpticsContainer {
 #define SRWL_AMD_ERR_GPU                              (300 + SRWL_FIRST_ERR)
 #define SRWL_AMD_ERR_UNSUPPORTED                      (301 + SRWL_FIRST_ERR)
 #define SRWL_AMD_ERR_BATCH                            (302 + SRWL_FIRST_ERR)
+#define SRWL_AMD_ERR_NOALLOC_INIT                     (303 + SRWL_FIRST_ERR)
+#define SRWL_AMD_ERR_NOALLOC_SIZE                     (304 + SRWL_FIRST_ERR)
 /* Warnings */
 #define SRWL_WARN_BAD_RAD_INT_TERMINATIONS            (12 + SRWL_WARN_OFFSET)
 #define SRWL_WARN_FLUX_ONLY_VS_PHOTON_ENERGY          (21 + SRWL_WARN_OFFSET)
@@ -306,6 +308,22 @@ SRWL_EXPORT int srwlCalcElecFieldSRBatch(SRWLWfr** arWfr, SRWLMagFldC** arMagFld
 SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagFldC* pMagFld,
                                    SRWLOptC* pOpt, double* precPar, int nPrecPar,
                                    long long nMacroElec, double* arPar, double* arOut);
+
+/* No counterpart in the reference: srwlCalcElecFieldSR for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and
+ * results as srwlCalcElecFieldSR; every device buffer comes from a fixed
+ * 64 MiB array per GPU (held until the process ends), so the call makes no
+ * hipMalloc / hipFree / pinned or event allocation.  Limits: method 0,
+ * ne*nx*ny <= 65536 field points, <= 20001 trajectory points, no automatic
+ * mesh (precPar[6] <= 0); larger requests return SRWL_AMD_ERR_NOALLOC_SIZE.
+ * srwlAmdNoAllocInit must have been called once on the device (it sets the
+ * library up and looks up the array's device address); otherwise
+ * SRWL_AMD_ERR_NOALLOC_INIT.  Calls on one GPU share its array and keep the
+ * last call's contents: calls that overlap on one GPU race. */
+SRWL_EXPORT int srwlAmdNoAllocInit(void);
+SRWL_EXPORT int srwlCalcElecFieldSRNoAlloc(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld, double* precPar,
+                                           int nPrecPar);
 
 #ifdef __cplusplus
 }

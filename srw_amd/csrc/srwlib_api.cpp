@@ -809,6 +809,11 @@ SRWL_EXPORT int srwlUtiGetErrText(char* t, int erNo)
     case SRWL_AMD_ERR_GPU: s = "GPU (HIP) error or no usable GPU"; break;
     case SRWL_AMD_ERR_UNSUPPORTED: s = "Parameter combination not supported by this build"; break;
     case SRWL_AMD_ERR_BATCH: s = "Inconsistent batch request"; break;
+    case SRWL_AMD_ERR_NOALLOC_INIT:
+        s = "srwlAmdNoAllocInit must be called on this device before srwlCalcElecFieldSRNoAlloc"; break;
+    case SRWL_AMD_ERR_NOALLOC_SIZE:
+        s = "Request exceeds the fixed scratch of srwlCalcElecFieldSRNoAlloc (method 0, at most 65536 field "
+            "points ne*nx*ny, at most 20001 trajectory points, no automatic mesh)"; break;
     case SRWL_WARN_FLUX_ONLY_VS_PHOTON_ENERGY: s = "Flux can only be extracted vs photon energy"; break;
     case SRWL_WARN_BAD_RAD_INT_TERMINATIONS: s = "Terminating terms of the radiation integral were not computed"; break;
     case SRWL_WARN_ELEC_BEAM_NOT_ULTRARELATIVISTIC: s = "Electron beam may be not ultra-relativistic"; break;
