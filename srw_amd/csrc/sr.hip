@@ -692,8 +692,8 @@ __global__ void __launch_bounds__(256) k_sr_tables(const srwd_trj_info* __restri
     // FP64 trajectory values at the point, for the FP64 recomputation of
     // ill-conditioned points (k_sr_fp64)
     if (T64) {
-        double* t = T64 + (tr * ldI + gi) * 5;
-        t[0] = Xv; t[1] = Zv; t[2] = Btx; t[3] = Btz; t[4] = IB;
+        double* t = T64 + (tr * ldI + gi) * SRWD_T64;
+        t[0] = Xv; t[1] = Zv; t[2] = Btx; t[3] = Btz; t[4] = s * I.gammaEm2 + IB; t[5] = inv;
     }
     if (c0 == gi) {
         srwd_chunk C; C.inv = inv; C.U = U; C.V = V; C.pad = 0;
@@ -880,7 +880,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 // that estimate exceeds condTol times the magnitude of its field (both
 // components, terminating terms included) is recomputed in FP64 (k_sr_fp64):
 // the points near the nodes of the pattern, where the sums cancel.
-#define SRWD_EPS_TERM 4e-6
+// Calibrated against FP64 recomputation of every point (201 x 201 meshes at
+// the ends and the centre of the +/-5 % band of the first harmonic; 48
+// requests of harmonics 1-5, K 0.5-2.5): an estimate built on 1e-6 still
+// bounded every measured error, 2e-6 leaves a factor 2.4 of margin.
+#define SRWD_EPS_TERM 2e-6
 
 // Terminating terms, the derivative correction, the float store and the
 // mirrored parts of one point; S = sum over the integration points of
@@ -1035,19 +1039,18 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
         const srwd_req& Rq = reqs[IT.traj];
         double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
         double kph = SRWD_KPH(IT.eph);
-        const double* T = T64 + (long long)IT.traj * ldI * 5;
+        const double* T = T64 + (long long)IT.traj * ldI * SRWD_T64;
         double acc[4] = {0, 0, 0, 0};
         for (long long i = lane; i < I.nInt; i += 64) {
-            double sv = (i == I.nInt - 1) ? I.sIntegFin : I.sIntegStart + (double)i * I.h;
-            const double* t = T + i * 5;
-            double Xv = t[0], Zv = t[1], Btx = t[2], Btz = t[3], IB = t[4];
-            double inv = 1. / (I.yObs - sv), dx = xo - Xv, dz = zo - Zv;
-            double ph = kph * (sv * I.gammaEm2 + IB + (dx * dx + dz * dz) * inv);
+            const double* t = T + i * SRWD_T64;
+            double Xv = t[0], Zv = t[1], Btx = t[2], Btz = t[3], sQ = t[4], inv = t[5];
+            double dx = xo - Xv, dz = zo - Zv;
+            double ph = kph * (sQ + (dx * dx + dz * dz) * inv);
             double c, sn;
             cos_sin_taylor(ph, c, sn);
-            double ax = (Btx - dx * inv) * inv, az = (Btz - dz * inv) * inv;
             double wt = (i == 0 || i == I.nInt - 1) ? 7. : ((i & 1) ? 16. : 14.);
-            acc[0] += wt * ax * c; acc[1] += wt * ax * sn; acc[2] += wt * az * c; acc[3] += wt * az * sn;
+            double ax = wt * ((Btx - dx * inv) * inv), az = wt * ((Btz - dz * inv) * inv);
+            acc[0] += ax * c; acc[1] += ax * sn; acc[2] += az * c; acc[3] += az * sn;
         }
         for (int m = 0; m < 4; m++)
             for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off);

@@ -110,6 +110,9 @@ struct srwd_req {
     long long nPts;           // nzc * nxc * ne
 };
 #define SRWD_EDGE_BLOCK 256
+// FP64 values per integration point for the FP64 recomputation: X, Z, X',
+// Z', s/gamma^2 + the integrals of the squared angles, 1/(yObs - s).
+#define SRWD_T64 6
 
 extern "C" {
 // Launch wrappers implemented in sr.hip; all asynchronous on stream st.
@@ -160,7 +163,7 @@ int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec*
 // FP64 recomputation and store of the listed points (block indices relative
 // to dBlockItem, as passed to srwd_launch_finalize).
 // dT64: FP64 trajectory values (X, Z, X', Z', integral of the squared angles)
-// per integration point, 5 doubles, as written by srwd_launch_tables.
+// per integration point, SRWD_T64 doubles, as written by srwd_launch_tables.
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
                         const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
                         float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, const srwd_req* dReq,

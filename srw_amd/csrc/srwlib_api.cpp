@@ -289,7 +289,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     srwd_edge* dEdge = (srwd_edge*)B.Edge.get(sizeof(srwd_edge) * nT);
     double* dMesh = (double*)B.Mesh.get(sizeof(double) * 4 * nT);
     double* dRx = (double*)B.Rx.get(sizeof(double) * 4 * nT);
-    double* dT64 = g_ctx.condTol > 0 ? (double*)B.T64.get(sizeof(double) * 5 * (size_t)ldI * nT) : nullptr;
+    double* dT64 = g_ctx.condTol > 0 ? (double*)B.T64.get(sizeof(double) * SRWD_T64 * (size_t)ldI * nT) : nullptr;
     if (!dRec || !dChunk || !dQ || !dEdge || !dMesh || !dRx || (g_ctx.condTol > 0 && !dT64))
         return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     if (srwd_launch_tables(dInfo, nT, maxInt, dX, dXp, dY, dYp, ldT, dIBx, dIBz, dRec, dChunk, dQ, ldI, ldC,
