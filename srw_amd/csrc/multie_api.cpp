@@ -100,12 +100,14 @@ void proposal_set(Proposal& Q, const float* o)
     }
 }
 
-// log p(g) and log q(g) of the defensive mixture (same formulas as k_gmm_grad)
-void proposal_logpq(const Proposal& Q, const double* g, double& lp, double& lq)
+// log p(g) and log q(g) of the defensive mixture (same formulas as
+// k_gmm_grad) over the first nd coordinates (a mixture of diagonal Gaussians
+// marginalised over the others)
+void proposal_logpq(const Proposal& Q, const double* g, double& lp, double& lq, int nd = SRWL_D)
 {
     const double L2P = 1.8378770664093453;
-    lp = -0.5 * SRWL_D * L2P;
-    for (int d = 0; d < SRWL_D; d++) lp -= 0.5 * g[d] * g[d];
+    lp = -0.5 * nd * L2P;
+    for (int d = 0; d < nd; d++) lp -= 0.5 * g[d] * g[d];
     double lmax = -1e300;
     for (int k = 0; k < SRWL_K; k++) lmax = std::max(lmax, Q.logit[k]);
     double se = 0;
@@ -114,7 +116,7 @@ void proposal_logpq(const Proposal& Q, const double* g, double& lp, double& lq)
     double lN[SRWL_K], mm = -1e300;
     for (int k = 0; k < SRWL_K; k++) {
         double s = Q.logit[k] - logZ;
-        for (int d = 0; d < SRWL_D; d++) {
+        for (int d = 0; d < nd; d++) {
             double z = (g[d] - Q.mu[k][d]) * std::exp(-Q.ls[k][d]);
             s += -0.5 * z * z - Q.ls[k][d] - 0.5 * L2P;
         }
@@ -147,6 +149,39 @@ void proposal_draw(const Proposal& Q, unsigned long long seed, unsigned long lon
     double cum = 0;
     for (int k = 0; k < SRWL_K; k++) { cum += pk[k] / se; if (b < cum) { kk = k; break; } }
     for (int d = 0; d < SRWL_D; d++) g[d] = Q.mu[kk][d] + std::exp(Q.ls[kk][d]) * z[d];
+}
+
+// The single Gaussian the quasi-Monte Carlo path draws from: the mixture's
+// Gaussians moment-matched per coordinate, widths held at >= 1 (the beam
+// distribution's), so that p/q stays bounded without a defensive part.  An
+// affine map of the normals keeps the integrand as smooth in the Sobol'
+// point as under p, which a mixture's component choice would not.
+void proposal_single(const Proposal& Q, double* m, double* sd)
+{
+    double lmax = -1e300, se = 0., pk[SRWL_K];
+    for (int k = 0; k < SRWL_K; k++) lmax = std::max(lmax, Q.logit[k]);
+    for (int k = 0; k < SRWL_K; k++) { pk[k] = std::exp(Q.logit[k] - lmax); se += pk[k]; }
+    for (int d = 0; d < 5; d++) {
+        double a = 0., b = 0.;
+        for (int k = 0; k < SRWL_K; k++) {
+            const double w = pk[k] / se, s2 = std::exp(2. * Q.ls[k][d]);
+            a += w * Q.mu[k][d];
+            b += w * (s2 + Q.mu[k][d] * Q.mu[k][d]);
+        }
+        m[d] = a;
+        sd[d] = std::max(1., std::sqrt(std::max(b - a * a, 0.)));
+    }
+}
+
+// log p(g) - log q(g) for the single Gaussian of proposal_single (5-D).
+double single_logw(const double* m, const double* sd, const double* g)
+{
+    double r = 0.;
+    for (int d = 0; d < 5; d++) {
+        const double z = (g[d] - m[d]) / sd[d];
+        r += -0.5 * g[d] * g[d] + 0.5 * z * z + std::log(sd[d]);
+    }
+    return r;
 }
 
 }  // namespace srwamd
@@ -277,7 +312,6 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     // replicate.
     const int rqmc = (arPar && arPar[6] > 0) ? 1 : 0;
     const int R = rqmc ? ((arPar && arPar[7] > 1) ? (int)arPar[7] : 16) : 1;
-    if (rqmc && useQ) return SRWL_AMD_ERR_UNSUPPORTED;
     if (rqmc) {
         if (nMacroElec < R) return SRWL_AMD_ERR_BATCH;
         B = std::max<long long>(R, (B / R) * R);
@@ -305,6 +339,15 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
     Proposal Q;
     std::vector<float> hOut(SRWL_OUT);
     double lastLoss = 0;
+    // The proposal is used only where the electrons drawn so far predict a
+    // gain: the variance of w I under the current q over that of I under p,
+    // both estimated from the replay set, must fall below kUseRatio;
+    // otherwise the batch is drawn from p itself, plain sampling being the
+    // fallback.
+    const double kUseRatio = 0.8;
+    double lastRatio = -1.;
+    double sgM[5] = {0, 0, 0, 0, 0}, sgS[5] = {1, 1, 1, 1, 1};   // single Gaussian of the RQMC path
+    long long mixBatches = 0;
     if (useQ) {
         float* base = (float*)Bf.Net.get(sizeof(float) * (4 * SRWL_NPARAM + SRWL_NACT + SRWL_OUT + 8 + 7 * kReplay + 64));
         if (!base) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
@@ -351,12 +394,49 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
             HIPCHK(hipMemcpyAsync(hOut.data(), Q.act + SRWL_ACT_OUT, sizeof(float) * SRWL_OUT, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             proposal_set(Q, hOut.data());
-            // the first batch, before any training, is drawn from p itself
-            Q.alpha = (Q.step == 0) ? 1.0 : 0.25;
+            // the first batch, before any training, is drawn from p itself;
+            // later ones from the mixture only where it predicts a gain
+            Q.alpha = 0.25;
+            if (rqmc) proposal_single(Q, sgM, sgS);
+            bool use = false;
+            if (Q.step > 0 && !repW.empty() && done > 0) {
+                const double m2 = lastA1 / ((double)done * (double)done);
+                double a = 0., b = 0.;
+                for (size_t j = 0; j < repW.size(); j++) {
+                    double gg[SRWL_D], lw;
+                    for (int d = 0; d < SRWL_D; d++) gg[d] = repG[6 * j + d];
+                    if (rqmc) lw = single_logw(sgM, sgS, gg);
+                    else {
+                        double lp, lq;
+                        proposal_logpq(Q, gg, lp, lq);
+                        lw = lp - lq;
+                    }
+                    const double n2 = repC[j] * repC[j];
+                    a += repW[j] * std::exp(lw) * n2;
+                    b += repW[j] * n2;
+                }
+                const double nrep = (double)repW.size();
+                const double vq = a / nrep - m2, vp = b / nrep - m2;
+                lastRatio = vp > 0. ? vq / vp : -1.;
+                use = lastRatio >= 0. && lastRatio < kUseRatio;
+            }
+            if (!use) {
+                Q.alpha = 1.0;
+                for (int d = 0; d < 5; d++) { sgM[d] = 0.; sgS[d] = 1.; }
+            } else mixBatches++;
         }
         for (long long j = 0; j < nb; j++) {
             double g[6];
-            if (useQ) {
+            if (useQ && rqmc) {
+                // slot j: replicate j / perRep, its point done / R + j % perRep
+                const unsigned r = (unsigned)(j / perRep);
+                const unsigned long long k = (unsigned long long)(done / R + j % perRep);
+                for (int d = 0; d < 5; d++)
+                    g[d] = sgM[d] + sgS[d] * srw_norm_inv(srw_u01(srw_sobol_word(sobolV, d, k) ^ srw_rqmc_shift(seed, r, d)));
+                g[5] = 0.;
+                w[j] = (float)std::exp(single_logw(sgM, sgS, g));
+                for (int d = 0; d < 6; d++) gsamp[6 * j + d] = (float)g[d];
+            } else if (useQ) {
                 proposal_draw(Q, seed, (unsigned long long)(done + j), g);
                 double lp, lq;
                 proposal_logpq(Q, g, lp, lq);
@@ -626,6 +706,10 @@ SRWL_EXPORT int srwlEmitPropMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLMagF
         arOut[6] = lastLoss;
         arOut[7] = useQ ? Q.step : 0;
         if (rqmc) arOut[9] = R;
+        if (useQ) {
+            arOut[10] = (double)mixBatches;   // batches drawn from the mixture (the others from p)
+            arOut[11] = lastRatio;            // last predicted variance ratio q vs p
+        }
         if (diag) {
             // N Var = E||I||^2 - ||mu||^2 for p; (E||I||)^2 - ||mu||^2 for q* ~ p ||I||
             double N = (double)done, m2 = lastA1 / (N * N);

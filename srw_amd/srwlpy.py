@@ -305,9 +305,12 @@ def EmitPropMultiE(_stokes, _wfr0, _mag, _opt_cnt, _prec_par, _n_macro, _seed=1,
     """Partially coherent intensity averaged over _n_macro electrons drawn from _wfr0.partBeam
     (replaces the per-electron loop of srwl_wfr_emit_prop_multi_e, reference srwlib.py:8506).
     _stokes.arS (float, nx*ny of _stokes.mesh) receives the averaged total intensity.
-    _proposal=1: learned importance-sampling proposal; _rqmc=1: randomized
-    quasi-Monte Carlo draws in _replicates independently shifted replicates.
-    Returns a dict of run information."""
+    _proposal=1: learned importance-sampling proposal (a batch is drawn from
+    it only where the electrons so far predict a variance gain, otherwise from
+    the beam distribution itself); _rqmc=1: randomized quasi-Monte Carlo draws
+    in _replicates independently shifted replicates (with _proposal=1 the
+    proposal is sampled through the same points).  Returns a dict of run
+    information."""
     keep = M.Keep()
     cs = M.stokes(_stokes, keep)
     cw = M.wfr(_wfr0, keep)
@@ -320,8 +323,8 @@ def EmitPropMultiE(_stokes, _wfr0, _mag, _opt_cnt, _prec_par, _n_macro, _seed=1,
     _check(lib.srwlEmitPropMultiE(C.byref(cs), C.byref(cw), C.byref(cm), C.byref(co) if co is not None else None,
                                   prec, len(_prec_par), int(_n_macro), par, out))
     keys = ("electrons", "ess", "sum_w", "seconds", "rel_l2_est", "batch", "train_loss", "train_steps",
-            "ideal_gain", "replicates")
-    return dict(zip(keys, list(out)[:10]))
+            "ideal_gain", "replicates", "mixture_batches", "predicted_ratio")
+    return dict(zip(keys, list(out)[:12]))
 
 
 def normals(seed, i):
