@@ -464,7 +464,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "precision": ("FP32 integrand, 32-point FP32 block sums added with Kahan compensation per 256-point chunk, "
+        "precision": ("FP32 integrand, 64-point FP32 block sums added with Kahan compensation per 256-point chunk, "
                       "FP64 chunk sums; FP64 trajectory, tables and per-chunk phase reference; points failing the "
                       "FP32 condition check recomputed in FP64 (%d in the timed calls' last pass)" % fp64_points),
         "data": "synthetic",
