@@ -300,7 +300,17 @@ __global__ void __launch_bounds__(1024) k_cols(float2* data, long long sliceStri
         cols_block(s, stw, S, base, out, outOff, c0 + colsPerWG, colsPerWG, lc, logny, lognx, P, op, inv);
         return;
     }
-    const int c0 = blockIdx.x * colsPerWG;
+    // XCD-aware order.  Workgroups go round-robin to the 8 XCDs, each with
+    // its own L2; the L = 128 / (8 colsPerWG) column blocks that share a
+    // row's 128-byte lines are given to workgroups 8 ids apart, which land on
+    // the same XCD at nearly the same time, so each line is fetched once.
+    int g = blockIdx.x;
+    const int L = colsPerWG >= 16 ? 1 : 16 / colsPerWG;
+    if (L > 1 && gridDim.x % (8 * L) == 0) {
+        const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+        g = ((k / L) * 8 + xcd) * L + (k % L);
+    }
+    const int c0 = g * colsPerWG;
     const int total = NY * colsPerWG;
     for (int e0 = threadIdx.x; e0 < total; e0 += SRWF_BATCH * blockDim.x) {
         float2 vb[SRWF_BATCH];

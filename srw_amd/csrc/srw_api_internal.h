@@ -79,6 +79,8 @@ struct Context {
     // row bands into pinned staging while the next band is computed; the
     // calling thread moves them on into the caller's arrays.
     hipStream_t copyStream = nullptr;
+    hipStream_t stream2 = nullptr;          // second compute stream: alternate row bands overlap their tails
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
     char* pin = nullptr;                    // pinned staging (grow-only)
     size_t pinBytes = 0;
     std::vector<hipEvent_t> bandEv;         // per band: computed, downloaded, SR start, SR end

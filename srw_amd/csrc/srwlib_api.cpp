@@ -37,6 +37,9 @@ int ensure_device()
     if (hipSetDevice(dev) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipStreamCreateWithFlags(&g_ctx.copyStream, hipStreamNonBlocking) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    if (hipStreamCreateWithFlags(&g_ctx.stream2, hipStreamNonBlocking) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    if (hipEventCreateWithFlags(&g_ctx.evFork, hipEventDisableTiming) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    if (hipEventCreateWithFlags(&g_ctx.evJoin, hipEventDisableTiming) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipHostMalloc((void**)&g_ctx.warnHost, sizeof(int), hipHostMallocDefault) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipHostMalloc((void**)&g_ctx.fpCountHost, sizeof(int) * 64, hipHostMallocDefault) != hipSuccess)
         return SRWL_AMD_ERR_GPU;
@@ -465,7 +468,9 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     int maxBandBlocks = 0;
     for (int b = 0; b < nBands; b++) maxBandBlocks = std::max(maxBandBlocks, bandBlock0[b + 1] - bandBlock0[b]);
     double* dCond = (double*)B.Cond.get(sizeof(double) * 5 * nT);
-    int* dFpList = (int*)B.FpList.get(sizeof(int) * 256 * (size_t)maxBandBlocks);
+    // one list per compute stream: consecutive bands run on alternate streams
+    const int nLists = nBands > 1 ? 2 : 1;
+    int* dFpList = (int*)B.FpList.get(sizeof(int) * 256 * (size_t)maxBandBlocks * nLists);
     int* dFpCount = (int*)B.FpCount.get(sizeof(int) * (size_t)nBands);
     if (!dItems || !dBlockItem || !dRowTraj || !dRowE || !dRowN || !dA || !dPart || !dWarn || !dCond || !dFpList ||
         !dFpCount)
@@ -520,25 +525,33 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             par_memcpy((char*)hostOut->ez + o, pin + compBytes + o, n);
         }
     };
+    if (nBands > 1) {
+        // bands alternate between the two compute streams, so that a band's
+        // last blocks share the GPU with the next band's first ones
+        HIPCHK(hipEventRecord(g_ctx.evFork, st));
+        HIPCHK(hipStreamWaitEvent(g_ctx.stream2, g_ctx.evFork, 0));
+    }
     for (int b = 0; b < nBands; b++) {
         int b0 = bandBlock0[b], nb = bandBlock0[b + 1] - b0;
+        hipStream_t sb = (nBands > 1 && (b & 1)) ? g_ctx.stream2 : st;
+        int* fpList = dFpList + (size_t)256 * maxBandBlocks * (b & (nLists - 1));
         if (nb > 0) {
-            if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 2] : g_ctx.ev0, st));
+            if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 2] : g_ctx.ev0, sb));
             if (srwd_launch_sr(dItems, dBlockItem + b0, nb, dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad,
-                               dPart, st))
+                               dPart, sb))
                 return SRWL_AMD_ERR_GPU;
-            if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 3] : g_ctx.ev1, st));
+            if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 3] : g_ctx.ev1, sb));
             if (srwd_launch_finalize(dItems, dBlockItem + b0, nb, dInfo, dEdge, termin, dPart, nSplit, nPointsPad,
-                                     dEx, dEz, dWarn, g_ctx.condTol > 0 ? dCond : nullptr, g_ctx.condTol, dFpList,
-                                     dFpCount + b, dReq, dCoord, dLast, st))
+                                     dEx, dEz, dWarn, g_ctx.condTol > 0 ? dCond : nullptr, g_ctx.condTol, fpList,
+                                     dFpCount + b, dReq, dCoord, dLast, sb))
                 return SRWL_AMD_ERR_GPU;
             if (g_ctx.condTol > 0 && srwd_launch_sr_fp64(dItems, dBlockItem + b0, dInfo, dEdge, termin, dT64, ldI, dEx,
-                                                         dEz, dWarn, dFpList, dFpCount + b, dReq, dCoord, dLast, st))
+                                                         dEz, dWarn, fpList, dFpCount + b, dReq, dCoord, dLast, sb))
                 return SRWL_AMD_ERR_GPU;
         }
         if (nBands > 1) {
             // download band b on the copy stream once computed
-            HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b], st));
+            HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b], sb));
             HIPCHK(hipStreamWaitEvent(g_ctx.copyStream, g_ctx.bandEv[4 * b], 0));
             int r[4];
             rows_of_band(b, r);
@@ -551,6 +564,10 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             }
             HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 1], g_ctx.copyStream));
         }
+    }
+    if (nBands > 1) {
+        HIPCHK(hipEventRecord(g_ctx.evJoin, g_ctx.stream2));
+        HIPCHK(hipStreamWaitEvent(st, g_ctx.evJoin, 0));
     }
     // the warning flag and the FP64 point counts are read back with the caller's final synchronisation
     HIPCHK(hipMemcpyAsync(g_ctx.warnHost, dWarn, sizeof(int), hipMemcpyDeviceToHost, st));
