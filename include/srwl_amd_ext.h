@@ -49,6 +49,11 @@ SRWL_EXPORT int srwlAmdSetTrajSerial(int on);
  * off. */
 SRWL_EXPORT int srwlAmdSetCondTol(double tol);
 
+/* e = 8 (default) or 4: requests with at least e photon energies integrate
+ * them e at a time, advancing each point's phasor from one energy to the
+ * next; 0: one energy per workgroup (for comparison). */
+SRWL_EXPORT int srwlAmdSetEnergyTiles(int e);
+
 /* Final level (1..20) of every computed point of the last method-1 field
  * computation (precPar[0] = 1), in the reference's point order: rows, then
  * columns, then photon energies of the computed region, request by request.

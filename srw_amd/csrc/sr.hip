@@ -758,7 +758,7 @@ __device__ __forceinline__ void kahan_add(float& sum, float& c, float v)
 static_assert(SRWD_CHUNK == 256, "k_sr_integral stages one chunk per 256-thread block");
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_sr_integral(const srwd_item* __restrict__ items,
-                                                     const int* __restrict__ blockItem,
+                                                     const int2* __restrict__ blocks,
                                                      const srwd_trj_info* __restrict__ info,
                                                      const srwd_rec* __restrict__ rec,
                                                      const srwd_chunk* __restrict__ chunks,
@@ -766,9 +766,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
                                                      long long ldI, long long ldC, int nSplit,
                                                      long long nPointsPad, double* __restrict__ part)
 {
-    const int it = blockItem[blockIdx.x];
-    const srwd_item& IT = items[it];
-    const int local = (blockIdx.x - IT.blockBase) * 256 + threadIdx.x;
+    const int2 bk = blocks[blockIdx.x];
+    const srwd_item& IT = items[bk.x];
+    const int local = bk.y * 256 + threadIdx.x;
     const long long flat = IT.pointBase + local;
     const bool valid = local < IT.npts;
     const srwd_trj_info& I = info[IT.traj];
@@ -792,7 +792,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     //   the phase is formed once per block instead of once per lane;
     // * otherwise one float4 per s = {A[s], b, b Px, b Pz}.
     __shared__ float4 sA[SRWD_ROWS][SRWD_CHUNK];
-    const int first = (blockIdx.x - IT.blockBase) * 256;
+    const int first = bk.y * 256;
     const int last = min(first + 255, IT.npts - 1);
     const int rLo = IT.iz0 + first / IT.nxc, nRows = IT.iz0 + last / IT.nxc - rLo + 1;
     const bool byRow = nRows <= SRWD_ROWS;
@@ -869,6 +869,148 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     o[0] = ox; o[1] = oxi; o[2] = oz; o[3] = ozi;
 }
 
+// The same integral for SRWD_TILE_E consecutive photon energies of one
+// request at once.  The phase of energy k is linear in k (K_k = K_0 + k dK,
+// and the phase table rows obey A_k = A_0 + k A_dK mod 1), so per s a lane
+// forms two phases, t_0 and the step dt, takes their sines and cosines and
+// advances the phasor exp(2 pi i t_k) by one complex multiply per energy:
+// per evaluation that replaces 2 transcendentals and 2 phase FMAs by 4 FP32
+// operations.  64-point FP32 block sums are turned by each energy's FP64
+// chunk phase and accumulated in FP64.
+template <int E>
+__global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __restrict__ items,
+                                                          const int2* __restrict__ blocks,
+                                                          const srwd_trj_info* __restrict__ info,
+                                                          const srwd_rec* __restrict__ rec,
+                                                          const srwd_chunk* __restrict__ chunks,
+                                                          const float* __restrict__ A,
+                                                          long long ldI, long long ldC, int nSplit,
+                                                          long long nPointsPad, double* __restrict__ part)
+{
+    const int2 bk = blocks[blockIdx.x];
+    const srwd_item& IT = items[bk.x];
+    const int local = bk.y * 256 + threadIdx.x;
+    const bool valid = local < IT.npts;
+    const srwd_trj_info& I = info[IT.traj];
+    int ix = valid ? local % IT.nxc : 0, iz = valid ? IT.iz0 + local / IT.nxc : 0;
+    double xo = IT.xStart + ix * IT.xStep, zo = IT.zStart + iz * IT.zStep;
+    const double k0 = SRWD_KPH(IT.eph) * (0.5 / M_PI), kd = SRWD_KPH(IT.de) * (0.5 / M_PI);
+    double r2 = xo * xo + zo * zo;
+    const float a0 = (float)(k0 * r2), b0 = (float)(-2. * k0 * xo), g0 = (float)(-2. * k0 * zo);
+    const float ad = (float)(kd * r2), bd = (float)(-2. * kd * xo), gd = (float)(-2. * kd * zo);
+    const float nfxo = -(float)xo, nfzo = -(float)zo;
+    long long nInt = I.nInt;
+    long long nChunk = (nInt + SRWD_CHUNK - 1) / SRWD_CHUNK;
+    long long cBeg = (nChunk * blockIdx.y) / nSplit, cEnd = (nChunk * (blockIdx.y + 1)) / nSplit;
+    const srwd_rec* R = rec + (long long)IT.traj * ldI;
+    const srwd_chunk* CH = chunks + (long long)IT.traj * ldC;
+    const float* A0 = A + (long long)IT.arow * ldI;
+    const float* Ad = A + (long long)IT.drow * ldI;
+    double out[E][4];
+#pragma unroll
+    for (int e = 0; e < E; e++) out[e][0] = out[e][1] = out[e][2] = out[e][3] = 0.;
+    // per (row, s): {A_0 + g0_row dV, A_dK + gd_row dV, b Px, b Pz} (rows as in
+    // k_sr_integral), and b
+    __shared__ float4 sA[SRWD_ROWS][SRWD_CHUNK];
+    __shared__ float sB[SRWD_CHUNK];
+    __shared__ float2 sRot[E][SRWD_CHUNK];
+    const int first = bk.y * 256;
+    const int last = min(first + 255, IT.npts - 1);
+    const int rLo = IT.iz0 + first / IT.nxc, nRows = IT.iz0 + last / IT.nxc - rLo + 1;
+    const bool byRow = nRows <= SRWD_ROWS;
+    const int rs = byRow && valid ? iz - rLo : 0;
+    for (long long c = cBeg; c < cEnd; c++) {
+        const srwd_chunk C = CH[c];
+        const double qc = r2 * C.inv - 2. * xo * C.U - 2. * zo * C.V;
+        long long i0 = c * SRWD_CHUNK, i1 = i0 + SRWD_CHUNK;
+        if (i1 > nInt) i1 = nInt;
+        const int n = (int)(i1 - i0);
+        const srwd_rec* Rc = R + i0;
+        __syncthreads();
+        // each energy's chunk phase, exactly (FP64), as a rotation (kept in
+        // LDS, per lane, to save registers)
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            double tc = SRWD_KPH(items[bk.x + e].eph) * (0.5 / M_PI) * qc;
+            float Tc = (float)(tc - rint(tc));
+            sRot[e][threadIdx.x] = make_float2(__builtin_amdgcn_cosf(Tc), __builtin_amdgcn_sinf(Tc));
+        }
+        if ((int)threadIdx.x < n) {
+            const srwd_rec& q = Rc[threadIdx.x];
+            const float u0 = A0[i0 + threadIdx.x], ud = Ad[i0 + threadIdx.x];
+            sB[threadIdx.x] = q.b;
+            if (byRow) {
+                for (int rr = 0; rr < nRows; rr++) {
+                    // the same floats as the lanes' g0, gd for that row
+                    const double zr = IT.zStart + (rLo + rr) * IT.zStep;
+                    const float g0r = (float)(-2. * k0 * zr), gdr = (float)(-2. * kd * zr);
+                    sA[rr][threadIdx.x] = make_float4(fmaf(g0r, q.dV, u0), fmaf(gdr, q.dV, ud), q.bPx, q.bPz);
+                }
+            } else {
+                sA[0][threadIdx.x] = make_float4(u0, ud, q.bPx, q.bPz);
+            }
+        }
+        __syncthreads();
+        const float4* S = sA[rs];
+        for (int bb = 0; bb < n; bb += SRWD_BLOCK) {
+            float acc[E][4];
+#pragma unroll
+            for (int e = 0; e < E; e++) acc[e][0] = acc[e][1] = acc[e][2] = acc[e][3] = 0.f;
+            const int b1 = bb + SRWD_BLOCK < n ? bb + SRWD_BLOCK : n;
+            // one evaluation of E energies at s index k; the dV terms are
+            // folded into the staged rows when byRow
+            auto eval = [&](int k, bool rowTerms) {
+                const srwd_rec r = Rc[k];        // scalar loads: dInv, dU, dV
+                const float4 L4 = S[k];
+                const float b = sB[k];
+                float t = fmaf(a0, r.dInv, L4.x), d = fmaf(ad, r.dInv, L4.y);
+                t = fmaf(b0, r.dU, t);
+                d = fmaf(bd, r.dU, d);
+                if (rowTerms) {
+                    t = fmaf(g0, r.dV, t);
+                    d = fmaf(gd, r.dV, d);
+                }
+                float zr = __builtin_amdgcn_cosf(t), zi = __builtin_amdgcn_sinf(t);
+                const float wr = __builtin_amdgcn_cosf(d), wi = __builtin_amdgcn_sinf(d);
+                const float wax = fmaf(b, nfxo, L4.z), waz = fmaf(b, nfzo, L4.w);   // b (P - r_o)
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    acc[e][0] = fmaf(wax, zr, acc[e][0]);
+                    acc[e][1] = fmaf(wax, zi, acc[e][1]);
+                    acc[e][2] = fmaf(waz, zr, acc[e][2]);
+                    acc[e][3] = fmaf(waz, zi, acc[e][3]);
+                    if (e + 1 < E) {
+                        const float nr = fmaf(zr, wr, -zi * wi), ni = fmaf(zr, wi, zi * wr);
+                        zr = nr;
+                        zi = ni;
+                    }
+                }
+            };
+            if (byRow) {
+#pragma unroll 2
+                for (int k = bb; k < b1; k++) eval(k, false);
+            } else {
+#pragma unroll 2
+                for (int k = bb; k < b1; k++) eval(k, true);
+            }
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const float2 rt = sRot[e][threadIdx.x];
+                const double ct = rt.x, st = rt.y;
+                const double xr = acc[e][0], xi = acc[e][1], zr = acc[e][2], zi = acc[e][3];
+                out[e][0] += xr * ct - xi * st; out[e][1] += xr * st + xi * ct;
+                out[e][2] += zr * ct - zi * st; out[e][3] += zr * st + zi * ct;
+            }
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        double* o = part + ((long long)blockIdx.y * nPointsPad + items[bk.x + e].pointBase + local) * 4;
+        o[0] = out[e][0]; o[1] = out[e][1]; o[2] = out[e][2]; o[3] = out[e][3];
+    }
+}
+
 // ------------------------------------------------------------------------
 // Finalisation: partial sums, terminating terms (srw_sr_dev.h), symmetry
 // ------------------------------------------------------------------------
@@ -885,6 +1027,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 // requests of harmonics 1-5, K 0.5-2.5): an estimate built on 1e-6 still
 // bounded every measured error, 2e-6 leaves a factor 2.4 of margin.
 #define SRWD_EPS_TERM 2e-6
+// Energy-tile items: the phasor of energy k carries k complex multiplies and
+// k times the rounding of the step's phase on top.
+#define SRWD_EPS_TERM_TILE 4e-6
 
 // Terminating terms, the derivative correction, the float store and the
 // mirrored parts of one point; S = sum over the integration points of
@@ -925,7 +1070,8 @@ __device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, con
     if (fpList) {
         // condition check of the FP32 sums against the point's field
         // magnitude: hand the point to the FP64 pass
-        const double e2 = SRWD_EPS_TERM * SRWD_EPS_TERM * w * w, t2 = condTol * condTol;
+        const double eps = IT.tiled ? SRWD_EPS_TERM_TILE : SRWD_EPS_TERM;
+        const double e2 = eps * eps * w * w, t2 = condTol * condTol;
         const double mag2 = ex.re * ex.re + ex.im * ex.im + ez.re * ez.re + ez.im * ez.im;
         if (e2 * (fmax(sumSqX, 0.) + fmax(sumSqZ, 0.)) > t2 * mag2) {
             int k = atomicAdd(fpCount, 1);
@@ -1185,12 +1331,22 @@ int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
+int srwd_launch_sr(const srwd_item* dItems, const int2* dBlocks, int nBlocks, int tiled, const srwd_trj_info* dInfo,
                    const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA, long long ldI,
                    long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sr_integral, dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlockItem, dInfo, dRec,
-                       dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart);
+    if (nBlocks <= 0) return 0;
+    if (tiled == 8)
+        hipLaunchKernelGGL(k_sr_integral_tile<8>, dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlocks,
+                           dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart);
+    else if (tiled == 4)
+        hipLaunchKernelGGL(k_sr_integral_tile<4>, dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlocks,
+                           dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart);
+    else if (tiled)
+        return -1;
+    else
+        hipLaunchKernelGGL(k_sr_integral, dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlocks, dInfo, dRec,
+                           dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

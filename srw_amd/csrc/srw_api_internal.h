@@ -39,6 +39,7 @@ struct Buffers {
     DBuf Cond, FpList, FpCount;   // condition check of the FP32 sums and its FP64 recomputation list
     DBuf T64;                     // FP64 trajectory values per integration point (FP64 recomputation)
     DBuf Req, Coord, BlkReq, Last;   // requests, observation coordinates, edge-derivative state
+    DBuf SrBlocks;                // SR launch lists {item, block}
     DBuf Ex, Ez;                  // field outputs
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
     DBuf Stokes, Draws, Wt, Net, Grad, Rep;
@@ -89,6 +90,7 @@ struct Context {
     int fpBands = 0;
     long long lastFp64Points = 0;           // points of the last SR pass recomputed in FP64
     double condTol = 1e-4;                  // condition check of the FP32 sums (0: off)
+    int tileE = SRWD_TILE_E;                // energies per energy-tile SR workgroup (0: single-energy kernel only)
     long long pendingEvals = 0;             // integrand evaluations of the last SR pass
     std::vector<int> autoLevels;            // final levels of the last method-1 pass, in point order
     int autoSweeps = 0;                     // fixed-point sweeps of the last method-1 pass (largest chunk)

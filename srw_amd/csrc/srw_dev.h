@@ -93,9 +93,16 @@ struct srwd_item {
     int blockBase;            // first 256-point block of this item
     int npts;                 // nxc * nzc
     double actNorm;           // normalisation constant * eph * 0.80654658e-3
+    int drow;                 // phase-table row of the energy step (energy tiles), -1 if none
+    int tiled;                // 1: computed by the energy-tile kernel (its FP32 error model differs)
+    double de;                // energy step of the request's mesh
 };
 
 #define SRWD_CHUNK 256
+// Photon energies per workgroup of the energy-tile SR kernel (4 or 8): the
+// phasor of energy k is advanced from that of energy k - 1 by one complex
+// multiply.
+#define SRWD_TILE_E 8
 
 // One request of a field computation, for the parts that follow the
 // reference's point loop (rows, then columns, then photon energies of the
@@ -137,7 +144,11 @@ int srwd_launch_rx(const srwd_trj_info* dInfo, int nTraj, long long maxNp, const
                    double* dRx, hipStream_t st);
 int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj, const double* dRowE,
                             const long long* dRowN, int nRows, long long maxInt, float* dA, hipStream_t st);
-int srwd_launch_sr(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
+// SR integral over a list of 256-point blocks: dBlocks[i] = {item, block of
+// the item}.  tiled = 0: one photon energy per block; tiled = 4 or 8: the
+// block's item and the tiled - 1 items after it (consecutive photon energies
+// of one request and band, with the same points).
+int srwd_launch_sr(const srwd_item* dItems, const int2* dBlocks, int nBlocks, int tiled, const srwd_trj_info* dInfo,
                    const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA, long long ldI,
                    long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st);
 // Edge-derivative state of the reference's point loop: per block of

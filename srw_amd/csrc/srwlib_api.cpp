@@ -400,6 +400,18 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             rowTraj.push_back(j); rowE.push_back(coord[reqs[j].eOff + ie]); rowN.push_back(info[j].nInt);
         }
     }
+    // energy tiles: a request with at least SRWD_TILE_E photon energies gets a
+    // phase-table row for its energy step, and its energies in groups of
+    // SRWD_TILE_E go to the energy-tile kernel
+    std::vector<int> dRowOf(nT, -1);
+    std::vector<double> eStepOf(nT, 0.);
+    for (int j = 0; j < nT; j++) {
+        const SRWLRadMesh& m = jobs[j].mesh;
+        if (g_ctx.tileE < 2 || m.ne < g_ctx.tileE) continue;
+        eStepOf[j] = (m.eFin - m.eStart) / (m.ne - 1);
+        dRowOf[j] = (int)rowTraj.size();
+        rowTraj.push_back(j); rowE.push_back(eStepOf[j]); rowN.push_back(info[j].nInt);
+    }
     int nBands = 1;
     if (hostOut && nT == 1 && !g_arena) {
         SrRegion G = sr_region(jobs[0].mesh, info[0]);
@@ -410,8 +422,14 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     std::vector<srwd_item> items;
     std::vector<int> blockItem;
     std::vector<int> bandBlock0(nBands + 1, 0), bandRow0(nBands + 1, 0);
+    // SR launch lists: {item, block of the item}; tile blocks name the first
+    // item of a group of SRWD_TILE_E energies
+    std::vector<int2> tileBlocks, restBlocks;
+    std::vector<int> bandTile0(nBands + 1, 0), bandRest0(nBands + 1, 0);
     for (int b = 0; b < nBands; b++) {
         bandBlock0[b] = (int)blockItem.size();
+        bandTile0[b] = (int)tileBlocks.size();
+        bandRest0[b] = (int)restBlocks.size();
         for (int j = 0; j < nT; j++) {
             SrJob& J = jobs[j];
             const srwd_trj_info& I = info[j];
@@ -422,6 +440,8 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             if (nBands > 1) { bandRow0[b] = r0; bandRow0[b + 1] = r1; }
             if (r1 <= r0) continue;
             double normConst = std::sqrt(normBase * J.Iavg * 1.E+09 / 1.602189246E-19);
+            const int tE = g_ctx.tileE;
+            const int nTiled = dRowOf[j] >= 0 ? (ne / tE) * tE : 0;
             for (int ie = 0; ie < ne; ie++) {
                 srwd_item it;
                 std::memset(&it, 0, sizeof(it));
@@ -438,13 +458,21 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
                 it.blockBase = (int)blockItem.size() - bandBlock0[b];   // relative to the band's first block
                 it.pointBase = (long long)blockItem.size() * 256;
                 it.actNorm = normConst * it.eph * 0.80654658E-03;
+                it.drow = dRowOf[j];
+                it.de = eStepOf[j];
+                it.tiled = ie < nTiled ? 1 : 0;
                 int nb = (it.npts + 255) / 256;
-                for (int q = 0; q < nb; q++) blockItem.push_back((int)items.size());
+                const int me = (int)items.size();
+                for (int q = 0; q < nb; q++) blockItem.push_back(me);
+                if (!it.tiled) for (int q = 0; q < nb; q++) restBlocks.push_back(make_int2(me, q));
+                else if (ie % tE == 0) for (int q = 0; q < nb; q++) tileBlocks.push_back(make_int2(me, q));
                 items.push_back(it);
             }
         }
     }
     bandBlock0[nBands] = (int)blockItem.size();
+    bandTile0[nBands] = (int)tileBlocks.size();
+    bandRest0[nBands] = (int)restBlocks.size();
     int nItems = (int)items.size();
     int nRows = (int)rowTraj.size();
     int nBlocks = (int)blockItem.size();
@@ -453,11 +481,17 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     // split the integration range when a band's mesh alone does not fill the GPU
     long long minChunks = ldC;
     for (auto& I : info) minChunks = std::min(minChunks, (I.nInt + SRWD_CHUNK - 1) / SRWD_CHUNK);
-    int bandBlocks = (nBlocks + nBands - 1) / nBands;
-    int nSplit = (int)std::min<long long>(std::max<long long>(1, (4096 + bandBlocks - 1) / bandBlocks), minChunks);
+    // workgroups of a band: every block of the single-energy items, one per
+    // group of SRWD_TILE_E energies for the tiles (the tile kernel's
+    // workgroups each do SRWD_TILE_E times the work, so fewer fill the GPU)
+    const int nWG = (int)(tileBlocks.size() + restBlocks.size());
+    const int bandWG = (nWG + nBands - 1) / nBands;
+    const int wgTarget = tileBlocks.empty() ? 4096 : 1024;
+    int nSplit = (int)std::min<long long>(std::max<long long>(1, (wgTarget + bandWG - 1) / bandWG), minChunks);
     if (g_ctx.forceSplit > 0) nSplit = (int)std::min<long long>(g_ctx.forceSplit, minChunks);
     srwd_item* dItems = (srwd_item*)B.Items.get(sizeof(srwd_item) * nItems);
     int* dBlockItem = (int*)B.BlockItem.get(sizeof(int) * nBlocks);
+    int2* dSrBlocks = (int2*)B.SrBlocks.get(sizeof(int2) * (size_t)std::max(1, nWG));
     int* dRowTraj = (int*)B.RowTraj.get(sizeof(int) * nRows);
     double* dRowE = (double*)B.RowE.get(sizeof(double) * nRows);
     long long* dRowN = (long long*)B.RowN.get(sizeof(long long) * nRows);
@@ -472,11 +506,19 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     const int nLists = nBands > 1 ? 2 : 1;
     int* dFpList = (int*)B.FpList.get(sizeof(int) * 256 * (size_t)maxBandBlocks * nLists);
     int* dFpCount = (int*)B.FpCount.get(sizeof(int) * (size_t)nBands);
-    if (!dItems || !dBlockItem || !dRowTraj || !dRowE || !dRowN || !dA || !dPart || !dWarn || !dCond || !dFpList ||
+    if (!dItems || !dBlockItem || !dSrBlocks || !dRowTraj || !dRowE || !dRowN || !dA || !dPart || !dWarn || !dCond || !dFpList ||
         !dFpCount)
         return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
     HIPCHK(hipMemcpyAsync(dItems, items.data(), sizeof(srwd_item) * nItems, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dBlockItem, blockItem.data(), sizeof(int) * nBlocks, hipMemcpyHostToDevice, st));
+    int2* dTileBlocks = dSrBlocks;
+    int2* dRestBlocks = dSrBlocks + tileBlocks.size();
+    if (!tileBlocks.empty())
+        HIPCHK(hipMemcpyAsync(dTileBlocks, tileBlocks.data(), sizeof(int2) * tileBlocks.size(), hipMemcpyHostToDevice,
+                              st));
+    if (!restBlocks.empty())
+        HIPCHK(hipMemcpyAsync(dRestBlocks, restBlocks.data(), sizeof(int2) * restBlocks.size(), hipMemcpyHostToDevice,
+                              st));
     HIPCHK(hipMemcpyAsync(dRowTraj, rowTraj.data(), sizeof(int) * nRows, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dRowE, rowE.data(), sizeof(double) * nRows, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dRowN, rowN.data(), sizeof(long long) * nRows, hipMemcpyHostToDevice, st));
@@ -537,8 +579,10 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         int* fpList = dFpList + (size_t)256 * maxBandBlocks * (b & (nLists - 1));
         if (nb > 0) {
             if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 2] : g_ctx.ev0, sb));
-            if (srwd_launch_sr(dItems, dBlockItem + b0, nb, dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad,
-                               dPart, sb))
+            if (srwd_launch_sr(dItems, dTileBlocks + bandTile0[b], bandTile0[b + 1] - bandTile0[b], g_ctx.tileE, dInfo, dRec,
+                               dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart, sb) ||
+                srwd_launch_sr(dItems, dRestBlocks + bandRest0[b], bandRest0[b + 1] - bandRest0[b], 0, dInfo, dRec,
+                               dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart, sb))
                 return SRWL_AMD_ERR_GPU;
             if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 3] : g_ctx.ev1, sb));
             if (srwd_launch_finalize(dItems, dBlockItem + b0, nb, dInfo, dEdge, termin, dPart, nSplit, nPointsPad,
@@ -1119,6 +1163,13 @@ SRWL_EXPORT int srwlAmdSetSplit(int nSplit)
 SRWL_EXPORT int srwlAmdSetTrajSerial(int on)
 {
     g_ctx.trajSerial = on != 0;
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetEnergyTiles(int e)
+{
+    if (e != 0 && e != 4 && e != 8) return SRWL_AMD_ERR_UNSUPPORTED;
+    g_ctx.tileE = e;
     return 0;
 }
 

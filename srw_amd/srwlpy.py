@@ -181,6 +181,11 @@ def set_traj_serial(on=True):
     lib.srwlAmdSetTrajSerial(1 if on else 0)
 
 
+def set_energy_tiles(e=8):
+    """Energies per workgroup of the energy-tile SR kernel: 8 (default), 4, or 0 (off)."""
+    _check(lib.srwlAmdSetEnergyTiles(int(e)))
+
+
 def set_cond_tol(tol=1e-4):
     """Tolerance of the FP32 condition check of the SR sums (0: off)."""
     _check(lib.srwlAmdSetCondTol(C.c_double(tol)))
