@@ -29,6 +29,11 @@ HBM_PEAK_GBS = 8000.0
 # phase FMAs, two amplitude FMAs and four accumulating FMAs (8 FMA = 16 flops);
 # the hardware sine and cosine are not counted.
 FLOPS_PER_EVAL = 16
+# the same for k_sr_integral_tile<8>, per energy of a group of 8: four
+# accumulating FMAs (8), the complex multiply that advances the phasor (2 mul
+# + 2 FMA = 6) for 7 of the 8 energies, and the per-s work shared by the 8
+# (four phase and two amplitude FMAs = 12, over 8)
+FLOPS_PER_EVAL_TILE = 8 + 6 * 7 / 8 + 12 / 8
 VALU_LANE_ISSUE_PER_S = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes per cycle x clock
 
 
@@ -45,6 +50,8 @@ def parse():
     ap.add_argument("--ne", type=int, default=201, help="photon energies")
     ap.add_argument("--cpu-energies", type=int, default=24, help="energies in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--api", type=int, default=1,
+                    help="time the API path into host arrays (0: only the device-resident leg, for profiling)")
     ap.add_argument("--multi-e", type=int, default=1, help="also measure the multi-electron time to 1%% rel-L2")
     ap.add_argument("--me-n", type=int, default=512, help="multi-electron emission mesh")
     ap.add_argument("--me-max", type=int, default=4096, help="electron cap per multi-electron run")
@@ -403,13 +410,14 @@ def main():
     # the API's own path: the field in the caller's host arrays on return
     srwlpy.set_keep_on_device(False)
     srwlpy.set_timing(True)
-    for _ in range(a.warmup):
+    n_api = a.steps if a.api else 1
+    for _ in range(a.warmup if a.api else 1):
         srwlpy.CalcElecFieldSR(w, 0, mag, prec)
     srwlpy.reset_stats()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for _ in range(n_api):
         srwlpy.CalcElecFieldSR(w, 0, mag, prec)
-    dt = time.perf_counter() - t0
+    dt = (time.perf_counter() - t0) * a.steps / n_api     # time of a.steps calls
     st = srwlpy.stats()
     value = pts * a.steps / dt
     k_ms = st["sum_sr_ms"] / max(st["n_sr"], 1)
@@ -428,7 +436,11 @@ def main():
     st_dev = srwlpy.stats()
     srwlpy.set_keep_on_device(False)
     k_ms_dev = st_dev["sum_sr_ms"] / max(st_dev["n_sr"], 1)
-    achieved = FLOPS_PER_EVAL * evals / (k_ms_dev * 1e-3) / 1e12
+    # executed FP32 flops per evaluation: the energy-tile kernel takes the
+    # energies of whole groups of 8, the single-energy kernel the rest
+    tiled = (a.ne // 8) * 8 / a.ne if a.ne >= 8 else 0.
+    flops_per_eval = tiled * FLOPS_PER_EVAL_TILE + (1. - tiled) * FLOPS_PER_EVAL
+    achieved = flops_per_eval * evals / (k_ms_dev * 1e-3) / 1e12
     vpe, tpe = sr_valu_per_eval(a.n, a.ne)
     # issue rate of the non-transcendental vector instructions (the transcendental
     # ones overlap them) over one wave64 instruction per 4 cycles per SIMD
@@ -474,8 +486,11 @@ def main():
                    "mesh": [a.n, a.n], "energies": a.ne, "trajectory_points": 10001, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": sr_traffic(a.n, a.ne),
-                     "kernel": "k_sr_integral", "kernel_ms": k_ms_dev, "evals_per_launch": evals,
-                     "flops_per_eval": FLOPS_PER_EVAL, "flops_counted": "executed FP32 FMAs x 2 (sin/cos excluded)",
+                     "kernel": "k_sr_integral_tile<8> + k_sr_integral", "kernel_ms": k_ms_dev,
+                     "evals_per_launch": evals, "flops_per_eval": flops_per_eval,
+                     "flops_counted": ("executed FP32 flops, an FMA counted as 2 (sin/cos excluded): 16 per evaluation "
+                                       "in the single-energy kernel; 14.75 in the energy-tile kernel (4 accumulating "
+                                       "FMAs, 7/8 of a complex multiply, 1/8 of 4 phase and 2 amplitude FMAs)"),
                      "valu_instr_per_eval": vpe, "valu_trans_per_eval": tpe, "valu_issue_frac": issue,
                      "nonpacked_fma_ceiling_frac": 0.5},
         "device_resident_points_per_s": pts * a.steps / dt_dev,

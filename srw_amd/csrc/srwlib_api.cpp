@@ -579,11 +579,23 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         int* fpList = dFpList + (size_t)256 * maxBandBlocks * (b & (nLists - 1));
         if (nb > 0) {
             if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 2] : g_ctx.ev0, sb));
-            if (srwd_launch_sr(dItems, dTileBlocks + bandTile0[b], bandTile0[b + 1] - bandTile0[b], g_ctx.tileE, dInfo, dRec,
-                               dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart, sb) ||
-                srwd_launch_sr(dItems, dRestBlocks + bandRest0[b], bandRest0[b + 1] - bandRest0[b], 0, dInfo, dRec,
-                               dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart, sb))
+            const int nTile = bandTile0[b + 1] - bandTile0[b], nRest = bandRest0[b + 1] - bandRest0[b];
+            // an unbanded pass runs the few single-energy blocks on the second
+            // stream, beside the energy tiles, instead of after them
+            const bool side = nBands == 1 && nTile > 0 && nRest > 0;
+            if (side) {
+                HIPCHK(hipEventRecord(g_ctx.evFork, sb));
+                HIPCHK(hipStreamWaitEvent(g_ctx.stream2, g_ctx.evFork, 0));
+            }
+            if (srwd_launch_sr(dItems, dRestBlocks + bandRest0[b], nRest, 0, dInfo, dRec, dChunk, dA, ldI, ldC, nSplit,
+                               nPointsPad, dPart, side ? g_ctx.stream2 : sb) ||
+                srwd_launch_sr(dItems, dTileBlocks + bandTile0[b], nTile, g_ctx.tileE, dInfo, dRec, dChunk, dA, ldI,
+                               ldC, nSplit, nPointsPad, dPart, sb))
                 return SRWL_AMD_ERR_GPU;
+            if (side) {
+                HIPCHK(hipEventRecord(g_ctx.evJoin, g_ctx.stream2));
+                HIPCHK(hipStreamWaitEvent(sb, g_ctx.evJoin, 0));
+            }
             if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 3] : g_ctx.ev1, sb));
             if (srwd_launch_finalize(dItems, dBlockItem + b0, nb, dInfo, dEdge, termin, dPart, nSplit, nPointsPad,
                                      dEx, dEz, dWarn, g_ctx.condTol > 0 ? dCond : nullptr, g_ctx.condTol, fpList,
