@@ -30,10 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # the hardware sine and cosine are not counted.
 FLOPS_PER_EVAL = 16
 # the same for k_sr_integral_tile<8>, per energy of a group of 8: four
-# accumulating FMAs (8), the complex multiply that advances the phasor (2 mul
-# + 2 FMA = 6) for 7 of the 8 energies, and the per-s work shared by the 8
-# (four phase and two amplitude FMAs = 12, over 8)
-FLOPS_PER_EVAL_TILE = 8 + 6 * 7 / 8 + 12 / 8
+# accumulating FMAs (8), the two recurrence FMAs for 4 of the 8 energies
+# (4, over 2), and the per-s work shared by the 8 (four phase FMAs, the
+# second chain's start phase FMA and two adds, the doubled cosine, two
+# amplitude FMAs: 17, over 8)
+FLOPS_PER_EVAL_TILE = 8 + 4 * 4 / 8 + 17 / 8
 VALU_LANE_ISSUE_PER_S = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes per cycle x clock
 
 
@@ -489,8 +490,9 @@ def main():
                      "kernel": "k_sr_integral_tile<8> + k_sr_integral", "kernel_ms": k_ms_dev,
                      "evals_per_launch": evals, "flops_per_eval": flops_per_eval,
                      "flops_counted": ("executed FP32 flops, an FMA counted as 2 (sin/cos excluded): 16 per evaluation "
-                                       "in the single-energy kernel; 14.75 in the energy-tile kernel (4 accumulating "
-                                       "FMAs, 7/8 of a complex multiply, 1/8 of 4 phase and 2 amplitude FMAs)"),
+                                       "in the single-energy kernel; 12.125 in the energy-tile kernel (4 accumulating "
+                                       "FMAs, 2 recurrence FMAs for half of the energies, 1/8 of the per-s phase, "
+                                       "chain-start and amplitude work)"),
                      "valu_instr_per_eval": vpe, "valu_trans_per_eval": tpe, "valu_issue_frac": issue,
                      "nonpacked_fma_ceiling_frac": 0.5},
         "device_resident_points_per_s": pts * a.steps / dt_dev,

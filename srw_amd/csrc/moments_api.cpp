@@ -17,6 +17,30 @@
 
 namespace srwamd {
 
+// Queues the moment sums of the slices on the library stream: slice table
+// in, sums and limits out through the host arrays given (pinned memory lets
+// the caller go on before the stream has reached them).
+static int queue_sums(const srwm_mslice* sl, int n, int nxMax, int nzMax, bool lanesOverSlices, double* sums,
+                      int* lims)
+{
+    hipStream_t st = g_ctx.stream;
+    Buffers& B = g_ctx.buf;
+    srwm_mslice* dSl = (srwm_mslice*)B.MomSl.get(sizeof(srwm_mslice) * n);
+    void* dWork = B.MomWork.get((size_t)srwm_work_bytes(n, nxMax, nzMax, lanesOverSlices ? 1 : 0));
+    double* dSums = (double*)B.MomSums.get(sizeof(double) * 24 * (size_t)n);
+    int* dLims = (int*)B.MomLims.get(sizeof(int) * 4 * (size_t)n);
+    if (!dSl || !dWork || !dSums || !dLims) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+    if (hipMemcpyAsync(dSl, sl, sizeof(srwm_mslice) * n, hipMemcpyHostToDevice, st) != hipSuccess)
+        return SRWL_AMD_ERR_GPU;
+    if (srwm_launch_moments(dSl, n, nxMax, nzMax, lanesOverSlices ? 1 : 0, dWork, dSums, dLims, st))
+        return SRWL_AMD_ERR_GPU;
+    if (hipMemcpyAsync(sums, dSums, sizeof(double) * 24 * n, hipMemcpyDeviceToHost, st) != hipSuccess)
+        return SRWL_AMD_ERR_GPU;
+    if (lims && hipMemcpyAsync(lims, dLims, sizeof(int) * 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess)
+        return SRWL_AMD_ERR_GPU;
+    return 0;
+}
+
 int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums, std::vector<int>& lims,
                    bool lanesOverSlices)
 {
@@ -26,25 +50,58 @@ int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums
     if (n == 0) return 0;
     int rc = ensure_device();
     if (rc) return rc;
-    hipStream_t st = g_ctx.stream;
     int nxMax = 1, nzMax = 1;
     for (auto& s : sl) { nxMax = std::max(nxMax, s.nx); nzMax = std::max(nzMax, s.nz); }
-    Buffers& B = g_ctx.buf;
-    srwm_mslice* dSl = (srwm_mslice*)B.MomSl.get(sizeof(srwm_mslice) * n);
-    void* dWork = B.MomWork.get((size_t)srwm_work_bytes(n, nxMax, nzMax, lanesOverSlices ? 1 : 0));
-    double* dSums = (double*)B.MomSums.get(sizeof(double) * 24 * (size_t)n);
-    int* dLims = (int*)B.MomLims.get(sizeof(int) * 4 * (size_t)n);
-    if (!dSl || !dWork || !dSums || !dLims) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    if (hipMemcpyAsync(dSl, sl.data(), sizeof(srwm_mslice) * n, hipMemcpyHostToDevice, st) != hipSuccess)
-        return SRWL_AMD_ERR_GPU;
-    if (srwm_launch_moments(dSl, n, nxMax, nzMax, lanesOverSlices ? 1 : 0, dWork, dSums, dLims, st))
-        return SRWL_AMD_ERR_GPU;
-    if (hipMemcpyAsync(sums.data(), dSums, sizeof(double) * 24 * n, hipMemcpyDeviceToHost, st) != hipSuccess)
-        return SRWL_AMD_ERR_GPU;
-    if (hipMemcpyAsync(lims.data(), dLims, sizeof(int) * 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess)
-        return SRWL_AMD_ERR_GPU;
-    if (hipStreamSynchronize(st) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    rc = queue_sums(sl.data(), n, nxMax, nzMax, lanesOverSlices, sums.data(), lims.data());
+    if (rc) return rc;
+    if (hipStreamSynchronize(g_ctx.stream) != hipSuccess) return SRWL_AMD_ERR_GPU;
     return 0;
+}
+
+int wfr_moments_queue(const SRWLWfr& w, const float2* dEx, const float2* dEz, long long perE, long long perX,
+                      long long perZ, MomQueued& q)
+{
+    q.n = 0;
+    if (w.mesh.nx <= 1 && w.mesh.ny <= 1) return 0;
+    int rc = ensure_device();
+    if (rc) return rc;
+    int n = (int)w.mesh.ne;
+    // pinned: slice table, then 24 sums per slice
+    size_t slBytes = ((sizeof(srwm_mslice) * n + 255) / 256) * 256;
+    size_t bytes = slBytes + sizeof(double) * 24 * n;
+    if (bytes > g_ctx.momPinBytes) {
+        if (g_ctx.momPin) (void)hipHostFree(g_ctx.momPin);
+        g_ctx.momPin = nullptr;
+        g_ctx.momPinBytes = 0;
+        if (hipHostMalloc((void**)&g_ctx.momPin, bytes, hipHostMallocDefault) != hipSuccess) {
+            g_ctx.momPin = nullptr;
+            return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        }
+        g_ctx.momPinBytes = bytes;
+    }
+    srwm_mslice* sl = (srwm_mslice*)g_ctx.momPin;
+    q.sums = (double*)(g_ctx.momPin + slBytes);
+    int nxMax = 1, nzMax = 1;
+    for (long ie = 0; ie < n; ie++) {
+        sl[ie] = wfr_slice(w, dEx, dEz, perE, perX, perZ, ie);
+        nxMax = std::max(nxMax, sl[ie].nx);
+        nzMax = std::max(nzMax, sl[ie].nz);
+    }
+    q.sl.assign(sl, sl + n);
+    rc = queue_sums(sl, n, nxMax, nzMax, perE == 1 && n > 1, q.sums, nullptr);
+    if (rc) return rc;
+    q.n = n;
+    q.hasX = dEx != nullptr;
+    q.hasZ = dEz != nullptr;
+    return 0;
+}
+
+void wfr_moments_collect(const MomQueued& q, double* momX, double* momZ)
+{
+    for (int i = 0; i < q.n; i++) {
+        if (momX && q.hasX) moments_from_sums(&q.sums[24 * i], q.sl[i], momX + 11 * i);
+        if (momZ && q.hasZ) moments_from_sums(&q.sums[24 * i + 11], q.sl[i], momZ + 11 * i);
+    }
 }
 
 // 11 moments of one component from its 11 sums; values pass through float as

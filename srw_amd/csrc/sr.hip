@@ -872,11 +872,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 // The same integral for SRWD_TILE_E consecutive photon energies of one
 // request at once.  The phase of energy k is linear in k (K_k = K_0 + k dK,
 // and the phase table rows obey A_k = A_0 + k A_dK mod 1), so per s a lane
-// forms two phases, t_0 and the step dt, takes their sines and cosines and
-// advances the phasor exp(2 pi i t_k) by one complex multiply per energy:
-// per evaluation that replaces 2 transcendentals and 2 phase FMAs by 4 FP32
-// operations.  64-point FP32 block sums are turned by each energy's FP64
-// chunk phase and accumulated in FP64.
+// forms two phases, t_0 and the step d, and gets the phasors z_k =
+// exp(2 pi i (t_0 + k d)) in chains of four: z_h and z_{h+1} from the
+// hardware sine and cosine, z_{h+2}, z_{h+3} from the three-term recurrence
+// z_{k+1} = 2 cos(2 pi d) z_k - z_{k-1} (2 FMAs each).  A rounding error in
+// the recurrence grows at most linearly along a chain (as sin(k d)/sin(d)),
+// so short chains keep it near the FP32 rounding of the phase itself.
+// 64-point FP32 block sums are turned by each energy's FP64 chunk phase and
+// accumulated in FP64.
 template <int E>
 __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __restrict__ items,
                                                           const int2* __restrict__ blocks,
@@ -887,6 +890,7 @@ __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __res
                                                           long long ldI, long long ldC, int nSplit,
                                                           long long nPointsPad, double* __restrict__ part)
 {
+    static_assert(E % 4 == 0, "energy tiles are chains of 4");
     const int2 bk = blocks[blockIdx.x];
     const srwd_item& IT = items[bk.x];
     const int local = bk.y * 256 + threadIdx.x;
@@ -970,19 +974,28 @@ __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __res
                     t = fmaf(g0, r.dV, t);
                     d = fmaf(gd, r.dV, d);
                 }
-                float zr = __builtin_amdgcn_cosf(t), zi = __builtin_amdgcn_sinf(t);
-                const float wr = __builtin_amdgcn_cosf(d), wi = __builtin_amdgcn_sinf(d);
+                const float c2 = 2.f * __builtin_amdgcn_cosf(d);
                 const float wax = fmaf(b, nfxo, L4.z), waz = fmaf(b, nfzo, L4.w);   // b (P - r_o)
+                // chains of 4 energies: the first two phasors of a chain
+                // directly, the others by z_{k+1} = 2 cos(2 pi d) z_k - z_{k-1}
 #pragma unroll
-                for (int e = 0; e < E; e++) {
-                    acc[e][0] = fmaf(wax, zr, acc[e][0]);
-                    acc[e][1] = fmaf(wax, zi, acc[e][1]);
-                    acc[e][2] = fmaf(waz, zr, acc[e][2]);
-                    acc[e][3] = fmaf(waz, zi, acc[e][3]);
-                    if (e + 1 < E) {
-                        const float nr = fmaf(zr, wr, -zi * wi), ni = fmaf(zr, wi, zi * wr);
-                        zr = nr;
-                        zi = ni;
+                for (int h = 0; h < E; h += 4) {
+                    const float th = h == 0 ? t : fmaf((float)h, d, t), th1 = th + d;
+                    float pr = __builtin_amdgcn_cosf(th), pi = __builtin_amdgcn_sinf(th);
+                    float qr = __builtin_amdgcn_cosf(th1), qi = __builtin_amdgcn_sinf(th1);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int e = h + k;
+                        if (k >= 2) {
+                            const float nr = fmaf(c2, qr, -pr), ni = fmaf(c2, qi, -pi);
+                            pr = qr; pi = qi;
+                            qr = nr; qi = ni;
+                        }
+                        const float zr = k == 0 ? pr : qr, zi = k == 0 ? pi : qi;
+                        acc[e][0] = fmaf(wax, zr, acc[e][0]);
+                        acc[e][1] = fmaf(wax, zi, acc[e][1]);
+                        acc[e][2] = fmaf(waz, zr, acc[e][2]);
+                        acc[e][3] = fmaf(waz, zi, acc[e][3]);
                     }
                 }
             };
@@ -1027,8 +1040,8 @@ __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __res
 // requests of harmonics 1-5, K 0.5-2.5): an estimate built on 1e-6 still
 // bounded every measured error, 2e-6 leaves a factor 2.4 of margin.
 #define SRWD_EPS_TERM 2e-6
-// Energy-tile items: the phasor of energy k carries k complex multiplies and
-// k times the rounding of the step's phase on top.
+// Energy-tile items: the phasor of energy k carries k times the rounding of
+// the step's phase and the recurrence's roundings on top.
 #define SRWD_EPS_TERM_TILE 4e-6
 
 // Terminating terms, the derivative correction, the float store and the
@@ -1121,8 +1134,13 @@ __global__ void __launch_bounds__(256) k_cond_sums(const srwd_trj_info* __restri
     if (threadIdx.x < 5) cond[5 * tr + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// One workgroup per (entry, part): an entry is (first item, 256-point block)
+// of a group of g items (consecutive photon energies of one request: the
+// energy tiles, g = SRWD_TILE_E; or single items, g = 1).  Lane t takes
+// energy t % g of point t / g of the part, so that a wave stores runs of g
+// consecutive energies (the field is stored photon energy fastest).
 __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict__ items,
-                                                     const int* __restrict__ blockItem,
+                                                     const int2* __restrict__ entries, int g,
                                                      const srwd_trj_info* __restrict__ info,
                                                      const srwd_edge* __restrict__ edges, int termin,
                                                      const double* __restrict__ part, int nSplit,
@@ -1132,17 +1150,19 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
                                                      const double* __restrict__ coord,
                                                      const long long* __restrict__ last)
 {
-    const int it = blockItem[blockIdx.x];
-    const srwd_item& IT = items[it];
-    const int local = (blockIdx.x - IT.blockBase) * 256 + threadIdx.x;
+    const int2 bk = entries[blockIdx.x / g];
+    const int q = blockIdx.x % g, e = threadIdx.x % g;
+    const int p = q * (256 / g) + threadIdx.x / g;   // point of the 256-point block
+    const srwd_item& IT = items[bk.x + e];
+    const int local = bk.y * 256 + p;
     if (local >= IT.npts) return;
     const long long flat = IT.pointBase + local;
     const srwd_trj_info& I = info[IT.traj];
     int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
     double S[4] = {0, 0, 0, 0};
     for (int k = 0; k < nSplit; k++) {
-        const double* p = part + ((long long)k * nPointsPad + flat) * 4;
-        S[0] += p[0]; S[1] += p[1]; S[2] += p[2]; S[3] += p[3];
+        const double* pp = part + ((long long)k * nPointsPad + flat) * 4;
+        S[0] += pp[0]; S[1] += pp[1]; S[2] += pp[2]; S[3] += pp[3];
     }
     const srwd_req Rq = reqs[IT.traj];
     double sumSqX = 0., sumSqZ = 0.;
@@ -1152,16 +1172,79 @@ __global__ void __launch_bounds__(256) k_sr_finalize(const srwd_item* __restrict
         sumSqX = c[0] * xo * xo - 2. * xo * c[1] + c[3];
         sumSqZ = c[0] * zo * zo - 2. * zo * c[2] + c[4];
     }
+    // list entry as the FP64 pass decodes it: the item's band-relative block and the point in it
     sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, S, Ex, Ez, warn, sumSqX, sumSqZ, condTol,
-                    cond ? fpList : nullptr, fpCount, (int)(blockIdx.x * 256 + threadIdx.x), Rq, coord, last);
+                    cond ? fpList : nullptr, fpCount, ((IT.blockBase + bk.y) << 8) | p, Rq, coord, last);
 }
 
-// FP64 recomputation of the points the condition check handed over: one wave
-// per point, the integration points split over the lanes, the trajectory
-// interpolated at each point as the reference's manual methods do
-// (srradint.cpp:1214-1258, 1379-1604), the phase reduced and its sine and
-// cosine taken with the reference's Taylor series.  A fixed grid of waves
-// walks the list (its length is read on the device).
+// FP64 sums of one point over the integration points of its trajectory
+// table T (n entries of SRWD_T64 doubles): the trajectory interpolated at
+// each point as the reference's manual methods do (srradint.cpp:1214-1258,
+// 1379-1604), the phase reduced and its sine and cosine taken with the
+// reference's Taylor series, the weights 7, 16, 14, ..., 16, 7 in units of
+// h/15.  With T and n uniform over the wave the table is read by scalar
+// loads, once per wave.
+// List length from which k_sr_fp64 gives each point one lane (below it, one
+// wave): about where a serial walk of 10^4 integration points per lane costs
+// as much as the per-point table reads of the wave-per-point form.
+#define SRWD_FP64_LANE_MIN 32768
+
+// Branch-free form of cos_sin_taylor (the same operations on the taken path,
+// so the same results).
+__device__ __forceinline__ void cos_sin_taylor_sel(double x, double& c, double& s)
+{
+    const double PI = 3.141592653590, TwoPI = 2. * PI, ThreePIdTwo = 1.5 * PI, HalfPI = 0.5 * PI;
+    const double One_dTwoPI = 1. / TwoPI;
+    const double a2c = -0.5, a4c = 1. / 24., a6c = -1. / 720., a8c = 1. / 40320., a10c = -1. / 3628800.;
+    const double a3s = -1. / 6., a5s = 1. / 120., a7s = -1. / 5040., a9s = 1. / 362880., a11s = -1. / 39916800.;
+    x -= TwoPI * (double)(int)(x * One_dTwoPI);
+    x = x < 0. ? x + TwoPI : x;
+    const bool hi = x > ThreePIdTwo, chs = !hi && x > HalfPI;
+    x = hi ? x - TwoPI : (chs ? x - PI : x);
+    double x2 = x * x;
+    c = 1. + x2 * (a2c + x2 * (a4c + x2 * (a6c + x2 * (a8c + x2 * a10c))));
+    s = x * (1. + x2 * (a3s + x2 * (a5s + x2 * (a7s + x2 * (a9s + x2 * a11s)))));
+    c = chs ? -c : c;
+    s = chs ? -s : s;
+}
+
+template <typename TP>
+__device__ __forceinline__ void fp64_point_sums(TP T, int n, double xo, double zo, double kph, double* acc)
+{
+#pragma unroll 2
+    for (int i = 0; i < n; i++) {
+        const int o = i * SRWD_T64;
+        double Xv = T[o], Zv = T[o + 1], Btx = T[o + 2], Btz = T[o + 3], sQ = T[o + 4], inv = T[o + 5];
+        double dx = xo - Xv, dz = zo - Zv;
+        double ph = kph * (sQ + (dx * dx + dz * dz) * inv);
+        double c, sn;
+        cos_sin_taylor_sel(ph, c, sn);
+        double wt = (i == 0 || i == n - 1) ? 7. : ((i & 1) ? 16. : 14.);
+        double ax = wt * ((Btx - dx * inv) * inv), az = wt * ((Btz - dz * inv) * inv);
+        acc[0] += ax * c; acc[1] += ax * sn; acc[2] += az * c; acc[3] += az * sn;
+    }
+}
+
+__device__ __forceinline__ const double* wave_first_ptr(const double* p)
+{
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (const double*)(((unsigned long long)hi << 32) | lo);
+}
+
+// FP64 recomputation of the points the condition check handed over.  A fixed
+// grid of waves walks the list (its length is read on the device).
+// * Long lists: one lane per point, 64 consecutive list entries per wave.
+//   Entries of one finalize wave are contiguous in the list and neighbouring
+//   blocks belong to the same request, so a wave's points nearly always
+//   share a trajectory; the table is then read once per wave (scalar loads)
+//   instead of once per point.  A wave whose points span trajectories reads
+//   per lane.
+// * Short lists (fewer points than SRWD_FP64_LANE_MIN), where one lane per
+//   point would leave the waves too few to hide the latency of a serial walk
+//   over the trajectory: one wave per point, the integration points split
+//   over its lanes.
 __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ items,
                                                  const int* __restrict__ blockItem,
                                                  const srwd_trj_info* __restrict__ info,
@@ -1175,34 +1258,61 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
     const int lane = threadIdx.x & 63;
     const int nWaves = gridDim.x * (blockDim.x >> 6);
     const int count = *fpCount;
-    for (int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < count; q += nWaves) {
-        const int entry = fpList[q];
-        const int blk = entry >> 8, thr = entry & 255;
-        const srwd_item& IT = items[blockItem[blk]];
-        const int local = (blk - IT.blockBase) * 256 + thr;
-        const srwd_trj_info I = info[IT.traj];
-        int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
-        const srwd_req& Rq = reqs[IT.traj];
-        double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
-        double kph = SRWD_KPH(IT.eph);
-        const double* T = T64 + (long long)IT.traj * ldI * SRWD_T64;
-        double acc[4] = {0, 0, 0, 0};
-        for (long long i = lane; i < I.nInt; i += 64) {
-            const double* t = T + i * SRWD_T64;
-            double Xv = t[0], Zv = t[1], Btx = t[2], Btz = t[3], sQ = t[4], inv = t[5];
-            double dx = xo - Xv, dz = zo - Zv;
-            double ph = kph * (sQ + (dx * dx + dz * dz) * inv);
-            double c, sn;
-            cos_sin_taylor(ph, c, sn);
-            double wt = (i == 0 || i == I.nInt - 1) ? 7. : ((i & 1) ? 16. : 14.);
-            double ax = wt * ((Btx - dx * inv) * inv), az = wt * ((Btz - dz * inv) * inv);
-            acc[0] += ax * c; acc[1] += ax * sn; acc[2] += az * c; acc[3] += az * sn;
+    if (count < SRWD_FP64_LANE_MIN) {
+        for (int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < count; q += nWaves) {
+            const int entry = fpList[q];
+            const int blk = entry >> 8, thr = entry & 255;
+            const srwd_item& IT = items[blockItem[blk]];
+            const int local = (blk - IT.blockBase) * 256 + thr;
+            const srwd_trj_info I = info[IT.traj];
+            int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
+            const srwd_req& Rq = reqs[IT.traj];
+            double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
+            double kph = SRWD_KPH(IT.eph);
+            const double* T = T64 + (long long)IT.traj * ldI * SRWD_T64;
+            double acc[4] = {0, 0, 0, 0};
+            for (long long i = lane; i < I.nInt; i += 64) {
+                const double* t = T + i * SRWD_T64;
+                double Xv = t[0], Zv = t[1], Btx = t[2], Btz = t[3], sQ = t[4], inv = t[5];
+                double dx = xo - Xv, dz = zo - Zv;
+                double ph = kph * (sQ + (dx * dx + dz * dz) * inv);
+                double c, sn;
+                cos_sin_taylor(ph, c, sn);
+                double wt = (i == 0 || i == I.nInt - 1) ? 7. : ((i & 1) ? 16. : 14.);
+                double ax = wt * ((Btx - dx * inv) * inv), az = wt * ((Btz - dz * inv) * inv);
+                acc[0] += ax * c; acc[1] += ax * sn; acc[2] += az * c; acc[3] += az * sn;
+            }
+            for (int m = 0; m < 4; m++)
+                for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off);
+            if (lane == 0)
+                sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr,
+                                nullptr, 0, Rq, coord, last);
         }
-        for (int m = 0; m < 4; m++)
-            for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off);
-        if (lane == 0)
+        return;
+    }
+    for (int seg = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); seg * 64 < count; seg += nWaves) {
+        const int q = seg * 64 + lane;
+        if (q < count) {
+            const int entry = fpList[q];
+            const int blk = entry >> 8, thr = entry & 255;
+            const srwd_item& IT = items[blockItem[blk]];
+            const int local = (blk - IT.blockBase) * 256 + thr;
+            const srwd_trj_info I = info[IT.traj];
+            int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
+            const srwd_req& Rq = reqs[IT.traj];
+            double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
+            double kph = SRWD_KPH(IT.eph);
+            const double* T = T64 + (long long)IT.traj * ldI * SRWD_T64;
+            const int n = (int)I.nInt;
+            double acc[4] = {0, 0, 0, 0};
+            if (__all(IT.traj == __builtin_amdgcn_readfirstlane(IT.traj)))
+                fp64_point_sums((const __attribute__((address_space(4))) double*)wave_first_ptr(T),
+                                __builtin_amdgcn_readfirstlane(n), xo, zo, kph, acc);
+            else
+                fp64_point_sums(T, n, xo, zo, kph, acc);
             sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr, nullptr,
                             0, Rq, coord, last);
+        }
     }
 }
 
@@ -1350,15 +1460,17 @@ int srwd_launch_sr(const srwd_item* dItems, const int2* dBlocks, int nBlocks, in
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
-                         const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
-                         long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
+int srwd_launch_finalize(const srwd_item* dItems, const int2* dEntries, int nEntries, int g,
+                         const srwd_trj_info* dInfo, const srwd_edge* dEdge, int termin, const double* dPart,
+                         int nSplit, long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
                          double condTol, int* dFpList, int* dFpCount, const srwd_req* dReq, const double* dCoord,
                          const long long* dLast, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sr_finalize, dim3(nBlocks), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin,
-                       dPart, nSplit, nPointsPad, (float2*)dEx, (float2*)dEz, dWarn, dCond, condTol, dFpList,
-                       dFpCount, dReq, dCoord, dLast);
+    if (nEntries <= 0) return 0;
+    if (g < 1 || 256 % g) return -1;
+    hipLaunchKernelGGL(k_sr_finalize, dim3((unsigned)nEntries * g), dim3(256), 0, st, dItems, dEntries, g, dInfo,
+                       dEdge, termin, dPart, nSplit, nPointsPad, (float2*)dEx, (float2*)dEz, dWarn, dCond, condTol,
+                       dFpList, dFpCount, dReq, dCoord, dLast);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

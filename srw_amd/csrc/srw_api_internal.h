@@ -84,6 +84,8 @@ struct Context {
     hipEvent_t evFork = nullptr, evJoin = nullptr;
     char* pin = nullptr;                    // pinned staging (grow-only)
     size_t pinBytes = 0;
+    char* momPin = nullptr;       // pinned slice table and sums of queued moments
+    size_t momPinBytes = 0;
     std::vector<hipEvent_t> bandEv;         // per band: computed, downloaded, SR start, SR end
     int nTimePairs = 0;                     // timed SR launches of the last pass (bandEv[4b+2], [4b+3])
     int* fpCountHost = nullptr;             // pinned: points recomputed in FP64, per band (at most 64 bands)
@@ -138,6 +140,10 @@ int setup_job_from_field(SrJob& J, const SRWLParticle& p, const SRWLMagFldC* mag
 struct HostOut {
     float* ex = nullptr;
     float* ez = nullptr;
+    // more device work to queue behind the field before the host starts
+    // moving bands (optional; gets the request's results so far)
+    int (*queueMore)(void* ctx, const std::vector<SrResult>& res) = nullptr;
+    void* queueCtx = nullptr;
 };
 int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, float* dEx, float* dEz,
            const std::vector<long long>& outOff, std::vector<SrResult>& res, int termin,
@@ -171,6 +177,18 @@ int wfr_moments(const SRWLWfr& w, const float2* dEx, const float2* dEz, long lon
 int device_moments(const std::vector<srwm_mslice>& sl, std::vector<double>& sums, std::vector<int>& lims,
                    bool lanesOverSlices = false);
 void moments_from_sums(const double* S, const srwm_mslice& s, double* mom);
+// The moments of wfr_moments split in two: kernels and read-back queued on
+// the library stream (into pinned memory, so the host is not held), then the
+// moments formed on the host once the stream has been synchronised.
+struct MomQueued {
+    std::vector<srwm_mslice> sl;
+    double* sums = nullptr;   // pinned, 24 per slice (the library's; valid until the next queue)
+    int n = 0;
+    bool hasX = false, hasZ = false;
+};
+int wfr_moments_queue(const SRWLWfr& w, const float2* dEx, const float2* dEz, long long perE, long long perX,
+                      long long perZ, MomQueued& q);
+void wfr_moments_collect(const MomQueued& q, double* momX, double* momZ);
 srwm_mslice wfr_slice(const SRWLWfr& w, const float2* dEx, const float2* dEz, long long perE, long long perX,
                       long long perZ, long ie);
 int drift_radius(const SRWLWfr& w, int sub, double L, double* momX, double* momZ, int (*recompute)(void*),

@@ -159,14 +159,17 @@ int srwd_launch_sr(const srwd_item* dItems, const int2* dBlocks, int nBlocks, in
 int srwd_launch_edge_state(const srwd_req* dReq, int nReq, const int* dBlkReq, long long nBlk,
                            const srwd_trj_info* dInfo, const srwd_edge* dEdge, const double* dCoord, int termin,
                            long long* dLast, hipStream_t st);
-// dCond (5 doubles per trajectory, from srwd_launch_cond_sums) enables the
-// condition check of the FP32 sums (estimated error <= condTol of the point's
-// field magnitude): points that fail it are appended to dFpList (block * 256 +
-// thread; *dFpCount entries, zeroed by the caller) instead of being stored,
-// for srwd_launch_sr_fp64.  dCond == nullptr stores every point.
-int srwd_launch_finalize(const srwd_item* dItems, const int* dBlockItem, int nBlocks, const srwd_trj_info* dInfo,
-                         const srwd_edge* dEdge, int termin, const double* dPart, int nSplit,
-                         long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
+// Entries (first item, 256-point block in the item) of groups of g items
+// (g consecutive photon energies of one request, g divides 256; 1 for single
+// items).  dCond (5 doubles per trajectory, from srwd_launch_cond_sums)
+// enables the condition check of the FP32 sums (estimated error <= condTol
+// of the point's field magnitude): points that fail it are appended to
+// dFpList (item's band-relative block * 256 + point in the block;
+// *dFpCount entries, zeroed by the caller) instead of being stored, for
+// srwd_launch_sr_fp64.  dCond == nullptr stores every point.
+int srwd_launch_finalize(const srwd_item* dItems, const int2* dEntries, int nEntries, int g,
+                         const srwd_trj_info* dInfo, const srwd_edge* dEdge, int termin, const double* dPart,
+                         int nSplit, long long nPointsPad, float* dEx, float* dEz, int* dWarn, const double* dCond,
                          double condTol, int* dFpList, int* dFpCount, const srwd_req* dReq, const double* dCoord,
                          const long long* dLast, hipStream_t st);
 int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec* dRec, long long ldI, double* dCond,

@@ -597,9 +597,14 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
                 HIPCHK(hipStreamWaitEvent(sb, g_ctx.evJoin, 0));
             }
             if (g_ctx.timing) HIPCHK(hipEventRecord(nBands > 1 ? g_ctx.bandEv[4 * b + 3] : g_ctx.ev1, sb));
-            if (srwd_launch_finalize(dItems, dBlockItem + b0, nb, dInfo, dEdge, termin, dPart, nSplit, nPointsPad,
-                                     dEx, dEz, dWarn, g_ctx.condTol > 0 ? dCond : nullptr, g_ctx.condTol, fpList,
-                                     dFpCount + b, dReq, dCoord, dLast, sb))
+            // energy tiles in groups (runs of consecutive photon energies stored together), then the rest
+            const double* cnd = g_ctx.condTol > 0 ? dCond : nullptr;
+            if (srwd_launch_finalize(dItems, dTileBlocks + bandTile0[b], nTile, g_ctx.tileE > 0 ? g_ctx.tileE : 1,
+                                     dInfo, dEdge, termin, dPart, nSplit, nPointsPad, dEx, dEz, dWarn, cnd,
+                                     g_ctx.condTol, fpList, dFpCount + b, dReq, dCoord, dLast, sb) ||
+                srwd_launch_finalize(dItems, dRestBlocks + bandRest0[b], nRest, 1, dInfo, dEdge, termin, dPart,
+                                     nSplit, nPointsPad, dEx, dEz, dWarn, cnd, g_ctx.condTol, fpList, dFpCount + b,
+                                     dReq, dCoord, dLast, sb))
                 return SRWL_AMD_ERR_GPU;
             if (g_ctx.condTol > 0 && srwd_launch_sr_fp64(dItems, dBlockItem + b0, dInfo, dEdge, termin, dT64, ldI, dEx,
                                                          dEz, dWarn, fpList, dFpCount + b, dReq, dCoord, dLast, sb))
@@ -633,6 +638,12 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     for (auto& it : items) ev += (long long)it.npts * info[it.traj].nInt;
     g_ctx.pendingEvals = ev;
     if (nBands > 1) {
+        // work that follows the field on the device is queued before the host
+        // starts waiting for bands
+        if (hostOut && hostOut->queueMore) {
+            rc = hostOut->queueMore(hostOut->queueCtx, res);
+            if (rc) return rc;
+        }
         // every band is queued; the host moves each downloaded band on into the
         // caller's arrays while the later ones are still computed
         for (int b = 0; b < nBands; b++) {
@@ -1027,20 +1038,42 @@ SRWL_EXPORT int srwlCalcElecFieldSR(SRWLWfr* pWfr, SRWLPrtTrj* pTrj, SRWLMagFldC
     float* dEx = (float*)g_ctx.buf.Ex.get(sizeof(float) * 2 * n);
     float* dEz = (float*)g_ctx.buf.Ez.get(sizeof(float) * 2 * n);
     if (!dEx || !dEz) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    // the field reaches the caller's arrays band by band while the rest is computed
+    // the field reaches the caller's arrays band by band while the rest is
+    // computed; the moments (srradint.cpp:3570-3574: every energy slice, then
+    // the mean energy) are queued behind the field before the host copies
+    // begin, so that they run while the last bands travel
+    struct MomCtx {
+        SRWLWfr* w;
+        const float2 *ex, *ez;
+        MomQueued q;
+        bool queued = false;
+    } mc{pWfr, (const float2*)dEx, (const float2*)dEz};
     HostOut ho;
     ho.ex = (float*)pWfr->arEx; ho.ez = (float*)pWfr->arEy;
+    if ((pWfr->arMomX || pWfr->arMomY) && !g_arena) {
+        ho.queueMore = [](void* p, const std::vector<SrResult>& r) -> int {
+            MomCtx& m = *(MomCtx*)p;
+            SRWLWfr w = *m.w;   // the radius and centre the moments use
+            w.Rx = w.Ry = r[0].Robs;
+            w.dRx = w.dRy = r[0].RobsErr;
+            w.xc = r[0].xc; w.yc = r[0].zc;
+            int rc = wfr_moments_queue(w, m.ex, m.ez, 1, w.mesh.ne, (long long)w.mesh.nx * w.mesh.ne, m.q);
+            m.queued = rc == 0;
+            return rc;
+        };
+        ho.queueCtx = &mc;
+    }
     rc = sr_run(jobs, fields, dEx, dEz, off, res, termin, true, false, g_ctx.keepOnDevice ? nullptr : &ho);
     if (rc) return rc;
     pWfr->Rx = pWfr->Ry = res[0].Robs;
     pWfr->dRx = pWfr->dRy = res[0].RobsErr;
     pWfr->xc = res[0].xc; pWfr->yc = res[0].zc;
-    // srradint.cpp:3570-3574: statistical moments of every energy slice, then the mean energy
-    if (pWfr->arMomX || pWfr->arMomY) {
+    if ((pWfr->arMomX || pWfr->arMomY) && !mc.queued) {
         rc = field_moments(*pWfr, (const float2*)dEx, (const float2*)dEz);
         if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    if (mc.queued) wfr_moments_collect(mc.q, pWfr->arMomX, pWfr->arMomY);
     sr_complete(res);
     pWfr->avgPhotEn = 0.5 * (pWfr->mesh.eStart + pWfr->mesh.eFin);
     pWfr->presCA = 0; pWfr->presFT = 0;
