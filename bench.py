@@ -443,9 +443,11 @@ def main():
     flops_per_eval = tiled * FLOPS_PER_EVAL_TILE + (1. - tiled) * FLOPS_PER_EVAL
     achieved = flops_per_eval * evals / (k_ms_dev * 1e-3) / 1e12
     vpe, tpe = sr_valu_per_eval(a.n, a.ne)
-    # issue rate of the non-transcendental vector instructions (the transcendental
-    # ones overlap them) over one wave64 instruction per 4 cycles per SIMD
-    issue = (vpe - tpe) * evals / (k_ms_dev * 1e-3) / VALU_LANE_ISSUE_PER_S if vpe and tpe is not None else None
+    # vector instructions issued per second over one wave64 instruction per 4
+    # cycles per SIMD; and the same with each transcendental taking two slots
+    # (they issue at a lower rate than FMAs)
+    issue = vpe * evals / (k_ms_dev * 1e-3) / VALU_LANE_ISSUE_PER_S if vpe else None
+    issue2 = (vpe + tpe) * evals / (k_ms_dev * 1e-3) / VALU_LANE_ISSUE_PER_S if vpe and tpe is not None else None
     c1 = c1_methods(srwlpy, W)
     log("101x101 methods 0 and 1 done")
     prop = None
@@ -494,6 +496,7 @@ def main():
                                        "FMAs, 2 recurrence FMAs for half of the energies, 1/8 of the per-s phase, "
                                        "chain-start and amplitude work)"),
                      "valu_instr_per_eval": vpe, "valu_trans_per_eval": tpe, "valu_issue_frac": issue,
+                     "valu_issue_frac_trans_two_slots": issue2,
                      "nonpacked_fma_ceiling_frac": 0.5},
         "device_resident_points_per_s": pts * a.steps / dt_dev,
         "sr_kernel_ms_in_api_run": k_ms,

@@ -323,28 +323,39 @@ __global__ void __launch_bounds__(256) k_resize_cells(const float2* __restrict__
                 re[4 * j + k] = v.x; im[4 * j + k] = v.y;
             }
         cell_intensity(re, im, In, norm, low);
-        double aRe[16], aIm[16], aIn[16];
-        if (!low) { bicubic(re, aRe); bicubic(im, aIm); bicubic(In, aIn); }
-        for (int iz = izF; iz <= izL; iz++) {
-            double zAbs = P.zStartN + iz * P.zStepN;
-            int izc, st; double zRel;
-            cell_of(zAbs, P.zStartO, P.zStepO, P.nzo, izc, st, zRel);
-            for (int ix = ixF; ix <= ixL; ix++) {
-                double xAbs = P.xStartN + ix * P.xStepN;
-                int ixc, st2; double xRel;
-                cell_of(xAbs, P.xStartO, P.xStepO, P.nxo, ixc, st2, xRel);
-                if (!filled_point(P, ix, iz, xAbs, zAbs)) continue;
-                float FI;
-                float2 F = low ? bilinear_point(re, im, In, ixc - ixSt, izc - izSt, xRel, zRel, FI)
-                               : bicubic_point(aRe, aIm, aIn, xRel, zRel, FI);
-                F = finish_point(F, FI, norm, P.improve != 0);
-                if (P.hasPost) {
-                    float c, s;
-                    phase_cs(P.post, ix, iz, ie, c, s);
-                    F = rotate_f(F, c, s);
+        // every new point of the cell, its value from `value` (the forms, or
+        // the bilinear fallback); the two cases in separate loops so that the
+        // cell values are dead once the forms are built
+        auto points = [&](auto value) {
+            for (int iz = izF; iz <= izL; iz++) {
+                double zAbs = P.zStartN + iz * P.zStepN;
+                int izc, st; double zRel;
+                cell_of(zAbs, P.zStartO, P.zStepO, P.nzo, izc, st, zRel);
+                for (int ix = ixF; ix <= ixL; ix++) {
+                    double xAbs = P.xStartN + ix * P.xStepN;
+                    int ixc, st2; double xRel;
+                    cell_of(xAbs, P.xStartO, P.xStepO, P.nxo, ixc, st2, xRel);
+                    if (!filled_point(P, ix, iz, xAbs, zAbs)) continue;
+                    float FI;
+                    float2 F = value(ixc - ixSt, izc - izSt, xRel, zRel, FI);
+                    F = finish_point(F, FI, norm, P.improve != 0);
+                    if (P.hasPost) {
+                        float c, s;
+                        phase_cs(P.post, ix, iz, ie, c, s);
+                        F = rotate_f(F, c, s);
+                    }
+                    d[ie * P.ln.perE + iz * P.ln.perZ + ix * P.ln.perX] = F;
                 }
-                d[ie * P.ln.perE + iz * P.ln.perZ + ix * P.ln.perX] = F;
             }
+        };
+        if (low) {
+            points([&](int ixr, int izr, double xr, double zr, float& FI) {
+                return bilinear_point(re, im, In, ixr, izr, xr, zr, FI);
+            });
+        } else {
+            double aRe[16], aIm[16], aIn[16];
+            bicubic(re, aRe); bicubic(im, aIm); bicubic(In, aIn);
+            points([&](int, int, double xr, double zr, float& FI) { return bicubic_point(aRe, aIm, aIn, xr, zr, FI); });
         }
     }
 }
