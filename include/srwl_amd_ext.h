@@ -38,6 +38,11 @@ SRWL_EXPORT int srwlAmdSetKeepOnDevice(int on);
  * (0: automatic). */
 SRWL_EXPORT int srwlAmdSetSplit(int nSplit);
 
+/* Forces the number of row bands in which a single request's field is
+ * computed and streamed into the caller's host arrays (0: automatic, at most
+ * 32; SRWL_AMD_ERR_UNSUPPORTED above). */
+SRWL_EXPORT int srwlAmdSetBands(int nBands);
+
 /* on != 0: integrate trajectories with the serial RK4 kernel (two threads per
  * trajectory) instead of the parallel-in-s solver; both give the serial RK4
  * recurrence, the switch exists to compare them. */

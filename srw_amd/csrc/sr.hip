@@ -1253,7 +1253,7 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
                                                  float2* Ez, int* warn, const int* __restrict__ fpList,
                                                  const int* __restrict__ fpCount, const srwd_req* __restrict__ reqs,
                                                  const double* __restrict__ coord,
-                                                 const long long* __restrict__ last)
+                                                 const long long* __restrict__ last, const double* __restrict__ sums)
 {
     const int lane = threadIdx.x & 63;
     const int nWaves = gridDim.x * (blockDim.x >> 6);
@@ -1290,6 +1290,70 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
         }
         return;
     }
+    if (sums == nullptr) {
+        // long list without a sums buffer: one lane per point, sums here
+        for (int seg = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); seg * 64 < count; seg += nWaves) {
+            const int q = seg * 64 + lane;
+            if (q < count) {
+                const int entry = fpList[q];
+                const int blk = entry >> 8, thr = entry & 255;
+                const srwd_item& IT = items[blockItem[blk]];
+                const int local = (blk - IT.blockBase) * 256 + thr;
+                const srwd_trj_info I = info[IT.traj];
+                int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
+                const srwd_req& Rq = reqs[IT.traj];
+                double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
+                double kph = SRWD_KPH(IT.eph);
+                const double* T = T64 + (long long)IT.traj * ldI * SRWD_T64;
+                const int n = (int)I.nInt;
+                double acc[4] = {0, 0, 0, 0};
+                if (__all(IT.traj == __builtin_amdgcn_readfirstlane(IT.traj)))
+                    fp64_point_sums((const __attribute__((address_space(4))) double*)wave_first_ptr(T),
+                                    __builtin_amdgcn_readfirstlane(n), xo, zo, kph, acc);
+                else
+                    fp64_point_sums(T, n, xo, zo, kph, acc);
+                sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr,
+                                nullptr, 0, Rq, coord, last);
+            }
+        }
+        return;
+    }
+    // long lists: the sums come from k_sr_fp64_sums (in sums, 4 per entry)
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        const int entry = fpList[q];
+        const int blk = entry >> 8, thr = entry & 255;
+        const srwd_item& IT = items[blockItem[blk]];
+        const int local = (blk - IT.blockBase) * 256 + thr;
+        const srwd_trj_info I = info[IT.traj];
+        int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
+        const srwd_req& Rq = reqs[IT.traj];
+        const double acc[4] = {sums[4 * (long long)q], sums[4 * (long long)q + 1], sums[4 * (long long)q + 2],
+                               sums[4 * (long long)q + 3]};
+        sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr, nullptr, 0,
+                        Rq, coord, last);
+    }
+}
+
+// The integration sums of a long FP64 list, one lane per point, 64
+// consecutive list entries per wave, into sums (4 per entry).  Entries of one
+// finalize wave are contiguous in the list and neighbouring blocks belong to
+// the same request, so a wave's points nearly always share a trajectory; the
+// table is then read once per wave (scalar loads) instead of once per point.
+// A wave whose points span trajectories reads per lane.  Kept apart from the
+// terminating terms of k_sr_fp64 so that it runs with few registers.
+__global__ void __launch_bounds__(256) k_sr_fp64_sums(const srwd_item* __restrict__ items,
+                                                      const int* __restrict__ blockItem,
+                                                      const srwd_trj_info* __restrict__ info,
+                                                      const double* __restrict__ T64, long long ldI,
+                                                      const int* __restrict__ fpList,
+                                                      const int* __restrict__ fpCount,
+                                                      const srwd_req* __restrict__ reqs,
+                                                      const double* __restrict__ coord, double* __restrict__ sums)
+{
+    const int lane = threadIdx.x & 63;
+    const int nWaves = gridDim.x * (blockDim.x >> 6);
+    const int count = *fpCount;
+    if (count < SRWD_FP64_LANE_MIN || sums == nullptr) return;
     for (int seg = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); seg * 64 < count; seg += nWaves) {
         const int q = seg * 64 + lane;
         if (q < count) {
@@ -1297,21 +1361,21 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
             const int blk = entry >> 8, thr = entry & 255;
             const srwd_item& IT = items[blockItem[blk]];
             const int local = (blk - IT.blockBase) * 256 + thr;
-            const srwd_trj_info I = info[IT.traj];
+            const int traj = IT.traj;
             int ix = local % IT.nxc, iz = IT.iz0 + local / IT.nxc;
-            const srwd_req& Rq = reqs[IT.traj];
+            const srwd_req& Rq = reqs[traj];
             double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
             double kph = SRWD_KPH(IT.eph);
-            const double* T = T64 + (long long)IT.traj * ldI * SRWD_T64;
-            const int n = (int)I.nInt;
+            const double* T = T64 + (long long)traj * ldI * SRWD_T64;
+            const int n = (int)info[traj].nInt;
             double acc[4] = {0, 0, 0, 0};
-            if (__all(IT.traj == __builtin_amdgcn_readfirstlane(IT.traj)))
+            if (__all(traj == __builtin_amdgcn_readfirstlane(traj)))
                 fp64_point_sums((const __attribute__((address_space(4))) double*)wave_first_ptr(T),
                                 __builtin_amdgcn_readfirstlane(n), xo, zo, kph, acc);
             else
                 fp64_point_sums(T, n, xo, zo, kph, acc);
-            sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr, nullptr,
-                            0, Rq, coord, last);
+            double* o = sums + 4 * (long long)q;
+            o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
         }
     }
 }
@@ -1484,10 +1548,14 @@ int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec*
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
                         const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
                         float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, const srwd_req* dReq,
-                        const double* dCoord, const long long* dLast, hipStream_t st)
+                        const double* dCoord, const long long* dLast, double* dSums, hipStream_t st)
 {
+    if (dSums)
+        hipLaunchKernelGGL(k_sr_fp64_sums, dim3(2048), dim3(256), 0, st, dItems, dBlockItem, dInfo, dT64, ldI, dFpList,
+                           dFpCount, dReq, dCoord, dSums);
     hipLaunchKernelGGL(k_sr_fp64, dim3(2048), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin, dT64, ldI,
-                       (float2*)dEx, (float2*)dEz, dWarn, dFpList, dFpCount, dReq, dCoord, dLast);
+                       (float2*)dEx, (float2*)dEz, dWarn, dFpList, dFpCount, dReq, dCoord, dLast,
+                       (const double*)dSums);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

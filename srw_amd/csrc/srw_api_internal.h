@@ -31,6 +31,9 @@ struct Arena {
     int nPieces = 0;
     void* get(const DBuf* b, size_t bytes);
 };
+// Most row bands of one request (the FP64 counts of the bands are read back
+// through a pinned array of 64).
+#define SRWL_AMD_MAX_BANDS 32
 extern Arena* g_arena;   // non-null while a no-allocation call runs
 
 struct Buffers {
@@ -62,6 +65,7 @@ struct Context {
     long long lastSrEvals = 0;
     int lastSrBlocks = 0, lastSrSplit = 0;
     int forceSplit = 0;
+    int forceBands = 0;           // row bands of a single request's streamed download (0: automatic)
     bool trajSerial = false;      // test switch: serial RK4 kernel instead of the parallel one
     bool keepOnDevice = false;    // benchmark switch: leave results in HBM
     bool resizePerPoint = false;  // test switch: resample per new point instead of per old cell

@@ -178,8 +178,10 @@ int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec*
 // to dBlockItem, as passed to srwd_launch_finalize).
 // dT64: FP64 trajectory values (X, Z, X', Z', integral of the squared angles)
 // per integration point, SRWD_T64 doubles, as written by srwd_launch_tables.
+// dSums: optional scratch of 4 doubles per list entry (the longest list the
+// call can produce); with it a long list's sums run in a kernel of their own.
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
                         const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
                         float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, const srwd_req* dReq,
-                        const double* dCoord, const long long* dLast, hipStream_t st);
+                        const double* dCoord, const long long* dLast, double* dSums, hipStream_t st);
 }

@@ -418,6 +418,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         const SRWLRadMesh& m = jobs[0].mesh;
         double outBytes = 16. * m.ne * m.nx * m.ny;
         if (outBytes >= 16. * (1 << 20)) nBands = std::min(8, G.nzc);
+        if (g_ctx.forceBands > 0) nBands = std::min(g_ctx.forceBands, G.nzc);
     }
     std::vector<srwd_item> items;
     std::vector<int> blockItem;
@@ -606,8 +607,11 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
                                      nSplit, nPointsPad, dEx, dEz, dWarn, cnd, g_ctx.condTol, fpList, dFpCount + b,
                                      dReq, dCoord, dLast, sb))
                 return SRWL_AMD_ERR_GPU;
+            // a single band's partial sums are consumed by now: their buffer
+            // (at least 4 doubles per point) holds the FP64 pass's sums
             if (g_ctx.condTol > 0 && srwd_launch_sr_fp64(dItems, dBlockItem + b0, dInfo, dEdge, termin, dT64, ldI, dEx,
-                                                         dEz, dWarn, fpList, dFpCount + b, dReq, dCoord, dLast, sb))
+                                                         dEz, dWarn, fpList, dFpCount + b, dReq, dCoord, dLast,
+                                                         nBands == 1 ? dPart : nullptr, sb))
                 return SRWL_AMD_ERR_GPU;
         }
         if (nBands > 1) {
@@ -1202,6 +1206,13 @@ SRWL_EXPORT int srwlAmdSetKeepOnDevice(int on)
 SRWL_EXPORT int srwlAmdSetSplit(int nSplit)
 {
     g_ctx.forceSplit = nSplit;
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetBands(int nBands)
+{
+    if (nBands < 0 || nBands > SRWL_AMD_MAX_BANDS) return SRWL_AMD_ERR_UNSUPPORTED;
+    g_ctx.forceBands = nBands;
     return 0;
 }
 

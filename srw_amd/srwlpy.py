@@ -181,6 +181,11 @@ def set_traj_serial(on=True):
     lib.srwlAmdSetTrajSerial(1 if on else 0)
 
 
+def set_bands(n=0):
+    """Row bands of a single request's streamed download (0: automatic)."""
+    _check(lib.srwlAmdSetBands(int(n)))
+
+
 def set_energy_tiles(e=8):
     """Energies per workgroup of the energy-tile SR kernel: 8 (default), 4, or 0 (off)."""
     _check(lib.srwlAmdSetEnergyTiles(int(e)))
