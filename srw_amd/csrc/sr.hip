@@ -999,8 +999,11 @@ __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __res
                     }
                 }
             };
+            // four s points in flight per wave (more independent work
+            // between the dependent phase, sine and recurrence steps at two
+            // waves per SIMD)
             if (byRow) {
-#pragma unroll 2
+#pragma unroll 4
                 for (int k = bb; k < b1; k++) eval(k, false);
             } else {
 #pragma unroll 2
