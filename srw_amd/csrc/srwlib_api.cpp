@@ -437,7 +437,13 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             const SRWLRadMesh& m = J.mesh;
             int nx = (int)m.nx, nz = (int)m.ny, ne = (int)m.ne;
             SrRegion G = sr_region(m, I);
-            int r0 = (int)(((long long)G.nzc * b) / nBands), r1 = (int)(((long long)G.nzc * (b + 1)) / nBands);
+            // bands shrink towards the end (row fraction 1 - (1 - b/n)^1.5 before
+            // band b): the last band's download, which nothing overlaps, is short
+            auto cut = [&](int q) {
+                return q >= nBands ? G.nzc
+                                   : (int)std::lround(G.nzc * (1. - std::pow(1. - (double)q / nBands, 1.5)));
+            };
+            int r0 = cut(b), r1 = cut(b + 1);
             if (nBands > 1) { bandRow0[b] = r0; bandRow0[b + 1] = r1; }
             if (r1 <= r0) continue;
             double normConst = std::sqrt(normBase * J.Iavg * 1.E+09 / 1.602189246E-19);
