@@ -274,6 +274,56 @@ SRWL_EXPORT int srwlUtiFFT(char* pcData, char typeData, double* arMesh, int nMes
 SRWL_EXPORT int srwlUtiGPUProc(int op, double* arParGPU);
 
 /* ---------------------------------------------------------------------- */
+/* The rest of the reference C API (other sources, magnetic field and     */
+/* transmission utilities, Stokes and power density, intensity            */
+/* processing).  These are outside this library's path: they keep the    */
+/* reference's names and signatures, so that a binding written against    */
+/* the reference (its srwlpy.cpp) links unchanged, and each returns       */
+/* SRWL_AMD_ERR_UNSUPPORTED without touching its arguments.  The          */
+/* structures they take are declared here by tag only.                    */
+/* ---------------------------------------------------------------------- */
+typedef struct SRWLStructKickMatrix SRWLKickM;
+typedef struct SRWLStructGaussianBeam SRWLGsnBm;
+typedef struct SRWLStructPointSource SRWLPtSrc;
+typedef struct SRWLStructOpticsTransmission SRWLOptT;
+
+/* reference srwlib.h:590 */
+SRWL_EXPORT int srwlCalcMagFld(SRWLMagFldC* pDispMagFld, SRWLMagFldC* pMagFld, double* precPar);
+/* reference srwlib.h:619 */
+SRWL_EXPORT int srwlCalcPartTrajFromKickMatr(SRWLPrtTrj* pTrj, SRWLKickM* arKickM, int nKickM, double* precPar);
+/* reference srwlib.h:649 */
+SRWL_EXPORT int srwlCalcElecFieldGaussian(SRWLWfr* pWfr, SRWLGsnBm* pGsnBm, double* precPar);
+/* reference srwlib.h:659 */
+SRWL_EXPORT int srwlCalcElecFieldPointSrc(SRWLWfr* pWfr, SRWLPtSrc* pPtSrc, double* precPar);
+/* reference srwlib.h:675 */
+SRWL_EXPORT int srwlCalcStokesUR(SRWLStokes* pStokes, SRWLPartBeam* pElBeam, SRWLMagFldU* pUnd, double* precPar);
+/* reference srwlib.h:692 */
+SRWL_EXPORT int srwlCalcPowDenSR(SRWLStokes* pStokes, SRWLPartBeam* pElBeam, SRWLPrtTrj* pTrj, SRWLMagFldC* pMagFld,
+                                 double* precPar);
+/* reference srwlib.h:802 */
+SRWL_EXPORT int srwlProcElecField(SRWLWfr* pWfr, double* par, SRWLWfr* pWfr2);
+/* reference srwlib.h:811 */
+SRWL_EXPORT int srwlSetRepresElecField(SRWLWfr* pWfr, char repr);
+/* reference srwlib.h:838 */
+SRWL_EXPORT int srwlPropagRadMultiE(SRWLStokes* pStokes, SRWLWfr* pWfr0, SRWLOptC* pOpt, double* precPar,
+                                    int (*pExtFunc)(int action, SRWLStokes* pStokesInst));
+/* reference srwlib.h:851 */
+SRWL_EXPORT int srwlCalcTransm(SRWLOptT* pOpTr, const double* pDelta, const double* pAttenLen, double** arObjShapeDefs,
+                               int nObj3D, const double* arPar);
+/* reference srwlib.h:893 */
+SRWL_EXPORT int srwlUtiConvWithGaussian(char* pcData, char typeData, double* arMesh, int nMesh, double* arSig);
+/* reference srwlib.h:920 */
+SRWL_EXPORT int srwlUtiIntInf(double* arInf, char* pcData, char typeData, SRWLRadMesh* pMesh, double* arPar, int nPar);
+/* reference srwlib.h:954 */
+SRWL_EXPORT int srwlUtiIntProc(char* pcI1, char typeI1, SRWLRadMesh* pMesh1, char* pcI2, char typeI2,
+                               SRWLRadMesh* pMesh2, double* arPar, int nPar);
+/* reference srwlib.h:968 */
+SRWL_EXPORT int srwlUtiUndFromMagFldTab(SRWLMagFldC* pUndCnt, SRWLMagFldC* pMagCnt, double* arPrecPar);
+/* reference srwlib.h:986 */
+SRWL_EXPORT int srwlUtiUndFindMagFldInterpInds(int* arResInds, int* pnResInds, double* arGaps, double* arPhases,
+                                               int nVals, double arPrecPar[5]);
+
+/* ---------------------------------------------------------------------- */
 /* Batch entry points (no counterpart in the reference C API; they replace */
 /* the per-electron loop of srwl_wfr_emit_prop_multi_e, srwlib.py:8506)    */
 /* ---------------------------------------------------------------------- */

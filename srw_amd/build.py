@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsrw_amd.so")
-SOURCES = ["sr.hip", "sr_auto.hip", "fft.hip", "fft_gen.hip", "multie.hip", "extract.hip", "optics.hip", "learn.hip", "moments.hip", "resize.hip", "noalloc.hip", "srwlib_api.cpp", "propag_api.cpp", "multie_api.cpp", "moments_api.cpp", "resize_api.cpp", "auto_api.cpp"]
+SOURCES = ["sr.hip", "sr_auto.hip", "fft.hip", "fft_gen.hip", "multie.hip", "extract.hip", "optics.hip", "learn.hip", "moments.hip", "resize.hip", "noalloc.hip", "srwlib_api.cpp", "propag_api.cpp", "multie_api.cpp", "moments_api.cpp", "resize_api.cpp", "auto_api.cpp", "other_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

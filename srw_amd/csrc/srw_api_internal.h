@@ -88,6 +88,8 @@ struct Context {
     hipEvent_t evFork = nullptr, evJoin = nullptr;
     char* pin = nullptr;                    // pinned staging (grow-only)
     size_t pinBytes = 0;
+    char* up = nullptr;           // pinned staging of setup uploads (bump allocated per SR call)
+    size_t upBytes = 0, upUsed = 0;
     char* momPin = nullptr;       // pinned slice table and sums of queued moments
     size_t momPinBytes = 0;
     std::vector<hipEvent_t> bandEv;         // per band: computed, downloaded, SR start, SR end

@@ -169,6 +169,34 @@ static int field_moments(SRWLWfr& w, const float2* dEx, const float2* dEz)
     return wfr_moments(w, dEx, dEz, 1, ne, nx * ne, w.arMomX, w.arMomY);
 }
 
+// Host-to-device copy of setup data through a pinned staging area (bump
+// allocated per sr_run call), so that the copy is queued without the host
+// waiting for the stream, as a copy out of pageable memory can.  In a
+// no-allocation call the data are copied straight from the caller's memory.
+static int h2d(void* dst, const void* src, size_t bytes, hipStream_t st)
+{
+    if (bytes == 0) return 0;
+    if (g_arena) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess ? 0 : SRWL_AMD_ERR_GPU;
+    size_t at = (g_ctx.upUsed + 255) & ~(size_t)255;
+    if (at + bytes > g_ctx.upBytes) {
+        // a larger area: the current one may still feed queued copies
+        if (hipStreamSynchronize(st) != hipSuccess) return SRWL_AMD_ERR_GPU;
+        if (g_ctx.up) (void)hipHostFree(g_ctx.up);
+        g_ctx.up = nullptr;
+        g_ctx.upBytes = 0;
+        size_t want = std::max<size_t>({(size_t)1 << 20, 2 * (at + bytes), bytes});
+        if (hipHostMalloc((void**)&g_ctx.up, want, hipHostMallocDefault) != hipSuccess) {
+            g_ctx.up = nullptr;
+            return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        }
+        g_ctx.upBytes = want;
+        at = 0;
+    }
+    std::memcpy(g_ctx.up + at, src, bytes);
+    g_ctx.upUsed = at + bytes;
+    return hipMemcpyAsync(dst, g_ctx.up + at, bytes, hipMemcpyHostToDevice, st) == hipSuccess ? 0 : SRWL_AMD_ERR_GPU;
+}
+
 // Runs the single-electron SR computation for a set of requests.  Outputs
 // stay on the device: request j writes ne*nx*ny complex values of Ex and Ez
 // starting at float2 offset outOff[j] of dEx / dEz.
@@ -182,6 +210,10 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     int nT = (int)jobs.size();
     if (nT == 0) return 0;
     res.assign(nT, SrResult());
+    if (g_ctx.upUsed) {   // the staging area is reused once its copies are done
+        HIPCHK(hipStreamSynchronize(st));
+        g_ctx.upUsed = 0;
+    }
     Buffers& B = g_ctx.buf;
     // --- trajectories -----------------------------------------------------
     long long ldT = 0;
@@ -245,19 +277,19 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     srwd_trj_info* dInfo = (srwd_trj_info*)B.Info.get(sizeof(srwd_trj_info) * nT);
     if (!dX || !dXp || !dY || !dYp || !dIBx || !dIBz || !dSamp || !dJobs || !dFields || !dInfo)
         return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    HIPCHK(hipMemcpyAsync(dJobs, tj.data(), sizeof(srwd_traj_job) * nT, hipMemcpyHostToDevice, st));
+    if (int e_ = h2d(dJobs, tj.data(), sizeof(srwd_traj_job) * nT, st)) return e_;
     if (!fields.empty())
-        HIPCHK(hipMemcpyAsync(dFields, fields.data(), sizeof(srwd_field) * fields.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dInfo, info.data(), sizeof(srwd_trj_info) * nT, hipMemcpyHostToDevice, st));
+        if (int e_ = h2d(dFields, fields.data(), sizeof(srwd_field) * fields.size(), st)) return e_;
+    if (int e_ = h2d(dInfo, info.data(), sizeof(srwd_trj_info) * nT, st)) return e_;
     bool anyRK = false;
     for (int j = 0; j < nT; j++) {
         if (jobs[j].userTrj) {
             const SRWLPrtTrj* u = jobs[j].userTrj;
             size_t b = sizeof(double) * u->np;
-            HIPCHK(hipMemcpyAsync(dX + j * ldT, u->arX, b, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(dXp + j * ldT, u->arXp, b, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(dY + j * ldT, u->arY, b, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(dYp + j * ldT, u->arYp, b, hipMemcpyHostToDevice, st));
+            if (int e_ = h2d(dX + j * ldT, u->arX, b, st)) return e_;
+            if (int e_ = h2d(dXp + j * ldT, u->arXp, b, st)) return e_;
+            if (int e_ = h2d(dY + j * ldT, u->arY, b, st)) return e_;
+            if (int e_ = h2d(dYp + j * ldT, u->arYp, b, st)) return e_;
         } else anyRK = true;
     }
     if (anyRK) {
@@ -274,10 +306,10 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             if (!jobs[j].userTrj) continue;
             const SRWLPrtTrj* u = jobs[j].userTrj;
             size_t b = sizeof(double) * u->np;
-            HIPCHK(hipMemcpyAsync(dX + j * ldT, u->arX, b, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(dXp + j * ldT, u->arXp, b, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(dY + j * ldT, u->arY, b, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(dYp + j * ldT, u->arYp, b, hipMemcpyHostToDevice, st));
+            if (int e_ = h2d(dX + j * ldT, u->arX, b, st)) return e_;
+            if (int e_ = h2d(dXp + j * ldT, u->arXp, b, st)) return e_;
+            if (int e_ = h2d(dY + j * ldT, u->arY, b, st)) return e_;
+            if (int e_ = h2d(dYp + j * ldT, u->arYp, b, st)) return e_;
         }
     }
     if (srwd_launch_nz(dInfo, nT, dX, dXp, dY, dYp, ldT, st)) return SRWL_AMD_ERR_GPU;
@@ -306,7 +338,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         meshLim[4 * j] = m.xStart; meshLim[4 * j + 1] = m.xStart + (m.nx - 1) * xStep;
         meshLim[4 * j + 2] = m.yStart; meshLim[4 * j + 3] = m.yStart + (m.ny - 1) * zStep;
     }
-    HIPCHK(hipMemcpyAsync(dMesh, meshLim.data(), sizeof(double) * 4 * nT, hipMemcpyHostToDevice, st));
+    if (int e_ = h2d(dMesh, meshLim.data(), sizeof(double) * 4 * nT, st)) return e_;
     if (srwd_launch_rx(dInfo, nT, ldT, dX, dY, dIBx, dIBz, ldT, dSamp, dMesh, dRx, st)) return SRWL_AMD_ERR_GPU;
     std::vector<double> rx(4 * nT);
     HIPCHK(hipMemcpyAsync(rx.data(), dRx, sizeof(double) * 4 * nT, hipMemcpyDeviceToHost, st));
@@ -352,11 +384,11 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     int* dBlkReq = nEdgeBlk ? (int*)B.BlkReq.get(sizeof(int) * (size_t)nEdgeBlk) : nullptr;
     long long* dLast = nEdgeBlk ? (long long*)B.Last.get(sizeof(long long) * 2 * (size_t)nEdgeBlk) : nullptr;
     if (!dReq || !dCoord || (nEdgeBlk && (!dBlkReq || !dLast))) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    HIPCHK(hipMemcpyAsync(dReq, reqs.data(), sizeof(srwd_req) * nT, hipMemcpyHostToDevice, st));
+    if (int e_ = h2d(dReq, reqs.data(), sizeof(srwd_req) * nT, st)) return e_;
     if (!coord.empty())
-        HIPCHK(hipMemcpyAsync(dCoord, coord.data(), sizeof(double) * coord.size(), hipMemcpyHostToDevice, st));
+        if (int e_ = h2d(dCoord, coord.data(), sizeof(double) * coord.size(), st)) return e_;
     if (nEdgeBlk) {
-        HIPCHK(hipMemcpyAsync(dBlkReq, blkReq.data(), sizeof(int) * (size_t)nEdgeBlk, hipMemcpyHostToDevice, st));
+        if (int e_ = h2d(dBlkReq, blkReq.data(), sizeof(int) * (size_t)nEdgeBlk, st)) return e_;
         if (srwd_launch_edge_state(dReq, nT, dBlkReq, nEdgeBlk, dInfo, dEdge, dCoord, termin, dLast, st))
             return SRWL_AMD_ERR_GPU;
     }
@@ -516,19 +548,17 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     if (!dItems || !dBlockItem || !dSrBlocks || !dRowTraj || !dRowE || !dRowN || !dA || !dPart || !dWarn || !dCond || !dFpList ||
         !dFpCount)
         return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
-    HIPCHK(hipMemcpyAsync(dItems, items.data(), sizeof(srwd_item) * nItems, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dBlockItem, blockItem.data(), sizeof(int) * nBlocks, hipMemcpyHostToDevice, st));
+    if (int e_ = h2d(dItems, items.data(), sizeof(srwd_item) * nItems, st)) return e_;
+    if (int e_ = h2d(dBlockItem, blockItem.data(), sizeof(int) * nBlocks, st)) return e_;
     int2* dTileBlocks = dSrBlocks;
     int2* dRestBlocks = dSrBlocks + tileBlocks.size();
     if (!tileBlocks.empty())
-        HIPCHK(hipMemcpyAsync(dTileBlocks, tileBlocks.data(), sizeof(int2) * tileBlocks.size(), hipMemcpyHostToDevice,
-                              st));
+        if (int e_ = h2d(dTileBlocks, tileBlocks.data(), sizeof(int2) * tileBlocks.size(), st)) return e_;
     if (!restBlocks.empty())
-        HIPCHK(hipMemcpyAsync(dRestBlocks, restBlocks.data(), sizeof(int2) * restBlocks.size(), hipMemcpyHostToDevice,
-                              st));
-    HIPCHK(hipMemcpyAsync(dRowTraj, rowTraj.data(), sizeof(int) * nRows, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dRowE, rowE.data(), sizeof(double) * nRows, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dRowN, rowN.data(), sizeof(long long) * nRows, hipMemcpyHostToDevice, st));
+        if (int e_ = h2d(dRestBlocks, restBlocks.data(), sizeof(int2) * restBlocks.size(), st)) return e_;
+    if (int e_ = h2d(dRowTraj, rowTraj.data(), sizeof(int) * nRows, st)) return e_;
+    if (int e_ = h2d(dRowE, rowE.data(), sizeof(double) * nRows, st)) return e_;
+    if (int e_ = h2d(dRowN, rowN.data(), sizeof(long long) * nRows, st)) return e_;
     HIPCHK(hipMemsetAsync(dWarn, 0, sizeof(int), st));
     if (srwd_launch_phase_table(dQ, ldI, dRowTraj, dRowE, dRowN, nRows, maxInt, dA, st)) return SRWL_AMD_ERR_GPU;
     if (srwd_launch_cond_sums(dInfo, nT, dRec, ldI, dCond, st)) return SRWL_AMD_ERR_GPU;

@@ -44,6 +44,32 @@ def test_reference_entry_points_are_declared():
     assert need <= set(_declared())
 
 
+def test_every_reference_entry_point_links_and_the_rest_report_unsupported(lib):
+    # every entry point of the reference C API (srwlib.h:542-998) is exported,
+    # so that a binding written against it links; those outside this
+    # library's path return SRWL_AMD_ERR_UNSUPPORTED (23301) and touch nothing
+    ref_api = {"srwlCalcElecFieldGaussian", "srwlCalcElecFieldPointSrc", "srwlCalcElecFieldSR",
+               "srwlCalcIntFromElecField", "srwlCalcMagFld", "srwlCalcPartTraj", "srwlCalcPartTrajFromKickMatr",
+               "srwlCalcPowDenSR", "srwlCalcStokesUR", "srwlCalcTransm", "srwlProcElecField", "srwlPropagElecField",
+               "srwlPropagRadMultiE", "srwlResizeElecField", "srwlResizeElecFieldMesh", "srwlSetRepresElecField",
+               "srwlUtiConvWithGaussian", "srwlUtiFFT", "srwlUtiGPUProc", "srwlUtiGetErrText", "srwlUtiIntInf",
+               "srwlUtiIntProc", "srwlUtiUndFindMagFldInterpInds", "srwlUtiUndFromMagFldTab", "srwlUtiVerNo",
+               "srwlUtiSetWfrModifFunc", "srwlUtiSetAllocArrayFunc", "srwlUtiSetProgrIndFunc"}
+    assert ref_api <= set(_declared())
+    for n in ref_api:
+        assert hasattr(lib, n), n
+    nulls = {"srwlCalcMagFld": 3, "srwlCalcPartTrajFromKickMatr": 4, "srwlCalcElecFieldGaussian": 3,
+             "srwlCalcElecFieldPointSrc": 3, "srwlCalcStokesUR": 4, "srwlCalcPowDenSR": 5, "srwlProcElecField": 3,
+             "srwlSetRepresElecField": 2, "srwlPropagRadMultiE": 5, "srwlCalcTransm": 6,
+             "srwlUtiConvWithGaussian": 5, "srwlUtiIntInf": 6, "srwlUtiIntProc": 8, "srwlUtiUndFromMagFldTab": 3,
+             "srwlUtiUndFindMagFldInterpInds": 6}
+    for n, k in nulls.items():
+        f = getattr(lib, n)
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p] * k
+        assert f(*([None] * k)) == 23301, n
+
+
 def test_error_texts_follow_reference_codes(lib):
     buf = C.create_string_buffer(2048)
     lib.srwlUtiGetErrText.argtypes = [C.c_char_p, C.c_int]
