@@ -43,6 +43,16 @@ SRWL_EXPORT int srwlAmdSetSplit(int nSplit);
  * 32; SRWL_AMD_ERR_UNSUPPORTED above). */
 SRWL_EXPORT int srwlAmdSetBands(int nBands);
 
+/* on != 0 (default): a single request whose field takes the banded download
+ * and has energy tiles is computed in one pass whose blocks finish their own
+ * points and count finished row bands, each band downloaded as soon as it is
+ * complete; 0: row bands computed one after another. */
+SRWL_EXPORT int srwlAmdSetStreamRows(int on);
+
+/* 1 when the last single-request call took the streamed-rows download to the
+ * end (no fall back to a plain download), 0 otherwise. */
+SRWL_EXPORT int srwlAmdLastStreamed(void);
+
 /* on != 0: integrate trajectories with the serial RK4 kernel (two threads per
  * trajectory) instead of the parallel-in-s solver; both give the serial RK4
  * recurrence, the switch exists to compare them. */

@@ -869,6 +869,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     o[0] = ox; o[1] = oxi; o[2] = oz; o[3] = ozi;
 }
 
+// Terminating terms, condition check and stores of one point (defined with
+// the finalize pass below).
+__device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, const srwd_edge& E, int termin,
+                                int ix, int iz, const double* S, float2* Ex, float2* Ez, int* warn,
+                                double sumSqX, double sumSqZ, double condTol, int* fpList, int* fpCount,
+                                int listEntry, const srwd_req& Rq, const double* coord, const long long* last,
+                                srwd_patch* patch = nullptr, int* patchCount = nullptr, int patchCap = 0);
+
 // The same integral for SRWD_TILE_E consecutive photon energies of one
 // request at once.  The phase of energy k is linear in k (K_k = K_0 + k dK,
 // and the phase table rows obey A_k = A_0 + k A_dK mod 1), so per s a lane
@@ -880,7 +888,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 // so short chains keep it near the FP32 rounding of the phase itself.
 // 64-point FP32 block sums are turned by each energy's FP64 chunk phase and
 // accumulated in FP64.
-template <int E>
+// FUSED: each block then finishes its points itself (terminating terms,
+// condition check, stores; the finalize pass of the two-pass form) and, once
+// its stores are visible to the system, sets its completion flag
+// (F.blockDone[blockIdx.x] = F.epoch); nSplit is 1.
+template <int E, bool FUSED>
 __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __restrict__ items,
                                                           const int2* __restrict__ blocks,
                                                           const srwd_trj_info* __restrict__ info,
@@ -888,7 +900,8 @@ __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __res
                                                           const srwd_chunk* __restrict__ chunks,
                                                           const float* __restrict__ A,
                                                           long long ldI, long long ldC, int nSplit,
-                                                          long long nPointsPad, double* __restrict__ part)
+                                                          long long nPointsPad, double* __restrict__ part,
+                                                          const srwd_fuse F)
 {
     static_assert(E % 4 == 0, "energy tiles are chains of 4");
     const int2 bk = blocks[blockIdx.x];
@@ -1019,11 +1032,39 @@ __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __res
             }
         }
     }
-    if (!valid) return;
+    if (valid) {
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        double* o = part + ((long long)blockIdx.y * nPointsPad + items[bk.x + e].pointBase + local) * 4;
-        o[0] = out[e][0]; o[1] = out[e][1]; o[2] = out[e][2]; o[3] = out[e][3];
+        for (int e = 0; e < E; e++) {
+            double* o = part + ((long long)blockIdx.y * nPointsPad + items[bk.x + e].pointBase + local) * 4;
+            o[0] = out[e][0]; o[1] = out[e][1]; o[2] = out[e][2]; o[3] = out[e][3];
+        }
+    }
+    if constexpr (FUSED) {
+        if (valid) {
+#pragma unroll 1
+            for (int e = 0; e < E; e++) {
+                const srwd_item& Ie = items[bk.x + e];
+                const double* pp = part + (Ie.pointBase + local) * 4;   // this thread's own sums
+                const double S[4] = {pp[0], pp[1], pp[2], pp[3]};
+                const srwd_req Rq = F.reqs[Ie.traj];
+                double sumSqX = 0., sumSqZ = 0.;
+                if (F.cond) {
+                    const double* c = F.cond + 5 * Ie.traj;
+                    const double xr = F.coord[Rq.xOff + ix], zr = F.coord[Rq.zOff + iz];
+                    sumSqX = c[0] * xr * xr - 2. * xr * c[1] + c[3];
+                    sumSqZ = c[0] * zr * zr - 2. * zr * c[2] + c[4];
+                }
+                sr_finish_point(Ie, info[Ie.traj], F.edges[Ie.traj], F.termin, ix, iz, S, F.Ex, F.Ez, F.warn, sumSqX,
+                                sumSqZ, F.condTol, F.cond ? F.fpList : nullptr, F.fpCount,
+                                ((Ie.blockBase + bk.y) << 8) | (int)threadIdx.x, Rq, F.coord, F.last);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // the block's stores reach memory before its flag says done
+            __threadfence_system();
+            __hip_atomic_store(F.blockDone + blockIdx.x, F.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -1056,7 +1097,8 @@ __global__ void __launch_bounds__(256) k_sr_integral_tile(const srwd_item* __res
 __device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, const srwd_edge& E, int termin,
                                 int ix, int iz, const double* S, float2* Ex, float2* Ez, int* warn,
                                 double sumSqX, double sumSqZ, double condTol, int* fpList, int* fpCount,
-                                int listEntry, const srwd_req& Rq, const double* coord, const long long* last)
+                                int listEntry, const srwd_req& Rq, const double* coord, const long long* last,
+                                srwd_patch* patch, int* patchCount, int patchCap)
 {
     double xo = coord[Rq.xOff + ix], zo = coord[Rq.zOff + iz];
     double h = I.h;
@@ -1099,8 +1141,13 @@ __device__ void sr_finish_point(const srwd_item& IT, const srwd_trj_info& I, con
     long long ne = IT.ne;
     auto put = [&](int jx, int jz, float sx, float sz) {
         long long o = IT.outOffset + ((long long)jz * IT.nx + jx) * ne;
-        Ex[o] = make_float2(sx * vx.x, sx * vx.y);
-        Ez[o] = make_float2(sz * vz.x, sz * vz.y);
+        const float2 a = make_float2(sx * vx.x, sx * vx.y), c = make_float2(sz * vz.x, sz * vz.y);
+        Ex[o] = a;
+        Ez[o] = c;
+        if (patch) {
+            const int k = atomicAdd(patchCount, 1);
+            if (k < patchCap) patch[k] = srwd_patch{o, a, c};
+        }
     };
     put(ix, iz, 1.f, 1.f);
     // symmetric parts (srradint.cpp:3264-3350): mirror in z flips Ez, in x flips Ex
@@ -1256,7 +1303,8 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
                                                  float2* Ez, int* warn, const int* __restrict__ fpList,
                                                  const int* __restrict__ fpCount, const srwd_req* __restrict__ reqs,
                                                  const double* __restrict__ coord,
-                                                 const long long* __restrict__ last, const double* __restrict__ sums)
+                                                 const long long* __restrict__ last, const double* __restrict__ sums,
+                                                 srwd_patch* patch, int* patchCount, int patchCap)
 {
     const int lane = threadIdx.x & 63;
     const int nWaves = gridDim.x * (blockDim.x >> 6);
@@ -1289,7 +1337,7 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
                 for (int off = 32; off > 0; off >>= 1) acc[m] += __shfl_down(acc[m], off);
             if (lane == 0)
                 sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr,
-                                nullptr, 0, Rq, coord, last);
+                                nullptr, 0, Rq, coord, last, patch, patchCount, patchCap);
         }
         return;
     }
@@ -1316,7 +1364,7 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
                 else
                     fp64_point_sums(T, n, xo, zo, kph, acc);
                 sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr,
-                                nullptr, 0, Rq, coord, last);
+                                nullptr, 0, Rq, coord, last, patch, patchCount, patchCap);
             }
         }
         return;
@@ -1333,7 +1381,7 @@ __global__ void __launch_bounds__(256) k_sr_fp64(const srwd_item* __restrict__ i
         const double acc[4] = {sums[4 * (long long)q], sums[4 * (long long)q + 1], sums[4 * (long long)q + 2],
                                sums[4 * (long long)q + 3]};
         sr_finish_point(IT, I, edges[IT.traj], termin, ix, iz, acc, Ex, Ez, warn, 0., 0., 0., nullptr, nullptr, 0,
-                        Rq, coord, last);
+                        Rq, coord, last, patch, patchCount, patchCap);
     }
 }
 
@@ -1513,17 +1561,35 @@ int srwd_launch_sr(const srwd_item* dItems, const int2* dBlocks, int nBlocks, in
                    long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st)
 {
     if (nBlocks <= 0) return 0;
+    const srwd_fuse none{};
     if (tiled == 8)
-        hipLaunchKernelGGL(k_sr_integral_tile<8>, dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlocks,
-                           dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart);
+        hipLaunchKernelGGL((k_sr_integral_tile<8, false>), dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlocks,
+                           dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart, none);
     else if (tiled == 4)
-        hipLaunchKernelGGL(k_sr_integral_tile<4>, dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlocks,
-                           dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart);
+        hipLaunchKernelGGL((k_sr_integral_tile<4, false>), dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlocks,
+                           dInfo, dRec, dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart, none);
     else if (tiled)
         return -1;
     else
         hipLaunchKernelGGL(k_sr_integral, dim3(nBlocks, nSplit), dim3(256), 0, st, dItems, dBlocks, dInfo, dRec,
                            dChunk, dA, ldI, ldC, nSplit, nPointsPad, dPart);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int srwd_launch_sr_fused(const srwd_item* dItems, const int2* dBlocks, int nBlocks, int tiled,
+                         const srwd_trj_info* dInfo, const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA,
+                         long long ldI, long long ldC, long long nPointsPad, double* dPart, const srwd_fuse& F,
+                         hipStream_t st)
+{
+    if (nBlocks <= 0) return 0;
+    if (tiled == 8)
+        hipLaunchKernelGGL((k_sr_integral_tile<8, true>), dim3(nBlocks, 1), dim3(256), 0, st, dItems, dBlocks, dInfo,
+                           dRec, dChunk, dA, ldI, ldC, 1, nPointsPad, dPart, F);
+    else if (tiled == 4)
+        hipLaunchKernelGGL((k_sr_integral_tile<4, true>), dim3(nBlocks, 1), dim3(256), 0, st, dItems, dBlocks, dInfo,
+                           dRec, dChunk, dA, ldI, ldC, 1, nPointsPad, dPart, F);
+    else
+        return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1551,14 +1617,15 @@ int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec*
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
                         const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
                         float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, const srwd_req* dReq,
-                        const double* dCoord, const long long* dLast, double* dSums, hipStream_t st)
+                        const double* dCoord, const long long* dLast, double* dSums, srwd_patch* dPatch,
+                        int* dPatchCount, int patchCap, hipStream_t st)
 {
     if (dSums)
         hipLaunchKernelGGL(k_sr_fp64_sums, dim3(2048), dim3(256), 0, st, dItems, dBlockItem, dInfo, dT64, ldI, dFpList,
                            dFpCount, dReq, dCoord, dSums);
     hipLaunchKernelGGL(k_sr_fp64, dim3(2048), dim3(256), 0, st, dItems, dBlockItem, dInfo, dEdge, termin, dT64, ldI,
                        (float2*)dEx, (float2*)dEz, dWarn, dFpList, dFpCount, dReq, dCoord, dLast,
-                       (const double*)dSums);
+                       (const double*)dSums, dPatch, dPatchCount, patchCap);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

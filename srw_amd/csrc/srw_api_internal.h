@@ -34,6 +34,11 @@ struct Arena {
 // Most row bands of one request (the FP64 counts of the bands are read back
 // through a pinned array of 64).
 #define SRWL_AMD_MAX_BANDS 32
+// FP64 store records a streamed-rows call patches into the caller's arrays
+// (more: the whole field is downloaded again), and the longest the host
+// waits for the blocks' flags before it falls back to a plain download.
+#define SRWL_AMD_PATCH_CAP (1 << 18)
+#define SRWL_AMD_STREAM_WAIT_S 60
 extern Arena* g_arena;   // non-null while a no-allocation call runs
 
 struct Buffers {
@@ -43,6 +48,7 @@ struct Buffers {
     DBuf T64;                     // FP64 trajectory values per integration point (FP64 recomputation)
     DBuf Req, Coord, BlkReq, Last;   // requests, observation coordinates, edge-derivative state
     DBuf SrBlocks;                // SR launch lists {item, block}
+    DBuf TileBands, Patch, PatchCount;   // streamed rows: row bands per tile block, FP64 store records
     DBuf Ex, Ez;                  // field outputs
     DBuf W0, W1, W2, W3;          // propagation / accumulation work buffers
     DBuf Stokes, Draws, Wt, Net, Grad, Rep;
@@ -66,6 +72,14 @@ struct Context {
     int lastSrBlocks = 0, lastSrSplit = 0;
     int forceSplit = 0;
     int forceBands = 0;           // row bands of a single request's streamed download (0: automatic)
+    bool streamRows = true;       // one pass counting finished row bands instead of banded passes
+    unsigned* blockDone = nullptr; // host-coherent completion flags of a streamed-rows pass, per tile block
+    size_t blockDoneCap = 0;
+    unsigned streamEpoch = 0;
+    hipEvent_t evTiles = nullptr;  // the streamed-rows tile kernel has finished
+    char* patchHost = nullptr;    // pinned: FP64 store count, then the store records
+    bool lastStreamFallback = false;
+    bool lastStreamed = false;    // the last single-request call streamed rows without falling back
     bool trajSerial = false;      // test switch: serial RK4 kernel instead of the parallel one
     bool keepOnDevice = false;    // benchmark switch: leave results in HBM
     bool resizePerPoint = false;  // test switch: resample per new point instead of per old cell

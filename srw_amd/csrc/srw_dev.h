@@ -144,6 +144,32 @@ int srwd_launch_rx(const srwd_trj_info* dInfo, int nTraj, long long maxNp, const
                    double* dRx, hipStream_t st);
 int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj, const double* dRowE,
                             const long long* dRowN, int nRows, long long maxInt, float* dA, hipStream_t st);
+// A store of the FP64 pass recorded for the host (offset in float2 units of
+// both components, the values stored).
+struct srwd_patch {
+    long long off;
+    float2 ex, ez;
+};
+
+// Finishing inside the energy-tile kernel (one pass without s-range split):
+// the arguments of the finalize pass; after its stores, block i writes
+// `epoch` to blockDone[i] (host-coherent memory, system scope).
+struct srwd_fuse {
+    const srwd_edge* edges;
+    int termin;
+    float2 *Ex, *Ez;
+    int* warn;
+    const double* cond;
+    double condTol;
+    int* fpList;
+    int* fpCount;
+    const srwd_req* reqs;
+    const double* coord;
+    const long long* last;
+    unsigned* blockDone;     // per launched block
+    unsigned epoch;
+};
+
 // SR integral over a list of 256-point blocks: dBlocks[i] = {item, block of
 // the item}.  tiled = 0: one photon energy per block; tiled = 4 or 8: the
 // block's item and the tiled - 1 items after it (consecutive photon energies
@@ -151,6 +177,13 @@ int srwd_launch_phase_table(const double* dQ, long long ldI, const int* dRowTraj
 int srwd_launch_sr(const srwd_item* dItems, const int2* dBlocks, int nBlocks, int tiled, const srwd_trj_info* dInfo,
                    const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA, long long ldI,
                    long long ldC, int nSplit, long long nPointsPad, double* dPart, hipStream_t st);
+// The energy-tile kernel (tiled = 4 or 8) with the finalize pass done by each
+// block after its sums (nSplit must be 1; dPart holds the sums as in the
+// two-pass form), and the per-block completion flags of F.
+int srwd_launch_sr_fused(const srwd_item* dItems, const int2* dBlocks, int nBlocks, int tiled,
+                         const srwd_trj_info* dInfo, const srwd_rec* dRec, const srwd_chunk* dChunk, const float* dA,
+                         long long ldI, long long ldC, long long nPointsPad, double* dPart, const srwd_fuse& F,
+                         hipStream_t st);
 // Edge-derivative state of the reference's point loop: per block of
 // SRWD_EDGE_BLOCK points of each request (dBlkReq: its request) and edge, the
 // last point at or before the block's end whose far-field expansion at that
@@ -180,8 +213,12 @@ int srwd_launch_cond_sums(const srwd_trj_info* dInfo, int nTraj, const srwd_rec*
 // per integration point, SRWD_T64 doubles, as written by srwd_launch_tables.
 // dSums: optional scratch of 4 doubles per list entry (the longest list the
 // call can produce); with it a long list's sums run in a kernel of their own.
+// dPatch (optional, capacity patchCap): every store is also recorded there
+// (*dPatchCount counts them, zeroed by the caller, and may exceed the
+// capacity: then the records past it are dropped).
 int srwd_launch_sr_fp64(const srwd_item* dItems, const int* dBlockItem, const srwd_trj_info* dInfo,
                         const srwd_edge* dEdge, int termin, const double* dT64, long long ldI, float* dEx,
                         float* dEz, int* dWarn, const int* dFpList, const int* dFpCount, const srwd_req* dReq,
-                        const double* dCoord, const long long* dLast, double* dSums, hipStream_t st);
+                        const double* dCoord, const long long* dLast, double* dSums, srwd_patch* dPatch,
+                        int* dPatchCount, int patchCap, hipStream_t st);
 }

@@ -12,6 +12,7 @@
 #include "srw_fft.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -40,6 +41,7 @@ int ensure_device()
     if (hipStreamCreateWithFlags(&g_ctx.stream2, hipStreamNonBlocking) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipEventCreateWithFlags(&g_ctx.evFork, hipEventDisableTiming) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipEventCreateWithFlags(&g_ctx.evJoin, hipEventDisableTiming) != hipSuccess) return SRWL_AMD_ERR_GPU;
+    if (hipEventCreateWithFlags(&g_ctx.evTiles, hipEventDisableTiming) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipHostMalloc((void**)&g_ctx.warnHost, sizeof(int), hipHostMallocDefault) != hipSuccess) return SRWL_AMD_ERR_GPU;
     if (hipHostMalloc((void**)&g_ctx.fpCountHost, sizeof(int) * 64, hipHostMallocDefault) != hipSuccess)
         return SRWL_AMD_ERR_GPU;
@@ -210,6 +212,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     int nT = (int)jobs.size();
     if (nT == 0) return 0;
     res.assign(nT, SrResult());
+    g_ctx.lastStreamed = false;
     if (g_ctx.upUsed) {   // the staging area is reused once its copies are done
         HIPCHK(hipStreamSynchronize(st));
         g_ctx.upUsed = 0;
@@ -452,6 +455,22 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         if (outBytes >= 16. * (1 << 20)) nBands = std::min(8, G.nzc);
         if (g_ctx.forceBands > 0) nBands = std::min(g_ctx.forceBands, G.nzc);
     }
+    // Streamed rows: instead of computing row bands one after another, one
+    // pass whose energy-tile blocks (in row order) finish their own points
+    // and then set a completion flag in host-coherent memory; the host
+    // downloads each band as soon as the flags of its blocks are set, and patches
+    // the points the FP64 pass recomputes afterwards.  Taken when the banded
+    // download would be, the request has energy tiles, and one pass without
+    // an s-range split fills the GPU.
+    int nDl = 0;
+    if (nBands > 1 && g_ctx.streamRows && dRowOf[0] >= 0 && g_ctx.forceSplit <= 1) {
+        SrRegion G = sr_region(jobs[0].mesh, info[0]);
+        const long long ne = jobs[0].mesh.ne, blocks = ((long long)G.nxc * G.nzc + 255) / 256;
+        if ((ne / g_ctx.tileE + ne % g_ctx.tileE) * blocks >= 1024) {
+            nDl = nBands;
+            nBands = 1;
+        }
+    }
     std::vector<srwd_item> items;
     std::vector<int> blockItem;
     std::vector<int> bandBlock0(nBands + 1, 0), bandRow0(nBands + 1, 0);
@@ -512,6 +531,30 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     bandBlock0[nBands] = (int)blockItem.size();
     bandTile0[nBands] = (int)tileBlocks.size();
     bandRest0[nBands] = (int)restBlocks.size();
+    std::vector<int> dlRow0, dlLast;
+    if (nDl) {
+        // the tile blocks in row order (every energy group of a 256-point
+        // block before the next block), so that rows are finished in order
+        std::stable_sort(tileBlocks.begin(), tileBlocks.end(), [](int2 a, int2 b) { return a.y < b.y; });
+        SrRegion G = sr_region(jobs[0].mesh, info[0]);
+        dlRow0.resize(nDl + 1);
+        for (int q = 0; q <= nDl; q++)
+            dlRow0[q] = q >= nDl ? G.nzc : (int)std::lround(G.nzc * (1. - std::pow(1. - (double)q / nDl, 1.5)));
+        auto bandOf = [&](int r) {
+            int b = 0;
+            while (b + 1 < nDl && dlRow0[b + 1] <= r) b++;
+            return b;
+        };
+        // last tile block (in launch order) whose points touch each band
+        dlLast.assign(nDl, -1);
+        for (int i = 0; i < (int)tileBlocks.size(); i++) {
+            const srwd_item& it = items[tileBlocks[i].x];
+            const int p0 = tileBlocks[i].y * 256, p1 = std::min(p0 + 255, it.npts - 1);
+            for (int q = bandOf(it.iz0 + p0 / it.nxc); q <= bandOf(it.iz0 + p1 / it.nxc); q++) dlLast[q] = i;
+        }
+        for (int q = 1; q < nDl; q++) dlLast[q] = std::max(dlLast[q], dlLast[q - 1]);
+        bandRow0 = dlRow0;
+    }
     int nItems = (int)items.size();
     int nRows = (int)rowTraj.size();
     int nBlocks = (int)blockItem.size();
@@ -604,6 +647,144 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             par_memcpy((char*)hostOut->ez + o, pin + compBytes + o, n);
         }
     };
+    if (nDl) {
+        // --- streamed rows (see above) ---------------------------------------
+        pin = pinned_staging(2 * compBytes);
+        if (!pin) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        const int nTile = (int)tileBlocks.size(), nRest = (int)restBlocks.size();
+        if ((size_t)nTile > g_ctx.blockDoneCap) {
+            if (g_ctx.blockDone) (void)hipHostFree(g_ctx.blockDone);
+            g_ctx.blockDone = nullptr;
+            g_ctx.blockDoneCap = 0;
+            if (hipHostMalloc((void**)&g_ctx.blockDone, sizeof(unsigned) * (size_t)nTile, hipHostMallocCoherent) !=
+                hipSuccess) {
+                g_ctx.blockDone = nullptr;
+                return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+            }
+            std::memset(g_ctx.blockDone, 0, sizeof(unsigned) * (size_t)nTile);
+            g_ctx.blockDoneCap = nTile;
+        }
+        // flags of this call carry its own epoch (never 0), so they need no clearing
+        const unsigned epoch = ++g_ctx.streamEpoch ? g_ctx.streamEpoch : ++g_ctx.streamEpoch;
+        if (!g_ctx.patchHost &&
+            hipHostMalloc((void**)&g_ctx.patchHost, sizeof(srwd_patch) * SRWL_AMD_PATCH_CAP + 256,
+                          hipHostMallocDefault) != hipSuccess) {
+            g_ctx.patchHost = nullptr;
+            return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        }
+        srwd_patch* dPatch = (srwd_patch*)B.Patch.get(sizeof(srwd_patch) * SRWL_AMD_PATCH_CAP);
+        int* dPatchCount = (int*)B.PatchCount.get(sizeof(int));
+        if (!dPatch || !dPatchCount) return SRWL_ERR_MEMORY_ALLOCATION_FAILURE;
+        HIPCHK(hipMemsetAsync(dPatchCount, 0, sizeof(int), st));
+        volatile unsigned* done = g_ctx.blockDone;
+        while (g_ctx.bandEv.size() < 4 * (size_t)nDl) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            g_ctx.bandEv.push_back(e);
+        }
+        const double* cnd = g_ctx.condTol > 0 ? dCond : nullptr;
+        // the single-energy items (energies left over from the groups) on the
+        // second stream, finished there by the two-pass form
+        HIPCHK(hipEventRecord(g_ctx.evFork, st));
+        HIPCHK(hipStreamWaitEvent(g_ctx.stream2, g_ctx.evFork, 0));
+        if (srwd_launch_sr(dItems, dRestBlocks, nRest, 0, dInfo, dRec, dChunk, dA, ldI, ldC, 1, nPointsPad, dPart,
+                           g_ctx.stream2) ||
+            srwd_launch_finalize(dItems, dRestBlocks, nRest, 1, dInfo, dEdge, termin, dPart, 1, nPointsPad, dEx, dEz,
+                                 dWarn, cnd, g_ctx.condTol, dFpList, dFpCount, dReq, dCoord, dLast, g_ctx.stream2))
+            return SRWL_AMD_ERR_GPU;
+        HIPCHK(hipEventRecord(g_ctx.evJoin, g_ctx.stream2));
+        srwd_fuse F{};
+        F.edges = dEdge; F.termin = termin; F.Ex = (float2*)dEx; F.Ez = (float2*)dEz; F.warn = dWarn;
+        F.cond = cnd; F.condTol = g_ctx.condTol; F.fpList = dFpList; F.fpCount = dFpCount;
+        F.reqs = dReq; F.coord = dCoord; F.last = dLast; F.blockDone = g_ctx.blockDone; F.epoch = epoch;
+        if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev0, st));
+        if (srwd_launch_sr_fused(dItems, dTileBlocks, nTile, g_ctx.tileE, dInfo, dRec, dChunk, dA, ldI, ldC,
+                                 nPointsPad, dPart, F, st))
+            return SRWL_AMD_ERR_GPU;
+        if (g_ctx.timing) HIPCHK(hipEventRecord(g_ctx.ev1, st));
+        HIPCHK(hipEventRecord(g_ctx.evTiles, st));
+        HIPCHK(hipStreamWaitEvent(st, g_ctx.evJoin, 0));
+        // FP64 recomputation, every store also recorded for the host
+        if (g_ctx.condTol > 0 &&
+            srwd_launch_sr_fp64(dItems, dBlockItem, dInfo, dEdge, termin, dT64, ldI, dEx, dEz, dWarn, dFpList,
+                                dFpCount, dReq, dCoord, dLast, dPart, dPatch, dPatchCount, SRWL_AMD_PATCH_CAP, st))
+            return SRWL_AMD_ERR_GPU;
+        HIPCHK(hipMemcpyAsync(g_ctx.warnHost, dWarn, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(g_ctx.fpCountHost, dFpCount, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(g_ctx.patchHost, dPatchCount, sizeof(int), hipMemcpyDeviceToHost, st));
+        g_ctx.fpBands = 1;
+        long long evs = 0;
+        for (auto& it : items) evs += (long long)it.npts * info[it.traj].nInt;
+        g_ctx.pendingEvals = evs;
+        if (hostOut->queueMore) {
+            rc = hostOut->queueMore(hostOut->queueCtx, res);
+            if (rc) return rc;
+        }
+        // each band's download once its blocks are counted; the host copy of
+        // band b - 1 while band b is waited for and travels
+        // A block's flag is visible once its stores are; blocks finish nearly
+        // in launch order, so a cursor walks the flags in that order.  Once
+        // the kernel is known to have finished, a flag still missing means
+        // the flags do not reach the host: fall back.
+        const auto t0 = std::chrono::steady_clock::now();
+        bool late = false;
+        int cursor = 0;
+        for (int b = 0; b < nDl && !late; b++) {
+            for (unsigned spin = 0; cursor <= dlLast[b];) {
+                if (done[cursor] == epoch) { cursor++; continue; }
+                if ((++spin & 1023) == 0) {
+                    const bool finished = hipEventQuery(g_ctx.evTiles) == hipSuccess;
+                    if ((finished && done[cursor] != epoch) ||
+                        std::chrono::steady_clock::now() - t0 > std::chrono::seconds(SRWL_AMD_STREAM_WAIT_S)) {
+                        late = true;
+                        break;
+                    }
+                }
+            }
+            if (late) break;
+            if (b == 0) HIPCHK(hipStreamWaitEvent(g_ctx.copyStream, g_ctx.evJoin, 0));
+            int r[4];
+            rows_of_band(b, r);
+            for (int k = 0; k < 4; k += 2) {
+                if (r[k + 1] <= r[k]) continue;
+                size_t o = rowBytes * r[k], n = rowBytes * (r[k + 1] - r[k]);
+                HIPCHK(hipMemcpyAsync(pin + o, (char*)dEx + o, n, hipMemcpyDeviceToHost, g_ctx.copyStream));
+                HIPCHK(hipMemcpyAsync(pin + compBytes + o, (char*)dEz + o, n, hipMemcpyDeviceToHost,
+                                      g_ctx.copyStream));
+            }
+            HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 1], g_ctx.copyStream));
+            if (b > 0) {
+                HIPCHK(hipEventSynchronize(g_ctx.bandEv[4 * (b - 1) + 1]));
+                host_copy_band(b - 1);
+            }
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        const int nPatch = *(volatile int*)g_ctx.patchHost;
+        g_ctx.lastStreamFallback = late || nPatch > SRWL_AMD_PATCH_CAP;
+        g_ctx.lastStreamed = !g_ctx.lastStreamFallback;
+        if (g_ctx.lastStreamFallback) {
+            // counts not complete in time, or more FP64 stores than the patch
+            // area holds: the whole field, now final, the plain way
+            HIPCHK(hipStreamSynchronize(g_ctx.copyStream));
+            rc = download(hostOut->ex, dEx, compBytes);
+            if (!rc) rc = download(hostOut->ez, dEz, compBytes);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipEventSynchronize(g_ctx.bandEv[4 * (nDl - 1) + 1]));
+            host_copy_band(nDl - 1);
+            // the FP64-recomputed values over what the bands carried
+            srwd_patch* P = (srwd_patch*)(g_ctx.patchHost + 256);
+            if (nPatch > 0)
+                HIPCHK(hipMemcpy(P, dPatch, sizeof(srwd_patch) * (size_t)nPatch, hipMemcpyDeviceToHost));
+            float2* hx = (float2*)hostOut->ex;
+            float2* hz = (float2*)hostOut->ez;
+            for (int k = 0; k < nPatch; k++) { hx[P[k].off] = P[k].ex; hz[P[k].off] = P[k].ez; }
+        }
+        if (defer) return 0;
+        HIPCHK(hipStreamSynchronize(st));
+        sr_complete(res);
+        return 0;
+    }
     if (nBands > 1) {
         // bands alternate between the two compute streams, so that a band's
         // last blocks share the GPU with the next band's first ones
@@ -647,7 +828,7 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             // (at least 4 doubles per point) holds the FP64 pass's sums
             if (g_ctx.condTol > 0 && srwd_launch_sr_fp64(dItems, dBlockItem + b0, dInfo, dEdge, termin, dT64, ldI, dEx,
                                                          dEz, dWarn, fpList, dFpCount + b, dReq, dCoord, dLast,
-                                                         nBands == 1 ? dPart : nullptr, sb))
+                                                         nBands == 1 ? dPart : nullptr, nullptr, nullptr, 0, sb))
                 return SRWL_AMD_ERR_GPU;
         }
         if (nBands > 1) {
@@ -1243,6 +1424,17 @@ SRWL_EXPORT int srwlAmdSetSplit(int nSplit)
 {
     g_ctx.forceSplit = nSplit;
     return 0;
+}
+
+SRWL_EXPORT int srwlAmdSetStreamRows(int on)
+{
+    g_ctx.streamRows = on != 0;
+    return 0;
+}
+
+SRWL_EXPORT int srwlAmdLastStreamed()
+{
+    return g_ctx.lastStreamed ? 1 : 0;
 }
 
 SRWL_EXPORT int srwlAmdSetBands(int nBands)

@@ -181,6 +181,16 @@ def set_traj_serial(on=True):
     lib.srwlAmdSetTrajSerial(1 if on else 0)
 
 
+def set_stream_rows(on=True):
+    """Streamed-rows download of a single request's field (default on)."""
+    _check(lib.srwlAmdSetStreamRows(1 if on else 0))
+
+
+def last_streamed():
+    """Whether the last single-request call streamed its rows to the end."""
+    return bool(lib.srwlAmdLastStreamed())
+
+
 def set_bands(n=0):
     """Row bands of a single request's streamed download (0: automatic)."""
     _check(lib.srwlAmdSetBands(int(n)))
