@@ -173,8 +173,15 @@ char* pinned_staging(size_t bytes);
 // Copies device memory to host memory through the pinned staging on the copy
 // stream, in pieces, after the work already queued on the compute stream.
 int download(void* dst, const void* src, size_t bytes);
-// Host memcpy split over several threads (large copies out of pinned memory).
+// Host memcpy split over a pool of threads (large copies out of pinned
+// memory): one range, or several ranges as one job.
+struct CopySeg {
+    void* dst;
+    const void* src;
+    size_t n;
+};
 void par_memcpy(void* dst, const void* src, size_t bytes);
+void par_memcpy(const CopySeg* seg, int nSeg);
 // Mesh size of one axis for the automatic mesh of precPar[6] > 0.
 long sr_auto_mesh_size(double start, double fin, long n, double c, double R, double lambda, double f);
 // Completes a deferred sr_run once the stream has been synchronised.

@@ -11,11 +11,16 @@
 #include "srw_api_internal.h"
 #include "srw_fft.h"
 
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -467,7 +472,9 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
         SrRegion G = sr_region(jobs[0].mesh, info[0]);
         const long long ne = jobs[0].mesh.ne, blocks = ((long long)G.nxc * G.nzc + 255) / 256;
         if ((ne / g_ctx.tileE + ne % g_ctx.tileE) * blocks >= 1024) {
-            nDl = nBands;
+            // bands only set the copies' granularity here: more of them keep
+            // the copy after the last flag short
+            nDl = g_ctx.forceBands > 0 ? nBands : std::min(16, G.nzc);
             nBands = 1;
         }
     }
@@ -640,12 +647,15 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
     auto host_copy_band = [&](int b) {
         int r[4];
         rows_of_band(b, r);
+        CopySeg seg[4];
+        int nSeg = 0;
         for (int k = 0; k < 4; k += 2) {
             if (r[k + 1] <= r[k]) continue;
             size_t o = rowBytes * r[k], n = rowBytes * (r[k + 1] - r[k]);
-            par_memcpy((char*)hostOut->ex + o, pin + o, n);
-            par_memcpy((char*)hostOut->ez + o, pin + compBytes + o, n);
+            seg[nSeg++] = CopySeg{(char*)hostOut->ex + o, pin + o, n};
+            seg[nSeg++] = CopySeg{(char*)hostOut->ez + o, pin + compBytes + o, n};
         }
+        par_memcpy(seg, nSeg);
     };
     if (nDl) {
         // --- streamed rows (see above) ---------------------------------------
@@ -758,20 +768,23 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
                 host_copy_band(b - 1);
             }
         }
+        // the last band's host copy while the FP64 pass and the moments run
+        if (!late) {
+            HIPCHK(hipEventSynchronize(g_ctx.bandEv[4 * (nDl - 1) + 1]));
+            host_copy_band(nDl - 1);
+        }
         HIPCHK(hipStreamSynchronize(st));
         const int nPatch = *(volatile int*)g_ctx.patchHost;
         g_ctx.lastStreamFallback = late || nPatch > SRWL_AMD_PATCH_CAP;
         g_ctx.lastStreamed = !g_ctx.lastStreamFallback;
         if (g_ctx.lastStreamFallback) {
-            // counts not complete in time, or more FP64 stores than the patch
-            // area holds: the whole field, now final, the plain way
+            // flags not all seen, or more FP64 stores than the patch area
+            // holds: the whole field, now final, the plain way
             HIPCHK(hipStreamSynchronize(g_ctx.copyStream));
             rc = download(hostOut->ex, dEx, compBytes);
             if (!rc) rc = download(hostOut->ez, dEz, compBytes);
             if (rc) return rc;
         } else {
-            HIPCHK(hipEventSynchronize(g_ctx.bandEv[4 * (nDl - 1) + 1]));
-            host_copy_band(nDl - 1);
             // the FP64-recomputed values over what the bands carried
             srwd_patch* P = (srwd_patch*)(g_ctx.patchHost + 256);
             if (nPatch > 0)
@@ -903,22 +916,90 @@ char* pinned_staging(size_t bytes)
     return g_ctx.pin;
 }
 
+namespace {
+// Host copies out of the pinned staging: a pool of worker threads started on
+// first use and kept for the life of the process (starting threads per copy
+// costs more than a small band's copy).  A job is a list of ranges cut into
+// 1 MiB pieces that the workers and the calling thread take in turn.  A forked
+// child, which has none of the workers, copies on its own thread.
+class CopyPool {
+public:
+    explicit CopyPool(unsigned nWorkers) : pid_(getpid())
+    {
+        for (unsigned i = 0; i < nWorkers; i++) std::thread([this] { worker(); }).detach();
+        workers_ = nWorkers;
+    }
+    void copy(const CopySeg* seg, int nSeg)
+    {
+        size_t total = 0;
+        for (int k = 0; k < nSeg; k++) total += seg[k].n;
+        if (workers_ == 0 || total <= kPiece || getpid() != pid_) {
+            for (int k = 0; k < nSeg; k++) std::memcpy(seg[k].dst, seg[k].src, seg[k].n);
+            return;
+        }
+        std::lock_guard<std::mutex> one(job_);   // one job at a time
+        pieces_.clear();
+        for (int k = 0; k < nSeg; k++)
+            for (size_t o = 0; o < seg[k].n; o += kPiece)
+                pieces_.push_back(CopySeg{(char*)seg[k].dst + o, (const char*)seg[k].src + o,
+                                          std::min(kPiece, seg[k].n - o)});
+        next_.store(0);
+        {
+            std::lock_guard<std::mutex> l(m_);
+            pending_ = workers_;
+            gen_++;
+        }
+        cv_.notify_all();
+        run();
+        std::unique_lock<std::mutex> l(m_);
+        cvDone_.wait(l, [&] { return pending_ == 0; });
+    }
+
+private:
+    static constexpr size_t kPiece = 1 << 20;
+    void run()
+    {
+        for (size_t k; (k = next_.fetch_add(1)) < pieces_.size();)
+            std::memcpy(pieces_[k].dst, pieces_[k].src, pieces_[k].n);
+    }
+    void worker()
+    {
+        unsigned seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(m_);
+                cv_.wait(l, [&] { return gen_ != seen; });
+                seen = gen_;
+            }
+            run();
+            std::lock_guard<std::mutex> l(m_);
+            if (--pending_ == 0) cvDone_.notify_one();
+        }
+    }
+    pid_t pid_;
+    unsigned workers_ = 0;
+    std::mutex job_, m_;
+    std::condition_variable cv_, cvDone_;
+    std::vector<CopySeg> pieces_;
+    std::atomic<size_t> next_{0};
+    unsigned pending_ = 0, gen_ = 0;
+};
+
+CopyPool& copy_pool()
+{
+    // never destroyed: the workers wait on it until the process ends
+    static CopyPool* pool =
+        new CopyPool(std::min(11u, std::max(1u, std::thread::hardware_concurrency()) - 1));
+    return *pool;
+}
+}  // namespace
+
+void par_memcpy(const CopySeg* seg, int nSeg) { copy_pool().copy(seg, nSeg); }
+
 void par_memcpy(void* dst, const void* src, size_t bytes)
 {
-    const size_t minPiece = 4 << 20;
-    unsigned nt = std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency()));
-    size_t pieces = std::min<size_t>(nt, (bytes + minPiece - 1) / minPiece);
-    if (pieces <= 1) { std::memcpy(dst, src, bytes); return; }
-    std::vector<std::thread> th;
-    size_t per = (bytes + pieces - 1) / pieces;
-    for (size_t k = 1; k < pieces; k++) {
-        size_t o = k * per;
-        if (o >= bytes) break;
-        size_t n = std::min(per, bytes - o);
-        th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, n); });
-    }
-    std::memcpy(dst, src, std::min(per, bytes));
-    for (auto& t : th) t.join();
+    CopySeg s{dst, src, bytes};
+    par_memcpy(&s, 1);
 }
 
 int download(void* dst, const void* src, size_t bytes)
