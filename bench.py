@@ -416,8 +416,12 @@ def main():
         srwlpy.CalcElecFieldSR(w, 0, mag, prec)
     srwlpy.reset_stats()
     t0 = time.perf_counter()
+    call_s, streamed = [], []
     for _ in range(n_api):
+        tc = time.perf_counter()
         srwlpy.CalcElecFieldSR(w, 0, mag, prec)
+        call_s.append(time.perf_counter() - tc)
+        streamed.append(srwlpy.last_streamed())
     dt = (time.perf_counter() - t0) * a.steps / n_api     # time of a.steps calls
     st = srwlpy.stats()
     value = pts * a.steps / dt
@@ -489,7 +493,7 @@ def main():
                    "mesh": [a.n, a.n], "energies": a.ne, "trajectory_points": 10001, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VALU_PEAK_TFLOPS, "traffic": sr_traffic(a.n, a.ne),
-                     "kernel": "k_sr_integral_tile<8> + k_sr_integral", "kernel_ms": k_ms_dev,
+                     "kernel": "k_sr_integral_tile<8, false> + k_sr_integral", "kernel_ms": k_ms_dev,
                      "evals_per_launch": evals, "flops_per_eval": flops_per_eval,
                      "flops_counted": ("executed FP32 flops, an FMA counted as 2 (sin/cos excluded): 16 per evaluation "
                                        "in the single-energy kernel; 12.125 in the energy-tile kernel (4 accumulating "
@@ -500,6 +504,8 @@ def main():
                      "nonpacked_fma_ceiling_frac": 0.5},
         "device_resident_points_per_s": pts * a.steps / dt_dev,
         "sr_kernel_ms_in_api_run": k_ms,
+        "api_call_ms": {"min": 1e3 * min(call_s), "median": 1e3 * sorted(call_s)[len(call_s) // 2],
+                        "max": 1e3 * max(call_s), "streamed_rows": sum(streamed), "calls": len(call_s)},
         "single_electron_101x101": c1,
         "propagation": prop,
         "multi_electron": me,

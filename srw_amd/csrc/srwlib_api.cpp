@@ -989,7 +989,7 @@ CopyPool& copy_pool()
 {
     // never destroyed: the workers wait on it until the process ends
     static CopyPool* pool =
-        new CopyPool(std::min(11u, std::max(1u, std::thread::hardware_concurrency()) - 1));
+        new CopyPool(std::min(7u, std::max(1u, std::thread::hardware_concurrency()) - 1));
     return *pool;
 }
 }  // namespace
