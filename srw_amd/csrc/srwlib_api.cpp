@@ -11,6 +11,7 @@
 #include "srw_api_internal.h"
 #include "srw_fft.h"
 
+#include <emmintrin.h>
 #include <unistd.h>
 
 #include <algorithm>
@@ -917,6 +918,32 @@ char* pinned_staging(size_t bytes)
 }
 
 namespace {
+// A copy whose stores bypass the caches (16-byte streaming stores once the
+// destination is aligned): the caller's arrays are written once and not read
+// back here, so reading each destination line before writing it (what an
+// ordinary store does) only costs bandwidth.
+void copy_stream(void* dst, const void* src, size_t n)
+{
+    char* d = (char*)dst;
+    const char* s = (const char*)src;
+    const size_t head = std::min(n, (size_t)((16 - ((uintptr_t)d & 15)) & 15));
+    std::memcpy(d, s, head);
+    d += head; s += head; n -= head;
+    size_t k = 0;
+    for (; k + 64 <= n; k += 64) {
+        const __m128i a = _mm_loadu_si128((const __m128i*)(s + k));
+        const __m128i b = _mm_loadu_si128((const __m128i*)(s + k + 16));
+        const __m128i c = _mm_loadu_si128((const __m128i*)(s + k + 32));
+        const __m128i e = _mm_loadu_si128((const __m128i*)(s + k + 48));
+        _mm_stream_si128((__m128i*)(d + k), a);
+        _mm_stream_si128((__m128i*)(d + k + 16), b);
+        _mm_stream_si128((__m128i*)(d + k + 32), c);
+        _mm_stream_si128((__m128i*)(d + k + 48), e);
+    }
+    std::memcpy(d + k, s + k, n - k);
+    _mm_sfence();
+}
+
 // Host copies out of the pinned staging: a pool of worker threads started on
 // first use and kept for the life of the process (starting threads per copy
 // costs more than a small band's copy).  A job is a list of ranges cut into
@@ -934,7 +961,7 @@ public:
         size_t total = 0;
         for (int k = 0; k < nSeg; k++) total += seg[k].n;
         if (workers_ == 0 || total <= kPiece || getpid() != pid_) {
-            for (int k = 0; k < nSeg; k++) std::memcpy(seg[k].dst, seg[k].src, seg[k].n);
+            for (int k = 0; k < nSeg; k++) copy_stream(seg[k].dst, seg[k].src, seg[k].n);
             return;
         }
         std::lock_guard<std::mutex> one(job_);   // one job at a time
@@ -960,7 +987,7 @@ private:
     void run()
     {
         for (size_t k; (k = next_.fetch_add(1)) < pieces_.size();)
-            std::memcpy(pieces_[k].dst, pieces_[k].src, pieces_[k].n);
+            copy_stream(pieces_[k].dst, pieces_[k].src, pieces_[k].n);
     }
     void worker()
     {
