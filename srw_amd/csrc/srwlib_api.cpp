@@ -731,49 +731,50 @@ int sr_run(std::vector<SrJob>& jobs, const std::vector<srwd_field>& fields, floa
             rc = hostOut->queueMore(hostOut->queueCtx, res);
             if (rc) return rc;
         }
-        // each band's download once its blocks are counted; the host copy of
-        // band b - 1 while band b is waited for and travels
-        // A block's flag is visible once its stores are; blocks finish nearly
-        // in launch order, so a cursor walks the flags in that order.  Once
-        // the kernel is known to have finished, a flag still missing means
-        // the flags do not reach the host: fall back.
+        // Each band's copy is queued as soon as the flags of its blocks are
+        // set, whatever the host copies' progress (the staging holds the
+        // whole field), and the bands are copied on into the caller's arrays
+        // in order as their copies land.  A block's flag is visible once its
+        // stores are; blocks finish nearly in launch order, so a cursor walks
+        // the flags in that order.  Once the kernel is known to have
+        // finished, a flag still missing means the flags do not reach the
+        // host: fall back.
         const auto t0 = std::chrono::steady_clock::now();
         bool late = false;
-        int cursor = 0;
-        for (int b = 0; b < nDl && !late; b++) {
-            for (unsigned spin = 0; cursor <= dlLast[b];) {
-                if (done[cursor] == epoch) { cursor++; continue; }
-                if ((++spin & 1023) == 0) {
-                    const bool finished = hipEventQuery(g_ctx.evTiles) == hipSuccess;
-                    if ((finished && done[cursor] != epoch) ||
-                        std::chrono::steady_clock::now() - t0 > std::chrono::seconds(SRWL_AMD_STREAM_WAIT_S)) {
-                        late = true;
-                        break;
-                    }
+        int cursor = 0, nextDl = 0, nextCopy = 0;
+        for (unsigned spin = 0; nextCopy < nDl && !late;) {
+            while (cursor < nTile && done[cursor] == epoch) cursor++;
+            bool progress = false;
+            while (nextDl < nDl && cursor > dlLast[nextDl]) {
+                const int b = nextDl++;
+                if (b == 0) HIPCHK(hipStreamWaitEvent(g_ctx.copyStream, g_ctx.evJoin, 0));
+                int r[4];
+                rows_of_band(b, r);
+                for (int k = 0; k < 4; k += 2) {
+                    if (r[k + 1] <= r[k]) continue;
+                    size_t o = rowBytes * r[k], n = rowBytes * (r[k + 1] - r[k]);
+                    HIPCHK(hipMemcpyAsync(pin + o, (char*)dEx + o, n, hipMemcpyDeviceToHost, g_ctx.copyStream));
+                    HIPCHK(hipMemcpyAsync(pin + compBytes + o, (char*)dEz + o, n, hipMemcpyDeviceToHost,
+                                          g_ctx.copyStream));
                 }
+                HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 1], g_ctx.copyStream));
+                progress = true;
             }
-            if (late) break;
-            if (b == 0) HIPCHK(hipStreamWaitEvent(g_ctx.copyStream, g_ctx.evJoin, 0));
-            int r[4];
-            rows_of_band(b, r);
-            for (int k = 0; k < 4; k += 2) {
-                if (r[k + 1] <= r[k]) continue;
-                size_t o = rowBytes * r[k], n = rowBytes * (r[k + 1] - r[k]);
-                HIPCHK(hipMemcpyAsync(pin + o, (char*)dEx + o, n, hipMemcpyDeviceToHost, g_ctx.copyStream));
-                HIPCHK(hipMemcpyAsync(pin + compBytes + o, (char*)dEz + o, n, hipMemcpyDeviceToHost,
-                                      g_ctx.copyStream));
+            if (nextCopy < nextDl && hipEventQuery(g_ctx.bandEv[4 * nextCopy + 1]) == hipSuccess) {
+                host_copy_band(nextCopy++);
+                progress = true;
             }
-            HIPCHK(hipEventRecord(g_ctx.bandEv[4 * b + 1], g_ctx.copyStream));
-            if (b > 0) {
-                HIPCHK(hipEventSynchronize(g_ctx.bandEv[4 * (b - 1) + 1]));
-                host_copy_band(b - 1);
+            if (progress || (++spin & 1023)) continue;
+            const bool timedOut =
+                std::chrono::steady_clock::now() - t0 > std::chrono::seconds(SRWL_AMD_STREAM_WAIT_S);
+            if (nextDl < nDl) {
+                const bool finished = hipEventQuery(g_ctx.evTiles) == hipSuccess;
+                if ((finished && done[cursor] != epoch) || timedOut) late = true;
+            } else if (timedOut) {
+                late = true;
             }
         }
-        // the last band's host copy while the FP64 pass and the moments run
-        if (!late) {
-            HIPCHK(hipEventSynchronize(g_ctx.bandEv[4 * (nDl - 1) + 1]));
-            host_copy_band(nDl - 1);
-        }
+        (void)hipGetLastError();   // the queries' "not ready"
         HIPCHK(hipStreamSynchronize(st));
         const int nPatch = *(volatile int*)g_ctx.patchHost;
         g_ctx.lastStreamFallback = late || nPatch > SRWL_AMD_PATCH_CAP;
